@@ -1,0 +1,129 @@
+/*
+ * hmotif.h — C ABI of libhmotif, an MI355X (gfx950) engine for exact hypergraph-motif counting.
+ *
+ * The ABI takes plain pointers and sizes only. The library owns all device memory; the caller
+ * owns host arrays, which are borrowed for the duration of a call. Every entry point returns an
+ * hm_status (never exits the process); hm_last_error() gives a message for the calling thread.
+ *
+ * Hypergraph input format (all entry points): hyperedge e has the nodes
+ * nodes[offsets[e] .. offsets[e+1]) — int32 node ids >= 0, offsets int64, offsets[0] == 0.
+ * Repeated nodes inside one hyperedge are dropped, as MoCHy's reader does
+ * (MoCHy-master/read_data.cpp:34). Hyperedges with identical node sets are distinct hyperedges
+ * (MoCHy keeps them) in static counts; in temporal counts they are copies of one static
+ * hyperedge (THyMe-main/code/main_thymeP.cpp:164-179).
+ *
+ * Output numbering:
+ *   static  — 30 bins in MoCHy order (MoCHy-master/motif_id.cpp:5-34). MoCHy prints 26 of them
+ *             ("motif 1..26" = bins 2,3,5,7,8,...,29; MoCHy-master/main_exact.cpp:133-137);
+ *             bins 0,1,4,6 only occur with duplicate hyperedges.
+ *   temporal — 96 bins in THyMe order (THyMe-main/code/motif_id.cpp:25-42), bin i = motif i+1.
+ */
+#ifndef HMOTIF_H
+#define HMOTIF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HM_STATIC_BINS 30
+#define HM_TEMPORAL_BINS 96
+/* Largest hyperedge (in nodes) the counting kernels accept. */
+#define HM_MAX_EDGE_SIZE 64
+
+typedef enum hm_status {
+  HM_OK = 0,
+  HM_ERR_OOM = 1,          /* a device or host allocation failed */
+  HM_ERR_OVER_BUDGET = 2,  /* the request does not fit the HBM budget / an index range */
+  HM_ERR_BAD_INPUT = 3,    /* malformed arguments */
+  HM_ERR_DEVICE = 4,       /* HIP runtime error or no usable gfx950 device */
+  HM_ERR_UNSUPPORTED = 5,  /* input outside what the engine handles (e.g. hyperedge > 64 nodes) */
+  HM_ERR_BUFFER = 6        /* a caller-provided buffer is too small */
+} hm_status;
+
+const char* hm_status_string(hm_status st);
+/* Message for the last failing call on this thread ("" when the last call succeeded). */
+const char* hm_last_error(void);
+
+/* Venn-id -> bin tables used by the kernels: out[id] = 0-based bin, -1 = not a connected triple.
+ * which = 0: static MoCHy bins (MoCHy-master/motif_id.cpp:5-22, minus 1);
+ * which = 1: temporal THyMe bins, ids in time order (THyMe-main/code/motif_id.cpp:25-42, minus 1). */
+hm_status hm_bin_table(int32_t which, int8_t out[128]);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Seeded synthetic generator (replaces dynamicHyperGraph/src/graphGeneration.cpp:10-51,      */
+/* which is unseeded). Sizes: power law P(k) ~ k^-alpha on [kmin, kmax]; members: k distinct  */
+/* nodes by successive sampling from Zipf(zipf_s) over ranks 1..num_nodes (id = rank - 1).    */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct hm_gen_params {
+  int64_t num_nodes;
+  int64_t num_edges;
+  double alpha;
+  int32_t kmin;
+  int32_t kmax;
+  double zipf_s;
+  uint64_t seed;
+} hm_gen_params;
+
+/* Number of incidences (sum of sizes) hm_gen_static will produce for these parameters. */
+hm_status hm_gen_static_size(const hm_gen_params* p, int64_t* num_incidences);
+/* offsets: num_edges + 1 entries; nodes: nodes_capacity >= hm_gen_static_size(). */
+hm_status hm_gen_static(const hm_gen_params* p, int64_t* offsets, int32_t* nodes,
+                        int64_t nodes_capacity);
+/* n strictly increasing int64 timestamps (microseconds): singleton gaps Exp(mean_gap_us);
+ * round(n*burst_fraction/burst_len) bursts of burst_len edges with Exp(burst_gap_us) gaps;
+ * every gap is rounded and at least 1 us. */
+hm_status hm_gen_timestamps(int64_t n, double mean_gap_us, double burst_fraction,
+                            int32_t burst_len, double burst_gap_us, uint64_t seed, int64_t* times);
+/* k distinct indices drawn uniformly from [0, n) (seeded partial Fisher-Yates). */
+hm_status hm_gen_pick(int64_t n, int64_t k, uint64_t seed, int64_t* idx);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Device runtime                                                                             */
+/* ------------------------------------------------------------------------------------------ */
+/* Selects the HIP device used by handles created afterwards on this thread (default 0) and
+ * checks that it is a gfx950 part. */
+hm_status hm_init(int32_t device);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Static hypergraph handle: flat sorted H2V / V2H arrays resident in HBM.                    */
+/* Replaces CBSTOperations::construct (dynamicHyperGraph/include/structure.hpp:45-64) and     */
+/* the host projection hyperedgeAdjacency (dynamicHyperGraph/utils/utils.cpp:399-427).        */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct hm_graph hm_graph;
+
+/* num_nodes <= 0 means max node id + 1 (MoCHy's V). Copies the CSR to the device. */
+hm_status hm_graph_create(const int64_t* offsets, const int32_t* nodes, int32_t num_edges,
+                          int32_t num_nodes, hm_graph** out);
+hm_status hm_graph_destroy(hm_graph* g);
+
+/* Exact static h-motif counts, 30 MoCHy bins.
+ * Replaces computeMotifCounts (dynamicHyperGraph/include/motif.hpp:10-13,
+ * dynamicHyperGraph/src/HMotifCount.cu:97-125), which counted closed triples only into int32
+ * and printed them; the count here follows MoCHy-E (MoCHy-master/main_exact_par.cpp:122-165). */
+hm_status hm_count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]);
+
+/* Graph statistics computed on the device by the last hm_count_static call on g. */
+typedef struct hm_graph_stats {
+  int64_t num_edges;          /* E */
+  int64_t num_nodes;          /* V */
+  int64_t num_incidences;     /* I */
+  int64_t proj_pairs;         /* P = sum_a d_a (directed projected pairs) */
+  int64_t max_proj_degree;    /* max_a d_a */
+  int64_t oriented_row_sum;   /* sum_a sum_{b in N(a), b > a} d_b */
+  int64_t wedges;             /* sum_a C(d_a, 2) */
+  int64_t triangles;          /* closed triples (including the 4 hidden bins) */
+  int64_t max_node_degree;    /* max_v |V2H(v)| */
+} hm_graph_stats;
+hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out);
+
+/* Kernel timing of the last hm_count_static on g, measured with HIP events on the handle's
+ * stream: names are "project" and "triangles"; ms receives the duration of that phase. */
+hm_status hm_graph_phase_ms(hm_graph* g, const char* phase, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMOTIF_H */

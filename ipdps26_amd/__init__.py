@@ -1,0 +1,256 @@
+"""ipdps26_amd — Python host side of libhmotif, an MI355X engine for exact hypergraph motifs.
+
+Everything here is a thin ctypes layer over the C ABI declared in ``include/hmotif.h``; the
+counting itself runs in the hand-written HIP kernels of ``libhmotif.so`` (built in-tree by
+``__graft_entry__.build()`` or ``make -C ipdps26_amd/csrc``). If the library is missing, every
+entry point raises: there is no CPU fallback.
+
+Output numbering follows the reference: static counts are the 30 MoCHy bins
+(MoCHy-master/motif_id.cpp:5-34; ``printed_motifs`` gives MoCHy's 26 printed ones), temporal
+counts the 96 THyMe bins (THyMe-main/code/motif_id.cpp:25-42).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhmotif.so")
+
+STATIC_BINS = 30
+TEMPORAL_BINS = 96
+HIDDEN_STATIC_BINS = (0, 1, 4, 6)
+PRINTED_STATIC_BINS = tuple(b for b in range(STATIC_BINS) if b not in HIDDEN_STATIC_BINS)
+
+_STATUS = {
+    0: "ok",
+    1: "out of memory",
+    2: "over budget",
+    3: "bad input",
+    4: "device error",
+    5: "unsupported input",
+    6: "buffer too small",
+}
+
+
+class HMotifError(RuntimeError):
+    """A libhmotif call returned a status other than HM_OK."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+
+
+class _GenParams(ctypes.Structure):
+    _fields_ = [
+        ("num_nodes", ctypes.c_int64),
+        ("num_edges", ctypes.c_int64),
+        ("alpha", ctypes.c_double),
+        ("kmin", ctypes.c_int32),
+        ("kmax", ctypes.c_int32),
+        ("zipf_s", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class _GraphStats(ctypes.Structure):
+    _fields_ = [
+        ("num_edges", ctypes.c_int64),
+        ("num_nodes", ctypes.c_int64),
+        ("num_incidences", ctypes.c_int64),
+        ("proj_pairs", ctypes.c_int64),
+        ("max_proj_degree", ctypes.c_int64),
+        ("oriented_row_sum", ctypes.c_int64),
+        ("wedges", ctypes.c_int64),
+        ("triangles", ctypes.c_int64),
+        ("max_node_degree", ctypes.c_int64),
+    ]
+
+
+_P = ctypes.POINTER
+_i32p = _P(ctypes.c_int32)
+_i64p = _P(ctypes.c_int64)
+_u64p = _P(ctypes.c_uint64)
+
+# (name, restype, argtypes) of every C entry point this module binds.
+_SIGNATURES = [
+    ("hm_status_string", ctypes.c_char_p, [ctypes.c_int]),
+    ("hm_last_error", ctypes.c_char_p, []),
+    ("hm_bin_table", ctypes.c_int, [ctypes.c_int32, _P(ctypes.c_int8)]),
+    ("hm_gen_static_size", ctypes.c_int, [_P(_GenParams), _i64p]),
+    ("hm_gen_static", ctypes.c_int, [_P(_GenParams), _i64p, _i32p, ctypes.c_int64]),
+    ("hm_gen_timestamps", ctypes.c_int,
+     [ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
+      ctypes.c_uint64, _i64p]),
+    ("hm_gen_pick", ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, _i64p]),
+    ("hm_init", ctypes.c_int, [ctypes.c_int32]),
+    ("hm_graph_create", ctypes.c_int,
+     [_i64p, _i32p, ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_void_p)]),
+    ("hm_graph_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("hm_count_static", ctypes.c_int, [ctypes.c_void_p, _u64p]),
+    ("hm_graph_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(_GraphStats)]),
+    ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
+]
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libhmotif.so from the package directory (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() or "
+                "`make -C ipdps26_amd/csrc`")
+        so = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGNATURES:
+            fn = getattr(so, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = so
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise HMotifError(status, lib().hm_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(_P(ctype))
+
+
+def bin_table(which: str = "static") -> np.ndarray:
+    """The kernels' Venn-id -> 0-based bin table (int8[128], -1 = not a connected triple)."""
+    out = np.zeros(128, dtype=np.int8)
+    _check(lib().hm_bin_table(0 if which == "static" else 1, _ptr(out, ctypes.c_int8)))
+    return out
+
+
+# ------------------------------------------------------------------------------------------
+# Generator
+# ------------------------------------------------------------------------------------------
+@dataclass(frozen=True)
+class GenParams:
+    """Synthetic hypergraph parameters: power-law sizes on [kmin, kmax], Zipf node popularity."""
+
+    num_nodes: int
+    num_edges: int
+    alpha: float
+    kmin: int
+    kmax: int
+    zipf_s: float
+    seed: int
+
+    def _c(self) -> _GenParams:
+        return _GenParams(self.num_nodes, self.num_edges, self.alpha, self.kmin, self.kmax,
+                          self.zipf_s, self.seed)
+
+
+def generate_static(p: GenParams) -> tuple[np.ndarray, np.ndarray]:
+    """Returns (offsets int64[E+1], nodes int32[I]) of the seeded synthetic hypergraph."""
+    cp = p._c()
+    total = ctypes.c_int64(0)
+    _check(lib().hm_gen_static_size(ctypes.byref(cp), ctypes.byref(total)))
+    offsets = np.zeros(p.num_edges + 1, dtype=np.int64)
+    nodes = np.zeros(max(total.value, 1), dtype=np.int32)
+    _check(lib().hm_gen_static(ctypes.byref(cp), _ptr(offsets, ctypes.c_int64),
+                               _ptr(nodes, ctypes.c_int32), nodes.size))
+    return offsets, nodes[: total.value]
+
+
+def generate_timestamps(n: int, seed: int, mean_gap_us: float = 1e6, burst_fraction: float = 0.3,
+                        burst_len: int = 50, burst_gap_us: float = 1e4) -> np.ndarray:
+    """Strictly increasing int64 microsecond timestamps (exponential gaps plus bursts)."""
+    out = np.zeros(max(n, 1), dtype=np.int64)
+    _check(lib().hm_gen_timestamps(n, mean_gap_us, burst_fraction, burst_len, burst_gap_us, seed,
+                                   _ptr(out, ctypes.c_int64)))
+    return out[:n]
+
+
+def pick(n: int, k: int, seed: int) -> np.ndarray:
+    """k distinct indices drawn uniformly from [0, n)."""
+    out = np.zeros(max(k, 1), dtype=np.int64)
+    _check(lib().hm_gen_pick(n, k, seed, _ptr(out, ctypes.c_int64)))
+    return out[:k]
+
+
+# ------------------------------------------------------------------------------------------
+# Static graph handle
+# ------------------------------------------------------------------------------------------
+def init(device: int = 0) -> None:
+    """Selects the HIP device for handles created afterwards (must be gfx950)."""
+    _check(lib().hm_init(device))
+
+
+def as_csr(edges) -> tuple[np.ndarray, np.ndarray]:
+    """Converts a list of node lists into (offsets int64, nodes int32)."""
+    offsets = np.zeros(len(edges) + 1, dtype=np.int64)
+    flat = []
+    for i, e in enumerate(edges):
+        flat.extend(int(v) for v in e)
+        offsets[i + 1] = len(flat)
+    return offsets, np.asarray(flat, dtype=np.int32)
+
+
+class Graph:
+    """A hypergraph resident in HBM (flat sorted H2V and V2H arrays)."""
+
+    def __init__(self, offsets: np.ndarray, nodes: np.ndarray, num_nodes: int = 0):
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        if nodes.size == 0:
+            nodes = np.zeros(1, dtype=np.int32)
+        self._h = ctypes.c_void_p()
+        _check(lib().hm_graph_create(_ptr(offsets, ctypes.c_int64), _ptr(nodes, ctypes.c_int32),
+                                     offsets.size - 1, num_nodes, ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().hm_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def count_static(self) -> np.ndarray:
+        """Exact static h-motif counts: uint64[30] in MoCHy bin order."""
+        out = np.zeros(STATIC_BINS, dtype=np.uint64)
+        _check(lib().hm_count_static(self._h, _ptr(out, ctypes.c_uint64)))
+        return out
+
+    def stats(self) -> dict:
+        st = _GraphStats()
+        _check(lib().hm_graph_get_stats(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in _GraphStats._fields_}
+
+    def phase_ms(self, phase: str) -> float:
+        ms = ctypes.c_double(0.0)
+        _check(lib().hm_graph_phase_ms(self._h, phase.encode(), ctypes.byref(ms)))
+        return ms.value
+
+
+def printed_motifs(counts30) -> list[int]:
+    """MoCHy's 26 printed motifs (motif 1..26) from the 30 bins (MoCHy-master/main_exact.cpp:133-137)."""
+    return [int(counts30[b]) for b in PRINTED_STATIC_BINS]
+
+
+def algorithmic_bytes_static(st: dict) -> int:
+    """Bytes a static count must move at minimum under the engine's cost model:
+    4(2I + E + V + 2) (CSR in) + 16 P (projection written and read at 8 B per pair)
+    + 8 * sum_a sum_{b in N(a), b > a} d_b (neighbour rows read for closed triples) + 240 (histogram)."""
+    return (4 * (2 * st["num_incidences"] + st["num_edges"] + st["num_nodes"] + 2)
+            + 16 * st["proj_pairs"] + 8 * st["oriented_row_sum"] + 240)

@@ -1,0 +1,98 @@
+// Host-side definition of the hm_graph handle and device buffer helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "hm_internal.hpp"
+
+namespace hm {
+
+// A device allocation that grows on demand and is reused across calls.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+  hm_status ensure(size_t need) {
+    if (need <= bytes && p) return HM_OK;
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    size_t want = need ? need : 16;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
+      return e == hipErrorOutOfMemory ? HM_ERR_OOM : HM_ERR_DEVICE;
+    }
+    bytes = want;
+    return HM_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// Flat, sorted, HBM-resident incidence arrays of one hypergraph.
+//   H2V: eoff[E+1] (uint32), enodes[I] (int32, ascending per row), esize[E] (uint8, 0 = deleted)
+//   V2H: voff[V+1] (uint32), vedges[I] (int32, ascending per row)
+struct DeviceCsr {
+  int32_t E = 0, V = 0;
+  int64_t I = 0;
+  int32_t max_size = 0;
+  DevBuf eoff, enodes, esize, voff, vedges;
+  void release() {
+    eoff.release();
+    enodes.release();
+    esize.release();
+    voff.release();
+    vedges.release();
+  }
+};
+
+// Scratch for one static count (reused across calls on the same handle).
+struct CountScratch {
+  DevBuf deg;      // uint32[E]   projected degree d_a (by id)
+  DevBuf ids;      // int32[E]    iota, radix-sort values
+  DevBuf perm;     // int32[E]    rank -> id (ascending degree, ties by id)
+  DevBuf rank;     // int32[E]    id -> rank
+  DevBuf degr;     // uint32[E+1] degree by rank (sorted keys), then scanned into roff
+  DevBuf roff;     // uint64[E+1] row offsets by rank (capacity = d_a)
+  DevBuf rlen;     // uint32[E]   upper-row length d+ by rank
+  DevBuf rows;     // uint32[P]   packed upper rows (rank << 7 | overlap)
+  DevBuf szr;      // uint8[E]    hyperedge size by rank
+  DevBuf work;     // uint64[E]   triangle-kernel work estimate by rank
+  DevBuf work2;    // uint64[E]   sorted work (discarded)
+  DevBuf order;    // int32[E]    centre ranks, most work first
+  DevBuf ranks;    // int32[E]    iota for the work sort
+  DevBuf orient;   // uint64[E]   sum_{b in N(a), b > a} d_b per a
+  DevBuf acc;      // uint64[64]  histogram (30 bins) + statistics
+  DevBuf tmp;      // hipcub temporary storage
+  void release() {
+    for (DevBuf* b : {&deg, &ids, &perm, &rank, &degr, &roff, &rlen, &rows, &szr, &work, &work2,
+                      &order, &ranks, &orient, &acc, &tmp})
+      b->release();
+  }
+};
+
+}  // namespace hm
+
+struct hm_graph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hm::DeviceCsr csr;
+  hm::CountScratch scratch;
+  // host mirror of the hyperedge lists (sorted, unique), kept for updates and statistics
+  std::vector<int64_t> h_off;
+  std::vector<int32_t> h_nodes;
+  hm_graph_stats stats{};
+  bool have_stats = false;
+  float ms_project = 0.f, ms_triangles = 0.f;
+};

@@ -1,0 +1,41 @@
+// Internal helpers shared by the libhmotif translation units: error reporting and checks.
+#pragma once
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/hmotif.h"
+
+namespace hm {
+
+// Per-thread message for hm_last_error(); every C entry point clears it on entry.
+void set_error(const char* fmt, ...);
+void clear_error();
+
+inline hm_status fail(hm_status st, const char* msg) {
+  set_error("%s", msg);
+  return st;
+}
+
+// Device selected by hm_init on this thread.
+int current_device();
+
+}  // namespace hm
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+namespace hm {
+hm_status check_hip(hipError_t e, const char* what, const char* file, int line);
+}
+// Evaluates a HIP call; on failure records the message and returns from the enclosing function.
+#define HM_HIP(call)                                                          \
+  do {                                                                        \
+    hm_status _st = hm::check_hip((call), #call, __FILE__, __LINE__);         \
+    if (_st != HM_OK) return _st;                                             \
+  } while (0)
+#define HM_TRY(expr)                  \
+  do {                                \
+    hm_status _st = (expr);           \
+    if (_st != HM_OK) return _st;     \
+  } while (0)
+#endif

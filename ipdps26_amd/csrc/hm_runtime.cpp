@@ -1,0 +1,79 @@
+// Error reporting and device selection for libhmotif.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "hm_internal.hpp"
+#include "hm_tables.hpp"
+
+namespace hm {
+
+static thread_local std::string t_error;
+static thread_local int t_device = 0;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  t_error = buf;
+}
+
+void clear_error() { t_error.clear(); }
+
+int current_device() { return t_device; }
+
+hm_status check_hip(hipError_t e, const char* what, const char* file, int line) {
+  if (e == hipSuccess) return HM_OK;
+  set_error("%s failed at %s:%d: %s", what, file, line, hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? HM_ERR_OOM : HM_ERR_DEVICE;
+}
+
+}  // namespace hm
+
+extern "C" {
+
+const char* hm_status_string(hm_status st) {
+  switch (st) {
+    case HM_OK: return "ok";
+    case HM_ERR_OOM: return "out of memory";
+    case HM_ERR_OVER_BUDGET: return "over budget";
+    case HM_ERR_BAD_INPUT: return "bad input";
+    case HM_ERR_DEVICE: return "device error";
+    case HM_ERR_UNSUPPORTED: return "unsupported input";
+    case HM_ERR_BUFFER: return "buffer too small";
+  }
+  return "unknown status";
+}
+
+const char* hm_last_error(void) { return hm::t_error.c_str(); }
+
+hm_status hm_bin_table(int32_t which, int8_t out[128]) {
+  hm::clear_error();
+  if (!out || (which != 0 && which != 1)) return hm::fail(HM_ERR_BAD_INPUT, "hm_bin_table: bad argument");
+  const int8_t* t = which == 0 ? hm::kStaticBin : hm::kTemporalBin;
+  for (int i = 0; i < 128; ++i) out[i] = t[i];
+  return HM_OK;
+}
+
+hm_status hm_init(int32_t device) {
+  hm::clear_error();
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return hm::fail(HM_ERR_DEVICE, "hm_init: no HIP device visible");
+  if (device < 0 || device >= n) return hm::fail(HM_ERR_BAD_INPUT, "hm_init: device index out of range");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+    return hm::fail(HM_ERR_DEVICE, "hm_init: hipGetDeviceProperties failed");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    hm::set_error("hm_init: device %d is %s; libhmotif is built for gfx950 only", device,
+                  prop.gcnArchName);
+    return HM_ERR_DEVICE;
+  }
+  hm::t_device = device;
+  return HM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+"""Pins the CPU oracle: reference golden vectors, brute force, closed forms and invariants.
+
+The oracle restates MoCHy-E (MoCHy-master/main_exact_par.cpp:74-173) and THyMe+
+(THyMe-main/code/main_thymeP.cpp:54-389). Golden vectors come from the reference binaries
+(tests/golden/make_golden.py); the brute-force definitions and closed forms pin the four bins
+MoCHy never prints and every temporal bin independently of the restatement's control flow.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import GOLDEN, csr, load_golden, random_hypergraph
+
+import ipdps26_amd as hm
+from ipdps26_amd import workloads as W
+
+HIDDEN = (0, 1, 4, 6)
+OPEN = range(20, 26)
+
+
+def printed(c30):
+    return [int(c30[b]) for b in range(30) if b not in HIDDEN]
+
+
+# ----------------------------------------------------------------------------- static goldens
+def test_dblp_matches_reference_mochy():
+    g = [x for x in load_golden("static") if x["name"] == "dblp"][0]
+    off, nodes = W.read_mochy_csv(os.path.join(GOLDEN, g["file"]))
+    assert printed(O.static_mochy(off, nodes)) == g["printed_motifs"]
+
+
+@pytest.mark.parametrize("name", ["c1_family_e800", "c2_family_e3000", "c3_family_e6000",
+                                  "tiny_dense_e60"])
+def test_generated_goldens(name):
+    g = [x for x in load_golden("static") if x["name"] == name][0]
+    p = hm.GenParams(**g["params"])
+    off, nodes = hm.generate_static(p)
+    assert printed(O.static_mochy(off, nodes)) == g["printed_motifs"]
+
+
+def test_golden_inputs_are_reproduced(tmp_path):
+    """The generator still emits byte-identical files for every generated golden."""
+    import hashlib
+
+    for g in load_golden("static"):
+        if g["kind"] != "generated":
+            continue
+        p = hm.GenParams(**g["params"])
+        path = tmp_path / (g["name"] + ".csv")
+        W.write_mochy_csv(str(path), *hm.generate_static(p))
+        assert hashlib.sha256(path.read_bytes()).hexdigest() == g["sha256_csv"], g["name"]
+
+
+# ----------------------------------------------------------------------------- static brute force
+@pytest.mark.parametrize("seed", range(12))
+def test_mochy_restatement_equals_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(3, 70)),
+                                   num_nodes=int(rng.integers(3, 25)), kmax=int(rng.integers(1, 8)))
+    assert np.array_equal(O.static_mochy(off, nodes), O.static_brute(off, nodes))
+
+
+def test_brute_force_on_golden_dblp_slice():
+    off, nodes = W.read_mochy_csv(os.path.join(GOLDEN, "dblp_graph.txt"))
+    k = 400
+    assert np.array_equal(O.static_mochy(off[: k + 1], nodes[: off[k]]),
+                          O.static_brute(off[: k + 1], nodes[: off[k]]))
+
+
+# ----------------------------------------------------------------------------- closed forms
+def test_star_sharing_one_node():
+    # k hyperedges {0, private_i}: every triple is closed, shares node 0 only, each has a private node
+    k = 9
+    off, nodes = csr([[0, 100 + i] for i in range(k)])
+    for fn in (O.static_mochy, O.static_brute):
+        c = fn(off, nodes)
+        assert c[3] == math.comb(k, 3) and c.sum() == math.comb(k, 3)
+
+
+def test_open_path():
+    # a-b-c with a ∩ c = ∅; b is covered by a ∪ c; a and c have private nodes -> bin 24
+    off, nodes = csr([[0, 1], [1, 2], [2, 3]])
+    for fn in (O.static_mochy, O.static_brute):
+        c = fn(off, nodes)
+        assert c[24] == 1 and c.sum() == 1
+
+
+def test_identical_edges_hidden_bin0():
+    off, nodes = csr([[4, 5, 6]] * 3)
+    for fn in (O.static_mochy, O.static_brute):
+        c = fn(off, nodes)
+        assert c[0] == 1 and c.sum() == 1
+
+
+def test_closed_without_common_node():
+    # pairwise overlaps {1}, {2}, {3}, no common node, private nodes -> MoCHy bin 29 (motif 26)
+    off, nodes = csr([[1, 3, 10], [1, 2, 11], [2, 3, 12]])
+    for fn in (O.static_mochy, O.static_brute):
+        c = fn(off, nodes)
+        assert c[29] == 1 and c.sum() == 1
+
+
+def test_disconnected_triples_not_counted():
+    off, nodes = csr([[0, 1], [2, 3], [4, 5], [1, 9]])
+    for fn in (O.static_mochy, O.static_brute):
+        assert fn(off, nodes).sum() == 0
+
+
+# ----------------------------------------------------------------------------- invariant
+@pytest.mark.parametrize("name", ["c1_family_e800", "tiny_dense_e60"])
+def test_wedge_invariant(name):
+    """sum(open) + 3 sum(closed) = sum_a C(d_a, 2): every wedge is one open triple at its centre
+    or one of the three wedges of a closed triple."""
+    g = [x for x in load_golden("static") if x["name"] == name][0]
+    off, nodes = hm.generate_static(hm.GenParams(**g["params"]))
+    c = O.static_mochy(off, nodes)
+    deg = O.projection_degrees(off, nodes)
+    open_ = sum(int(c[b]) for b in OPEN)
+    closed = int(c.sum()) - open_
+    assert open_ + 3 * closed == int((deg * (deg - 1) // 2).sum())
+
+
+def test_tables_have_expected_shape():
+    s = O.mochy_table()
+    t = O.thyme_temporal_table()
+    assert sum(1 for x in s if x > 0) == 96 and sorted(set(s)) == list(range(31))
+    assert sorted(x for x in t if x > 0) == list(range(1, 97))
+    # static bins are invariant under relabelling the three hyperedges
+    perms = [(0, 1, 2), (1, 0, 2), (0, 2, 1), (2, 1, 0), (1, 2, 0), (2, 0, 1)]
+
+    def permute(i, p):
+        single = [(i >> k) & 1 for k in range(3)]
+        pair = {(0, 1): (i >> 3) & 1, (1, 2): (i >> 4) & 1, (2, 0): (i >> 5) & 1}
+        pair.update({(1, 0): pair[(0, 1)], (2, 1): pair[(1, 2)], (0, 2): pair[(2, 0)]})
+        out = sum(single[p[k]] << k for k in range(3))
+        out |= pair[(p[0], p[1])] << 3 | pair[(p[1], p[2])] << 4 | pair[(p[2], p[0])] << 5
+        return out | (i & 64)
+
+    for i in range(128):
+        assert len({s[permute(i, p)] for p in perms}) == 1
+
+
+# ----------------------------------------------------------------------------- temporal
+@pytest.mark.parametrize("name", ["enron_distinct_d100000", "c4_family_e3000"])
+def test_temporal_goldens(name):
+    g = [x for x in load_golden("temporal") if x["name"] == name][0]
+    if g["kind"] == "fixture":
+        off, nodes, _ = W.read_thyme_tsv(os.path.join(GOLDEN, g["file"]))
+        times = (np.arange(off.size - 1, dtype=np.int64) + 1) * 1000
+    else:
+        p = g["params"]
+        off, nodes, times = W.temporal_c4(p["num_edges"], p["num_nodes"])
+    assert O.thyme_temporal(off, nodes, times, g["delta"]).tolist() == g["counts96"]
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_thyme_restatement_equals_brute_force(seed):
+    rng = np.random.default_rng(100 + seed)
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(3, 60)),
+                                   num_nodes=int(rng.integers(3, 20)), kmax=int(rng.integers(1, 6)),
+                                   dup_frac=0.25)
+    E = off.size - 1
+    times = np.cumsum(rng.integers(1, 20, size=E)).astype(np.int64)
+    delta = int(rng.integers(5, 200))
+    assert np.array_equal(O.thyme_temporal(off, nodes, times, delta),
+                          O.temporal_brute(off, nodes, times, delta))
+
+
+def test_temporal_copies_of_one_edge():
+    m = 7
+    off, nodes = csr([[1, 2, 3]] * m)
+    times = np.arange(1, m + 1, dtype=np.int64) * 10
+    for fn in (O.thyme_temporal, O.temporal_brute):
+        c = fn(off, nodes, times, 10 * m)
+        assert c[95] == math.comb(m, 3) and c.sum() == math.comb(m, 3)
+    # a window of 2 steps keeps only consecutive triples
+    for fn in (O.thyme_temporal, O.temporal_brute):
+        assert fn(off, nodes, times, 20)[95] == m - 2
+
+
+def test_temporal_window_is_inclusive():
+    off, nodes = csr([[1, 2], [2, 3], [3, 1]])
+    times = np.array([0, 5, 10], dtype=np.int64)
+    for fn in (O.thyme_temporal, O.temporal_brute):
+        assert fn(off, nodes, times, 10).sum() == 1
+        assert fn(off, nodes, times, 9).sum() == 0
