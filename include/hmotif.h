@@ -122,6 +122,65 @@ hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out);
  * stream: names are "project" and "triangles"; ms receives the duration of that phase. */
 hm_status hm_graph_phase_ms(hm_graph* g, const char* phase, double* ms);
 
+/* ------------------------------------------------------------------------------------------ */
+/* Dynamic hypergraphs                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+/* Applies one batch in place on the device: deletes the live hyperedges del_ids (their rows
+ * become empty) and inserts n_ins hyperedges given as a CSR (ins_offsets[0] == 0), which receive
+ * the fresh ids num_edges, num_edges + 1, ... written to assigned_ids (ids are never reused).
+ * If delta_out is not NULL it receives the exact change of the 30 static bins,
+ * counts(after) - counts(before), computed incrementally from the changed hyperedges only.
+ * Replaces the reference's CBSTOperations::insert / erase and fillCBST / unfillCBST
+ * (dynamicHyperGraph/include/structure.hpp:39-55, structure/operations.cu:82-453) as driven by
+ * applyIncrementalUpdates (dynamicHyperGraph/src/main.cu:238-250), plus its delta count. */
+hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
+                          const int64_t* ins_offsets, const int32_t* ins_nodes, int32_t n_ins,
+                          int32_t* assigned_ids, int64_t delta_out[HM_STATIC_BINS]);
+
+/* Exact delta between two snapshots of one id space, counts(new_g) - counts(old_g), given the
+ * ids deleted from old_g and the ids inserted into new_g (an id may be deleted and re-inserted;
+ * all other hyperedges must be identical in both). Same arguments as computeMotifCountsDelta
+ * (dynamicHyperGraph/include/motif_update.hpp:13-19), with 0-based ids and int64 output; the
+ * reference's anchor rule (dynamicHyperGraph/src/HMotifCountUpdate.cu:99-226) is replaced by
+ * charging each changed triple to its smallest changed id. */
+hm_status hm_count_delta(hm_graph* old_g, hm_graph* new_g, const int32_t* del_ids, int32_t n_del,
+                         const int32_t* ins_ids, int32_t n_ins, int64_t delta_out[HM_STATIC_BINS]);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Temporal hypergraph handle: temporal hyperedges (node set, int64 timestamp) in time order  */
+/* (ties by input order), resident in HBM.                                                    */
+/* Replaces the 3-layer ring buffer TemporalHypergraphIndex /                                 */
+/* TemporalCBSTOperations (temporalDynamicHMotif/include/temporal_adjacency.hpp:9-18,         */
+/* temporal_structure.hpp:24-78) and the incremental projection of THyMe+                     */
+/* (THyMe-main/code/main_thymeP.cpp:54-115).                                                  */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct hm_tgraph hm_tgraph;
+
+hm_status hm_tgraph_create(const int64_t* offsets, const int32_t* nodes, const int64_t* times,
+                           int32_t num_edges, hm_tgraph** out);
+hm_status hm_tgraph_destroy(hm_tgraph* g);
+
+/* Exact temporal h-motif counts within window delta (same unit as the timestamps), 96 THyMe
+ * bins: every triple of temporal hyperedges with t_max - t_min <= delta and at least two
+ * overlapping pairs, classified by its Venn pattern in time order.
+ * Replaces computeTemporalMotifCountsStrictInc (temporalDynamicHMotif/include/
+ * temporal_count.hpp:10-11), which counted a different, time-free 30-bin pattern; the count here
+ * is THyMe+'s (THyMe-main/code/main_thymeP.cpp:219-367). Equal to THyMe+ whenever timestamps are
+ * distinct (with ties THyMe's result depends on random tie-breaking; here ties follow input order). */
+hm_status hm_count_temporal(hm_tgraph* g, int64_t delta, uint64_t out[HM_TEMPORAL_BINS]);
+
+typedef struct hm_temporal_stats {
+  int64_t num_edges;        /* temporal hyperedges */
+  int64_t num_incidences;
+  int64_t proj_pairs;       /* overlapping pairs of temporal hyperedges within delta */
+  int64_t max_window;       /* max hyperedges strictly older than an anchor inside its window */
+  int64_t neighbour_pairs;  /* sum over anchors of C(#window neighbours, 2) */
+  int64_t instances;        /* sum of the 96 bins */
+} hm_temporal_stats;
+hm_status hm_tgraph_get_stats(hm_tgraph* g, hm_temporal_stats* out);
+/* Phases "project" (window bounds + temporal projection rows) and "count" (k_tcount). */
+hm_status hm_tgraph_phase_ms(hm_tgraph* g, const char* phase, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

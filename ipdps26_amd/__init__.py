@@ -93,6 +93,16 @@ _SIGNATURES = [
     ("hm_count_static", ctypes.c_int, [ctypes.c_void_p, _u64p]),
     ("hm_graph_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(_GraphStats)]),
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
+    ("hm_graph_update", ctypes.c_int,
+     [ctypes.c_void_p, _i32p, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _i32p, _i64p]),
+    ("hm_count_delta", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, _i32p, ctypes.c_int32, _i32p, ctypes.c_int32, _i64p]),
+    ("hm_tgraph_create", ctypes.c_int,
+     [_i64p, _i32p, _i64p, ctypes.c_int32, _P(ctypes.c_void_p)]),
+    ("hm_tgraph_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("hm_count_temporal", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _u64p]),
+    ("hm_tgraph_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("hm_tgraph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
 ]
 
 _lib = None
@@ -240,6 +250,95 @@ class Graph:
     def phase_ms(self, phase: str) -> float:
         ms = ctypes.c_double(0.0)
         _check(lib().hm_graph_phase_ms(self._h, phase.encode(), ctypes.byref(ms)))
+        return ms.value
+
+    def update(self, deleted, ins_offsets=None, ins_nodes=None, delta: bool = True):
+        """Deletes the ids in `deleted`, inserts the CSR (ins_offsets, ins_nodes); returns
+        (assigned ids int32[n_ins], delta int64[30] or None)."""
+        dels = np.ascontiguousarray(np.asarray(deleted, dtype=np.int32).reshape(-1))
+        if ins_offsets is None:
+            ins_offsets, ins_nodes = np.zeros(1, dtype=np.int64), np.zeros(0, dtype=np.int32)
+        ioff = np.ascontiguousarray(ins_offsets, dtype=np.int64)
+        inodes = np.ascontiguousarray(ins_nodes, dtype=np.int32)
+        n_ins = ioff.size - 1
+        assigned = np.zeros(max(n_ins, 1), dtype=np.int32)
+        out = np.zeros(STATIC_BINS, dtype=np.int64)
+        _check(lib().hm_graph_update(
+            self._h, _ptr(dels if dels.size else np.zeros(1, np.int32), ctypes.c_int32), dels.size,
+            _ptr(ioff, ctypes.c_int64), _ptr(inodes if inodes.size else np.zeros(1, np.int32), ctypes.c_int32),
+            n_ins, _ptr(assigned, ctypes.c_int32), _ptr(out, ctypes.c_int64) if delta else None))
+        return assigned[:n_ins], (out if delta else None)
+
+
+def count_delta(old: Graph, new: Graph, deleted, inserted) -> np.ndarray:
+    """counts(new) - counts(old) (int64[30]) from the deleted / inserted ids of two snapshots."""
+    d = np.ascontiguousarray(np.asarray(deleted, dtype=np.int32).reshape(-1))
+    i = np.ascontiguousarray(np.asarray(inserted, dtype=np.int32).reshape(-1))
+    out = np.zeros(STATIC_BINS, dtype=np.int64)
+    _check(lib().hm_count_delta(old._h, new._h, _ptr(d if d.size else np.zeros(1, np.int32), ctypes.c_int32),
+                                d.size, _ptr(i if i.size else np.zeros(1, np.int32), ctypes.c_int32),
+                                i.size, _ptr(out, ctypes.c_int64)))
+    return out
+
+
+class _TemporalStats(ctypes.Structure):
+    _fields_ = [
+        ("num_edges", ctypes.c_int64),
+        ("num_incidences", ctypes.c_int64),
+        ("proj_pairs", ctypes.c_int64),
+        ("max_window", ctypes.c_int64),
+        ("neighbour_pairs", ctypes.c_int64),
+        ("instances", ctypes.c_int64),
+    ]
+
+
+class TemporalGraph:
+    """Temporal hyperedges (node sets with int64 timestamps) resident in HBM, in time order."""
+
+    def __init__(self, offsets: np.ndarray, nodes: np.ndarray, times: np.ndarray):
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        times = np.ascontiguousarray(times, dtype=np.int64)
+        if nodes.size == 0:
+            nodes = np.zeros(1, dtype=np.int32)
+        if times.size == 0:
+            times = np.zeros(1, dtype=np.int64)
+        self._h = ctypes.c_void_p()
+        _check(lib().hm_tgraph_create(_ptr(offsets, ctypes.c_int64), _ptr(nodes, ctypes.c_int32),
+                                      _ptr(times, ctypes.c_int64), offsets.size - 1,
+                                      ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().hm_tgraph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def count(self, delta: int) -> np.ndarray:
+        """Exact temporal h-motif counts within delta: uint64[96] in THyMe order (motif i+1)."""
+        out = np.zeros(TEMPORAL_BINS, dtype=np.uint64)
+        _check(lib().hm_count_temporal(self._h, int(delta), _ptr(out, ctypes.c_uint64)))
+        return out
+
+    def stats(self) -> dict:
+        st = _TemporalStats()
+        _check(lib().hm_tgraph_get_stats(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in _TemporalStats._fields_}
+
+    def phase_ms(self, phase: str) -> float:
+        ms = ctypes.c_double(0.0)
+        _check(lib().hm_tgraph_phase_ms(self._h, phase.encode(), ctypes.byref(ms)))
         return ms.value
 
 

@@ -1,0 +1,548 @@
+// Dynamic hypergraphs: device-side batch insert/delete on the flat sorted arrays and the exact
+// incremental (delta) h-motif count of a batch.
+//
+// Replaces the reference's CBST update path (dynamicHyperGraph/structure/operations.cu:82-453:
+// fillCBST / unfillCBST / deleteCBST / insertCBST, kernels in dynamicHyperGraph/kernel/) and its
+// delta count computeMotifCountsDelta (dynamicHyperGraph/include/motif_update.hpp:13-19,
+// dynamicHyperGraph/src/HMotifCountUpdate.cu:99-301), whose anchor rule adds or subtracts every
+// closed triangle anchored at a flagged edge and so is not the difference of two counts.
+//
+// Delta rule. For a set S of changed hyperedges in a graph G, the count of the triples of G that
+// contain at least one member of S is split over S: a triple T is charged to min(T ∩ S) (by id).
+// Deleted hyperedges are charged on the graph before the batch (subtracted), inserted ones on the
+// graph after it (added), so  counts(after) = counts(before) - L(D, before) + L(A, after).
+// L(S, G) is computed per changed hyperedge f (one workgroup) from the LDS mask table of f
+// (M[x] = bitmask of f's nodes contained in x) and full projection rows N(b) of G:
+//   * pairs {b, c} of neighbours of f: counted as if open at f from the neighbour classes
+//     (C_fb, [b ⊄ f]), then every b < c with b ∩ c ≠ ∅ (c in N(b) ∩ N(f)) is re-booked from the
+//     open bin to its closed bin;
+//   * triples open at a neighbour b with f at one end: c in N(b) \ N(f), c ≠ f.
+// Neighbours that are in S with a smaller id than f are skipped, which is the charging rule.
+//
+// Store update. Deleted ids keep an empty row (an empty hyperedge overlaps nothing, so it is in no
+// triple); inserted hyperedges get fresh ids E, E+1, ... . H2V is rebuilt by a scan of the new
+// sizes and a row copy; V2H by a stable radix sort of the (node, hyperedge) incidences, which
+// keeps every node's hyperedge list ascending.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "hm_device.hpp"
+#include "hm_graph.hpp"
+#include "hm_internal.hpp"
+#include "hm_tables.hpp"
+
+namespace hm {
+
+__constant__ int8_t c_delta_bin[128] = HM_STATIC_BIN_TABLE;
+
+// projected degree d_a (all neighbours, by id)
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_deg(int32_t E, const uint32_t* __restrict__ eoff,
+                                                const int32_t* __restrict__ enodes,
+                                                const uint32_t* __restrict__ voff,
+                                                const int32_t* __restrict__ vedges,
+                                                unsigned long long* __restrict__ deg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ uint32_t red[kWavesPerBlock];
+  const int32_t a = blockIdx.x;
+  if (eoff[a + 1] == eoff[a]) {  // empty (deleted) hyperedge
+    if (threadIdx.x == 0) deg[a] = 0;
+    return;
+  }
+  build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, nullptr);
+  uint32_t d = 0;
+  for (int32_t x = threadIdx.x; x < E; x += blockDim.x) d += M[x] != (MT)0;
+  d = wave_sum(d);
+  if (lane_id() == 0) red[wave_id()] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int q = 0; q < kWavesPerBlock; ++q) t += red[q];
+    deg[a] = t;
+  }
+}
+
+// full row of a by id: (x << 7 | C_ax) for every x != a with C_ax > 0, ascending
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_full_fill(int32_t E, const uint32_t* __restrict__ eoff,
+                                                      const int32_t* __restrict__ enodes,
+                                                      const uint32_t* __restrict__ voff,
+                                                      const int32_t* __restrict__ vedges,
+                                                      const unsigned long long* __restrict__ roff,
+                                                      uint32_t* __restrict__ rows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ uint32_t wcnt[kWavesPerBlock];
+  const int32_t a = blockIdx.x;
+  if (eoff[a + 1] == eoff[a]) return;
+  build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, nullptr);
+  const int lane = lane_id(), w = wave_id();
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t* out = rows + roff[a];
+  uint32_t running = 0;
+  for (int32_t x0 = 0; x0 < E; x0 += blockDim.x) {
+    const int32_t x = x0 + (int32_t)threadIdx.x;
+    const MT m = x < E ? M[x] : (MT)0;
+    const unsigned long long bal = __ballot(m != (MT)0);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int q = 0; q < kWavesPerBlock; ++q) {
+      woff += q < w ? wcnt[q] : 0;
+      tot += wcnt[q];
+    }
+    if (m) out[running + woff + (uint32_t)__popcll(bal & below)] =
+        ((uint32_t)x << kCntBits) | (uint32_t)popc(m);
+    running += tot;
+    __syncthreads();
+  }
+}
+
+// L(S, G) for the changed hyperedges in `changed` (flag[x] = 1 iff x in S); adds sign * L to acc.
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_delta(
+    int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
+    const int32_t* __restrict__ vedges, const unsigned long long* __restrict__ roff,
+    const unsigned long long* __restrict__ rlen, const uint32_t* __restrict__ rows,
+    const int32_t* __restrict__ changed, const uint8_t* __restrict__ flag, long long sign,
+    unsigned long long* __restrict__ acc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  uint8_t* sz = reinterpret_cast<uint8_t*>(M + E);
+  uint8_t* fl = sz + E;
+  __shared__ long long h[kWavesPerBlock][32];
+  __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
+  __shared__ unsigned long long red[6][kWavesPerBlock];
+  __shared__ int8_t tbl[128];
+  __shared__ uint32_t s_next;
+  const int32_t f = changed[blockIdx.x];
+  if (eoff[f + 1] == eoff[f]) return;
+  for (int k = threadIdx.x; k < kWavesPerBlock * 32; k += blockDim.x) (&h[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
+  if (threadIdx.x < 128) tbl[threadIdx.x] = c_delta_bin[threadIdx.x];
+  if (threadIdx.x == 0) s_next = 0;
+  for (int32_t x = threadIdx.x; x < E; x += blockDim.x) {
+    sz[x] = esize[x];
+    fl[x] = flag[x];
+  }
+  build_masks<MT>(M, E, f, eoff, enodes, voff, vedges, nullptr);
+  const int sf = (int)(eoff[f + 1] - eoff[f]);
+  // classes of the allowed neighbours
+  for (int32_t x = threadIdx.x; x < E; x += blockDim.x) {
+    const MT m = M[x];
+    if (m && (!fl[x] || x > f)) {
+      const int C = popc(m);
+      atomicAdd(&cls[2 * (C - 1) + (sz[x] > C ? 1 : 0)], 1u);
+    }
+  }
+  __syncthreads();
+  {  // every pair of allowed neighbours as if open at f
+    const int K = 2 * sf;
+    unsigned long long o[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = threadIdx.x; p < K * K; p += blockDim.x) {
+      const int k1 = p / K, k2 = p - k1 * K;
+      if (k1 > k2) continue;
+      const unsigned long long n1 = cls[k1], n2 = cls[k2];
+      const unsigned long long pairs = (k1 == k2) ? n1 * (n1 - (n1 > 0)) / 2 : n1 * n2;
+      const int b6 = (int)((k1 >> 1) + (k2 >> 1) + 2 < sf) + 2 * ((k1 & 1) + (k2 & 1));
+#pragma unroll
+      for (int q = 0; q < 6; ++q) o[q] += b6 == q ? pairs : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const unsigned long long v = wave_sum(o[q]);
+      if (lane_id() == 0) red[q][wave_id()] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    unsigned long long t = 0;
+    for (int q = 0; q < kWavesPerBlock; ++q) t += red[threadIdx.x][q];
+    h[0][kOpenBin0 + threadIdx.x] += (long long)t;
+  }
+  __syncthreads();
+  // neighbours b of f: closed pairs (re-booked) and triples open at b
+  const int lane = lane_id(), w = wave_id();
+  long long* hw = h[w];
+  const uint32_t* rowF = rows + roff[f];
+  const uint32_t nF = (uint32_t)rlen[f];
+  for (;;) {
+    uint32_t ib = 0;
+    if (lane == 0) ib = atomicAdd(&s_next, 1u);
+    ib = __shfl(ib, 0);
+    if (ib >= nF) break;
+    const int32_t b = (int32_t)(rowF[ib] >> kCntBits);
+    if (fl[b] && b < f) continue;
+    const MT mb = M[b];
+    const int cfb = popc(mb);
+    const int sb = sz[b];
+    const uint32_t* rowB = rows + roff[b];
+    const uint32_t nB = (uint32_t)rlen[b];
+    for (uint32_t j0 = 0; j0 < nB; j0 += kWave) {
+      const uint32_t j = j0 + lane;
+      bool plus = false, minus = false;
+      int bp = 0, bm = 0;
+      if (j < nB) {
+        const uint32_t e = rowB[j];
+        const int32_t c = (int32_t)(e >> kCntBits);
+        const int cbc = (int)(e & kCntMask);
+        if (c != f && (!fl[c] || c > f)) {
+          const MT mc = M[c];
+          const int sc = sz[c];
+          if (mc) {
+            if (c > b) {
+              const int cfc = popc(mc);
+              plus = minus = true;
+              bp = tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))];
+              bm = open_bin(sf, sb, sc, cfb, cfc);
+            }
+          } else {
+            plus = true;
+            bp = open_bin(sb, sf, sc, cfb, cbc);
+          }
+        }
+      }
+      wave_hist_add<long long>(hw, bp, plus, 1);
+      wave_hist_add<long long>(hw, bm, minus, -1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kStaticBins) {
+    long long s = 0;
+    for (int q = 0; q < kWavesPerBlock; ++q) s += h[q][threadIdx.x];
+    if (s) atomicAdd(&acc[threadIdx.x], (unsigned long long)(s * sign));
+  }
+}
+
+// ---- store update kernels -------------------------------------------------------------------
+__global__ void k_new_sizes(int32_t E, int32_t nins, const uint32_t* __restrict__ eoff,
+                            const uint8_t* __restrict__ del, const uint32_t* __restrict__ ins_off,
+                            unsigned long long* __restrict__ sizes) {
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < E) sizes[e] = del[e] ? 0 : (eoff[e + 1] - eoff[e]);
+  else if (e < E + nins) sizes[e] = ins_off[e - E + 1] - ins_off[e - E];
+  else if (e == E + nins) sizes[e] = 0;
+}
+
+__global__ void k_copy_rows(int32_t E, int32_t nins, const uint32_t* __restrict__ eoff,
+                            const int32_t* __restrict__ enodes, const uint8_t* __restrict__ del,
+                            const uint32_t* __restrict__ ins_off, const int32_t* __restrict__ ins_nodes,
+                            const unsigned long long* __restrict__ noff, int32_t* __restrict__ nnodes,
+                            uint32_t* __restrict__ neoff, uint8_t* __restrict__ nsize,
+                            int32_t* __restrict__ inc_node, int32_t* __restrict__ inc_edge) {
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e > E + nins) return;
+  neoff[e] = (uint32_t)noff[e];
+  if (e == E + nins) return;
+  const int32_t* src;
+  uint32_t n;
+  if (e < E) {
+    src = enodes + eoff[e];
+    n = del[e] ? 0 : eoff[e + 1] - eoff[e];
+  } else {
+    src = ins_nodes + ins_off[e - E];
+    n = ins_off[e - E + 1] - ins_off[e - E];
+  }
+  const unsigned long long o = noff[e];
+  nsize[e] = (uint8_t)(n > 255 ? 255 : n);
+  for (uint32_t k = 0; k < n; ++k) {
+    nnodes[o + k] = src[k];
+    inc_node[o + k] = src[k];
+    inc_edge[o + k] = e;
+  }
+}
+
+__global__ void k_node_count(int64_t I, const int32_t* __restrict__ node, uint32_t* __restrict__ cnt) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < I) atomicAdd(&cnt[node[k]], 1u);
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+__global__ void k_set_flags(const int32_t* __restrict__ ids, int32_t n, uint8_t* __restrict__ flag) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) flag[ids[k]] = 1;
+}
+
+__global__ void k_zero_acc(unsigned long long* p, int n) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0;
+}
+
+static unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+static DynScratch& dyn_scratch(hm_graph* g) { return g->dyn; }
+
+// Full projection rows of the graph currently on the handle.
+template <typename MT>
+static hm_status build_full_rows(hm_graph* g, DynScratch& d) {
+  DeviceCsr& c = g->csr;
+  const int32_t E = c.E;
+  hipStream_t st = g->stream;
+  const size_t lds = ((size_t)E * sizeof(MT) + 15) & ~(size_t)15;
+  if (lds > 65536) {
+    HM_HIP(hipFuncSetAttribute((const void*)k_deg<MT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HM_HIP(hipFuncSetAttribute((const void*)k_full_fill<MT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  HM_TRY(d.deg.ensure(sizeof(unsigned long long) * (E + 1)));
+  HM_TRY(d.roff.ensure(sizeof(unsigned long long) * (E + 1)));
+  k_zero_acc<<<1, 1, 0, st>>>(d.deg.as<unsigned long long>() + E, 1);
+  k_deg<MT><<<E, kBlock, lds, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                    c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+                                    d.deg.as<unsigned long long>());
+  HM_HIP(hipGetLastError());
+  size_t tb = 0;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d.deg.as<unsigned long long>(),
+                                          d.roff.as<unsigned long long>(), E + 1, st));
+  HM_TRY(d.tmp.ensure(tb));
+  tb = d.tmp.bytes;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(d.tmp.p, tb, d.deg.as<unsigned long long>(),
+                                          d.roff.as<unsigned long long>(), E + 1, st));
+  unsigned long long P = 0;
+  HM_HIP(hipMemcpyAsync(&P, d.roff.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  HM_TRY(d.rows.ensure(sizeof(uint32_t) * std::max<unsigned long long>(P, 1)));
+  k_full_fill<MT><<<E, kBlock, lds, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                          c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+                                          d.roff.as<unsigned long long>(), d.rows.as<uint32_t>());
+  HM_HIP(hipGetLastError());
+  return HM_OK;
+}
+
+// acc (device, 32 x u64, two's complement) += sign * L(ids, current graph).
+template <typename MT>
+static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,
+                             long long sign) {
+  if (n == 0) return HM_OK;
+  DeviceCsr& c = g->csr;
+  const int32_t E = c.E;
+  hipStream_t st = g->stream;
+  HM_TRY(build_full_rows<MT>(g, d));
+  HM_TRY(d.flag.ensure((size_t)std::max(E, 1)));
+  HM_TRY(d.ids.ensure(sizeof(int32_t) * n));
+  HM_HIP(hipMemcpyAsync(d.ids.p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  k_fill_u8<<<grid_for(E), kBlock, 0, st>>>(d.flag.as<uint8_t>(), E, 0);
+  k_set_flags<<<grid_for(n), kBlock, 0, st>>>(d.ids.as<int32_t>(), n, d.flag.as<uint8_t>());
+  const size_t lds = (((size_t)E * sizeof(MT) + 2 * (size_t)E) + 15) & ~(size_t)15;
+  if (lds > 65536)
+    HM_HIP(hipFuncSetAttribute((const void*)k_delta<MT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  k_delta<MT><<<n, kBlock, lds, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                      c.esize.as<uint8_t>(), c.voff.as<uint32_t>(),
+                                      c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
+                                      d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(),
+                                      d.ids.as<int32_t>(), d.flag.as<uint8_t>(), sign,
+                                      d.acc.as<unsigned long long>());
+  HM_HIP(hipGetLastError());
+  return HM_OK;
+}
+
+static bool dense_fits(hm_graph* g, int32_t E, bool wide) {
+  int lim = 65536;
+  (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
+  return (size_t)E * ((wide ? 8 : 4) + 2) + 64 <= (size_t)lim;
+}
+
+static hm_status local_count_any(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,
+                                 long long sign) {
+  if (g->csr.max_size > HM_MAX_EDGE_SIZE)
+    return fail(HM_ERR_UNSUPPORTED, "delta count: hyperedge larger than 64 nodes");
+  const bool wide = g->csr.max_size > 32;
+  if (!dense_fits(g, g->csr.E, wide)) {
+    set_error("delta count: %d hyperedge ids exceed the dense LDS table", g->csr.E);
+    return HM_ERR_UNSUPPORTED;
+  }
+  return wide ? local_count<mask64_t>(g, d, ids, n, sign) : local_count<uint32_t>(g, d, ids, n, sign);
+}
+
+// Applies the batch to the device arrays (deleted rows emptied, inserted rows appended).
+static hm_status apply_batch(hm_graph* g, DynScratch& d, const std::vector<uint8_t>& delmask,
+                             const std::vector<uint32_t>& ins_off, const std::vector<int32_t>& ins_nodes,
+                             int32_t nins, int32_t newV) {
+  DeviceCsr& c = g->csr;
+  const int32_t E = c.E, E2 = E + nins;
+  hipStream_t st = g->stream;
+  const int64_t Iins = ins_off.back();
+  HM_TRY(d.flag.ensure((size_t)std::max(E, 1)));
+  HM_HIP(hipMemcpyAsync(d.flag.p, delmask.data(), (size_t)E, hipMemcpyHostToDevice, st));
+  HM_TRY(d.ins_off.ensure(sizeof(uint32_t) * ins_off.size()));
+  HM_TRY(d.ins_nodes.ensure(sizeof(int32_t) * std::max<int64_t>(Iins, 1)));
+  HM_HIP(hipMemcpyAsync(d.ins_off.p, ins_off.data(), sizeof(uint32_t) * ins_off.size(), hipMemcpyHostToDevice, st));
+  if (Iins > 0)
+    HM_HIP(hipMemcpyAsync(d.ins_nodes.p, ins_nodes.data(), sizeof(int32_t) * Iins, hipMemcpyHostToDevice, st));
+  HM_TRY(d.sizes.ensure(sizeof(unsigned long long) * (E2 + 1)));
+  HM_TRY(d.noff.ensure(sizeof(unsigned long long) * (E2 + 1)));
+  k_new_sizes<<<grid_for(E2 + 1), kBlock, 0, st>>>(E, nins, c.eoff.as<uint32_t>(), d.flag.as<uint8_t>(),
+                                                   d.ins_off.as<uint32_t>(), d.sizes.as<unsigned long long>());
+  size_t tb = 0;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d.sizes.as<unsigned long long>(),
+                                          d.noff.as<unsigned long long>(), E2 + 1, st));
+  HM_TRY(d.tmp.ensure(tb));
+  tb = d.tmp.bytes;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(d.tmp.p, tb, d.sizes.as<unsigned long long>(),
+                                          d.noff.as<unsigned long long>(), E2 + 1, st));
+  unsigned long long I2 = 0;
+  HM_HIP(hipMemcpyAsync(&I2, d.noff.as<unsigned long long>() + E2, 8, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  if (I2 >= (1ull << 32)) return fail(HM_ERR_OVER_BUDGET, "update: more than 2^32 incidences");
+  const size_t I2b = (size_t)std::max<unsigned long long>(I2, 1);
+  HM_TRY(d.nnodes.ensure(sizeof(int32_t) * I2b));
+  HM_TRY(d.neoff.ensure(sizeof(uint32_t) * (E2 + 1)));
+  HM_TRY(d.nsize.ensure((size_t)std::max(E2, 1)));
+  HM_TRY(d.inc_node.ensure(sizeof(int32_t) * I2b));
+  HM_TRY(d.inc_edge.ensure(sizeof(int32_t) * I2b));
+  HM_TRY(d.srt_node.ensure(sizeof(int32_t) * I2b));
+  HM_TRY(d.srt_edge.ensure(sizeof(int32_t) * I2b));
+  HM_TRY(d.vcnt.ensure(sizeof(uint32_t) * ((size_t)newV + 1)));
+  HM_TRY(d.nvoff.ensure(sizeof(uint32_t) * ((size_t)newV + 1)));
+  k_copy_rows<<<grid_for(E2 + 1), kBlock, 0, st>>>(
+      E, nins, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), d.flag.as<uint8_t>(),
+      d.ins_off.as<uint32_t>(), d.ins_nodes.as<int32_t>(), d.noff.as<unsigned long long>(),
+      d.nnodes.as<int32_t>(), d.neoff.as<uint32_t>(), d.nsize.as<uint8_t>(),
+      d.inc_node.as<int32_t>(), d.inc_edge.as<int32_t>());
+  HM_HIP(hipGetLastError());
+  // V2H: stable sort of incidences by node (hyperedge ids stay ascending within a node)
+  int bits = 1;
+  while (bits < 31 && ((int64_t)1 << bits) <= newV) ++bits;
+  tb = 0;
+  HM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d.inc_node.as<uint32_t>(), d.srt_node.as<uint32_t>(),
+                                            d.inc_edge.as<int32_t>(), d.srt_edge.as<int32_t>(),
+                                            (int)I2, 0, bits, st));
+  HM_TRY(d.tmp.ensure(tb));
+  tb = d.tmp.bytes;
+  HM_HIP(hipcub::DeviceRadixSort::SortPairs(d.tmp.p, tb, d.inc_node.as<uint32_t>(), d.srt_node.as<uint32_t>(),
+                                            d.inc_edge.as<int32_t>(), d.srt_edge.as<int32_t>(),
+                                            (int)I2, 0, bits, st));
+  k_fill_u32<<<grid_for((int64_t)newV + 1), kBlock, 0, st>>>(d.vcnt.as<uint32_t>(), (int64_t)newV + 1, 0);
+  k_node_count<<<grid_for((int64_t)I2), kBlock, 0, st>>>((int64_t)I2, d.inc_node.as<int32_t>(),
+                                                          d.vcnt.as<uint32_t>());
+  tb = 0;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d.vcnt.as<uint32_t>(), d.nvoff.as<uint32_t>(),
+                                          newV + 1, st));
+  HM_TRY(d.tmp.ensure(tb));
+  tb = d.tmp.bytes;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(d.tmp.p, tb, d.vcnt.as<uint32_t>(), d.nvoff.as<uint32_t>(),
+                                          newV + 1, st));
+  // swap the new arrays in
+  std::swap(c.enodes, d.nnodes);
+  std::swap(c.eoff, d.neoff);
+  std::swap(c.esize, d.nsize);
+  std::swap(c.vedges, d.srt_edge);
+  std::swap(c.voff, d.nvoff);
+  c.E = E2;
+  c.V = newV;
+  c.I = (int64_t)I2;
+  HM_HIP(hipStreamSynchronize(st));
+  return HM_OK;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+extern "C" {
+
+hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
+                          const int64_t* ins_offsets, const int32_t* ins_nodes, int32_t n_ins,
+                          int32_t* assigned_ids, int64_t delta_out[HM_STATIC_BINS]) {
+  clear_error();
+  if (!g) return fail(HM_ERR_BAD_INPUT, "hm_graph_update: null graph");
+  if (n_del < 0 || n_ins < 0) return fail(HM_ERR_BAD_INPUT, "hm_graph_update: negative count");
+  if ((n_del > 0 && !del_ids) || (n_ins > 0 && (!ins_offsets || !assigned_ids)))
+    return fail(HM_ERR_BAD_INPUT, "hm_graph_update: null batch array");
+  const int32_t E = g->csr.E;
+  if ((int64_t)E + n_ins >= kMaxPackedId) return fail(HM_ERR_OVER_BUDGET, "hm_graph_update: id space exhausted");
+  std::vector<uint8_t> delmask((size_t)std::max(E, 1), 0);
+  for (int32_t k = 0; k < n_del; ++k) {
+    const int32_t x = del_ids[k];
+    if (x < 0 || x >= E) return fail(HM_ERR_BAD_INPUT, "hm_graph_update: deleted id out of range");
+    if (!g->live[(size_t)x]) return fail(HM_ERR_BAD_INPUT, "hm_graph_update: deleted id is not live");
+    if (delmask[(size_t)x]) return fail(HM_ERR_BAD_INPUT, "hm_graph_update: deleted id repeated");
+    delmask[(size_t)x] = 1;
+  }
+  std::vector<int64_t> ioff;
+  std::vector<int32_t> inodes;
+  int32_t vmax = -1;
+  hm_status st = prepare_host_rows(ins_offsets, ins_nodes, n_ins, &ioff, &inodes, &vmax);
+  if (st != HM_OK) return st;
+  std::vector<uint32_t> ins_off32(ioff.size());
+  int32_t maxsz = g->csr.max_size;
+  for (size_t k = 0; k < ioff.size(); ++k) ins_off32[k] = (uint32_t)ioff[k];
+  for (int32_t k = 0; k < n_ins; ++k) maxsz = std::max<int32_t>(maxsz, (int32_t)(ioff[(size_t)k + 1] - ioff[(size_t)k]));
+  const int32_t newV = std::max(g->csr.V, vmax + 1);
+  HM_HIP(hipSetDevice(g->device));
+  DynScratch& d = dyn_scratch(g);
+  const bool want = delta_out != nullptr;
+  if (want) {
+    HM_TRY(d.acc.ensure(sizeof(unsigned long long) * 32));
+    k_zero_acc<<<1, 32, 0, g->stream>>>(d.acc.as<unsigned long long>(), 32);
+    std::vector<int32_t> dels(del_ids, del_ids + n_del);
+    std::sort(dels.begin(), dels.end());
+    HM_TRY(local_count_any(g, d, dels.data(), n_del, -1));
+  }
+  HM_TRY(apply_batch(g, d, delmask, ins_off32, inodes, n_ins, newV));
+  g->csr.max_size = maxsz;
+  for (int32_t k = 0; k < n_del; ++k) g->live[(size_t)del_ids[k]] = 0;
+  g->live.resize((size_t)E + n_ins, 1);
+  for (int32_t k = 0; k < n_ins; ++k) assigned_ids[k] = E + k;
+  if (want) {
+    std::vector<int32_t> ins((size_t)n_ins);
+    for (int32_t k = 0; k < n_ins; ++k) ins[(size_t)k] = E + k;
+    HM_TRY(local_count_any(g, d, ins.data(), n_ins, +1));
+    unsigned long long h[32];
+    HM_HIP(hipMemcpyAsync(h, d.acc.p, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+    HM_HIP(hipStreamSynchronize(g->stream));
+    for (int k = 0; k < HM_STATIC_BINS; ++k) delta_out[k] = (int64_t)h[k];
+  }
+  g->have_stats = false;
+  return HM_OK;
+}
+
+hm_status hm_count_delta(hm_graph* old_g, hm_graph* new_g, const int32_t* del_ids, int32_t n_del,
+                         const int32_t* ins_ids, int32_t n_ins, int64_t delta_out[HM_STATIC_BINS]) {
+  clear_error();
+  if (!old_g || !new_g || !delta_out) return fail(HM_ERR_BAD_INPUT, "hm_count_delta: null argument");
+  if (old_g == new_g) return fail(HM_ERR_BAD_INPUT, "hm_count_delta: old and new must be two snapshots");
+  if (n_del < 0 || n_ins < 0 || (n_del > 0 && !del_ids) || (n_ins > 0 && !ins_ids))
+    return fail(HM_ERR_BAD_INPUT, "hm_count_delta: bad id list");
+  std::vector<int32_t> dels(del_ids, del_ids + n_del), ins(ins_ids, ins_ids + n_ins);
+  std::sort(dels.begin(), dels.end());
+  std::sort(ins.begin(), ins.end());
+  if (std::adjacent_find(dels.begin(), dels.end()) != dels.end() ||
+      std::adjacent_find(ins.begin(), ins.end()) != ins.end())
+    return fail(HM_ERR_BAD_INPUT, "hm_count_delta: repeated id");
+  for (int32_t x : dels)
+    if (x < 0 || x >= old_g->csr.E) return fail(HM_ERR_BAD_INPUT, "hm_count_delta: deleted id out of range");
+  for (int32_t x : ins)
+    if (x < 0 || x >= new_g->csr.E) return fail(HM_ERR_BAD_INPUT, "hm_count_delta: inserted id out of range");
+  unsigned long long h[32] = {0};
+  for (hm_graph* g : {old_g, new_g}) {
+    HM_HIP(hipSetDevice(g->device));
+    DynScratch& d = dyn_scratch(g);
+    HM_TRY(d.acc.ensure(sizeof(unsigned long long) * 32));
+    k_zero_acc<<<1, 32, 0, g->stream>>>(d.acc.as<unsigned long long>(), 32);
+    const bool is_old = g == old_g;
+    HM_TRY(local_count_any(g, d, is_old ? dels.data() : ins.data(), is_old ? n_del : n_ins,
+                           is_old ? -1 : +1));
+    unsigned long long part[32];
+    HM_HIP(hipMemcpyAsync(part, d.acc.p, sizeof(part), hipMemcpyDeviceToHost, g->stream));
+    HM_HIP(hipStreamSynchronize(g->stream));
+    for (int k = 0; k < 32; ++k) h[k] += part[k];
+  }
+  for (int k = 0; k < HM_STATIC_BINS; ++k) delta_out[k] = (int64_t)h[k];
+  return HM_OK;
+}
+
+}  // extern "C"

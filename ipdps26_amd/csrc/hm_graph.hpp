@@ -81,6 +81,19 @@ struct CountScratch {
   }
 };
 
+// Scratch of the dynamic path (full projection rows, batch staging), reused across updates.
+struct DynScratch {
+  DevBuf deg, roff, rows, flag, ids, acc, tmp;
+  DevBuf ins_off, ins_nodes, sizes, noff, nnodes, neoff, nsize, inc_node, inc_edge, srt_node,
+      srt_edge, vcnt, nvoff;
+  void release() {
+    for (DevBuf* b : {&deg, &roff, &rows, &flag, &ids, &acc, &tmp, &ins_off, &ins_nodes, &sizes,
+                      &noff, &nnodes, &neoff, &nsize, &inc_node, &inc_edge, &srt_node, &srt_edge,
+                      &vcnt, &nvoff})
+      b->release();
+  }
+};
+
 }  // namespace hm
 
 struct hm_graph {
@@ -89,9 +102,9 @@ struct hm_graph {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hm::DeviceCsr csr;
   hm::CountScratch scratch;
-  // host mirror of the hyperedge lists (sorted, unique), kept for updates and statistics
-  std::vector<int64_t> h_off;
-  std::vector<int32_t> h_nodes;
+  hm::DynScratch dyn;
+  // live[id] = 1 for hyperedges that have not been deleted
+  std::vector<uint8_t> live;
   hm_graph_stats stats{};
   bool have_stats = false;
   float ms_project = 0.f, ms_triangles = 0.f;

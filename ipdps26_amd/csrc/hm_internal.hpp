@@ -3,6 +3,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/hmotif.h"
 
@@ -19,6 +20,11 @@ inline hm_status fail(hm_status st, const char* msg) {
 
 // Device selected by hm_init on this thread.
 int current_device();
+
+// Validates a caller CSR (offsets[0] == 0, non-decreasing, node ids >= 0) and returns its rows
+// sorted with repeated nodes removed; *vmax_out = largest node id (-1 if none).
+hm_status prepare_host_rows(const int64_t* offsets, const int32_t* nodes, int32_t E,
+                            std::vector<int64_t>* off, std::vector<int32_t>* out, int32_t* vmax_out);
 
 }  // namespace hm
 

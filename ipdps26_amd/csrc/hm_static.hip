@@ -45,32 +45,9 @@ enum AccSlot : int {
   kAccOrient = 34,     // sum_a sum_{b in N(a), b > a} d_b
   kAccWedges = 35,     // sum_a C(d_a, 2)
   kAccUpper = 36,      // sum_a d+_a (upper-row entries)
+  kAccMaxNodeDeg = 37, // max_v |V2H(v)|
   kAccCount = 40
 };
-
-// ---------------------------------------------------------------------------------------------
-// Mask table: M[key(x)] |= bit(i) for every node a_i of hyperedge a and every x in V2H(a_i), x != a.
-// ---------------------------------------------------------------------------------------------
-template <typename MT>
-__device__ inline void build_masks(MT* M, int32_t E, int32_t a, const uint32_t* __restrict__ eoff,
-                                   const int32_t* __restrict__ enodes,
-                                   const uint32_t* __restrict__ voff,
-                                   const int32_t* __restrict__ vedges,
-                                   const int32_t* __restrict__ rank) {
-  for (int x = threadIdx.x; x < E; x += blockDim.x) M[x] = (MT)0;
-  __syncthreads();
-  const uint32_t s = eoff[a], t = eoff[a + 1];
-  for (uint32_t i = s; i < t; ++i) {
-    const int32_t v = enodes[i];
-    const MT bit = (MT)1 << (i - s);
-    const uint32_t ks = voff[v], kt = voff[v + 1];
-    for (uint32_t k = ks + threadIdx.x; k < kt; k += blockDim.x) {
-      const int32_t x = vedges[k];
-      if (x != a) atomicOr(&M[rank ? rank[x] : x], bit);
-    }
-  }
-  __syncthreads();
-}
 
 template <typename MT>
 __global__ __launch_bounds__(kBlock) void k_proj_count(
@@ -304,7 +281,8 @@ __global__ void k_rank_from_perm(const int32_t* __restrict__ perm, const uint8_t
 }
 
 // Graph statistics: P, max degree, wedges, upper entries, oriented row sum.
-__global__ __launch_bounds__(kBlock) void k_stats(int32_t E, const uint32_t* __restrict__ deg,
+__global__ __launch_bounds__(kBlock) void k_stats(int32_t E, int32_t V, const uint32_t* __restrict__ voff,
+                                                  const uint32_t* __restrict__ deg,
                                                   const unsigned long long* __restrict__ orient,
                                                   const uint32_t* __restrict__ rlen,
                                                   unsigned long long* __restrict__ acc) {
@@ -317,13 +295,18 @@ __global__ __launch_bounds__(kBlock) void k_stats(int32_t E, const uint32_t* __r
     up += rlen[a];
     orr += orient[a];
   }
+  unsigned long long mv = 0;
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+    mv = max(mv, (unsigned long long)(voff[v + 1] - voff[v]));
   p = wave_sum(p); wd = wave_sum(wd); up = wave_sum(up); orr = wave_sum(orr); mx = wave_max(mx);
+  mv = wave_max(mv);
   if (lane_id() == 0) {
     atomicAdd(&acc[kAccP], p);
     atomicMax(&acc[kAccMaxDeg], mx);
     atomicAdd(&acc[kAccWedges], wd);
     atomicAdd(&acc[kAccUpper], up);
     atomicAdd(&acc[kAccOrient], orr);
+    atomicMax(&acc[kAccMaxNodeDeg], mv);
   }
 }
 
@@ -444,7 +427,7 @@ static hm_status count_static_dense(hm_graph* g, unsigned long long* h_acc) {
       s.order.as<int32_t>(), acc);
   HM_HIP(hipGetLastError());
   HM_HIP(hipEventRecord(g->ev[2], st));
-  k_stats<<<64, kBlock, 0, st>>>(E, s.deg.as<uint32_t>(), s.orient.as<unsigned long long>(),
+  k_stats<<<64, kBlock, 0, st>>>(E, c.V, c.voff.as<uint32_t>(), s.deg.as<uint32_t>(), s.orient.as<unsigned long long>(),
                                  s.rlen.as<uint32_t>(), acc);
   HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
@@ -483,30 +466,31 @@ hm_status prepare_host_rows(const int64_t* offsets, const int32_t* nodes, int32_
   return HM_OK;
 }
 
-hm_status upload_csr(hm_graph* g) {
+hm_status upload_csr(hm_graph* g, const std::vector<int64_t>& h_off,
+                     const std::vector<int32_t>& h_nodes) {
   DeviceCsr& c = g->csr;
   const int32_t E = c.E, V = c.V;
-  const int64_t I = (int64_t)g->h_nodes.size();
+  const int64_t I = (int64_t)h_nodes.size();
   if (I >= ((int64_t)1 << 32)) return fail(HM_ERR_OVER_BUDGET, "more than 2^32 incidences");
   c.I = I;
   std::vector<uint32_t> eoff((size_t)E + 1), voff((size_t)V + 1, 0);
   std::vector<uint8_t> esz((size_t)std::max(E, 1), 0);
-  for (int32_t e = 0; e <= E; ++e) eoff[(size_t)e] = (uint32_t)g->h_off[(size_t)e];
+  for (int32_t e = 0; e <= E; ++e) eoff[(size_t)e] = (uint32_t)h_off[(size_t)e];
   for (int32_t e = 0; e < E; ++e) {
-    int64_t sz = g->h_off[(size_t)e + 1] - g->h_off[(size_t)e];
+    int64_t sz = h_off[(size_t)e + 1] - h_off[(size_t)e];
     esz[(size_t)e] = (uint8_t)std::min<int64_t>(sz, 255);
   }
   c.max_size = 0;
   for (int32_t e = 0; e < E; ++e)
-    c.max_size = std::max<int32_t>(c.max_size, (int32_t)(g->h_off[(size_t)e + 1] - g->h_off[(size_t)e]));
-  for (int64_t i = 0; i < I; ++i) voff[(size_t)g->h_nodes[(size_t)i] + 1]++;
+    c.max_size = std::max<int32_t>(c.max_size, (int32_t)(h_off[(size_t)e + 1] - h_off[(size_t)e]));
+  for (int64_t i = 0; i < I; ++i) voff[(size_t)h_nodes[(size_t)i] + 1]++;
   for (int32_t v = 0; v < V; ++v) voff[(size_t)v + 1] += voff[(size_t)v];
   std::vector<int32_t> vedges((size_t)std::max<int64_t>(I, 1));
   {
     std::vector<uint32_t> fill(voff.begin(), voff.end() - 1);
     for (int32_t e = 0; e < E; ++e)
-      for (int64_t i = g->h_off[(size_t)e]; i < g->h_off[(size_t)e + 1]; ++i)
-        vedges[fill[(size_t)g->h_nodes[(size_t)i]]++] = e;
+      for (int64_t i = h_off[(size_t)e]; i < h_off[(size_t)e + 1]; ++i)
+        vedges[fill[(size_t)h_nodes[(size_t)i]]++] = e;
   }
   HM_TRY(c.eoff.ensure(sizeof(uint32_t) * (E + 1)));
   HM_TRY(c.enodes.ensure(sizeof(int32_t) * std::max<int64_t>(I, 1)));
@@ -515,7 +499,7 @@ hm_status upload_csr(hm_graph* g) {
   HM_TRY(c.vedges.ensure(sizeof(int32_t) * std::max<int64_t>(I, 1)));
   HM_HIP(hipMemcpyAsync(c.eoff.p, eoff.data(), sizeof(uint32_t) * (E + 1), hipMemcpyHostToDevice, g->stream));
   if (I > 0) {
-    HM_HIP(hipMemcpyAsync(c.enodes.p, g->h_nodes.data(), sizeof(int32_t) * I, hipMemcpyHostToDevice, g->stream));
+    HM_HIP(hipMemcpyAsync(c.enodes.p, h_nodes.data(), sizeof(int32_t) * I, hipMemcpyHostToDevice, g->stream));
     HM_HIP(hipMemcpyAsync(c.vedges.p, vedges.data(), sizeof(int32_t) * I, hipMemcpyHostToDevice, g->stream));
   }
   HM_HIP(hipMemcpyAsync(c.esize.p, esz.data(), (size_t)std::max(E, 1), hipMemcpyHostToDevice, g->stream));
@@ -557,13 +541,7 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
   for (int k = 0; k < HM_STATIC_BINS; ++k)
     if (k < kOpenBin0 || k >= kOpenBin0 + 6) closed += (int64_t)out[k];
   st.triangles = closed;
-  int64_t mxv = 0;
-  {
-    // max node degree from the host mirror
-    std::vector<int32_t> cnt((size_t)std::max(g->csr.V, 1), 0);
-    for (int32_t x : g->h_nodes) mxv = std::max<int64_t>(mxv, ++cnt[(size_t)x]);
-  }
-  st.max_node_degree = mxv;
+  st.max_node_degree = (int64_t)acc[kAccMaxNodeDeg];
   g->have_stats = true;
   return HM_OK;
 }
@@ -580,7 +558,9 @@ hm_status hm_graph_create(const int64_t* offsets, const int32_t* nodes, int32_t 
   hm_graph* g = new (std::nothrow) hm_graph();
   if (!g) return hm::fail(HM_ERR_OOM, "hm_graph_create: host allocation failed");
   int32_t vmax = -1;
-  hm_status st = hm::prepare_host_rows(offsets, nodes, num_edges, &g->h_off, &g->h_nodes, &vmax);
+  std::vector<int64_t> h_off;
+  std::vector<int32_t> h_nodes;
+  hm_status st = hm::prepare_host_rows(offsets, nodes, num_edges, &h_off, &h_nodes, &vmax);
   if (st != HM_OK) { delete g; return st; }
   if (num_nodes > 0 && vmax >= num_nodes) {
     delete g;
@@ -595,7 +575,8 @@ hm_status hm_graph_create(const int64_t* offsets, const int32_t* nodes, int32_t 
     return cleanup(hm::fail(HM_ERR_DEVICE, "hipStreamCreate failed"));
   for (auto& e : g->ev)
     if (hipEventCreate(&e) != hipSuccess) return cleanup(hm::fail(HM_ERR_DEVICE, "hipEventCreate failed"));
-  st = hm::upload_csr(g);
+  st = hm::upload_csr(g, h_off, h_nodes);
+  g->live.assign((size_t)num_edges, 1);
   if (st != HM_OK) return cleanup(st);
   *out = g;
   return HM_OK;
@@ -605,6 +586,7 @@ hm_status hm_graph_destroy(hm_graph* g) {
   if (!g) return HM_OK;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   g->scratch.release();
+  g->dyn.release();
   g->csr.release();
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);

@@ -1,0 +1,715 @@
+// Temporal h-motif counting (THyMe's 96 temporal h-motifs) on MI355X (gfx950).
+//
+// Semantics follow THyMe+ (THyMe-main/code/main_thymeP.cpp:54-389): a temporal h-motif instance is
+// three temporal hyperedges whose time span is at most delta (window kept inclusive,
+// main_thymeP.cpp:228) and of which at least two pairs overlap; it is classified by the Venn
+// pattern of the three node sets written in time order (THyMe-main/code/motif_id.cpp:57-108).
+// Temporal hyperedges with identical node sets are copies of one static hyperedge; THyMe+ counts
+// their combinations through per-static-edge time lists (instances :299-334, motifs 87-95
+// :338-363, motif 96 :366-367). Counting triples of temporal hyperedges directly, as done here,
+// gives the same numbers (a copy pair is just a pair whose overlap is the whole node set).
+//
+// Layout: temporal hyperedges in time order (ties by input order) are "positions".
+//   H2V by position: eoff/enodes/esize; V2H: voff/vpos (positions ascending); times[E].
+//   Window of anchor i: positions [lo_i, i) with t >= t_i - delta.
+//   Temporal projection rows (built per count): F(x) = positions y > x with t_y <= t_x + delta
+//   that overlap x, B(x) = positions y < x with t_y >= t_x - delta that overlap x; entries packed
+//   as (y << 7 | |x ∩ y|), ascending.
+//
+// Counting kernel (one workgroup per anchor i; every instance is counted at its newest edge i):
+//   M[y] = bitmask of i's nodes contained in y, for y in the window (LDS, dense over the window);
+//   NL = window positions with M != 0 (neighbours of i), ascending.
+//   For every x in NL (one wave per x):
+//     (b) y in F(x), y < i:   record C_xy in the wave's table; if y is not a neighbour of i the
+//                             triple (x, y, i) is open at x.
+//     (d) y in NL, y > x:     triple (x, y, i) with C_xy from the table (closed, or open at i).
+//     (e) y in B(x), y >= lo_i, y not a neighbour of i: triple (y, x, i), open at x.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "hm_device.hpp"
+#include "hm_graph.hpp"
+#include "hm_internal.hpp"
+#include "hm_tables.hpp"
+
+namespace hm {
+
+__constant__ int8_t c_temporal_bin[128] = HM_TEMPORAL_BIN_TABLE;
+
+constexpr int kTemporalBins = 96;
+enum TAccSlot : int {
+  kTAccHist = 0,        // 96 bins
+  kTAccPairs = 96,      // sum_x |F(x)| (temporal projection pairs)
+  kTAccMaxWin = 97,     // max_i (i - lo_i)
+  kTAccNbrPairs = 98,   // sum_i C(|N(i) ∩ window|, 2)
+  kTAccFwdScan = 99,    // sum_i sum_{x in N(i)} |F(x) ∩ (x, i)|
+  kTAccCount = 104
+};
+
+// lower_bound over an ascending array
+template <typename T>
+__device__ inline uint32_t lower_bound_dev(const T* a, uint32_t n, T key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Window bounds: lo[x] = first position with t >= t_x - delta, hi[x] = last with t <= t_x + delta.
+__global__ void k_twindow(int32_t E, const long long* __restrict__ times, long long delta,
+                          int32_t* __restrict__ lo, int32_t* __restrict__ hi) {
+  const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= E) return;
+  const long long t = times[x];
+  lo[x] = (int32_t)lower_bound_dev<long long>(times, (uint32_t)x + 1, t - delta);
+  // first position with time > t + delta, minus one
+  uint32_t l = (uint32_t)x, h = (uint32_t)E;
+  while (l < h) {
+    uint32_t mid = (l + h) >> 1;
+    if (times[mid] <= t + delta) l = mid + 1;
+    else h = mid;
+  }
+  hi[x] = (int32_t)l - 1;
+}
+
+// M[y - base] |= bit(k) for node k of hyperedge x and every position y in [base, last] holding it,
+// y != x. Positions per node are ascending, so the range is found by binary search.
+template <typename MT>
+__device__ inline void build_window_masks(MT* M, int32_t x, int32_t base, int32_t last,
+                                          const uint32_t* __restrict__ eoff,
+                                          const int32_t* __restrict__ enodes,
+                                          const uint32_t* __restrict__ voff,
+                                          const int32_t* __restrict__ vpos) {
+  const int32_t R = last - base + 1;
+  for (int k = threadIdx.x; k < R; k += blockDim.x) M[k] = (MT)0;
+  __syncthreads();
+  const uint32_t s = eoff[x], t = eoff[x + 1];
+  for (uint32_t i = s; i < t; ++i) {
+    const int32_t v = enodes[i];
+    const MT bit = (MT)1 << (i - s);
+    const int32_t* row = vpos + voff[v];
+    const uint32_t n = voff[v + 1] - voff[v];
+    const uint32_t k0 = lower_bound_dev<int32_t>(row, n, base);
+    const uint32_t k1 = lower_bound_dev<int32_t>(row, n, last + 1);
+    for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+      const int32_t y = row[k];
+      if (y != x) atomicOr(&M[y - base], bit);
+    }
+  }
+  __syncthreads();
+}
+
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_tproj_count(
+    int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vpos,
+    const int32_t* __restrict__ lo, const int32_t* __restrict__ hi,
+    unsigned long long* __restrict__ fdeg, unsigned long long* __restrict__ bdeg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ uint32_t red[2][kWavesPerBlock];
+  const int32_t x = blockIdx.x;
+  const int32_t base = lo[x], last = hi[x];
+  build_window_masks<MT>(M, x, base, last, eoff, enodes, voff, vpos);
+  uint32_t f = 0, b = 0;
+  for (int32_t y = base + (int32_t)threadIdx.x; y <= last; y += blockDim.x) {
+    if (M[y - base]) {
+      if (y > x) ++f;
+      else ++b;
+    }
+  }
+  f = wave_sum(f);
+  b = wave_sum(b);
+  if (lane_id() == 0) {
+    red[0][wave_id()] = f;
+    red[1][wave_id()] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ff = 0, bb = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+      ff += red[0][w];
+      bb += red[1][w];
+    }
+    fdeg[x] = ff;
+    bdeg[x] = bb;
+  }
+}
+
+// Compacts positions y in [from, to] with M[y - base] != 0 into out[] (ascending).
+template <typename MT>
+__device__ inline uint32_t compact_rows(const MT* M, int32_t base, int32_t from, int32_t to,
+                                        uint32_t* __restrict__ out, uint32_t* wcnt) {
+  const int lane = lane_id(), w = wave_id();
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t running = 0;
+  for (int32_t y0 = from; y0 <= to; y0 += blockDim.x) {
+    const int32_t y = y0 + (int32_t)threadIdx.x;
+    const MT m = (y <= to) ? M[y - base] : (MT)0;
+    const unsigned long long bal = __ballot(m != (MT)0);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int q = 0; q < kWavesPerBlock; ++q) {
+      woff += q < w ? wcnt[q] : 0;
+      tot += wcnt[q];
+    }
+    if (m) out[running + woff + (uint32_t)__popcll(bal & below)] =
+        ((uint32_t)y << kCntBits) | (uint32_t)popc(m);
+    running += tot;
+    __syncthreads();
+  }
+  return running;
+}
+
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_tproj_fill(
+    int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vpos,
+    const int32_t* __restrict__ lo, const int32_t* __restrict__ hi,
+    const unsigned long long* __restrict__ foff, const unsigned long long* __restrict__ boff,
+    uint32_t* __restrict__ frows, uint32_t* __restrict__ brows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ uint32_t wcnt[kWavesPerBlock];
+  const int32_t x = blockIdx.x;
+  const int32_t base = lo[x], last = hi[x];
+  build_window_masks<MT>(M, x, base, last, eoff, enodes, voff, vpos);
+  compact_rows<MT>(M, base, x + 1, last, frows + foff[x], wcnt);
+  compact_rows<MT>(M, base, base, x - 1, brows + boff[x], wcnt);
+}
+
+// Layout of the counting kernel's dynamic LDS for a window of W positions.
+template <typename MT>
+struct TCountLds {
+  static size_t bytes(int32_t W) {
+    size_t b = ((size_t)W * sizeof(MT) + 15) & ~(size_t)15;  // M
+    b += ((size_t)W * sizeof(uint16_t) + 15) & ~(size_t)15;  // NL (relative positions)
+    b += ((size_t)W + 15) & ~(size_t)15;                     // sizes
+    return b;
+  }
+};
+
+// As-if-open bin of the ordered pair (x earlier, y later) of neighbours of anchor i, i.e. the bin
+// the triple (x, y, i) would have if x ∩ y = ∅: Venn id 48 + [x ⊄ i] + 2 [y ⊄ i] + 4 [i ⊄ x ∪ y].
+__device__ inline int as_open_id(int fx, int fy, int cxi, int cyi, int si) {
+  return 48 + fx + 2 * fy + 4 * (int)(cxi + cyi < si);
+}
+
+// Counts, for every ordered pair (x earlier, y later) of entries of NL, the as-if-open id
+// (8 cases: fx, fy, [C_xi + C_yi < si]) with one wave. The running class histogram of earlier
+// entries is kept across lanes (lane c holds the number of earlier entries with C = c), and
+// earlier entries of the same 64-entry chunk are counted with one ballot per (f, C) class.
+template <typename MT>
+__device__ inline void ordered_open_pairs(const MT* M, const uint16_t* NL, const uint8_t* sz,
+                                          uint32_t nN, int si, unsigned long long cnt[8]) {
+  const int lane = lane_id();
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long R0 = 0, R1 = 0;  // earlier entries with f = 0 / 1 and C == lane + 1
+  for (int k = 0; k < 8; ++k) cnt[k] = 0;
+  for (uint32_t c0 = 0; c0 < nN; c0 += kWave) {
+    const uint32_t k = c0 + lane;
+    const bool valid = k < nN;
+    int C = 0, f = 0;
+    if (valid) {
+      const int yr = NL[k];
+      C = popc(M[yr]);
+      f = (int)sz[yr] > C ? 1 : 0;
+    }
+    const int T = si - C;  // earlier x count towards [C_x + C < si] iff C_x < T
+    // earlier chunks: prefix over the lane-distributed histograms
+    unsigned long long P0 = R0, P1 = R1;  // inclusive prefix over lanes 0..lane
+    for (int o = 1; o < kWave; o <<= 1) {
+      const unsigned long long q0 = __shfl_up(P0, o), q1 = __shfl_up(P1, o);
+      if (lane >= o) { P0 += q0; P1 += q1; }
+    }
+    const unsigned long long tot0 = __shfl(P0, kWave - 1), tot1 = __shfl(P1, kWave - 1);
+    // earlier entries with C_x < T, i.e. histogram lanes 0 .. T - 2
+    const int src = T >= 2 ? T - 2 : 0;
+    unsigned long long a0 = __shfl(P0, src), a1 = __shfl(P1, src);
+    if (T < 2) { a0 = 0; a1 = 0; }
+    unsigned long long n0 = tot0, n1 = tot1;
+    // same chunk
+    const unsigned long long all0 = __ballot(valid && f == 0), all1 = __ballot(valid && f == 1);
+    n0 += (unsigned long long)__popcll(all0 & below);
+    n1 += (unsigned long long)__popcll(all1 & below);
+    for (int c = 1; c <= si; ++c) {
+      const unsigned long long m0 = __ballot(valid && f == 0 && C == c);
+      const unsigned long long m1 = __ballot(valid && f == 1 && C == c);
+      if (c < T) {
+        a0 += (unsigned long long)__popcll(m0 & below);
+        a1 += (unsigned long long)__popcll(m1 & below);
+      }
+      if (lane == c - 1) {
+        R0 += (unsigned long long)__popcll(m0);
+        R1 += (unsigned long long)__popcll(m1);
+      }
+    }
+    if (valid) {
+      // id bits: fx (x earlier) + 2 fy (this entry) + 4 [C_x + C < si]
+      cnt[0 + 2 * f + 4] += a0;
+      cnt[0 + 2 * f] += n0 - a0;
+      cnt[1 + 2 * f + 4] += a1;
+      cnt[1 + 2 * f] += n1 - a1;
+    }
+  }
+  for (int k = 0; k < 8; ++k) cnt[k] = wave_sum(cnt[k]);
+}
+
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_tcount(
+    int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
+    const int32_t* __restrict__ vpos, const int32_t* __restrict__ lo,
+    const unsigned long long* __restrict__ foff, const unsigned long long* __restrict__ fdeg,
+    const uint32_t* __restrict__ frows, const unsigned long long* __restrict__ boff,
+    const unsigned long long* __restrict__ bdeg, const uint32_t* __restrict__ brows,
+    const int32_t* __restrict__ order, int32_t Wmax, unsigned long long* __restrict__ acc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ long long hist[kWavesPerBlock][kTemporalBins];
+  __shared__ int8_t tbl[128];
+  __shared__ uint32_t wcnt[kWavesPerBlock];
+  __shared__ uint32_t s_nN, s_next;
+  const int32_t i = order[blockIdx.x];
+  const int32_t base = lo[i];
+  const int32_t W = i - base;
+  if (W < 2 || bdeg[i] == 0) return;
+  size_t off = ((size_t)Wmax * sizeof(MT) + 15) & ~(size_t)15;
+  MT* M = reinterpret_cast<MT*>(smem);
+  uint16_t* NL = reinterpret_cast<uint16_t*>(smem + off);
+  off += ((size_t)Wmax * sizeof(uint16_t) + 15) & ~(size_t)15;
+  uint8_t* sz = reinterpret_cast<uint8_t*>(smem + off);
+
+  for (int k = threadIdx.x; k < kWavesPerBlock * kTemporalBins; k += blockDim.x)
+    (&hist[0][0])[k] = 0;
+  if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
+  if (threadIdx.x == 0) s_next = 0;
+  for (int k = threadIdx.x; k < W; k += blockDim.x) sz[k] = esize[base + k];
+  build_window_masks<MT>(M, i, base, i - 1, eoff, enodes, voff, vpos);
+
+  // NL: neighbours of i inside the window, ascending (relative positions)
+  {
+    const int lane = lane_id(), w = wave_id();
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t running = 0;
+    for (int32_t k0 = 0; k0 < W; k0 += blockDim.x) {
+      const int32_t k = k0 + (int32_t)threadIdx.x;
+      const bool hit = k < W && M[k] != (MT)0;
+      const unsigned long long bal = __ballot(hit);
+      if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+      __syncthreads();
+      uint32_t woff = 0, tot = 0;
+      for (int q = 0; q < kWavesPerBlock; ++q) {
+        woff += q < w ? wcnt[q] : 0;
+        tot += wcnt[q];
+      }
+      if (hit) NL[running + woff + (uint32_t)__popcll(bal & below)] = (uint16_t)k;
+      running += tot;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) s_nN = running;
+  }
+  __syncthreads();
+  const uint32_t nN = s_nN;
+  const int lane = lane_id(), w = wave_id();
+  const int si = (int)esize[i];
+  long long* h = hist[w];
+
+  // open at i, counted for all ordered neighbour pairs as if disjoint (closed ones are taken
+  // back below when their triangle is met)
+  if (w == kWavesPerBlock - 1) {
+    unsigned long long cnt[8];
+    ordered_open_pairs<MT>(M, NL, sz, nN, si, cnt);
+    if (lane == 0)
+      for (int k = 0; k < 8; ++k) h[tbl[48 + k]] += (long long)cnt[k];
+  }
+
+  for (;;) {
+    uint32_t xi = 0;
+    if (lane == 0) xi = atomicAdd(&s_next, 1u);
+    xi = __shfl(xi, 0);
+    if (xi >= nN) break;
+    const int32_t xr = NL[xi];
+    const int32_t x = base + xr;
+    const MT mx = M[xr];
+    const int sx = sz[xr];
+    const int cxi = popc(mx);
+    const int fx = sx > cxi ? 1 : 0;
+    // (b) forward row of x inside (x, i): closed triangles and triples open at x
+    {
+      const uint32_t* row = frows + foff[x];
+      const uint32_t n = (uint32_t)fdeg[x];
+      for (uint32_t j0 = 0; j0 < n; j0 += kWave) {
+        const uint32_t j = j0 + lane;
+        const uint32_t e = j < n ? row[j] : 0xFFFFFFFFu;
+        const int32_t y = (int32_t)(e >> kCntBits);
+        const bool ok = j < n && y < i;
+        if (__ballot(ok) == 0ull) break;
+        int b1 = 0, b2 = 0;
+        bool closed = false;
+        if (ok) {
+          const int32_t yr = y - base;
+          const int cxy = (int)(e & kCntMask);
+          const MT my = M[yr];
+          const int sy = sz[yr];
+          if (my) {
+            closed = true;
+            const int cyi = popc(my);
+            b1 = tbl[venn_id(sx, sy, si, cxy, cyi, cxi, popc((MT)(mx & my)))];
+            b2 = tbl[as_open_id(fx, sy > cyi ? 1 : 0, cxi, cyi, si)];
+          } else {
+            b1 = tbl[venn_id(sx, sy, si, cxy, 0, cxi, 0)];
+          }
+        }
+        wave_hist_add<long long>(h, b1, ok, 1);
+        wave_hist_add<long long>(h, b2, ok && closed, -1);
+      }
+    }
+    // (e) backward row of x inside [lo_i, x): triples (y, x, i) open at x
+    {
+      const uint32_t* row = brows + boff[x];
+      const uint32_t n = (uint32_t)bdeg[x];
+      const uint32_t j0 = lower_bound_dev<uint32_t>(row, n, (uint32_t)base << kCntBits);
+      for (uint32_t jb = j0; jb < n; jb += kWave) {
+        const uint32_t j = jb + lane;
+        bool ok = false;
+        int b = 0;
+        if (j < n) {
+          const uint32_t e = row[j];
+          const int32_t yr = (int32_t)(e >> kCntBits) - base;
+          if (M[yr] == (MT)0) {
+            ok = true;
+            b = tbl[venn_id(sz[yr], sx, si, (int)(e & kCntMask), cxi, 0, 0)];
+          }
+        }
+        wave_hist_add<long long>(h, b, ok, 1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x) {
+    long long s = 0;
+    for (int q = 0; q < kWavesPerBlock; ++q) s += hist[q][b];
+    if (s) atomicAdd(&acc[kTAccHist + b], (unsigned long long)s);
+  }
+}
+
+// Per-anchor work estimate (for launch order) and statistics.
+__global__ void k_tstats(int32_t E, const int32_t* __restrict__ lo,
+                         const unsigned long long* __restrict__ fdeg,
+                         const unsigned long long* __restrict__ bdeg,
+                         unsigned long long* __restrict__ work, unsigned long long* __restrict__ acc) {
+  unsigned long long pairs = 0, mxw = 0, np = 0;
+  for (int32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < E; x += gridDim.x * blockDim.x) {
+    const unsigned long long b = bdeg[x];
+    pairs += fdeg[x];
+    mxw = max(mxw, (unsigned long long)(x - lo[x]));
+    np += b * (b - (b > 0)) / 2;
+    work[x] = b * b + b;
+  }
+  pairs = wave_sum(pairs);
+  np = wave_sum(np);
+  mxw = wave_max(mxw);
+  if (lane_id() == 0) {
+    atomicAdd(&acc[kTAccPairs], pairs);
+    atomicAdd(&acc[kTAccNbrPairs], np);
+    atomicMax(&acc[kTAccMaxWin], mxw);
+  }
+}
+
+__global__ void k_tzero(unsigned long long* p, int n) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0;
+}
+__global__ void k_tiota(int32_t* p, int32_t n) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = k;
+}
+
+}  // namespace hm
+
+// ---------------------------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------------------------
+struct hm_tgraph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int32_t E = 0, V = 0, max_size = 0;
+  int64_t I = 0;
+  hm::DevBuf eoff, enodes, esize, voff, vpos, times;
+  // scratch
+  hm::DevBuf lo, hi, fdeg, bdeg, foff, boff, frows, brows, work, work2, ids, order, acc, tmp;
+  std::vector<int32_t> order_to_input;  // position -> input index
+  hm_temporal_stats stats{};
+  bool have_stats = false;
+  float ms_project = 0.f, ms_count = 0.f;
+};
+
+namespace hm {
+
+static size_t win_lds_bytes(int32_t R, bool wide) {
+  return (((size_t)R * (wide ? 8 : 4)) + 15) & ~(size_t)15;
+}
+
+template <typename MT>
+static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned long long* h_acc) {
+  const int32_t E = g->E;
+  hipStream_t st = g->stream;
+  const unsigned eb = (unsigned)((E + kBlock - 1) / kBlock);
+  HM_TRY(g->lo.ensure(sizeof(int32_t) * E));
+  HM_TRY(g->hi.ensure(sizeof(int32_t) * E));
+  HM_TRY(g->fdeg.ensure(sizeof(unsigned long long) * (E + 1)));
+  HM_TRY(g->bdeg.ensure(sizeof(unsigned long long) * (E + 1)));
+  HM_TRY(g->foff.ensure(sizeof(unsigned long long) * (E + 1)));
+  HM_TRY(g->boff.ensure(sizeof(unsigned long long) * (E + 1)));
+  HM_TRY(g->work.ensure(sizeof(unsigned long long) * E));
+  HM_TRY(g->work2.ensure(sizeof(unsigned long long) * E));
+  HM_TRY(g->ids.ensure(sizeof(int32_t) * E));
+  HM_TRY(g->order.ensure(sizeof(int32_t) * E));
+  HM_TRY(g->acc.ensure(sizeof(unsigned long long) * kTAccCount));
+  unsigned long long* acc = g->acc.as<unsigned long long>();
+
+  HM_HIP(hipEventRecord(g->ev[0], st));
+  k_tzero<<<1, 128, 0, st>>>(acc, kTAccCount);
+  k_twindow<<<eb, kBlock, 0, st>>>(E, g->times.as<long long>(), delta, g->lo.as<int32_t>(),
+                                   g->hi.as<int32_t>());
+  // widest projection range decides the LDS of the projection kernels
+  std::vector<int32_t> hlo((size_t)E), hhi((size_t)E);
+  HM_HIP(hipMemcpyAsync(hlo.data(), g->lo.p, sizeof(int32_t) * E, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipMemcpyAsync(hhi.data(), g->hi.p, sizeof(int32_t) * E, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  int32_t Rmax = 1, Wmax = 1;
+  for (int32_t x = 0; x < E; ++x) {
+    Rmax = std::max(Rmax, hhi[(size_t)x] - hlo[(size_t)x] + 1);
+    Wmax = std::max(Wmax, x - hlo[(size_t)x]);
+  }
+  int lim = 65536;
+  (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
+  const size_t lds_proj = win_lds_bytes(Rmax, sizeof(MT) == 8);
+  const size_t lds_cnt = TCountLds<MT>::bytes(Wmax);
+  const size_t static_cnt = sizeof(long long) * kWavesPerBlock * kTemporalBins + 256;
+  if (lds_proj > (size_t)lim || lds_cnt + static_cnt > (size_t)lim || Wmax > 65535) {
+    set_error("temporal count: a delta window of %d hyperedges exceeds the LDS tables", Wmax);
+    return HM_ERR_UNSUPPORTED;
+  }
+  if (lds_proj > 65536)
+    HM_HIP(hipFuncSetAttribute((const void*)k_tproj_count<MT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_proj));
+  if (lds_proj > 65536)
+    HM_HIP(hipFuncSetAttribute((const void*)k_tproj_fill<MT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_proj));
+  if (lds_cnt > 65536)
+    HM_HIP(hipFuncSetAttribute((const void*)k_tcount<MT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cnt));
+
+  k_tproj_count<MT><<<E, kBlock, lds_proj, st>>>(
+      E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
+      g->vpos.as<int32_t>(), g->lo.as<int32_t>(), g->hi.as<int32_t>(),
+      g->fdeg.as<unsigned long long>(), g->bdeg.as<unsigned long long>());
+  HM_HIP(hipGetLastError());
+  size_t tb1 = 0, tb2 = 0;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb1, g->fdeg.as<unsigned long long>(),
+                                          g->foff.as<unsigned long long>(), E + 1, st));
+  HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(
+      nullptr, tb2, g->work.as<unsigned long long>(), g->work2.as<unsigned long long>(),
+      g->ids.as<int32_t>(), g->order.as<int32_t>(), E, 0, 64, st));
+  HM_TRY(g->tmp.ensure(std::max(tb1, tb2)));
+  // fdeg[E] and bdeg[E] are the scan's trailing zero
+  k_tzero<<<1, 1, 0, st>>>(g->fdeg.as<unsigned long long>() + E, 1);
+  k_tzero<<<1, 1, 0, st>>>(g->bdeg.as<unsigned long long>() + E, 1);
+  size_t tb = g->tmp.bytes;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(g->tmp.p, tb, g->fdeg.as<unsigned long long>(),
+                                          g->foff.as<unsigned long long>(), E + 1, st));
+  tb = g->tmp.bytes;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(g->tmp.p, tb, g->bdeg.as<unsigned long long>(),
+                                          g->boff.as<unsigned long long>(), E + 1, st));
+  unsigned long long tot[2] = {0, 0};
+  HM_HIP(hipMemcpyAsync(&tot[0], g->foff.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipMemcpyAsync(&tot[1], g->boff.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  HM_TRY(g->frows.ensure(sizeof(uint32_t) * std::max<unsigned long long>(tot[0], 1)));
+  HM_TRY(g->brows.ensure(sizeof(uint32_t) * std::max<unsigned long long>(tot[1], 1)));
+  k_tproj_fill<MT><<<E, kBlock, lds_proj, st>>>(
+      E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
+      g->vpos.as<int32_t>(), g->lo.as<int32_t>(), g->hi.as<int32_t>(),
+      g->foff.as<unsigned long long>(), g->boff.as<unsigned long long>(),
+      g->frows.as<uint32_t>(), g->brows.as<uint32_t>());
+  HM_HIP(hipGetLastError());
+  // anchor order: most work first
+  k_tstats<<<64, kBlock, 0, st>>>(E, g->lo.as<int32_t>(), g->fdeg.as<unsigned long long>(),
+                                  g->bdeg.as<unsigned long long>(), g->work.as<unsigned long long>(),
+                                  acc);
+  k_tiota<<<eb, kBlock, 0, st>>>(g->ids.as<int32_t>(), E);
+  tb = g->tmp.bytes;
+  HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(
+      g->tmp.p, tb, g->work.as<unsigned long long>(), g->work2.as<unsigned long long>(),
+      g->ids.as<int32_t>(), g->order.as<int32_t>(), E, 0, 64, st));
+  HM_HIP(hipEventRecord(g->ev[1], st));
+  k_tcount<MT><<<E, kBlock, lds_cnt, st>>>(
+      E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->esize.as<uint8_t>(),
+      g->voff.as<uint32_t>(), g->vpos.as<int32_t>(), g->lo.as<int32_t>(),
+      g->foff.as<unsigned long long>(), g->fdeg.as<unsigned long long>(), g->frows.as<uint32_t>(),
+      g->boff.as<unsigned long long>(), g->bdeg.as<unsigned long long>(), g->brows.as<uint32_t>(),
+      g->order.as<int32_t>(), Wmax, acc);
+  HM_HIP(hipGetLastError());
+  HM_HIP(hipEventRecord(g->ev[2], st));
+  HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kTAccCount, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  HM_HIP(hipEventElapsedTime(&g->ms_project, g->ev[0], g->ev[1]));
+  HM_HIP(hipEventElapsedTime(&g->ms_count, g->ev[1], g->ev[2]));
+  return HM_OK;
+}
+
+}  // namespace hm
+
+extern "C" {
+
+hm_status hm_tgraph_create(const int64_t* offsets, const int32_t* nodes, const int64_t* times,
+                           int32_t num_edges, hm_tgraph** out) {
+  hm::clear_error();
+  if (!out) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_create: null out");
+  *out = nullptr;
+  if (num_edges > 0 && !times) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_create: null times");
+  if ((int64_t)num_edges >= hm::kMaxPackedId)
+    return hm::fail(HM_ERR_UNSUPPORTED, "hm_tgraph_create: more than 2^25 temporal hyperedges");
+  std::vector<int64_t> off;
+  std::vector<int32_t> flat;
+  int32_t vmax = -1;
+  hm_status st = hm::prepare_host_rows(offsets, nodes, num_edges, &off, &flat, &vmax);
+  if (st != HM_OK) return st;
+  const int32_t E = num_edges;
+  for (int32_t e = 0; e < E; ++e)
+    if (times[e] < -((int64_t)1 << 62) || times[e] > ((int64_t)1 << 62))
+      return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_create: timestamps must lie within +-2^62");
+  hm_tgraph* g = new (std::nothrow) hm_tgraph();
+  if (!g) return hm::fail(HM_ERR_OOM, "hm_tgraph_create: host allocation failed");
+  auto cleanup = [&](hm_status s) { hm_tgraph_destroy(g); return s; };
+  // time order, ties by input index
+  std::vector<int32_t> order((size_t)E);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return times[a] < times[b]; });
+  g->order_to_input = order;
+  g->E = E;
+  g->V = vmax + 1;
+  std::vector<uint32_t> eoff((size_t)E + 1, 0), voff((size_t)g->V + 1, 0);
+  std::vector<int32_t> en;
+  en.reserve(flat.size());
+  std::vector<uint8_t> esz((size_t)std::max(E, 1), 0);
+  std::vector<long long> tpos((size_t)std::max(E, 1), 0);
+  for (int32_t p = 0; p < E; ++p) {
+    const int32_t e = order[(size_t)p];
+    const int64_t s = off[(size_t)e], t = off[(size_t)e + 1];
+    if (t - s > HM_MAX_EDGE_SIZE) {
+      delete g;
+      return hm::fail(HM_ERR_UNSUPPORTED, "hm_tgraph_create: hyperedge larger than 64 nodes");
+    }
+    for (int64_t k = s; k < t; ++k) en.push_back(flat[(size_t)k]);
+    eoff[(size_t)p + 1] = (uint32_t)en.size();
+    esz[(size_t)p] = (uint8_t)(t - s);
+    g->max_size = std::max<int32_t>(g->max_size, (int32_t)(t - s));
+    tpos[(size_t)p] = times[e];
+  }
+  g->I = (int64_t)en.size();
+  if (g->I >= ((int64_t)1 << 32)) return cleanup(hm::fail(HM_ERR_OVER_BUDGET, "more than 2^32 incidences"));
+  for (int32_t v : en) voff[(size_t)v + 1]++;
+  for (int32_t v = 0; v < g->V; ++v) voff[(size_t)v + 1] += voff[(size_t)v];
+  std::vector<int32_t> vp((size_t)std::max<int64_t>(g->I, 1));
+  {
+    std::vector<uint32_t> fill(voff.begin(), voff.end() - 1);
+    for (int32_t p = 0; p < E; ++p)
+      for (uint32_t k = eoff[(size_t)p]; k < eoff[(size_t)p + 1]; ++k) vp[fill[(size_t)en[k]]++] = p;
+  }
+  g->device = hm::current_device();
+  if (hipSetDevice(g->device) != hipSuccess) return cleanup(hm::fail(HM_ERR_DEVICE, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(hm::fail(HM_ERR_DEVICE, "hipStreamCreate failed"));
+  for (auto& e : g->ev)
+    if (hipEventCreate(&e) != hipSuccess) return cleanup(hm::fail(HM_ERR_DEVICE, "hipEventCreate failed"));
+  const size_t I1 = (size_t)std::max<int64_t>(g->I, 1);
+  if ((st = g->eoff.ensure(sizeof(uint32_t) * ((size_t)E + 1))) != HM_OK ||
+      (st = g->enodes.ensure(sizeof(int32_t) * I1)) != HM_OK ||
+      (st = g->esize.ensure((size_t)std::max(E, 1))) != HM_OK ||
+      (st = g->voff.ensure(sizeof(uint32_t) * ((size_t)g->V + 1))) != HM_OK ||
+      (st = g->vpos.ensure(sizeof(int32_t) * I1)) != HM_OK ||
+      (st = g->times.ensure(sizeof(long long) * (size_t)std::max(E, 1))) != HM_OK)
+    return cleanup(st);
+  if (hipMemcpyAsync(g->eoff.p, eoff.data(), sizeof(uint32_t) * ((size_t)E + 1), hipMemcpyHostToDevice, g->stream) != hipSuccess ||
+      (g->I > 0 && hipMemcpyAsync(g->enodes.p, en.data(), sizeof(int32_t) * (size_t)g->I, hipMemcpyHostToDevice, g->stream) != hipSuccess) ||
+      (g->I > 0 && hipMemcpyAsync(g->vpos.p, vp.data(), sizeof(int32_t) * (size_t)g->I, hipMemcpyHostToDevice, g->stream) != hipSuccess) ||
+      hipMemcpyAsync(g->esize.p, esz.data(), (size_t)std::max(E, 1), hipMemcpyHostToDevice, g->stream) != hipSuccess ||
+      hipMemcpyAsync(g->voff.p, voff.data(), sizeof(uint32_t) * ((size_t)g->V + 1), hipMemcpyHostToDevice, g->stream) != hipSuccess ||
+      hipMemcpyAsync(g->times.p, tpos.data(), sizeof(long long) * (size_t)std::max(E, 1), hipMemcpyHostToDevice, g->stream) != hipSuccess ||
+      hipStreamSynchronize(g->stream) != hipSuccess)
+    return cleanup(hm::fail(HM_ERR_DEVICE, "hm_tgraph_create: upload failed"));
+  *out = g;
+  return HM_OK;
+}
+
+hm_status hm_tgraph_destroy(hm_tgraph* g) {
+  if (!g) return HM_OK;
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  for (hm::DevBuf* b : {&g->eoff, &g->enodes, &g->esize, &g->voff, &g->vpos, &g->times, &g->lo,
+                        &g->hi, &g->fdeg, &g->bdeg, &g->foff, &g->boff, &g->frows, &g->brows,
+                        &g->work, &g->work2, &g->ids, &g->order, &g->acc, &g->tmp})
+    b->release();
+  for (auto& e : g->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  return HM_OK;
+}
+
+hm_status hm_count_temporal(hm_tgraph* g, int64_t delta, uint64_t out[HM_TEMPORAL_BINS]) {
+  hm::clear_error();
+  if (!g || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_count_temporal: null argument");
+  if (delta < 0 || delta > ((int64_t)1 << 62))
+    return hm::fail(HM_ERR_BAD_INPUT, "hm_count_temporal: delta must lie in [0, 2^62]");
+  unsigned long long acc[hm::kTAccCount] = {0};
+  g->ms_project = g->ms_count = 0.f;
+  if (g->E >= 3) {
+    if (hipSetDevice(g->device) != hipSuccess) return hm::fail(HM_ERR_DEVICE, "hipSetDevice failed");
+    const hm_status st = g->max_size <= 32 ? hm::count_temporal_impl<uint32_t>(g, delta, acc)
+                                           : hm::count_temporal_impl<hm::mask64_t>(g, delta, acc);
+    if (st != HM_OK) return st;
+  }
+  for (int k = 0; k < HM_TEMPORAL_BINS; ++k) out[k] = acc[hm::kTAccHist + k];
+  hm_temporal_stats& s = g->stats;
+  s.num_edges = g->E;
+  s.num_incidences = g->I;
+  s.proj_pairs = (int64_t)acc[hm::kTAccPairs];
+  s.max_window = (int64_t)acc[hm::kTAccMaxWin];
+  s.neighbour_pairs = (int64_t)acc[hm::kTAccNbrPairs];
+  int64_t tot = 0;
+  for (int k = 0; k < HM_TEMPORAL_BINS; ++k) tot += (int64_t)out[k];
+  s.instances = tot;
+  g->have_stats = true;
+  return HM_OK;
+}
+
+hm_status hm_tgraph_get_stats(hm_tgraph* g, hm_temporal_stats* out) {
+  hm::clear_error();
+  if (!g || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_get_stats: null argument");
+  if (!g->have_stats) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_get_stats: no count has run");
+  *out = g->stats;
+  return HM_OK;
+}
+
+hm_status hm_tgraph_phase_ms(hm_tgraph* g, const char* phase, double* ms) {
+  hm::clear_error();
+  if (!g || !phase || !ms) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_phase_ms: null argument");
+  if (std::strcmp(phase, "project") == 0) *ms = g->ms_project;
+  else if (std::strcmp(phase, "count") == 0) *ms = g->ms_count;
+  else return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_phase_ms: unknown phase");
+  return HM_OK;
+}
+
+}  // extern "C"

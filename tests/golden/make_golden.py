@@ -75,30 +75,51 @@ def gen_entry(name: str, p: hm.GenParams, td: str) -> dict:
             "printed_motifs": counts, "reference": "MoCHy main_exact_par", "ref_seconds": rt}
 
 
+def _load(path: str) -> list:
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return []
+
+
+def _save(path: str, entries: list) -> None:
+    with open(path, "w") as f:
+        json.dump(entries, f, indent=1)
+
+
 def main() -> None:
+    """Computes every vector not yet present in the JSON files (each is saved once computed)."""
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-c1", action="store_true", help="skip the ~1 min full-size c1 run")
     args = ap.parse_args()
-    static = []
-    temporal = []
+    spath = os.path.join(HERE, "static_golden.json")
+    tpath = os.path.join(HERE, "temporal_golden.json")
+    static, temporal = _load(spath), _load(tpath)
+    have_s = {g["name"] for g in static}
+    have_t = {g["name"] for g in temporal}
     with tempfile.TemporaryDirectory() as td:
-        dblp = os.path.join(HERE, "dblp_graph.txt")
-        counts, rt = run_mochy(dblp)
-        static.append({"name": "dblp", "kind": "fixture", "file": "dblp_graph.txt",
-                       "sha256_csv": sha256(dblp), "printed_motifs": counts,
-                       "reference": "MoCHy main_exact_par", "ref_seconds": rt})
-        for name, p in [
-            ("c1_family_e800", W.scaled(W.STATIC_C1, 800, 400)),
-            ("c2_family_e3000", W.scaled(W.STATIC_C2, 3000, 1500)),
-            ("c2_family_e8000", W.scaled(W.STATIC_C2, 8000, 4000)),
-            ("c3_family_e6000", W.scaled(W.DYNAMIC_C3, 6000, 2400)),
-            ("c5_family_e10000", W.scaled(W.STATIC_C2, 10000, 5000, seed=5)),
+        if "dblp" not in have_s:
+            dblp = os.path.join(HERE, "dblp_graph.txt")
+            counts, rt = run_mochy(dblp)
+            static.append({"name": "dblp", "kind": "fixture", "file": "dblp_graph.txt",
+                           "sha256_csv": sha256(dblp), "printed_motifs": counts,
+                           "reference": "MoCHy main_exact_par", "ref_seconds": rt})
+            _save(spath, static)
+        gens = [
             ("tiny_dense_e60", hm.GenParams(num_nodes=12, num_edges=60, alpha=1.5, kmin=1, kmax=6,
                                             zipf_s=0.8, seed=11)),
-        ]:
-            static.append(gen_entry(name, p, td))
+            ("c1_family_e800", W.scaled(W.STATIC_C1, 800, 400)),
+            ("c2_family_e3000", W.scaled(W.STATIC_C2, 3000, 1500)),
+            ("c3_family_e6000", W.scaled(W.DYNAMIC_C3, 6000, 2400)),
+            ("c2_family_e8000", W.scaled(W.STATIC_C2, 8000, 4000)),
+            ("c5_family_e10000", W.scaled(W.STATIC_C2, 10000, 5000, seed=5)),
+        ]
         if not args.skip_c1:
-            static.append(gen_entry("static_c1", W.STATIC_C1, td))
+            gens.append(("static_c1", W.STATIC_C1))
+        for name, p in gens:
+            if name not in have_s:
+                static.append(gen_entry(name, p, td))
+                _save(spath, static)
 
         # temporal: email-Enron re-timed to 1000 * line number (distinct), two windows
         enron = os.path.join(HERE, "email-Enron-full.txt")
@@ -107,14 +128,22 @@ def main() -> None:
         tsv = os.path.join(td, "enron_distinct.tsv")
         W.write_thyme_tsv(tsv, off, nodes, times)
         for delta in (100_000, 1_000_000):
+            name = f"enron_distinct_d{delta}"
+            if name in have_t:
+                continue
             counts, rt = run_thymep(tsv, delta)
             print(f"enron delta={delta}: thymeP {rt:.2f}s", flush=True)
-            temporal.append({"name": f"enron_distinct_d{delta}", "kind": "fixture",
+            temporal.append({"name": name, "kind": "fixture",
                              "file": "email-Enron-full.txt", "retime": "1000*line_number",
                              "delta": delta, "sha256_tsv": sha256(tsv), "counts96": counts,
                              "reference": "THyMe+ main_thymeP", "ref_seconds": rt})
+            _save(tpath, temporal)
+        # reduced instances of the temporal workload (THyMe+ enumerates serially, so the windows
+        # are kept to a few hundred hyperedges)
         for name, ne, nv, delta in [("c4_family_e3000", 3000, 1500, 600_000_000),
-                                    ("c4_family_e20000", 20000, 10000, 3_600_000_000)]:
+                                    ("c4_family_e12000", 12000, 6000, 120_000_000)]:
+            if name in have_t:
+                continue
             off, nodes, times = W.temporal_c4(ne, nv)
             tsv = os.path.join(td, name + ".tsv")
             W.write_thyme_tsv(tsv, off, nodes, times)
@@ -124,10 +153,7 @@ def main() -> None:
                              "params": W.scaled(W.TEMPORAL_C4, ne, nv).__dict__, "delta": delta,
                              "sha256_tsv": sha256(tsv), "counts96": counts,
                              "reference": "THyMe+ main_thymeP", "ref_seconds": rt})
-    with open(os.path.join(HERE, "static_golden.json"), "w") as f:
-        json.dump(static, f, indent=1)
-    with open(os.path.join(HERE, "temporal_golden.json"), "w") as f:
-        json.dump(temporal, f, indent=1)
+            _save(tpath, temporal)
 
 
 if __name__ == "__main__":

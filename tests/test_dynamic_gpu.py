@@ -1,0 +1,118 @@
+"""GPU parity of the dynamic path (hm_graph_update, hm_count_delta through the C ABI): every
+batch's incremental delta equals the difference of two full recounts, by the oracle and by the
+engine itself, and the updated store equals a graph rebuilt from scratch. Integer counts: exact."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import csr, random_hypergraph
+
+import ipdps26_amd as hm
+from ipdps26_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rows_of(off, nodes):
+    return [list(nodes[off[e]:off[e + 1]]) for e in range(off.size - 1)]
+
+
+def random_batch(rng, rows, live, n_del, n_ins, num_nodes, kmax):
+    dels = sorted(rng.choice(sorted(live), size=min(n_del, len(live)), replace=False).tolist())
+    new_rows = []
+    for _ in range(n_ins):
+        r = rng.random()
+        if r < 0.2 and live:  # duplicate of a live hyperedge
+            new_rows.append(list(rows[int(rng.choice(sorted(live)))]))
+        else:
+            k = int(rng.integers(1, kmax + 1))
+            new_rows.append(list(rng.choice(num_nodes, size=min(k, num_nodes), replace=False)))
+    return dels, new_rows
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_batches_match_oracle_recount(gpu_ready, seed):
+    rng = np.random.default_rng(5000 + seed)
+    nv, kmax = int(rng.integers(10, 80)), [4, 8, 40][seed % 3]
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(20, 250)), num_nodes=nv, kmax=kmax)
+    rows = rows_of(off, nodes)
+    live = set(range(len(rows)))
+    g = hm.Graph(off, nodes)
+    before = O.static_mochy(*csr(rows))
+    for b in range(4):
+        dels, new_rows = random_batch(rng, rows, live, int(rng.integers(0, 15)),
+                                      int(rng.integers(0, 15)), nv, kmax)
+        ioff, inodes = csr(new_rows)
+        ids, delta = g.update(dels, ioff, inodes)
+        assert ids.tolist() == list(range(len(rows), len(rows) + len(new_rows)))
+        for x in dels:
+            rows[x] = []
+            live.discard(x)
+        rows.extend(new_rows)
+        live.update(ids.tolist())
+        after = O.static_mochy(*csr(rows))
+        print(f"seed={seed} batch={b} E={len(rows)}", flush=True)
+        assert np.array_equal(delta, after - before)
+        assert np.array_equal(g.count_static().astype(np.int64), after)
+        before = after
+    g.close()
+
+
+def test_snapshot_delta_with_id_reuse(gpu_ready):
+    """The reference's batch reuses deleted ids for inserts (dynamicHyperGraph/src/main.cu:196-199)."""
+    rng = np.random.default_rng(77)
+    off, nodes = random_hypergraph(rng, num_edges=180, num_nodes=40, kmax=7)
+    old_rows = rows_of(off, nodes)
+    dels = sorted(rng.choice(len(old_rows), size=25, replace=False).tolist())
+    new_rows = [list(r) for r in old_rows]
+    ins = dels[:15] + [len(old_rows) + k for k in range(10)]
+    for x in dels:
+        new_rows[x] = []
+    new_rows.extend([[] for _ in range(10)])
+    for x in ins:
+        k = int(rng.integers(1, 7))
+        new_rows[x] = list(rng.choice(40, size=k, replace=False))
+    old_rows.extend([[] for _ in range(10)])  # same id space in both snapshots
+    with hm.Graph(*csr(old_rows)) as go, hm.Graph(*csr(new_rows)) as gn:
+        delta = hm.count_delta(go, gn, dels, ins)
+    assert np.array_equal(delta, O.static_mochy(*csr(new_rows)) - O.static_mochy(*csr(old_rows)))
+
+
+def test_c3_family_chain_against_recount(gpu_ready):
+    """Reduced dynamic workload: c3 generator at 4,000 hyperedges, 6 batches of 200 changes
+    (70% inserts of fresh generator hyperedges, 30% deletes of uniformly chosen live ones)."""
+    p = W.scaled(W.DYNAMIC_C3, 4000, 1600)
+    off, nodes = hm.generate_static(p)
+    g = hm.Graph(off, nodes)
+    live = np.arange(off.size - 1, dtype=np.int64)
+    counts = g.count_static().astype(np.int64)
+    assert np.array_equal(counts, O.static_mochy(off, nodes))
+    rows = rows_of(off, nodes)
+    for b in range(1, 7):
+        dels, ioff, inodes = W.dynamic_batch(live, b, p, inserts=140, deletes=60)
+        ids, delta = g.update(dels, ioff, inodes)
+        counts_after = g.count_static().astype(np.int64)
+        assert np.array_equal(counts + delta, counts_after), f"batch {b}"
+        counts = counts_after
+        live = np.setdiff1d(live, dels.astype(np.int64))
+        live = np.concatenate([live, ids.astype(np.int64)])
+        for x in dels:
+            rows[x] = []
+        rows.extend(rows_of(ioff, inodes))
+    assert np.array_equal(counts, O.static_mochy(*csr(rows)))
+    # the updated store equals a store built from scratch
+    with hm.Graph(*csr(rows)) as fresh:
+        assert np.array_equal(fresh.count_static().astype(np.int64), counts)
+    g.close()
+
+
+def test_update_rejects_bad_ids(gpu_ready):
+    off, nodes = csr([[0, 1], [1, 2], [2, 3]])
+    with hm.Graph(off, nodes) as g:
+        with pytest.raises(hm.HMotifError):
+            g.update([5])
+        g.update([1])
+        with pytest.raises(hm.HMotifError):
+            g.update([1])  # already deleted
+        ids, delta = g.update([], *csr([[3, 4]]))
+        assert ids.tolist() == [3]

@@ -1,0 +1,86 @@
+"""GPU parity of the temporal count (hm_count_temporal, through the C ABI) against the THyMe+
+oracle, the reference THyMe+ golden vectors and the temporal brute force. Integer counts:
+bit-exact."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import GOLDEN, csr, load_golden, random_hypergraph
+
+import ipdps26_amd as hm
+from ipdps26_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def tcount(off, nodes, times, delta):
+    with hm.TemporalGraph(off, nodes, times) as g:
+        return g.count(delta).astype(np.int64)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_matches_thyme_oracle(gpu_ready, seed):
+    rng = np.random.default_rng(3000 + seed)
+    kmax = [3, 6, 12, 40][seed % 4]
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(3, 500)),
+                                   num_nodes=int(rng.integers(kmax, 120)), kmax=kmax, dup_frac=0.25)
+    E = off.size - 1
+    times = np.cumsum(rng.integers(1, 50, size=E)).astype(np.int64)
+    perm = rng.permutation(E)  # input order is not time order
+    off2, nodes2 = csr([nodes[off[e]:off[e + 1]] for e in perm])
+    times2 = times[perm]
+    delta = int(rng.integers(10, 3000))
+    print(f"seed={seed} E={E} delta={delta}", flush=True)
+    assert np.array_equal(tcount(off2, nodes2, times2, delta),
+                          O.thyme_temporal(off2, nodes2, times2, delta))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_tiny_matches_brute_force(gpu_ready, seed):
+    rng = np.random.default_rng(4000 + seed)
+    off, nodes = random_hypergraph(rng, num_edges=50, num_nodes=10, kmax=5, dup_frac=0.35)
+    times = np.cumsum(rng.integers(1, 5, size=off.size - 1)).astype(np.int64)
+    assert np.array_equal(tcount(off, nodes, times, 25), O.temporal_brute(off, nodes, times, 25))
+
+
+@pytest.mark.parametrize("name", [g["name"] for g in load_golden("temporal")])
+def test_reference_goldens(gpu_ready, name):
+    g = [x for x in load_golden("temporal") if x["name"] == name][0]
+    if g["kind"] == "fixture":
+        off, nodes, _ = W.read_thyme_tsv(os.path.join(GOLDEN, g["file"]))
+        times = (np.arange(off.size - 1, dtype=np.int64) + 1) * 1000
+    else:
+        p = g["params"]
+        off, nodes, times = W.temporal_c4(p["num_edges"], p["num_nodes"])
+    print(f"golden={name} E={off.size - 1}", flush=True)
+    assert tcount(off, nodes, times, g["delta"]).tolist() == g["counts96"]
+
+
+def test_copies_closed_form(gpu_ready):
+    m = 40
+    off, nodes = csr([[1, 2, 3]] * m)
+    times = np.arange(1, m + 1, dtype=np.int64) * 10
+    c = tcount(off, nodes, times, 10 * m)
+    assert c[95] == math.comb(m, 3) and c.sum() == math.comb(m, 3)
+    assert tcount(off, nodes, times, 20)[95] == m - 2
+
+
+def test_window_inclusive_and_small_inputs(gpu_ready):
+    off, nodes = csr([[1, 2], [2, 3], [3, 1]])
+    times = np.array([0, 5, 10], dtype=np.int64)
+    assert tcount(off, nodes, times, 10).sum() == 1
+    assert tcount(off, nodes, times, 9).sum() == 0
+    assert tcount(*csr([[1, 2], [2, 3]]), np.array([0, 1]), 10).sum() == 0
+
+
+def test_stats_and_repeat(gpu_ready):
+    off, nodes, times = W.temporal_c4(3000, 1500)
+    with hm.TemporalGraph(off, nodes, times) as g:
+        a = g.count(600_000_000)
+        st = g.stats()
+        b = g.count(600_000_000)
+    assert np.array_equal(a, b)
+    assert st["instances"] == int(a.astype(np.int64).sum())
