@@ -1,16 +1,28 @@
 #!/usr/bin/env python3
-"""Throughput of exact static h-motif counting on one MI355X.
+"""Throughput of exact h-motif counting on one MI355X.
 
-A step is one hm_count_static call on a hypergraph already resident in HBM: projection,
-combinatorial open-triple count and oriented-triangle enumeration, ending with the 30-bin uint64
-histogram on the host. `value` = triples classified per second = (sum of the 30 bins) / step time.
+Workloads (inputs are built and copied to HBM before the timed region):
+  temporal_c4 (default)  THyMe's 96 temporal h-motifs on the temporal workload: 200,000 temporal
+                         hyperedges over 100,000 nodes (sizes ~k^-2.2 on [2,20], Zipf s=1.2,
+                         seed 4), int64 microsecond timestamps (Exp(1 s) gaps, 1,200 bursts of 50
+                         with Exp(10 ms) gaps), window delta = 3,600 s. A step is one
+                         hm_count_temporal call (window bounds, temporal projection, counting).
+  static_c1              MoCHy's 30 static bins on 5,000 hyperedges over 2,000 nodes (sizes
+                         ~k^-2.5 on [2,25], Zipf s=1.2, seed 1). A step is one hm_count_static call.
+The full static 1M-node / 2M-hyperedge graph is out of reach of exact per-triple counting (its
+largest node sits in ~780k hyperedges, i.e. >7e16 triples); the temporal workload is the
+largest configuration the engine counts at full size on one GPU.
 
-Prints one JSON line. The roofline entry is for the dominant kernel (k_tri): its algorithmic bytes
-per launch are 8 P (projection read once, 8 B per pair) + 8 * sum_a sum_{b in N(a), b > a} d_b
-(neighbour rows read for closed triples) + 240 B (histogram), divided by its average duration
-measured with HIP events on the stream it runs on. The CPU baseline is the reference MoCHy
-main_exact_par (built from its sources into oracle/_ref) timed on this host on a bounded prefix
-of the same hypergraph.
+value = h-motif instances classified per second = (sum of the bins) / step time.
+roofline = the dominant kernel (k_tcount or k_tri): algorithmic bytes per launch divided by its
+average duration from HIP events on the stream it runs on. Its algorithmic bytes are
+8 P (projection rows read once, 8 B per directed pair) + 8 * sum over pairs (x, y), y after x,
+of d_y (neighbour rows read for closed triples) + 8 B per histogram bin. `traffic` is the HBM
+bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known
+size) when profiles/traffic_<workload>.txt holds it, else null.
+cpu_baseline = the reference program timed on this host on a bounded prefix of the same input:
+THyMe+ main_thymeP (temporal) or MoCHy main_exact_par (static), built from the reference sources
+into oracle/_ref.
 """
 from __future__ import annotations
 
@@ -32,95 +44,196 @@ import ipdps26_amd as hm  # noqa: E402
 from ipdps26_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-WORKLOADS = {
-    "static_c1": (W.STATIC_C1, "static_c1: 5,000 hyperedges over 2,000 nodes, sizes ~k^-2.5 on "
-                               "[2,25], Zipf s=1.2, seed 1"),
+DESC = {
+    "temporal_c4": "temporal_c4: 200,000 temporal hyperedges over 100,000 nodes, sizes ~k^-2.2 on "
+                   "[2,20], Zipf s=1.2, seed 4, int64 us timestamps, delta=3600 s",
+    "static_c1": "static_c1: 5,000 hyperedges over 2,000 nodes, sizes ~k^-2.5 on [2,25], "
+                 "Zipf s=1.2, seed 1",
 }
-# Bounded CPU sample: the first CPU_SAMPLE_EDGES hyperedges of the same workload.
-CPU_SAMPLE_EDGES = {"static_c1": 3500}
+# Bounded CPU samples: a prefix of the same input sized for ~10-30 s of reference-program time.
+CPU_SAMPLE_EDGES = {"temporal_c4": 600, "static_c1": 3500}
+DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20}
 
 
-def cpu_baseline(workload: str, off: np.ndarray, nodes: np.ndarray) -> dict | None:
-    """Times MoCHy main_exact_par (reference program) on a prefix of the workload."""
-    k = CPU_SAMPLE_EDGES[workload]
+def cpu_threads() -> int:
+    return min(os.cpu_count() or 1, 16)
+
+
+def cpu_baseline_static(off, nodes) -> dict:
+    k = CPU_SAMPLE_EDGES["static_c1"]
     soff, snodes = off[: k + 1], nodes[: off[k]]
-    with hm.Graph(soff, snodes) as g:
-        triples = int(g.count_static().sum())
-    threads = min(os.cpu_count() or 1, 16)
+    with hm.Graph(soff, snodes) as g:  # MoCHy prints 26 of 30 bins; the total comes from here
+        triples = int(g.count_static().astype(np.int64).sum())
+    threads = cpu_threads()
     ref = os.path.join(ROOT, "oracle", "_ref", "mochy_exact_par")
     with tempfile.TemporaryDirectory() as td:
         csv = os.path.join(td, "sample.csv")
         W.write_mochy_csv(csv, soff, snodes)
         if os.path.exists(ref):
+            t0 = time.perf_counter()
             out = subprocess.run([ref, str(threads), csv], capture_output=True, text=True,
-                                 check=True, timeout=600).stdout
+                                 check=True, timeout=900).stdout
+            wall = time.perf_counter() - t0
             secs = float(re.search(r"TOTAL_RUNTIME_SECONDS ([0-9.eE+-]+)", out).group(1))
-            kind = "reference"
-        else:  # the oracle's restatement of the same program
+            kind, prog = "reference", "MoCHy main_exact_par"
+        else:
             sys.path.insert(0, os.path.join(ROOT, "tests"))
             import oracle_lib
 
             t0 = time.perf_counter()
             oracle_lib.static_mochy(soff, snodes, threads)
-            secs = time.perf_counter() - t0
-            kind = "port"
+            secs = wall = time.perf_counter() - t0
+            kind, prog = "port", "oracle restatement of MoCHy-E"
     return {"value": triples / secs, "unit": "triples/s", "cores": threads, "kind": kind,
-            "sample": f"first {k} of {off.size - 1} hyperedges of {workload} ({triples} triples), "
-                      f"MoCHy main_exact_par TOTAL_RUNTIME_SECONDS={secs:.2f}"}
+            "sample": f"first {k} of {off.size - 1} hyperedges of static_c1 ({triples} triples); "
+                      f"{prog} TOTAL_RUNTIME_SECONDS={secs:.2f} (wall {wall:.2f} s)"}
+
+
+def cpu_baseline_temporal(off, nodes, times, delta) -> dict:
+    k = CPU_SAMPLE_EDGES["temporal_c4"]
+    soff, snodes, stimes = off[: k + 1], nodes[: off[k]], times[:k]
+    threads = cpu_threads()
+    ref = os.path.join(ROOT, "oracle", "_ref", "thymeP")
+    with tempfile.TemporaryDirectory() as td:
+        if os.path.exists(ref):
+            tsv, res = os.path.join(td, "sample.tsv"), os.path.join(td, "res.txt")
+            W.write_thyme_tsv(tsv, soff, snodes, stimes)
+            t0 = time.perf_counter()
+            subprocess.run([ref, "x", str(delta), str(threads), tsv, res], capture_output=True,
+                           check=True, timeout=900)
+            secs = time.perf_counter() - t0
+            inst = sum(int(l.split("\t")[1]) for l in open(res).read().split("\n")[1:97])
+            kind, prog = "reference", "THyMe+ main_thymeP"
+        else:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_lib
+
+            t0 = time.perf_counter()
+            inst = int(oracle_lib.thyme_temporal(soff, snodes, stimes, delta).sum())
+            secs = time.perf_counter() - t0
+            kind, prog = "port", "oracle restatement of THyMe+"
+    return {"value": inst / secs, "unit": "instances/s", "cores": threads, "kind": kind,
+            "sample": f"first {k} of {off.size - 1} temporal hyperedges of temporal_c4, "
+                      f"delta=3600 s ({inst} instances); {prog} wall time {secs:.2f} s"}
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                if key.strip() == "hbm_bytes_per_launch":
+                    return int(val.strip())
+    return None
+
+
+def run_temporal(steps: int, warmup: int, with_cpu: bool) -> dict:
+    off, nodes, times = W.temporal_c4()
+    delta = W.TEMPORAL_C4_DELTA_US
+    g = hm.TemporalGraph(off, nodes, times)
+    for _ in range(warmup):
+        counts = g.count(delta)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        counts = g.count(delta)
+        kms.append(g.phase_ms("count"))
+    t1 = time.perf_counter()
+    st = g.stats()
+    g.close()
+    inst = int(counts.astype(np.int64).sum())
+    kbytes = 8 * 2 * st["proj_pairs"] + 8 * st["oriented_row_sum"] + 8 * hm.TEMPORAL_BINS
+    return {"ms": (t1 - t0) * 1e3 / steps, "total": inst, "unit": "instances/s",
+            "kernel": "k_tcount", "kernel_ms": float(np.mean(kms)), "kernel_bytes": kbytes,
+            "extra": {"temporal_hyperedges": int(off.size - 1), "instances": inst,
+                      "window_pairs": st["proj_pairs"], "max_window": st["max_window"],
+                      "bytes_model_total": hm.algorithmic_bytes_temporal(st)},
+            "cpu": cpu_baseline_temporal(off, nodes, times, delta) if with_cpu else None}
+
+
+def run_static(steps: int, warmup: int, with_cpu: bool) -> dict:
+    off, nodes = hm.generate_static(W.STATIC_C1)
+    g = hm.Graph(off, nodes)
+    for _ in range(warmup):
+        counts = g.count_static()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        counts = g.count_static()
+        kms.append(g.phase_ms("triangles"))
+    t1 = time.perf_counter()
+    st = g.stats()
+    g.close()
+    triples = int(counts.astype(np.int64).sum())
+    kbytes = 8 * st["proj_pairs"] + 8 * st["oriented_row_sum"] + 8 * hm.STATIC_BINS
+    return {"ms": (t1 - t0) * 1e3 / steps, "total": triples, "unit": "triples/s",
+            "kernel": "k_tri", "kernel_ms": float(np.mean(kms)), "kernel_bytes": kbytes,
+            "extra": {"hyperedges": int(off.size - 1), "triples": triples,
+                      "bytes_model_total": hm.algorithmic_bytes_static(st)},
+            "cpu": cpu_baseline_static(off, nodes) if with_cpu else None}
+
+
+def copy_probe(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB through the HIP runtime: the reference kernel whose
+    rocprofv3 FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    assert hip.hipMemset(a, 1, n) == 0
+    assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": mib << 20}), flush=True)
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="static_c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="temporal_c4", choices=sorted(DESC))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no "
               "multi-GPU path", file=sys.stderr)
         return 2
-
-    params, desc = WORKLOADS[args.workload]
-    off, nodes = hm.generate_static(params)
+    steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.workload]
     hm.init(0)
-    g = hm.Graph(off, nodes)  # H2D copy outside the timed region
-    for _ in range(args.warmup):
-        counts = g.count_static()
-    tri_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        counts = g.count_static()
-        tri_ms.append(g.phase_ms("triangles"))
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    st = g.stats()
-    triples = int(counts.astype(np.int64).sum())
-    tri_bytes = 8 * st["proj_pairs"] + 8 * st["oriented_row_sum"] + 240
-    tri_avg_ms = float(np.mean(tri_ms))
-    achieved = tri_bytes / (tri_avg_ms * 1e-3) / 1e9
+    run = run_temporal if args.workload == "temporal_c4" else run_static
+    r = run(steps, args.warmup, not args.no_cpu)
+    achieved = r["kernel_bytes"] / (r["kernel_ms"] * 1e-3) / 1e9
     line = {
         "metric": "exact h-motif triples classified/sec",
-        "value": triples / (ms_per_step * 1e-3),
-        "unit": "triples/s",
+        "value": r["total"] / (r["ms"] * 1e-3),
+        "unit": r["unit"],
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": r["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": desc, "triples": triples, "hyperedges": int(off.size - 1),
-                   "bytes_model_total": hm.algorithmic_bytes_static(st)},
-        "roofline": {"bound": "hbm", "kernel": "k_tri", "achieved": achieved,
+        "config": {"workload": DESC[args.workload], **r["extra"]},
+        "roofline": {"bound": "hbm", "kernel": r["kernel"], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "kernel_ms": tri_avg_ms, "algorithmic_bytes": tri_bytes},
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(args.workload, off, nodes),
+                     "traffic": measured_traffic(args.workload), "kernel_ms": r["kernel_ms"],
+                     "algorithmic_bytes": r["kernel_bytes"]},
+        "cpu_baseline": r["cpu"],
     }
-    g.close()
     print(json.dumps(line), flush=True)
     return 0
 

@@ -176,6 +176,8 @@ typedef struct hm_temporal_stats {
   int64_t max_window;       /* max hyperedges strictly older than an anchor inside its window */
   int64_t neighbour_pairs;  /* sum over anchors of C(#window neighbours, 2) */
   int64_t instances;        /* sum of the 96 bins */
+  int64_t oriented_row_sum; /* sum over later-overlapping pairs (x, y) of y's window degree */
+  int64_t num_nodes;        /* max node id + 1 */
 } hm_temporal_stats;
 hm_status hm_tgraph_get_stats(hm_tgraph* g, hm_temporal_stats* out);
 /* Phases "project" (window bounds + temporal projection rows) and "count" (k_tcount). */

@@ -289,6 +289,8 @@ class _TemporalStats(ctypes.Structure):
         ("max_window", ctypes.c_int64),
         ("neighbour_pairs", ctypes.c_int64),
         ("instances", ctypes.c_int64),
+        ("oriented_row_sum", ctypes.c_int64),
+        ("num_nodes", ctypes.c_int64),
     ]
 
 
@@ -345,6 +347,14 @@ class TemporalGraph:
 def printed_motifs(counts30) -> list[int]:
     """MoCHy's 26 printed motifs (motif 1..26) from the 30 bins (MoCHy-master/main_exact.cpp:133-137)."""
     return [int(counts30[b]) for b in PRINTED_STATIC_BINS]
+
+
+def algorithmic_bytes_temporal(st: dict) -> int:
+    """The static byte model over the delta-window projection, plus 8 B per timestamp and the
+    96-bin histogram: 4(2I + E + V + 2) + 16 P + 8 * sum_{x < y overlapping} d_y + 8 E + 768,
+    with P = 2 * proj_pairs directed pairs."""
+    return (4 * (2 * st["num_incidences"] + st["num_edges"] + st["num_nodes"] + 2)
+            + 16 * 2 * st["proj_pairs"] + 8 * st["oriented_row_sum"] + 8 * st["num_edges"] + 768)
 
 
 def algorithmic_bytes_static(st: dict) -> int:

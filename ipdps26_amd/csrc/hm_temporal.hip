@@ -47,7 +47,7 @@ enum TAccSlot : int {
   kTAccPairs = 96,      // sum_x |F(x)| (temporal projection pairs)
   kTAccMaxWin = 97,     // max_i (i - lo_i)
   kTAccNbrPairs = 98,   // sum_i C(|N(i) ∩ window|, 2)
-  kTAccFwdScan = 99,    // sum_i sum_{x in N(i)} |F(x) ∩ (x, i)|
+  kTAccOrient = 99,     // sum_x sum_{y in F(x)} d_y  (d = fdeg + bdeg, the delta-window degree)
   kTAccCount = 104
 };
 
@@ -425,6 +425,26 @@ __global__ void k_tstats(int32_t E, const int32_t* __restrict__ lo,
   }
 }
 
+// sum over forward pairs (x, y), y > x, of the window degree of y (byte model of the count kernel)
+__global__ __launch_bounds__(kBlock) void k_torient(int32_t E, const unsigned long long* __restrict__ foff,
+                                                    const unsigned long long* __restrict__ fdeg,
+                                                    const unsigned long long* __restrict__ bdeg,
+                                                    const uint32_t* __restrict__ frows,
+                                                    unsigned long long* __restrict__ acc) {
+  const int32_t x = blockIdx.x * kWavesPerBlock + wave_id();
+  unsigned long long s = 0;
+  if (x < E) {
+    const uint32_t* row = frows + foff[x];
+    const uint32_t n = (uint32_t)fdeg[x];
+    for (uint32_t j = lane_id(); j < n; j += kWave) {
+      const int32_t y = (int32_t)(row[j] >> kCntBits);
+      s += fdeg[y] + bdeg[y];
+    }
+  }
+  s = wave_sum(s);
+  if (lane_id() == 0 && s) atomicAdd(&acc[kTAccOrient], s);
+}
+
 __global__ void k_tzero(unsigned long long* p, int n) {
   for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0;
 }
@@ -552,6 +572,9 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
   HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(
       g->tmp.p, tb, g->work.as<unsigned long long>(), g->work2.as<unsigned long long>(),
       g->ids.as<int32_t>(), g->order.as<int32_t>(), E, 0, 64, st));
+  k_torient<<<(unsigned)((E + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, st>>>(
+      E, g->foff.as<unsigned long long>(), g->fdeg.as<unsigned long long>(),
+      g->bdeg.as<unsigned long long>(), g->frows.as<uint32_t>(), acc);
   HM_HIP(hipEventRecord(g->ev[1], st));
   k_tcount<MT><<<E, kBlock, lds_cnt, st>>>(
       E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->esize.as<uint8_t>(),
@@ -688,6 +711,8 @@ hm_status hm_count_temporal(hm_tgraph* g, int64_t delta, uint64_t out[HM_TEMPORA
   s.proj_pairs = (int64_t)acc[hm::kTAccPairs];
   s.max_window = (int64_t)acc[hm::kTAccMaxWin];
   s.neighbour_pairs = (int64_t)acc[hm::kTAccNbrPairs];
+  s.oriented_row_sum = (int64_t)acc[hm::kTAccOrient];
+  s.num_nodes = g->V;
   int64_t tot = 0;
   for (int k = 0; k < HM_TEMPORAL_BINS; ++k) tot += (int64_t)out[k];
   s.instances = tot;
