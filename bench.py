@@ -49,10 +49,13 @@ DESC = {
                    "[2,20], Zipf s=1.2, seed 4, int64 us timestamps, delta=3600 s",
     "static_c1": "static_c1: 5,000 hyperedges over 2,000 nodes, sizes ~k^-2.5 on [2,25], "
                  "Zipf s=1.2, seed 1",
+    "serving_c5": "serving_c5 (reduced): 64 graphs of the static_c2 family, E log-uniform on "
+                  "[1k, 6k], V = E/2, seed 5; 400 requests, half count queries, half update "
+                  "batches of 0.5-5% of E (70/30 insert/delete), seed 5000",
 }
 # Bounded CPU samples: a prefix of the same input sized for ~10-30 s of reference-program time.
 CPU_SAMPLE_EDGES = {"temporal_c4": 600, "static_c1": 3500}
-DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20}
+DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20, "serving_c5": 1}
 
 
 def cpu_threads() -> int:
@@ -192,6 +195,43 @@ def copy_probe(mib: int) -> None:
     print(json.dumps({"copy_probe_bytes": mib << 20}), flush=True)
 
 
+def run_serving(slots: int = 4, n_requests: int = 400) -> dict:
+    """Plain (one request at a time) versus served (hm_server, `slots` concurrent) on one request
+    stream: throughput with every request queued at t=0, then p50/p99 latency with Poisson
+    arrivals at 70% of the plain path's capacity. Results must agree request by request."""
+    from ipdps26_amd import serving
+
+    graphs = W.serving_graphs()
+    reqs = W.serving_requests(graphs, n_requests)
+    def note(msg):
+        print(f"[serving] {msg}", file=sys.stderr, flush=True)
+
+    plain = serving.run_plain(graphs, reqs, realtime=False)
+    note(f"plain saturated: {plain['requests_per_s']:.1f} req/s")
+    served = serving.run_served(graphs, reqs, realtime=False, slots=slots)
+    note(f"served saturated: {served['requests_per_s']:.1f} req/s")
+    same = all(np.array_equal(a, b) for a, b in zip(plain["results"], served["results"]))
+    rate = 0.7 * plain["requests_per_s"]
+    timed = W.serving_requests(graphs, n_requests, rate_per_s=rate)
+    plain_rt = serving.run_plain(graphs, timed, realtime=True)
+    note(f"plain at {rate:.1f} req/s: p50 {plain_rt['p50_ms']:.1f} ms p99 {plain_rt['p99_ms']:.1f} ms")
+    served_rt = serving.run_served(graphs, timed, realtime=True, slots=slots)
+    note(f"served at {rate:.1f} req/s: p50 {served_rt['p50_ms']:.1f} ms p99 {served_rt['p99_ms']:.1f} ms")
+    same = same and all(np.array_equal(a, b) for a, b in zip(plain_rt["results"], served_rt["results"]))
+    triples = serving.count_triples(reqs, served["results"])
+    drop = ("results",)
+    return {"ms": served["makespan_s"] * 1e3, "total": triples, "unit": "triples/s",
+            "extra": {"requests": n_requests, "slots": slots,
+                      "results_identical_plain_vs_served": bool(same),
+                      "saturated": {"plain": {k: v for k, v in plain.items() if k not in drop},
+                                    "served": {k: v for k, v in served.items() if k not in drop},
+                                    "speedup": served["requests_per_s"] / plain["requests_per_s"]},
+                      "poisson_at_70pct_plain_capacity": {
+                          "rate_per_s": rate,
+                          "plain": {k: v for k, v in plain_rt.items() if k not in drop},
+                          "served": {k: v for k, v in served_rt.items() if k not in drop}}}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +251,17 @@ def main() -> int:
         return 2
     steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.workload]
     hm.init(0)
+    if args.workload == "serving_c5":
+        r = run_serving()
+        line = {"metric": "exact h-motif triples classified/sec; p50/p99 latency, served vs plain",
+                "value": r["total"] / (r["ms"] * 1e-3), "unit": r["unit"], "n_gpus": 1,
+                "steps": 1, "warmup": 0, "ms_per_step": r["ms"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+                "data": "synthetic (seeded generator)",
+                "config": {"workload": DESC[args.workload], **r["extra"]},
+                "roofline": None, "cpu_baseline": None}
+        print(json.dumps(line), flush=True)
+        return 0
     run = run_temporal if args.workload == "temporal_c4" else run_static
     r = run(steps, args.warmup, not args.no_cpu)
     achieved = r["kernel_bytes"] / (r["kernel_ms"] * 1e-3) / 1e9

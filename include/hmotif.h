@@ -183,6 +183,55 @@ hm_status hm_tgraph_get_stats(hm_tgraph* g, hm_temporal_stats* out);
 /* Phases "project" (window bounds + temporal projection rows) and "count" (k_tcount). */
 hm_status hm_tgraph_phase_ms(hm_tgraph* g, const char* phase, double* ms);
 
+/* ------------------------------------------------------------------------------------------ */
+/* Serving: many device-resident hypergraphs, requests admitted and run as slots free up.     */
+/* No counterpart in the reference (its drivers run one graph, one request, one stream).      */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct hm_server hm_server;
+
+#define HM_REQ_COUNT 0  /* exact static count of the graph as of this request */
+#define HM_REQ_UPDATE 1 /* one batch (deletes + inserts); result = delta of the 30 bins */
+
+typedef struct hm_server_config {
+  int64_t hbm_budget_bytes; /* graphs are admitted while their estimated footprint fits */
+  int32_t slots;            /* requests running at once (worker threads, one stream per graph) */
+} hm_server_config;
+
+typedef struct hm_result {
+  int32_t status;                /* hm_status of the request */
+  int32_t kind;                  /* HM_REQ_COUNT or HM_REQ_UPDATE */
+  int64_t counts[HM_STATIC_BINS];/* counts (COUNT) or counts(after) - counts(before) (UPDATE) */
+  double queued_ms;              /* submit -> start */
+  double run_ms;                 /* start -> finish */
+  double latency_ms;             /* submit -> finish */
+  int32_t first_new_id;          /* UPDATE: id of the first inserted hyperedge (-1 if none) */
+  int32_t n_new;                 /* UPDATE: number of inserted hyperedges (ids are consecutive) */
+} hm_result;
+
+typedef struct hm_server_stats {
+  int64_t graphs;
+  int64_t reserved_bytes;
+  int64_t budget_bytes;
+  int64_t completed;
+  int64_t coalesced; /* count requests answered by another request's count */
+} hm_server_stats;
+
+hm_status hm_server_create(const hm_server_config* cfg, hm_server** out);
+/* Waits for the workers to finish the queued requests and frees every graph. */
+hm_status hm_server_destroy(hm_server* s);
+/* HM_ERR_OVER_BUDGET if the graph's estimated device footprint does not fit the budget. */
+hm_status hm_server_add_graph(hm_server* s, const int64_t* offsets, const int32_t* nodes,
+                              int32_t num_edges, int32_t* graph_id);
+hm_status hm_server_remove_graph(hm_server* s, int32_t graph_id);
+/* Non-blocking; requests on one graph run in submission order. */
+hm_status hm_server_submit_count(hm_server* s, int32_t graph_id, uint64_t* ticket);
+hm_status hm_server_submit_update(hm_server* s, int32_t graph_id, const int32_t* del_ids,
+                                  int32_t n_del, const int64_t* ins_offsets,
+                                  const int32_t* ins_nodes, int32_t n_ins, uint64_t* ticket);
+/* Blocks until the request is done; each ticket can be waited for once. */
+hm_status hm_server_wait(hm_server* s, uint64_t ticket, hm_result* out);
+hm_status hm_server_get_stats(hm_server* s, hm_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

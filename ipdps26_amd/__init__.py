@@ -97,6 +97,15 @@ _SIGNATURES = [
      [ctypes.c_void_p, _i32p, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _i32p, _i64p]),
     ("hm_count_delta", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, _i32p, ctypes.c_int32, _i32p, ctypes.c_int32, _i64p]),
+    ("hm_server_create", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_void_p)]),
+    ("hm_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("hm_server_add_graph", ctypes.c_int, [ctypes.c_void_p, _i64p, _i32p, ctypes.c_int32, _i32p]),
+    ("hm_server_remove_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("hm_server_submit_count", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _u64p]),
+    ("hm_server_submit_update", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int32, _i32p, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _u64p]),
+    ("hm_server_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+    ("hm_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("hm_tgraph_create", ctypes.c_int,
      [_i64p, _i32p, _i64p, ctypes.c_int32, _P(ctypes.c_void_p)]),
     ("hm_tgraph_destroy", ctypes.c_int, [ctypes.c_void_p]),
@@ -342,6 +351,90 @@ class TemporalGraph:
         ms = ctypes.c_double(0.0)
         _check(lib().hm_tgraph_phase_ms(self._h, phase.encode(), ctypes.byref(ms)))
         return ms.value
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("slots", ctypes.c_int32)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("kind", ctypes.c_int32),
+        ("counts", ctypes.c_int64 * STATIC_BINS),
+        ("queued_ms", ctypes.c_double),
+        ("run_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+        ("first_new_id", ctypes.c_int32),
+        ("n_new", ctypes.c_int32),
+    ]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in
+                ("graphs", "reserved_bytes", "budget_bytes", "completed", "coalesced")]
+
+
+class Server:
+    """Request loop over many device-resident hypergraphs (see hm_server_* in hmotif.h)."""
+
+    COUNT, UPDATE = 0, 1
+
+    def __init__(self, hbm_budget_bytes: int, slots: int = 4):
+        cfg = _ServerConfig(int(hbm_budget_bytes), int(slots))
+        self._h = ctypes.c_void_p()
+        _check(lib().hm_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().hm_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_graph(self, offsets: np.ndarray, nodes: np.ndarray) -> int:
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        nodes = np.ascontiguousarray(nodes if nodes.size else np.zeros(1), dtype=np.int32)
+        gid = ctypes.c_int32(-1)
+        _check(lib().hm_server_add_graph(self._h, _ptr(offsets, ctypes.c_int64),
+                                         _ptr(nodes, ctypes.c_int32), offsets.size - 1,
+                                         ctypes.byref(gid)))
+        return gid.value
+
+    def remove_graph(self, gid: int) -> None:
+        _check(lib().hm_server_remove_graph(self._h, gid))
+
+    def submit_count(self, gid: int) -> int:
+        t = ctypes.c_uint64(0)
+        _check(lib().hm_server_submit_count(self._h, gid, ctypes.byref(t)))
+        return t.value
+
+    def submit_update(self, gid: int, deleted, ins_offsets, ins_nodes) -> int:
+        d = np.ascontiguousarray(np.asarray(deleted, dtype=np.int32).reshape(-1))
+        o = np.ascontiguousarray(ins_offsets, dtype=np.int64)
+        n = np.ascontiguousarray(ins_nodes if np.size(ins_nodes) else np.zeros(1), dtype=np.int32)
+        t = ctypes.c_uint64(0)
+        _check(lib().hm_server_submit_update(
+            self._h, gid, _ptr(d if d.size else np.zeros(1, np.int32), ctypes.c_int32), d.size,
+            _ptr(o, ctypes.c_int64), _ptr(n, ctypes.c_int32), o.size - 1, ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        r = _Result()
+        st = lib().hm_server_wait(self._h, ticket, ctypes.byref(r))
+        _check(st)
+        return {"kind": r.kind, "counts": np.array(r.counts[:], dtype=np.int64),
+                "queued_ms": r.queued_ms, "run_ms": r.run_ms, "latency_ms": r.latency_ms,
+                "first_new_id": r.first_new_id, "n_new": r.n_new}
+
+    def stats(self) -> dict:
+        s = _ServerStats()
+        _check(lib().hm_server_get_stats(self._h, ctypes.byref(s)))
+        return {n: getattr(s, n) for n, _ in _ServerStats._fields_}
 
 
 def printed_motifs(counts30) -> list[int]:

@@ -293,10 +293,8 @@ static hm_status build_full_rows(hm_graph* g, DynScratch& d) {
   const int32_t E = c.E;
   hipStream_t st = g->stream;
   const size_t lds = ((size_t)E * sizeof(MT) + 15) & ~(size_t)15;
-  if (lds > 65536) {
-    HM_HIP(hipFuncSetAttribute((const void*)k_deg<MT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    HM_HIP(hipFuncSetAttribute((const void*)k_full_fill<MT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  }
+  HM_TRY(allow_lds((const void*)k_deg<MT>, lds));
+  HM_TRY(allow_lds((const void*)k_full_fill<MT>, lds));
   HM_TRY(d.deg.ensure(sizeof(unsigned long long) * (E + 1)));
   HM_TRY(d.roff.ensure(sizeof(unsigned long long) * (E + 1)));
   k_zero_acc<<<1, 1, 0, st>>>(d.deg.as<unsigned long long>() + E, 1);
@@ -337,8 +335,7 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
   k_fill_u8<<<grid_for(E), kBlock, 0, st>>>(d.flag.as<uint8_t>(), E, 0);
   k_set_flags<<<grid_for(n), kBlock, 0, st>>>(d.ids.as<int32_t>(), n, d.flag.as<uint8_t>());
   const size_t lds = (((size_t)E * sizeof(MT) + 2 * (size_t)E) + 15) & ~(size_t)15;
-  if (lds > 65536)
-    HM_HIP(hipFuncSetAttribute((const void*)k_delta<MT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  HM_TRY(allow_lds((const void*)k_delta<MT>, lds));
   k_delta<MT><<<n, kBlock, lds, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
                                       c.esize.as<uint8_t>(), c.voff.as<uint32_t>(),
                                       c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -15,14 +16,17 @@ struct DevBuf {
   size_t bytes = 0;
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
+  // Regrowth adds 50% headroom: graphs grow a little with every update batch, and hipFree
+  // synchronises the whole device, which would serialise concurrent requests.
   hm_status ensure(size_t need) {
     if (need <= bytes && p) return HM_OK;
+    size_t want = need ? need : 16;
     if (p) {
+      want = std::max(want, bytes + bytes / 2);
       (void)hipFree(p);
       p = nullptr;
       bytes = 0;
     }
-    size_t want = need ? need : 16;
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
       p = nullptr;

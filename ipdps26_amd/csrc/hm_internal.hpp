@@ -32,6 +32,9 @@ hm_status prepare_host_rows(const int64_t* offsets, const int32_t* nodes, int32_
 #include <hip/hip_runtime.h>
 namespace hm {
 hm_status check_hip(hipError_t e, const char* what, const char* file, int line);
+// Raises a kernel's dynamic-LDS limit to at least `bytes` (once per kernel and size; the
+// attribute call is not repeated on every launch).
+hm_status allow_lds(const void* kernel, size_t bytes);
 }
 // Evaluates a HIP call; on failure records the message and returns from the enclosing function.
 #define HM_HIP(call)                                                          \

@@ -2,7 +2,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "hm_internal.hpp"
 #include "hm_tables.hpp"
@@ -29,6 +31,19 @@ hm_status check_hip(hipError_t e, const char* what, const char* file, int line) 
   if (e == hipSuccess) return HM_OK;
   set_error("%s failed at %s:%d: %s", what, file, line, hipGetErrorString(e));
   return e == hipErrorOutOfMemory ? HM_ERR_OOM : HM_ERR_DEVICE;
+}
+
+hm_status allow_lds(const void* kernel, size_t bytes) {
+  if (bytes <= 65536) return HM_OK;
+  static std::mutex mu;
+  static std::unordered_map<const void*, size_t> granted;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& g = granted[kernel];
+  if (bytes <= g) return HM_OK;
+  hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return check_hip(e, "hipFuncSetAttribute", __FILE__, __LINE__);
+  g = bytes;
+  return HM_OK;
 }
 
 }  // namespace hm

@@ -1,0 +1,309 @@
+// Serving: a request loop over many device-resident hypergraphs.
+//
+// The reference has no serving layer (single host thread, default stream, a
+// synchronize after every launch). This one keeps many graphs resident in HBM and runs their
+// requests concurrently:
+//   * admission — a graph is admitted only if its estimated device footprint (CSR, projection
+//     rows bounded by sum_v deg_v (deg_v - 1), per-hyperedge scratch) fits the remaining HBM
+//     budget; otherwise hm_server_add_graph returns HM_ERR_OVER_BUDGET;
+//   * continuous batching — `slots` worker threads take whichever graph has pending work as soon
+//     as they are free; every graph owns its HIP stream, so requests on different graphs overlap
+//     on the device while requests on one graph keep their arrival order (an update is seen by
+//     every later count);
+//   * coalescing — consecutive count queries on an unchanged graph are answered by one count;
+//   * latency — each request records submit, start and finish times (steady clock).
+// Counts are exact, so a result is the same alone and in the mix.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "hm_internal.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  uint64_t ticket = 0;
+  int32_t graph = -1;
+  int32_t kind = 0;  // HM_REQ_COUNT or HM_REQ_UPDATE
+  std::vector<int32_t> dels;
+  std::vector<int64_t> ins_off;
+  std::vector<int32_t> ins_nodes;
+  Clock::time_point submit, start, finish;
+  hm_status status = HM_OK;
+  std::string error;
+  int64_t counts[HM_STATIC_BINS] = {0};
+  int32_t first_new_id = -1, n_new = 0;
+  bool done = false;
+};
+
+struct GraphSlot {
+  hm_graph* g = nullptr;
+  int64_t bytes = 0;
+  std::deque<std::shared_ptr<Request>> pending;
+  bool busy = false;     // a worker is running one of its requests
+  bool queued = false;   // the graph id sits in the ready queue
+  bool removed = false;
+};
+
+int64_t estimate_bytes(const int64_t* off, const int32_t* nodes, int32_t E) {
+  const int64_t I = off[E];
+  std::unordered_map<int32_t, int64_t> deg;
+  deg.reserve((size_t)std::min<int64_t>(I, 1 << 22));
+  for (int64_t k = 0; k < I; ++k) deg[nodes[k]]++;
+  int64_t pairs = 0;
+  for (auto& kv : deg) pairs += kv.second * (kv.second - 1);
+  const int64_t csr = 4 * (2 * I + 2 * (int64_t)E + 2 + 2 * (int64_t)deg.size());
+  const int64_t rows = 4 * std::min<int64_t>(pairs, (int64_t)E * (int64_t)E);
+  return csr + rows + 64 * (int64_t)E + (1 << 20);
+}
+
+}  // namespace
+
+struct hm_server {
+  hm_server_config cfg{};
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable work_cv, done_cv;
+  std::vector<GraphSlot> graphs;
+  std::deque<int32_t> ready;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> requests;
+  std::vector<std::thread> workers;
+  uint64_t next_ticket = 1;
+  int64_t used_bytes = 0;
+  int64_t completed = 0, coalesced = 0;
+  bool stop = false;
+
+  void enqueue_locked(int32_t gid) {
+    GraphSlot& s = graphs[(size_t)gid];
+    if (!s.busy && !s.queued && !s.pending.empty()) {
+      s.queued = true;
+      ready.push_back(gid);
+      work_cv.notify_one();
+    }
+  }
+
+  void worker_main() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      work_cv.wait(lk, [&] { return stop || !ready.empty(); });
+      if (stop && ready.empty()) return;
+      const int32_t gid = ready.front();
+      ready.pop_front();
+      GraphSlot& s = graphs[(size_t)gid];
+      s.queued = false;
+      if (s.pending.empty()) continue;
+      s.busy = true;
+      // head request, plus every directly following count when the head is a count
+      std::vector<std::shared_ptr<Request>> batch{s.pending.front()};
+      s.pending.pop_front();
+      if (batch[0]->kind == HM_REQ_COUNT)
+        while (!s.pending.empty() && s.pending.front()->kind == HM_REQ_COUNT) {
+          batch.push_back(s.pending.front());
+          s.pending.pop_front();
+        }
+      hm_graph* g = s.g;
+      lk.unlock();
+      const auto t0 = Clock::now();
+      Request& r = *batch[0];
+      hm_status st;
+      if (r.kind == HM_REQ_COUNT) {
+        uint64_t out[HM_STATIC_BINS];
+        st = hm_count_static(g, out);
+        for (int k = 0; k < HM_STATIC_BINS; ++k) r.counts[k] = (int64_t)out[k];
+      } else {
+        std::vector<int32_t> ids(r.ins_off.size() - 1);
+        st = hm_graph_update(g, r.dels.data(), (int32_t)r.dels.size(), r.ins_off.data(),
+                             r.ins_nodes.data(), (int32_t)ids.size(), ids.data(), r.counts);
+        r.n_new = (int32_t)ids.size();
+        r.first_new_id = ids.empty() ? -1 : ids[0];
+      }
+      const std::string err = st == HM_OK ? std::string() : std::string(hm_last_error());
+      const auto t1 = Clock::now();
+      lk.lock();
+      for (auto& q : batch) {
+        q->start = t0;
+        q->finish = t1;
+        q->status = st;
+        q->error = err;
+        if (q.get() != &r) std::memcpy(q->counts, r.counts, sizeof(r.counts));
+        q->done = true;
+      }
+      completed += (int64_t)batch.size();
+      coalesced += (int64_t)batch.size() - 1;
+      s.busy = false;
+      enqueue_locked(gid);
+      done_cv.notify_all();
+    }
+  }
+};
+
+extern "C" {
+
+hm_status hm_server_create(const hm_server_config* cfg, hm_server** out) {
+  hm::clear_error();
+  if (!cfg || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_create: null argument");
+  if (cfg->slots < 1 || cfg->slots > 64 || cfg->hbm_budget_bytes <= 0)
+    return hm::fail(HM_ERR_BAD_INPUT, "hm_server_create: need 1 <= slots <= 64 and a positive budget");
+  *out = nullptr;
+  size_t free_b = 0, total_b = 0;
+  if (hipSetDevice(hm::current_device()) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+    return hm::fail(HM_ERR_DEVICE, "hm_server_create: no usable device");
+  if ((uint64_t)cfg->hbm_budget_bytes > (uint64_t)total_b)
+    return hm::fail(HM_ERR_OVER_BUDGET, "hm_server_create: budget exceeds device memory");
+  hm_server* s = new (std::nothrow) hm_server();
+  if (!s) return hm::fail(HM_ERR_OOM, "hm_server_create: host allocation failed");
+  s->cfg = *cfg;
+  s->device = hm::current_device();
+  for (int32_t k = 0; k < cfg->slots; ++k) s->workers.emplace_back([s] { s->worker_main(); });
+  *out = s;
+  return HM_OK;
+}
+
+hm_status hm_server_destroy(hm_server* s) {
+  if (!s) return HM_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->work_cv.notify_all();
+  }
+  for (auto& t : s->workers) t.join();
+  for (auto& g : s->graphs)
+    if (g.g) hm_graph_destroy(g.g);
+  delete s;
+  return HM_OK;
+}
+
+hm_status hm_server_add_graph(hm_server* s, const int64_t* offsets, const int32_t* nodes,
+                              int32_t num_edges, int32_t* graph_id) {
+  hm::clear_error();
+  if (!s || !graph_id || !offsets || num_edges < 0)
+    return hm::fail(HM_ERR_BAD_INPUT, "hm_server_add_graph: bad argument");
+  const int64_t need = estimate_bytes(offsets, nodes, num_edges);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->used_bytes + need > s->cfg.hbm_budget_bytes) {
+      hm::set_error("hm_server_add_graph: needs ~%lld bytes, %lld of %lld in use",
+                    (long long)need, (long long)s->used_bytes, (long long)s->cfg.hbm_budget_bytes);
+      return HM_ERR_OVER_BUDGET;
+    }
+    s->used_bytes += need;  // reserved while the graph is built
+  }
+  hm_graph* g = nullptr;
+  hm_status st = hm_graph_create(offsets, nodes, num_edges, 0, &g);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (st != HM_OK) {
+    s->used_bytes -= need;
+    return st;
+  }
+  GraphSlot slot;
+  slot.g = g;
+  slot.bytes = need;
+  s->graphs.push_back(std::move(slot));
+  *graph_id = (int32_t)s->graphs.size() - 1;
+  return HM_OK;
+}
+
+hm_status hm_server_remove_graph(hm_server* s, int32_t gid) {
+  hm::clear_error();
+  if (!s) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_remove_graph: null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (gid < 0 || gid >= (int32_t)s->graphs.size() || s->graphs[(size_t)gid].removed)
+    return hm::fail(HM_ERR_BAD_INPUT, "hm_server_remove_graph: unknown graph");
+  GraphSlot& g = s->graphs[(size_t)gid];
+  s->done_cv.wait(lk, [&] { return !g.busy && g.pending.empty(); });
+  hm_graph_destroy(g.g);
+  g.g = nullptr;
+  g.removed = true;
+  s->used_bytes -= g.bytes;
+  return HM_OK;
+}
+
+static hm_status submit(hm_server* s, std::shared_ptr<Request> r, uint64_t* ticket) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (r->graph < 0 || r->graph >= (int32_t)s->graphs.size() || s->graphs[(size_t)r->graph].removed)
+    return hm::fail(HM_ERR_BAD_INPUT, "hm_server_submit: unknown graph");
+  r->ticket = s->next_ticket++;
+  r->submit = Clock::now();
+  s->requests[r->ticket] = r;
+  s->graphs[(size_t)r->graph].pending.push_back(r);
+  s->enqueue_locked(r->graph);
+  *ticket = r->ticket;
+  return HM_OK;
+}
+
+hm_status hm_server_submit_count(hm_server* s, int32_t graph_id, uint64_t* ticket) {
+  hm::clear_error();
+  if (!s || !ticket) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_submit_count: null argument");
+  auto r = std::make_shared<Request>();
+  r->graph = graph_id;
+  r->kind = HM_REQ_COUNT;
+  return submit(s, r, ticket);
+}
+
+hm_status hm_server_submit_update(hm_server* s, int32_t graph_id, const int32_t* del_ids,
+                                  int32_t n_del, const int64_t* ins_offsets,
+                                  const int32_t* ins_nodes, int32_t n_ins, uint64_t* ticket) {
+  hm::clear_error();
+  if (!s || !ticket || n_del < 0 || n_ins < 0 || (n_del && !del_ids) || !ins_offsets)
+    return hm::fail(HM_ERR_BAD_INPUT, "hm_server_submit_update: bad argument");
+  auto r = std::make_shared<Request>();
+  r->graph = graph_id;
+  r->kind = HM_REQ_UPDATE;
+  r->dels.assign(del_ids, del_ids + n_del);
+  r->ins_off.assign(ins_offsets, ins_offsets + n_ins + 1);
+  const int64_t I = ins_offsets[n_ins];
+  if (I < 0 || (I > 0 && !ins_nodes)) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_submit_update: bad CSR");
+  r->ins_nodes.assign(ins_nodes, ins_nodes + I);
+  return submit(s, r, ticket);
+}
+
+hm_status hm_server_wait(hm_server* s, uint64_t ticket, hm_result* out) {
+  hm::clear_error();
+  if (!s || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_wait: null argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->requests.find(ticket);
+  if (it == s->requests.end()) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_wait: unknown ticket");
+  std::shared_ptr<Request> r = it->second;
+  s->done_cv.wait(lk, [&] { return r->done; });
+  s->requests.erase(it);
+  using ms = std::chrono::duration<double, std::milli>;
+  out->status = r->status;
+  out->kind = r->kind;
+  std::memcpy(out->counts, r->counts, sizeof(out->counts));
+  out->queued_ms = ms(r->start - r->submit).count();
+  out->run_ms = ms(r->finish - r->start).count();
+  out->latency_ms = ms(r->finish - r->submit).count();
+  out->first_new_id = r->first_new_id;
+  out->n_new = r->n_new;
+  if (r->status != HM_OK) {
+    hm::set_error("%s", r->error.c_str());
+    return r->status;
+  }
+  return HM_OK;
+}
+
+hm_status hm_server_get_stats(hm_server* s, hm_server_stats* out) {
+  hm::clear_error();
+  if (!s || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_get_stats: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  out->graphs = 0;
+  for (auto& g : s->graphs) out->graphs += g.removed ? 0 : 1;
+  out->reserved_bytes = s->used_bytes;
+  out->budget_bytes = s->cfg.hbm_budget_bytes;
+  out->completed = s->completed;
+  out->coalesced = s->coalesced;
+  return HM_OK;
+}
+
+}  // extern "C"

@@ -330,9 +330,7 @@ static size_t dense_lds_bytes(int32_t E, bool with_sizes) {
 
 template <typename MT>
 static hm_status set_lds_attr(const void* fn, size_t bytes) {
-  if (bytes > 65536)
-    HM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-  return HM_OK;
+  return allow_lds(fn, bytes);
 }
 
 template <typename MT>

@@ -520,15 +520,9 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
     set_error("temporal count: a delta window of %d hyperedges exceeds the LDS tables", Wmax);
     return HM_ERR_UNSUPPORTED;
   }
-  if (lds_proj > 65536)
-    HM_HIP(hipFuncSetAttribute((const void*)k_tproj_count<MT>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_proj));
-  if (lds_proj > 65536)
-    HM_HIP(hipFuncSetAttribute((const void*)k_tproj_fill<MT>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_proj));
-  if (lds_cnt > 65536)
-    HM_HIP(hipFuncSetAttribute((const void*)k_tcount<MT>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cnt));
+  HM_TRY(allow_lds((const void*)k_tproj_count<MT>, lds_proj));
+  HM_TRY(allow_lds((const void*)k_tproj_fill<MT>, lds_proj));
+  HM_TRY(allow_lds((const void*)k_tcount<MT>, lds_cnt));
 
   k_tproj_count<MT><<<E, kBlock, lds_proj, st>>>(
       E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),

@@ -75,6 +75,63 @@ def serving_c5_sizes(n_graphs: int = SERVING_C5_GRAPHS, lo: int = 10_000, hi: in
     return [int(round(math.exp(math.log(lo) + x * (math.log(hi) - math.log(lo))))) for x in u]
 
 
+# Reduced serving workload: the static_c2 generator family at sizes the exact engine counts per
+# request in milliseconds (the full [10k, 1M] range holds graphs with > 1e13 triples each).
+SERVING_REDUCED_LO, SERVING_REDUCED_HI = 1_000, 6_000
+
+
+def serving_graphs(n_graphs: int = SERVING_C5_GRAPHS, lo: int = SERVING_REDUCED_LO,
+                   hi: int = SERVING_REDUCED_HI, seed: int = 5):
+    """n_graphs hypergraphs of the static_c2 family, E_i log-uniform on [lo, hi], V_i = E_i / 2."""
+    out = []
+    for i, e in enumerate(serving_c5_sizes(n_graphs, lo, hi, seed)):
+        p = GenParams(num_nodes=max(e // 2, STATIC_C2.kmax), num_edges=e, alpha=STATIC_C2.alpha,
+                      kmin=STATIC_C2.kmin, kmax=STATIC_C2.kmax, zipf_s=STATIC_C2.zipf_s,
+                      seed=seed * 1000 + i)
+        out.append(generate_static(p))
+    return out
+
+
+def serving_requests(graphs, n_requests: int, seed: int = 5000, count_frac: float = 0.5,
+                     insert_frac: float = 0.7, min_batch_frac: float = 0.005,
+                     max_batch_frac: float = 0.05, rate_per_s: float = 0.0):
+    """A seeded request stream over `graphs`: count queries and update batches (batch sizes
+    log-uniform on [min, max] x E, insert_frac of the changes are fresh generator hyperedges, the
+    rest deletes of uniformly chosen live hyperedges), graphs chosen uniformly, Poisson arrival
+    times at rate_per_s (all at 0 when rate_per_s is 0). Live sets are tracked as the engine
+    assigns ids (inserted hyperedges get the next ids), so both a plain and a served run apply
+    exactly the same updates."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    live = [list(range(off.size - 1)) for off, _ in graphs]
+    nxt = [off.size - 1 for off, _ in graphs]
+    reqs, t = [], 0.0
+    for r in range(n_requests):
+        if rate_per_s > 0:
+            t += float(rng.exponential(1.0 / rate_per_s))
+        gi = int(rng.integers(len(graphs)))
+        if rng.random() < count_frac:
+            reqs.append({"t": t, "graph": gi, "kind": "count"})
+            continue
+        e0 = graphs[gi][0].size - 1
+        lo, hi = max(1.0, min_batch_frac * e0), max(1.0, max_batch_frac * e0)
+        size = int(round(math.exp(math.log(lo) + rng.random() * (math.log(hi) - math.log(lo)))))
+        n_ins = int(round(size * insert_frac))
+        n_del = min(size - n_ins, len(live[gi]))
+        pick_idx = rng.choice(len(live[gi]), size=n_del, replace=False) if n_del else []
+        dels = sorted(live[gi][k] for k in pick_idx)
+        keep = set(dels)
+        live[gi] = [x for x in live[gi] if x not in keep]
+        p = GenParams(num_nodes=max(e0 // 2, STATIC_C2.kmax), num_edges=n_ins, alpha=STATIC_C2.alpha,
+                      kmin=STATIC_C2.kmin, kmax=STATIC_C2.kmax, zipf_s=STATIC_C2.zipf_s,
+                      seed=seed * 100_000 + r)
+        ioff, inodes = generate_static(p)
+        live[gi].extend(range(nxt[gi], nxt[gi] + n_ins))
+        nxt[gi] += n_ins
+        reqs.append({"t": t, "graph": gi, "kind": "update", "dels": np.asarray(dels, dtype=np.int32),
+                     "ins_off": ioff, "ins_nodes": inodes})
+    return reqs
+
+
 # ------------------------------------------------------------------------------------------
 # Reference file formats
 # ------------------------------------------------------------------------------------------

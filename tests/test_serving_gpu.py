@@ -1,0 +1,62 @@
+"""Serving: every request's result is the same alone (plain, one request at a time) and in the mix
+(served, concurrent slots, coalesced counts), admission respects the HBM budget."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+import ipdps26_amd as hm
+from ipdps26_amd import serving, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stream(n_graphs=10, n_req=120, rate=0.0):
+    graphs = W.serving_graphs(n_graphs, lo=200, hi=1500, seed=7)
+    reqs = W.serving_requests(graphs, n_req, seed=77, rate_per_s=rate)
+    return graphs, reqs
+
+
+def test_served_equals_plain(gpu_ready):
+    graphs, reqs = small_stream()
+    plain = serving.run_plain(graphs, reqs, realtime=False)
+    served = serving.run_served(graphs, reqs, realtime=False, slots=4)
+    assert len(plain["results"]) == len(served["results"]) == len(reqs)
+    for k, (a, b) in enumerate(zip(plain["results"], served["results"])):
+        assert np.array_equal(a, b), f"request {k} ({reqs[k]['kind']})"
+    # a count that precedes every update of its graph is the oracle's count of the input graph
+    touched, checked = set(), 0
+    for q, res in zip(reqs, served["results"]):
+        if q["kind"] == "update":
+            touched.add(q["graph"])
+        elif q["graph"] not in touched:
+            assert np.array_equal(res, O.static_mochy(*graphs[q["graph"]]))
+            checked += 1
+    assert checked > 0
+
+
+def test_served_with_arrivals_and_one_slot(gpu_ready):
+    graphs, reqs = small_stream(n_graphs=6, n_req=60, rate=2000.0)
+    plain = serving.run_plain(graphs, reqs, realtime=True)
+    served = serving.run_served(graphs, reqs, realtime=True, slots=1)
+    for a, b in zip(plain["results"], served["results"]):
+        assert np.array_equal(a, b)
+
+
+def test_admission_budget(gpu_ready):
+    off, nodes = hm.generate_static(W.scaled(W.STATIC_C2, 2000, 1000))
+    with hm.Server(hbm_budget_bytes=256 << 10, slots=2) as srv:
+        with pytest.raises(hm.HMotifError) as ei:
+            srv.add_graph(off, nodes)
+        assert ei.value.status == 2
+    with hm.Server(hbm_budget_bytes=1 << 30, slots=2) as srv:
+        gid = srv.add_graph(off, nodes)
+        st = srv.stats()
+        assert st["graphs"] == 1 and 0 < st["reserved_bytes"] <= st["budget_bytes"]
+        c = srv.wait(srv.submit_count(gid))["counts"]
+        with hm.Graph(off, nodes) as g:
+            assert np.array_equal(c, g.count_static().astype(np.int64))
+        srv.remove_graph(gid)
+        assert srv.stats()["reserved_bytes"] == 0
+        with pytest.raises(hm.HMotifError):
+            srv.submit_count(gid)
