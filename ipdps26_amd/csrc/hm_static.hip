@@ -191,8 +191,11 @@ __global__ __launch_bounds__(kBlock) void k_work(int32_t E, const unsigned long 
   if (lane_id() == 0) work[r] = s + n;
 }
 
-template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_tri(
+// 8 waves per centre workgroup (c1: 4 waves 9.03 ms, 8 waves 7.85 ms, 16 waves 8.09 ms).
+constexpr int kTriWaves = 8;
+
+template <typename MT, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_tri(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges,
     const int32_t* __restrict__ rank, const int32_t* __restrict__ perm,
@@ -203,13 +206,13 @@ __global__ __launch_bounds__(kBlock) void k_tri(
   MT* M = reinterpret_cast<MT*>(smem);
   uint8_t* sz = reinterpret_cast<uint8_t*>(M + E);
   constexpr int kCopies = 8;
-  __shared__ int32_t h[kWavesPerBlock][32][kCopies];
+  __shared__ int32_t h[NW][32][kCopies];
   __shared__ int8_t tbl[128];
   const int32_t r = order[blockIdx.x];
   const uint32_t nA = rlen[r];
   if (nA < 2) return;  // no triangle has r as its lowest-ranked member
   const int32_t a = perm[r];
-  for (int k = threadIdx.x; k < kWavesPerBlock * 32 * kCopies; k += blockDim.x)
+  for (int k = threadIdx.x; k < NW * 32 * kCopies; k += blockDim.x)
     (&h[0][0][0])[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_static_bin[threadIdx.x];
   for (int x = threadIdx.x; x < E; x += blockDim.x) sz[x] = szr[x];
@@ -219,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_tri(
   const int copy = lane & (kCopies - 1);
   const int sa = sz[r];
   const uint32_t* rowA = rows + roff[r];
-  for (uint32_t ib = w; ib < nA; ib += kWavesPerBlock) {
+  for (uint32_t ib = w; ib < nA; ib += NW) {
     const uint32_t eb = rowA[ib];
     const int32_t rb = (int32_t)(eb >> kCntBits);
     const int cab = (int)(eb & kCntMask);
@@ -227,8 +230,8 @@ __global__ __launch_bounds__(kBlock) void k_tri(
     const int sb = sz[rb];
     const uint32_t* rowB = rows + roff[rb];
     const uint32_t nB = rlen[rb];
-    for (uint32_t j = lane; j < nB; j += kWave) {
-      const uint32_t ec = rowB[j];
+    int32_t* hc = &h[w][0][copy];
+    auto visit = [&](uint32_t ec) {
       const int32_t rc = (int32_t)(ec >> kCntBits);
       const MT mc = M[rc];
       if (mc) {
@@ -237,17 +240,25 @@ __global__ __launch_bounds__(kBlock) void k_tri(
         const int g = popc((MT)(mab & mc));
         const int sc = sz[rc];
         const int closed = tbl[venn_id(sa, sb, sc, cab, cbc, cac, g)];
-        atomicAdd(&h[w][closed][copy], 1);
-        atomicSub(&h[w][open_bin(sa, sb, sc, cab, cac)][copy], 1);
-        atomicSub(&h[w][open_bin(sb, sa, sc, cab, cbc)][copy], 1);
-        atomicSub(&h[w][open_bin(sc, sa, sb, cac, cbc)][copy], 1);
+        atomicAdd(&hc[closed * kCopies], 1);
+        atomicSub(&hc[open_bin(sa, sb, sc, cab, cac) * kCopies], 1);
+        atomicSub(&hc[open_bin(sb, sa, sc, cab, cbc) * kCopies], 1);
+        atomicSub(&hc[open_bin(sc, sa, sb, cac, cbc) * kCopies], 1);
       }
+    };
+    // two row loads in flight per lane before either is used (lane l reads l, l + 64, ...)
+    uint32_t j = lane;
+    for (; j + kWave < nB; j += 2 * kWave) {
+      const uint32_t e0 = rowB[j], e1 = rowB[j + kWave];
+      visit(e0);
+      visit(e1);
     }
+    if (j < nB) visit(rowB[j]);
   }
   __syncthreads();
   if (threadIdx.x < kStaticBins) {
     long long s = 0;
-    for (int q = 0; q < kWavesPerBlock; ++q)
+    for (int q = 0; q < NW; ++q)
       for (int c = 0; c < kCopies; ++c) s += h[q][threadIdx.x][c];
     if (s) atomicAdd(&acc[kAccHist + threadIdx.x], (unsigned long long)s);
   }
@@ -343,7 +354,7 @@ static hm_status count_static_dense(hm_graph* g, unsigned long long* h_acc) {
   const size_t lds_tri = dense_lds_bytes<MT>(E, true);
   HM_TRY(set_lds_attr<MT>((const void*)k_proj_count<MT>, lds_cnt));
   HM_TRY(set_lds_attr<MT>((const void*)k_proj_fill<MT>, lds_cnt));
-  HM_TRY(set_lds_attr<MT>((const void*)k_tri<MT>, lds_tri));
+  HM_TRY(set_lds_attr<MT>((const void*)k_tri<MT, kTriWaves>, lds_tri));
 
   HM_TRY(s.deg.ensure(sizeof(uint32_t) * E));
   HM_TRY(s.ids.ensure(sizeof(int32_t) * E));
@@ -418,11 +429,13 @@ static hm_status count_static_dense(hm_graph* g, unsigned long long* h_acc) {
                                                       E, 0, 64, st));
   HM_HIP(hipEventRecord(g->ev[1], st));
   // 6. triangles
-  k_tri<MT><<<E, kBlock, lds_tri, st>>>(
-      E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(),
-      c.vedges.as<int32_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), s.szr.as<uint8_t>(),
-      s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
-      s.order.as<int32_t>(), acc);
+#define HM_TRI_ARGS                                                                           \
+  E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(),                        \
+      c.vedges.as<int32_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), s.szr.as<uint8_t>(),   \
+      s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),             \
+      s.order.as<int32_t>(), acc
+  k_tri<MT, kTriWaves><<<E, kTriWaves * kWave, lds_tri, st>>>(HM_TRI_ARGS);
+#undef HM_TRI_ARGS
   HM_HIP(hipGetLastError());
   HM_HIP(hipEventRecord(g->ev[2], st));
   k_stats<<<64, kBlock, 0, st>>>(E, c.V, c.voff.as<uint32_t>(), s.deg.as<uint32_t>(), s.orient.as<unsigned long long>(),

@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <numeric>
 #include <vector>
 
@@ -264,8 +265,10 @@ __device__ inline void ordered_open_pairs(const MT* M, const uint16_t* NL, const
   for (int k = 0; k < 8; ++k) cnt[k] = wave_sum(cnt[k]);
 }
 
-template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_tcount(
+// Histogram: per wave, two interleaved 64-bit copies updated with per-lane LDS atomics (measured
+// faster here than aggregating equal bins across the wave with ballots).
+template <typename MT, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_tcount(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
     const int32_t* __restrict__ vpos, const int32_t* __restrict__ lo,
@@ -273,10 +276,11 @@ __global__ __launch_bounds__(kBlock) void k_tcount(
     const uint32_t* __restrict__ frows, const unsigned long long* __restrict__ boff,
     const unsigned long long* __restrict__ bdeg, const uint32_t* __restrict__ brows,
     const int32_t* __restrict__ order, int32_t Wmax, unsigned long long* __restrict__ acc) {
+  constexpr int kC = 2;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ long long hist[kWavesPerBlock][kTemporalBins];
+  __shared__ unsigned long long hist[NW][kTemporalBins * kC];
   __shared__ int8_t tbl[128];
-  __shared__ uint32_t wcnt[kWavesPerBlock];
+  __shared__ uint32_t wcnt[NW];
   __shared__ uint32_t s_nN, s_next;
   const int32_t i = order[blockIdx.x];
   const int32_t base = lo[i];
@@ -288,16 +292,15 @@ __global__ __launch_bounds__(kBlock) void k_tcount(
   off += ((size_t)Wmax * sizeof(uint16_t) + 15) & ~(size_t)15;
   uint8_t* sz = reinterpret_cast<uint8_t*>(smem + off);
 
-  for (int k = threadIdx.x; k < kWavesPerBlock * kTemporalBins; k += blockDim.x)
-    (&hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < NW * kTemporalBins * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
   if (threadIdx.x == 0) s_next = 0;
   for (int k = threadIdx.x; k < W; k += blockDim.x) sz[k] = esize[base + k];
   build_window_masks<MT>(M, i, base, i - 1, eoff, enodes, voff, vpos);
 
+  const int lane = lane_id(), w = wave_id();
   // NL: neighbours of i inside the window, ascending (relative positions)
   {
-    const int lane = lane_id(), w = wave_id();
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t running = 0;
     for (int32_t k0 = 0; k0 < W; k0 += blockDim.x) {
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_tcount(
       if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
       __syncthreads();
       uint32_t woff = 0, tot = 0;
-      for (int q = 0; q < kWavesPerBlock; ++q) {
+      for (int q = 0; q < NW; ++q) {
         woff += q < w ? wcnt[q] : 0;
         tot += wcnt[q];
       }
@@ -319,17 +322,18 @@ __global__ __launch_bounds__(kBlock) void k_tcount(
   }
   __syncthreads();
   const uint32_t nN = s_nN;
-  const int lane = lane_id(), w = wave_id();
   const int si = (int)esize[i];
-  long long* h = hist[w];
+  unsigned long long* hw = hist[w];
+  const int cp = lane & (kC - 1);
 
   // open at i, counted for all ordered neighbour pairs as if disjoint (closed ones are taken
   // back below when their triangle is met)
-  if (w == kWavesPerBlock - 1) {
+  if (w == NW - 1) {
     unsigned long long cnt[8];
     ordered_open_pairs<MT>(M, NL, sz, nN, si, cnt);
     if (lane == 0)
-      for (int k = 0; k < 8; ++k) h[tbl[48 + k]] += (long long)cnt[k];
+      for (int k = 0; k < 8; ++k)
+        if (cnt[k]) atomicAdd(&acc[kTAccHist + tbl[48 + k]], cnt[k]);
   }
 
   for (;;) {
@@ -343,61 +347,64 @@ __global__ __launch_bounds__(kBlock) void k_tcount(
     const int sx = sz[xr];
     const int cxi = popc(mx);
     const int fx = sx > cxi ? 1 : 0;
-    // (b) forward row of x inside (x, i): closed triangles and triples open at x
+    // (b) forward row of x inside (x, i): closed triangles and triples open at x. Two row loads
+    // are in flight per lane; the row is ascending, so the scan stops at the first entry >= i.
     {
       const uint32_t* row = frows + foff[x];
       const uint32_t n = (uint32_t)fdeg[x];
-      for (uint32_t j0 = 0; j0 < n; j0 += kWave) {
-        const uint32_t j = j0 + lane;
-        const uint32_t e = j < n ? row[j] : 0xFFFFFFFFu;
+      auto visit = [&](uint32_t j, uint32_t e) {
         const int32_t y = (int32_t)(e >> kCntBits);
-        const bool ok = j < n && y < i;
-        if (__ballot(ok) == 0ull) break;
-        int b1 = 0, b2 = 0;
-        bool closed = false;
-        if (ok) {
-          const int32_t yr = y - base;
-          const int cxy = (int)(e & kCntMask);
-          const MT my = M[yr];
-          const int sy = sz[yr];
-          if (my) {
-            closed = true;
-            const int cyi = popc(my);
-            b1 = tbl[venn_id(sx, sy, si, cxy, cyi, cxi, popc((MT)(mx & my)))];
-            b2 = tbl[as_open_id(fx, sy > cyi ? 1 : 0, cxi, cyi, si)];
-          } else {
-            b1 = tbl[venn_id(sx, sy, si, cxy, 0, cxi, 0)];
-          }
+        if (j >= n || y >= i) return;
+        const int32_t yr = y - base;
+        const int cxy = (int)(e & kCntMask);
+        const MT my = M[yr];
+        const int sy = sz[yr];
+        if (my) {
+          const int cyi = popc(my);
+          atomicAdd(&hw[tbl[venn_id(sx, sy, si, cxy, cyi, cxi, popc((MT)(mx & my)))] * kC + cp], 1ull);
+          atomicAdd(&hw[tbl[as_open_id(fx, sy > cyi ? 1 : 0, cxi, cyi, si)] * kC + cp], ~0ull);  // -1
+        } else {
+          atomicAdd(&hw[tbl[venn_id(sx, sy, si, cxy, 0, cxi, 0)] * kC + cp], 1ull);
         }
-        wave_hist_add<long long>(h, b1, ok, 1);
-        wave_hist_add<long long>(h, b2, ok && closed, -1);
+      };
+      for (uint32_t j0 = 0; j0 < n; j0 += 2 * kWave) {
+        const uint32_t ja = j0 + lane, jb = ja + kWave;
+        const uint32_t ea = ja < n ? row[ja] : 0xFFFFFFFFu;
+        const uint32_t eb = jb < n ? row[jb] : 0xFFFFFFFFu;
+        visit(ja, ea);
+        visit(jb, eb);
+        // stop once this pair of chunks reached past the anchor (or the row's end)
+        if (__ballot(jb >= n || (int32_t)(eb >> kCntBits) >= i) != 0ull) break;
       }
     }
-    // (e) backward row of x inside [lo_i, x): triples (y, x, i) open at x
+    // (e) backward row of x inside [lo_i, x): triples (y, x, i) open at x. The row is ascending,
+    // so it is read from its end until it leaves the window.
     {
       const uint32_t* row = brows + boff[x];
-      const uint32_t n = (uint32_t)bdeg[x];
-      const uint32_t j0 = lower_bound_dev<uint32_t>(row, n, (uint32_t)base << kCntBits);
-      for (uint32_t jb = j0; jb < n; jb += kWave) {
-        const uint32_t j = jb + lane;
-        bool ok = false;
+      const int32_t n = (int32_t)bdeg[x];
+      for (int32_t jend = n; jend > 0; jend -= kWave) {
+        const int32_t j = jend - kWave + lane;
+        bool ok = false, inside = false;
         int b = 0;
-        if (j < n) {
+        if (j >= 0) {
           const uint32_t e = row[j];
           const int32_t yr = (int32_t)(e >> kCntBits) - base;
-          if (M[yr] == (MT)0) {
+          inside = yr >= 0;
+          if (inside && M[yr] == (MT)0) {
             ok = true;
             b = tbl[venn_id(sz[yr], sx, si, (int)(e & kCntMask), cxi, 0, 0)];
           }
         }
-        wave_hist_add<long long>(h, b, ok, 1);
+        if (ok) atomicAdd(&hw[b * kC + cp], 1ull);
+        if (__ballot(j >= 0 && !inside)) break;
       }
     }
   }
   __syncthreads();
   for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x) {
     long long s = 0;
-    for (int q = 0; q < kWavesPerBlock; ++q) s += hist[q][b];
+    for (int q = 0; q < NW; ++q)
+      for (int c = 0; c < kC; ++c) s += (long long)hist[q][b * kC + c];
     if (s) atomicAdd(&acc[kTAccHist + b], (unsigned long long)s);
   }
 }
@@ -479,6 +486,25 @@ static size_t win_lds_bytes(int32_t R, bool wide) {
   return (((size_t)R * (wide ? 8 : 4)) + 15) & ~(size_t)15;
 }
 
+// Launch of the counting kernel: 16 waves per anchor workgroup and per-lane 64-bit LDS atomics
+// for the histogram. Measured on the 200k-edge temporal workload (c4): 4 waves with wave-aggregated
+// histograms 7.04 s, 8 waves aggregated 3.74 s, 8 waves atomics 2.46 s, 16 waves atomics 2.12 s.
+constexpr int kTCountWaves = 16;
+
+template <typename MT>
+static hm_status launch_tcount(hm_tgraph* g, int32_t Wmax, size_t lds, unsigned long long* acc) {
+  const int32_t E = g->E;
+  HM_TRY(allow_lds((const void*)k_tcount<MT, kTCountWaves>, lds));
+  k_tcount<MT, kTCountWaves><<<E, kTCountWaves * kWave, lds, g->stream>>>(
+      E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->esize.as<uint8_t>(),
+      g->voff.as<uint32_t>(), g->vpos.as<int32_t>(), g->lo.as<int32_t>(),
+      g->foff.as<unsigned long long>(), g->fdeg.as<unsigned long long>(), g->frows.as<uint32_t>(),
+      g->boff.as<unsigned long long>(), g->bdeg.as<unsigned long long>(), g->brows.as<uint32_t>(),
+      g->order.as<int32_t>(), Wmax, acc);
+  HM_HIP(hipGetLastError());
+  return HM_OK;
+}
+
 template <typename MT>
 static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned long long* h_acc) {
   const int32_t E = g->E;
@@ -515,14 +541,14 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
   (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
   const size_t lds_proj = win_lds_bytes(Rmax, sizeof(MT) == 8);
   const size_t lds_cnt = TCountLds<MT>::bytes(Wmax);
-  const size_t static_cnt = sizeof(long long) * kWavesPerBlock * kTemporalBins + 256;
+  const size_t static_cnt = sizeof(long long) * 16 * kTemporalBins * 2 + 256;
   if (lds_proj > (size_t)lim || lds_cnt + static_cnt > (size_t)lim || Wmax > 65535) {
     set_error("temporal count: a delta window of %d hyperedges exceeds the LDS tables", Wmax);
     return HM_ERR_UNSUPPORTED;
   }
   HM_TRY(allow_lds((const void*)k_tproj_count<MT>, lds_proj));
   HM_TRY(allow_lds((const void*)k_tproj_fill<MT>, lds_proj));
-  HM_TRY(allow_lds((const void*)k_tcount<MT>, lds_cnt));
+
 
   k_tproj_count<MT><<<E, kBlock, lds_proj, st>>>(
       E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
@@ -570,13 +596,7 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
       E, g->foff.as<unsigned long long>(), g->fdeg.as<unsigned long long>(),
       g->bdeg.as<unsigned long long>(), g->frows.as<uint32_t>(), acc);
   HM_HIP(hipEventRecord(g->ev[1], st));
-  k_tcount<MT><<<E, kBlock, lds_cnt, st>>>(
-      E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->esize.as<uint8_t>(),
-      g->voff.as<uint32_t>(), g->vpos.as<int32_t>(), g->lo.as<int32_t>(),
-      g->foff.as<unsigned long long>(), g->fdeg.as<unsigned long long>(), g->frows.as<uint32_t>(),
-      g->boff.as<unsigned long long>(), g->bdeg.as<unsigned long long>(), g->brows.as<uint32_t>(),
-      g->order.as<int32_t>(), Wmax, acc);
-  HM_HIP(hipGetLastError());
+  HM_TRY(launch_tcount<MT>(g, Wmax, lds_cnt, acc));
   HM_HIP(hipEventRecord(g->ev[2], st));
   HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kTAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
