@@ -93,6 +93,7 @@ _SIGNATURES = [
     ("hm_count_static", ctypes.c_int, [ctypes.c_void_p, _u64p]),
     ("hm_graph_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(_GraphStats)]),
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
+    ("hm_graph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_update", ctypes.c_int,
      [ctypes.c_void_p, _i32p, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _i32p, _i64p]),
     ("hm_count_delta", ctypes.c_int,
@@ -260,6 +261,10 @@ class Graph:
         ms = ctypes.c_double(0.0)
         _check(lib().hm_graph_phase_ms(self._h, phase.encode(), ctypes.byref(ms)))
         return ms.value
+
+    def set_tile(self, tile_ids: int) -> None:
+        """Key tile of count_static: 0 = automatic, > 0 = forced tiles of that many ids."""
+        _check(lib().hm_graph_set_tile(self._h, int(tile_ids)))
 
     def update(self, deleted, ins_offsets=None, ins_nodes=None, delta: bool = True):
         """Deletes the ids in `deleted`, inserts the CSR (ins_offsets, ins_nodes); returns

@@ -4,6 +4,8 @@
 
 #include <cstdint>
 
+#include "hmotif.h"
+
 namespace hm {
 
 constexpr int kWave = 64;
@@ -89,6 +91,76 @@ __device__ inline void build_masks(MT* M, int32_t E, int32_t a, const uint32_t* 
     }
   }
   __syncthreads();
+}
+
+}  // namespace hm
+
+namespace hm {
+
+// Index of the lowest set bit of a non-zero mask.
+__device__ inline int low_bit(uint32_t m) { return __builtin_ctz(m); }
+__device__ inline int low_bit(mask64_t m) { return __builtin_ctzll(m); }
+
+// First position p in [lo, hi) with keys[p] >= t (keys ascending on [lo, hi)), else hi.
+__device__ inline uint32_t lower_bound_key(const int32_t* __restrict__ keys, uint32_t lo, uint32_t hi,
+                                           int32_t t) {
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (keys[mid] < t) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The part of a hyperedge's V2H rows that falls in one key tile [k0, k1): for node a_i (i < sa),
+// positions lo[i] .. lo[i] + (pre[i + 1] - pre[i]) of its row, whose keys (hyperedge ids or ranks,
+// ascending within the row) lie in the tile. pre[sa] is the total.
+struct TileRanges {
+  uint32_t lo[HM_MAX_EDGE_SIZE];
+  uint32_t pre[HM_MAX_EDGE_SIZE + 1];
+};
+
+// Fills tr for the nodes anodes[0 .. sa) (sa <= 64, handled by wave 0) and returns the total, the
+// same on every lane. Starts and ends with a workgroup barrier, so it may follow any use of tr.
+__device__ inline uint32_t tile_ranges(TileRanges& tr, int sa, const int32_t* __restrict__ anodes,
+                                       const uint32_t* __restrict__ voff,
+                                       const int32_t* __restrict__ keys, int32_t k0, int32_t k1) {
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    const int i = (int)threadIdx.x;
+    uint32_t n = 0;
+    if (i < sa && k0 < k1) {
+      const int32_t v = anodes[i];
+      const uint32_t s = voff[v], t = voff[v + 1];
+      const uint32_t lo = lower_bound_key(keys, s, t, k0);
+      n = lower_bound_key(keys, lo, t, k1) - lo;
+      tr.lo[i] = lo;
+    }
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(n, o);
+      if (i >= o) n += y;
+    }
+    if (i < sa) tr.pre[i + 1] = n;
+    if (i == 0) tr.pre[0] = 0;
+  }
+  __syncthreads();
+  return tr.pre[sa];
+}
+
+// Calls f(i, key) for every entry of the tile ranges, spread over all lanes of the workgroup.
+template <typename F>
+__device__ inline void for_tile_entries(const TileRanges& tr, int sa, uint32_t total,
+                                        const int32_t* __restrict__ keys, F&& f) {
+  for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
+    int lo = 0, hi = sa;  // largest i with pre[i] <= q
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (tr.pre[mid] <= q) lo = mid;
+      else hi = mid;
+    }
+    f(lo, keys[tr.lo[lo] + (q - tr.pre[lo])]);
+  }
 }
 
 }  // namespace hm

@@ -75,12 +75,19 @@ struct CountScratch {
   DevBuf work2;    // uint64[E]   sorted work (discarded)
   DevBuf order;    // int32[E]    centre ranks, most work first
   DevBuf ranks;    // int32[E]    iota for the work sort
-  DevBuf orient;   // uint64[E]   sum_{b in N(a), b > a} d_b per a
+  DevBuf dlow;     // uint32[E]   neighbours with a smaller id, per a
   DevBuf acc;      // uint64[64]  histogram (30 bins) + statistics
   DevBuf tmp;      // hipcub temporary storage
+  // tiled path only
+  DevBuf vkeys;    // uint64[I]   (node << 32 | rank) of every incidence, then sorted
+  DevBuf vkeys2;   // uint64[I]
+  DevBuf vrank;    // int32[I]    V2H relabelled to ranks, ascending per node
+  DevBuf uoff;     // uint64[E+1] exclusive scan of the upper-row lengths (state offsets)
+  DevBuf cur;      // uint32[U]   read position in b's row, per upper-row entry (a, b)
+  DevBuf pm;       // mask[U]     a's nodes contained in b, per upper-row entry (a, b)
   void release() {
     for (DevBuf* b : {&deg, &ids, &perm, &rank, &degr, &roff, &rlen, &rows, &szr, &work, &work2,
-                      &order, &ranks, &orient, &acc, &tmp})
+                      &order, &ranks, &dlow, &acc, &tmp, &vkeys, &vkeys2, &vrank, &uoff, &cur, &pm})
       b->release();
   }
 };
@@ -112,4 +119,7 @@ struct hm_graph {
   hm_graph_stats stats{};
   bool have_stats = false;
   float ms_project = 0.f, ms_triangles = 0.f;
+  // id tile of the static count: 0 = automatic (one LDS table when E fits), > 0 = forced tiles
+  int32_t tile = 0;
+  int32_t last_tile = 0;  // tile used by the last count (0 = single table)
 };

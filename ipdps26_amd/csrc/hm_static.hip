@@ -49,37 +49,10 @@ enum AccSlot : int {
   kAccCount = 40
 };
 
-template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_proj_count(
-    int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
-    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
-    const int32_t* __restrict__ vedges, uint32_t* __restrict__ deg,
-    unsigned long long* __restrict__ acc) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  MT* M = reinterpret_cast<MT*>(smem);
-  __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
+// OpenAll(a): every unordered pair of neighbours of a, classified as an open triple centred at a
+// from the neighbour classes cls[2 (C_ab - 1) + [b ⊄ a]] (sa = |a|). Adds to acc bins 20..25.
+__device__ inline void open_all_pairs(const uint32_t* cls, int sa, unsigned long long* acc) {
   __shared__ unsigned long long red[6][kWavesPerBlock];
-  __shared__ uint32_t dred[kWavesPerBlock];
-  const int32_t a = blockIdx.x;
-  for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
-  build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, nullptr);
-
-  // neighbour classes: k = 2 (C_ab - 1) + [|b| > C_ab]
-  uint32_t d = 0;
-  for (int x = threadIdx.x; x < E; x += blockDim.x) {
-    const MT m = M[x];
-    if (m) {
-      const int C = popc(m);
-      atomicAdd(&cls[2 * (C - 1) + (esize[x] > C ? 1 : 0)], 1u);
-      ++d;
-    }
-  }
-  d = wave_sum(d);
-  if (lane_id() == 0) dred[wave_id()] = d;
-  __syncthreads();
-
-  // OpenAll(a): every unordered pair of neighbours, classified as an open triple centred at a.
-  const int sa = (int)(eoff[a + 1] - eoff[a]);
   const int K = 2 * sa;
   unsigned long long o0 = 0, o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
   for (int p = threadIdx.x; p < K * K; p += blockDim.x) {
@@ -106,14 +79,54 @@ __global__ __launch_bounds__(kBlock) void k_proj_count(
   }
   __syncthreads();
   if (threadIdx.x < 6) {
-    unsigned long long s = 0;
-    for (int w = 0; w < kWavesPerBlock; ++w) s += red[threadIdx.x][w];
-    if (s) atomicAdd(&acc[kAccHist + kOpenBin0 + threadIdx.x], s);
+    unsigned long long t = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) t += red[threadIdx.x][w];
+    if (t) atomicAdd(&acc[kAccHist + kOpenBin0 + threadIdx.x], t);
   }
+}
+
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_proj_count(
+    int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
+    const int32_t* __restrict__ vedges, uint32_t* __restrict__ deg, uint32_t* __restrict__ dlow,
+    unsigned long long* __restrict__ acc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
+  __shared__ uint32_t dred[2][kWavesPerBlock];
+  const int32_t a = blockIdx.x;
+  for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
+  build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, nullptr);
+
+  // neighbour classes: k = 2 (C_ab - 1) + [|b| > C_ab]
+  uint32_t d = 0, dl = 0;
+  for (int x = threadIdx.x; x < E; x += blockDim.x) {
+    const MT m = M[x];
+    if (m) {
+      const int C = popc(m);
+      atomicAdd(&cls[2 * (C - 1) + (esize[x] > C ? 1 : 0)], 1u);
+      ++d;
+      dl += x < a ? 1u : 0u;
+    }
+  }
+  d = wave_sum(d);
+  dl = wave_sum(dl);
+  if (lane_id() == 0) {
+    dred[0][wave_id()] = d;
+    dred[1][wave_id()] = dl;
+  }
+  __syncthreads();
+
+  open_all_pairs(cls, (int)(eoff[a + 1] - eoff[a]), acc);
   if (threadIdx.x == 0) {
-    uint32_t dd = 0;
-    for (int w = 0; w < kWavesPerBlock; ++w) dd += dred[w];
+    uint32_t dd = 0, ll = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+      dd += dred[0][w];
+      ll += dred[1][w];
+    }
     deg[a] = dd;
+    dlow[a] = ll;
   }
 }
 
@@ -121,14 +134,11 @@ template <typename MT>
 __global__ __launch_bounds__(kBlock) void k_proj_fill(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges,
-    const int32_t* __restrict__ rank, const int32_t* __restrict__ perm,
-    const uint32_t* __restrict__ deg, const unsigned long long* __restrict__ roff,
-    uint32_t* __restrict__ rlen, uint32_t* __restrict__ rows,
-    unsigned long long* __restrict__ orient) {
+    const int32_t* __restrict__ rank, const unsigned long long* __restrict__ roff,
+    uint32_t* __restrict__ rlen, uint32_t* __restrict__ rows) {
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
   __shared__ uint32_t wcnt[kWavesPerBlock];
-  __shared__ unsigned long long ored[kWavesPerBlock];
   const int32_t a = blockIdx.x;
   const int32_t ra = rank[a];
   build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, rank);
@@ -157,23 +167,6 @@ __global__ __launch_bounds__(kBlock) void k_proj_fill(
     __syncthreads();
   }
   if (threadIdx.x == 0) rlen[ra] = running;
-
-  // statistic for the roofline model: sum of d_b over neighbours b with id > a
-  unsigned long long s = 0;
-  for (int32_t r = threadIdx.x; r < E; r += blockDim.x) {
-    if (M[r]) {
-      const int32_t b = perm[r];
-      if (b > a) s += deg[b];
-    }
-  }
-  s = wave_sum(s);
-  if (lane == 0) ored[w] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int q = 0; q < kWavesPerBlock; ++q) t += ored[q];
-    orient[a] = t;
-  }
 }
 
 // Work estimate of centre r for the triangle kernel: sum of upper-row lengths of its upper row.
@@ -264,6 +257,297 @@ __global__ __launch_bounds__(NW * kWave) void k_tri(
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Tiled kernels: graphs whose hyperedge ids do not fit one LDS table. The key space (ids, or
+// degree ranks) is cut into tiles of T keys; a workgroup walks the tiles of its hyperedge a,
+// building the mask table of one tile at a time from the part of each node's V2H row that falls
+// in the tile (binary search, rows ascending) and visiting only the entries it set: an entry x is
+// visited by the lane holding the lowest node of a in x, which also clears it. The cost per
+// (a, tile) is the tile's incidences plus |a| binary searches, never the tile size.
+// ---------------------------------------------------------------------------------------------
+
+// Projected degree, lower degree and neighbour classes of a, over id tiles.
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_proj_count_tiled(
+    int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
+    const int32_t* __restrict__ vedges, uint32_t* __restrict__ deg, uint32_t* __restrict__ dlow,
+    unsigned long long* __restrict__ acc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
+  __shared__ uint32_t dred[2][kWavesPerBlock];
+  __shared__ TileRanges tr;
+  const int32_t a = blockIdx.x;
+  const uint32_t s = eoff[a];
+  const int sa = (int)(eoff[a + 1] - s);
+  for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
+  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+  uint32_t d = 0, dl = 0;
+  for (int32_t t0 = 0; t0 < E; t0 += T) {
+    const int32_t t1 = min(E, t0 + T);
+    const uint32_t total = tile_ranges(tr, sa, enodes + s, voff, vedges, t0, t1);
+    if (total == 0) continue;
+    for_tile_entries(tr, sa, total, vedges, [&](int i, int32_t x) {
+      if (x != a) atomicOr(&M[x - t0], (MT)1 << i);
+    });
+    __syncthreads();
+    for_tile_entries(tr, sa, total, vedges, [&](int i, int32_t x) {
+      if (x == a) return;
+      const MT m = M[x - t0];
+      if (m && low_bit(m) == i) {
+        const int C = popc(m);
+        atomicAdd(&cls[2 * (C - 1) + (esize[x] > C ? 1 : 0)], 1u);
+        ++d;
+        dl += x < a ? 1u : 0u;
+        M[x - t0] = (MT)0;
+      }
+    });
+  }
+  d = wave_sum(d);
+  dl = wave_sum(dl);
+  if (lane_id() == 0) {
+    dred[0][wave_id()] = d;
+    dred[1][wave_id()] = dl;
+  }
+  __syncthreads();
+  open_all_pairs(cls, sa, acc);
+  if (threadIdx.x == 0) {
+    uint32_t dd = 0, ll = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+      dd += dred[0][w];
+      ll += dred[1][w];
+    }
+    deg[a] = dd;
+    dlow[a] = ll;
+  }
+}
+
+// Upper row of a over rank tiles (vrank = V2H relabelled to ranks, ascending per node): entries
+// (rank << 7 | C) grouped by tile in ascending tile order, in any order within a tile.
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_proj_fill_tiled(
+    int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vrank,
+    const int32_t* __restrict__ rank, const unsigned long long* __restrict__ roff,
+    uint32_t* __restrict__ rlen, uint32_t* __restrict__ rows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ TileRanges tr;
+  __shared__ uint32_t s_cnt;
+  const int32_t a = blockIdx.x;
+  const int32_t ra = rank[a];
+  const uint32_t s = eoff[a];
+  const int sa = (int)(eoff[a + 1] - s);
+  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+  if (threadIdx.x == 0) s_cnt = 0;
+  uint32_t* out = rows + roff[ra];
+  for (int32_t t0 = ((ra + 1) / T) * T; t0 < E; t0 += T) {
+    const int32_t t1 = min(E, t0 + T);
+    const uint32_t total = tile_ranges(tr, sa, enodes + s, voff, vrank, max(t0, ra + 1), t1);
+    if (total == 0) continue;
+    for_tile_entries(tr, sa, total, vrank,
+                     [&](int i, int32_t x) { atomicOr(&M[x - t0], (MT)1 << i); });
+    __syncthreads();
+    for_tile_entries(tr, sa, total, vrank, [&](int i, int32_t x) {
+      const MT m = M[x - t0];
+      if (m && low_bit(m) == i) {
+        out[atomicAdd(&s_cnt, 1u)] = ((uint32_t)x << kCntBits) | (uint32_t)popc(m);
+        M[x - t0] = (MT)0;
+      }
+    });
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) rlen[ra] = s_cnt;
+}
+
+// Mask of a's nodes (s_an[0 .. sa), ascending) contained in hyperedge b.
+template <typename MT>
+__device__ inline MT pair_mask(const int32_t* s_an, int sa, const uint32_t* __restrict__ eoff,
+                               const int32_t* __restrict__ enodes, int32_t b) {
+  MT m = (MT)0;
+  for (uint32_t q = eoff[b], qe = eoff[b + 1]; q < qe; ++q) {
+    const int32_t v = enodes[q];
+    int lo = 0, hi = sa;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_an[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < sa && s_an[lo] == v) m |= (MT)1 << lo;
+  }
+  return m;
+}
+
+// Triangles of centre rank r over rank tiles of c. For each b in r's upper row, state at
+// uoff[r] + ib keeps a's mask in b (computed once) and the read position in b's row, which is
+// grouped by tile, so the entries of b's row in the current tile are the next ones from there.
+// Tiles without work are skipped: the next tile is the smallest one any cursor points into.
+template <typename MT, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_tri_tiled(
+    int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vrank,
+    const int32_t* __restrict__ perm, const uint8_t* __restrict__ szr,
+    const unsigned long long* __restrict__ roff, const uint32_t* __restrict__ rlen,
+    const uint32_t* __restrict__ rows, const unsigned long long* __restrict__ uoff,
+    uint32_t* __restrict__ cur, MT* __restrict__ pm, const int32_t* __restrict__ order,
+    unsigned long long* __restrict__ acc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  uint8_t* szt = reinterpret_cast<uint8_t*>(M + T);
+  constexpr int kCopies = 8;
+  __shared__ int32_t h[NW][32][kCopies];
+  __shared__ int8_t tbl[128];
+  __shared__ int32_t s_an[HM_MAX_EDGE_SIZE];
+  __shared__ TileRanges tr;
+  __shared__ uint32_t s_end;
+  __shared__ int32_t s_next;
+  const int32_t r = order[blockIdx.x];
+  const uint32_t nA = rlen[r];
+  if (nA < 2) return;
+  const int32_t a = perm[r];
+  const uint32_t s = eoff[a];
+  const int sa = (int)(eoff[a + 1] - s);
+  for (int k = threadIdx.x; k < NW * 32 * kCopies; k += blockDim.x) (&h[0][0][0])[k] = 0;
+  if (threadIdx.x < 128) tbl[threadIdx.x] = c_static_bin[threadIdx.x];
+  if ((int)threadIdx.x < sa) s_an[threadIdx.x] = enodes[s + threadIdx.x];
+  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+
+  const int lane = lane_id(), w = wave_id();
+  const int copy = lane & (kCopies - 1);
+  const uint32_t* rowA = rows + roff[r];
+  const unsigned long long st = uoff[r];
+  uint32_t prev_end = 0;
+  int32_t k = (int32_t)(rowA[0] >> kCntBits) / T;
+  for (;;) {
+    const int32_t t0 = k * T, t1 = min(E, t0 + T);
+    if (threadIdx.x == 0) {  // end of the b entries whose rank is below t1 (row grouped by tile)
+      uint32_t lo = prev_end, hi = nA;
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if ((int32_t)(rowA[mid] >> kCntBits) < t1) lo = mid + 1;
+        else hi = mid;
+      }
+      s_end = lo;
+    }
+    const uint32_t total = tile_ranges(tr, sa, enodes + s, voff, vrank, max(t0, r + 1), t1);
+    if (threadIdx.x == 0) s_next = INT32_MAX;
+    for_tile_entries(tr, sa, total, vrank, [&](int i, int32_t x) {
+      atomicOr(&M[x - t0], (MT)1 << i);
+      szt[x - t0] = szr[x];
+    });
+    __syncthreads();
+    const uint32_t end = s_end;
+    int32_t* hc = &h[w][0][copy];
+    for (uint32_t base = (uint32_t)w * kWave; base < end; base += NW * kWave) {
+      const uint32_t ib = base + lane;
+      const bool have = ib < end;
+      int32_t rb = 0;
+      int cab = 0;
+      uint32_t cb = 0, nB = 0;
+      unsigned long long ob = 0;
+      MT mab = (MT)0;
+      bool work = false;
+      if (have) {
+        const uint32_t eb = rowA[ib];
+        rb = (int32_t)(eb >> kCntBits);
+        cab = (int)(eb & kCntMask);
+        ob = roff[rb];
+        nB = rlen[rb];
+        if (ib >= prev_end) {
+          mab = pair_mask<MT>(s_an, sa, eoff, enodes, perm[rb]);
+          pm[st + ib] = mab;
+        } else {
+          mab = pm[st + ib];
+          cb = cur[st + ib];
+        }
+        work = cb < nB && (int32_t)(rows[ob + cb] >> kCntBits) < t1;
+      }
+      unsigned long long wl = __ballot(work);
+      while (wl) {
+        const int leader = __ffsll((long long)wl) - 1;
+        wl &= wl - 1;
+        const int32_t rbL = __shfl(rb, leader);
+        const int cabL = __shfl(cab, leader);
+        const uint32_t nBL = __shfl(nB, leader);
+        const unsigned long long obL = __shfl(ob, leader);
+        const MT mabL = __shfl(mab, leader);
+        uint32_t pos = __shfl(cb, leader);
+        const int sbL = szr[rbL];
+        const uint32_t* rowB = rows + obL;
+        for (;;) {
+          const uint32_t j = pos + lane;
+          bool v = false;
+          uint32_t e = 0;
+          if (j < nBL) {
+            e = rowB[j];
+            v = (int32_t)(e >> kCntBits) < t1;
+          }
+          if (v) {
+            const int32_t rc = (int32_t)(e >> kCntBits);
+            const MT mc = M[rc - t0];
+            if (mc) {
+              const int cbc = (int)(e & kCntMask);
+              const int cac = popc(mc);
+              const int g = popc((MT)(mabL & mc));
+              const int sc = szt[rc - t0];
+              atomicAdd(&hc[tbl[venn_id(sa, sbL, sc, cabL, cbc, cac, g)] * kCopies], 1);
+              atomicSub(&hc[open_bin(sa, sbL, sc, cabL, cac) * kCopies], 1);
+              atomicSub(&hc[open_bin(sbL, sa, sc, cabL, cbc) * kCopies], 1);
+              atomicSub(&hc[open_bin(sc, sa, sbL, cac, cbc) * kCopies], 1);
+            }
+          }
+          const uint32_t nv = (uint32_t)__popcll(__ballot(v));
+          pos += nv;
+          if (nv < (uint32_t)kWave) break;
+        }
+        if (lane == leader) cb = pos;
+      }
+      if (have) {
+        cur[st + ib] = cb;
+        if (cb < nB) atomicMin(&s_next, (int32_t)(rows[ob + cb] >> kCntBits) / T);
+      }
+    }
+    if (threadIdx.x == 0 && end < nA) atomicMin(&s_next, (int32_t)(rowA[end] >> kCntBits) / T);
+    __syncthreads();
+    if (threadIdx.x < kStaticBins) {  // flush this tile's histogram (bounds the int32 counters)
+      long long sum = 0;
+      for (int q = 0; q < NW; ++q)
+        for (int c = 0; c < kCopies; ++c) {
+          sum += h[q][threadIdx.x][c];
+          h[q][threadIdx.x][c] = 0;
+        }
+      if (sum) atomicAdd(&acc[kAccHist + threadIdx.x], (unsigned long long)sum);
+    }
+    for_tile_entries(tr, sa, total, vrank, [&](int, int32_t x) { M[x - t0] = (MT)0; });
+    const int32_t kn = s_next;
+    prev_end = end;
+    if (kn == INT32_MAX) break;
+    k = kn;
+  }
+}
+
+// Node id of every V2H position: node[k] = v for voff[v] <= k < voff[v + 1].
+__global__ void k_inc_node_keys(int32_t V, int64_t I, const uint32_t* __restrict__ voff,
+                                const int32_t* __restrict__ vedges, const int32_t* __restrict__ rank,
+                                unsigned long long* __restrict__ keys) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= I) return;
+  int32_t lo = 0, hi = V;  // largest v with voff[v] <= k
+  while (hi - lo > 1) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if ((int64_t)voff[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  keys[k] = ((unsigned long long)lo << 32) | (unsigned long long)(uint32_t)rank[vedges[k]];
+}
+
+__global__ void k_low_words(const unsigned long long* __restrict__ keys, int32_t* __restrict__ out,
+                            int64_t I) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < I) out[k] = (int32_t)(uint32_t)keys[k];
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
@@ -294,7 +578,7 @@ __global__ void k_rank_from_perm(const int32_t* __restrict__ perm, const uint8_t
 // Graph statistics: P, max degree, wedges, upper entries, oriented row sum.
 __global__ __launch_bounds__(kBlock) void k_stats(int32_t E, int32_t V, const uint32_t* __restrict__ voff,
                                                   const uint32_t* __restrict__ deg,
-                                                  const unsigned long long* __restrict__ orient,
+                                                  const uint32_t* __restrict__ dlow,
                                                   const uint32_t* __restrict__ rlen,
                                                   unsigned long long* __restrict__ acc) {
   unsigned long long p = 0, mx = 0, wd = 0, up = 0, orr = 0;
@@ -304,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(int32_t E, int32_t V, const ui
     mx = d > mx ? d : mx;
     wd += d * (d - (d > 0)) / 2;
     up += rlen[a];
-    orr += orient[a];
+    orr += d * dlow[a];  // sum_a sum_{b in N(a), b > a} d_b = sum_b d_b #{a in N(b), a < b}
   }
   unsigned long long mv = 0;
   for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
@@ -344,19 +628,39 @@ static hm_status set_lds_attr(const void* fn, size_t bytes) {
   return allow_lds(fn, bytes);
 }
 
+// LDS reserved for the static arrays of the tiled kernels (k_tri_tiled: 8 KB histogram + tables).
+constexpr size_t kTiledStaticLds = 12 * 1024;
+
+// Largest tile (multiple of 64 ids) whose table and size bytes fit the LDS limit.
 template <typename MT>
-static hm_status count_static_dense(hm_graph* g, unsigned long long* h_acc) {
+static int32_t auto_tile(int lim) {
+  const size_t avail = (size_t)lim > kTiledStaticLds ? (size_t)lim - kTiledStaticLds : 0;
+  return (int32_t)((avail / (sizeof(MT) + 1)) & ~(size_t)63);
+}
+
+// Static count. T == 0: one LDS table over all ids (dense kernels); T > 0: tiles of T ids/ranks.
+template <typename MT>
+static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32_t T) {
   DeviceCsr& c = g->csr;
   CountScratch& s = g->scratch;
   const int32_t E = c.E;
+  const int64_t I = c.I;
   hipStream_t st = g->stream;
-  const size_t lds_cnt = dense_lds_bytes<MT>(E, false);
-  const size_t lds_tri = dense_lds_bytes<MT>(E, true);
-  HM_TRY(set_lds_attr<MT>((const void*)k_proj_count<MT>, lds_cnt));
-  HM_TRY(set_lds_attr<MT>((const void*)k_proj_fill<MT>, lds_cnt));
-  HM_TRY(set_lds_attr<MT>((const void*)k_tri<MT, kTriWaves>, lds_tri));
+  const bool tiled = T > 0;
+  const size_t lds_cnt = tiled ? dense_lds_bytes<MT>(T, false) : dense_lds_bytes<MT>(E, false);
+  const size_t lds_tri = tiled ? dense_lds_bytes<MT>(T, true) : dense_lds_bytes<MT>(E, true);
+  if (tiled) {
+    HM_TRY(set_lds_attr<MT>((const void*)k_proj_count_tiled<MT>, lds_cnt));
+    HM_TRY(set_lds_attr<MT>((const void*)k_proj_fill_tiled<MT>, lds_cnt));
+    HM_TRY(set_lds_attr<MT>((const void*)k_tri_tiled<MT, kTriWaves>, lds_tri));
+  } else {
+    HM_TRY(set_lds_attr<MT>((const void*)k_proj_count<MT>, lds_cnt));
+    HM_TRY(set_lds_attr<MT>((const void*)k_proj_fill<MT>, lds_cnt));
+    HM_TRY(set_lds_attr<MT>((const void*)k_tri<MT, kTriWaves>, lds_tri));
+  }
 
   HM_TRY(s.deg.ensure(sizeof(uint32_t) * E));
+  HM_TRY(s.dlow.ensure(sizeof(uint32_t) * E));
   HM_TRY(s.ids.ensure(sizeof(int32_t) * E));
   HM_TRY(s.perm.ensure(sizeof(int32_t) * E));
   HM_TRY(s.rank.ensure(sizeof(int32_t) * E));
@@ -368,8 +672,15 @@ static hm_status count_static_dense(hm_graph* g, unsigned long long* h_acc) {
   HM_TRY(s.work2.ensure(sizeof(unsigned long long) * E));
   HM_TRY(s.order.ensure(sizeof(int32_t) * E));
   HM_TRY(s.ranks.ensure(sizeof(int32_t) * E));
-  HM_TRY(s.orient.ensure(sizeof(unsigned long long) * E));
   HM_TRY(s.acc.ensure(sizeof(unsigned long long) * kAccCount));
+  if (tiled && I >= ((int64_t)1 << 31))
+    return fail(HM_ERR_OVER_BUDGET, "static count: tiled path supports fewer than 2^31 incidences");
+  if (tiled) {
+    HM_TRY(s.vkeys.ensure(sizeof(unsigned long long) * std::max<int64_t>(I, 1)));
+    HM_TRY(s.vkeys2.ensure(sizeof(unsigned long long) * std::max<int64_t>(I, 1)));
+    HM_TRY(s.vrank.ensure(sizeof(int32_t) * std::max<int64_t>(I, 1)));
+    HM_TRY(s.uoff.ensure(sizeof(unsigned long long) * (E + 1)));
+  }
 
   unsigned long long* acc = s.acc.as<unsigned long long>();
   const unsigned eb = (unsigned)((E + kBlock - 1) / kBlock);
@@ -377,13 +688,20 @@ static hm_status count_static_dense(hm_graph* g, unsigned long long* h_acc) {
   HM_HIP(hipEventRecord(g->ev[0], st));
   k_zero_u64<<<1, 64, 0, st>>>(acc, kAccCount);
   // 1. projected degrees + combinatorial open counts
-  k_proj_count<MT><<<E, kBlock, lds_cnt, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
-                                                c.esize.as<uint8_t>(), c.voff.as<uint32_t>(),
-                                                c.vedges.as<int32_t>(), s.deg.as<uint32_t>(), acc);
+  if (tiled)
+    k_proj_count_tiled<MT><<<E, kBlock, lds_cnt, st>>>(
+        E, T, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
+        c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), s.deg.as<uint32_t>(), s.dlow.as<uint32_t>(), acc);
+  else
+    k_proj_count<MT><<<E, kBlock, lds_cnt, st>>>(
+        E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(), c.voff.as<uint32_t>(),
+        c.vedges.as<int32_t>(), s.deg.as<uint32_t>(), s.dlow.as<uint32_t>(), acc);
   HM_HIP(hipGetLastError());
   // 2. degree order: stable radix sort of (d_a, a)
   k_iota<<<eb, kBlock, 0, st>>>(s.ids.as<int32_t>(), E);
-  size_t tb1 = 0, tb2 = 0, tb3 = 0;
+  int vbits = 1;
+  while (vbits < 31 && ((int64_t)1 << vbits) <= c.V) ++vbits;
+  size_t tb1 = 0, tb2 = 0, tb3 = 0, tb4 = 0;
   HM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, s.deg.as<uint32_t>(), s.work2.as<uint32_t>(),
                                             s.ids.as<int32_t>(), s.perm.as<int32_t>(), E, 0, 32, st));
   HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, s.degr.as<unsigned long long>(),
@@ -392,7 +710,10 @@ static hm_status count_static_dense(hm_graph* g, unsigned long long* h_acc) {
                                                       s.work2.as<unsigned long long>(),
                                                       s.ranks.as<int32_t>(), s.order.as<int32_t>(),
                                                       E, 0, 64, st));
-  HM_TRY(s.tmp.ensure(std::max(tb1, std::max(tb2, tb3))));
+  if (tiled)
+    HM_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb4, s.vkeys.as<unsigned long long>(),
+                                             s.vkeys2.as<unsigned long long>(), (int)I, 0, 32 + vbits, st));
+  HM_TRY(s.tmp.ensure(std::max(std::max(tb1, tb2), std::max(tb3, tb4))));
   size_t tb = s.tmp.bytes;
   // sorted keys land in work2 (as uint32), values in perm
   HM_HIP(hipcub::DeviceRadixSort::SortPairs(s.tmp.p, tb, s.deg.as<uint32_t>(), s.work2.as<uint32_t>(),
@@ -405,17 +726,31 @@ static hm_status count_static_dense(hm_graph* g, unsigned long long* h_acc) {
   tb = s.tmp.bytes;
   HM_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp.p, tb, s.degr.as<unsigned long long>(),
                                           s.roff.as<unsigned long long>(), E + 1, st));
+  if (tiled && I > 0) {  // V2H relabelled to ranks, ascending within every node's row
+    const unsigned ib = (unsigned)((I + kBlock - 1) / kBlock);
+    k_inc_node_keys<<<ib, kBlock, 0, st>>>(c.V, I, c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+                                           s.rank.as<int32_t>(), s.vkeys.as<unsigned long long>());
+    tb = s.tmp.bytes;
+    HM_HIP(hipcub::DeviceRadixSort::SortKeys(s.tmp.p, tb, s.vkeys.as<unsigned long long>(),
+                                             s.vkeys2.as<unsigned long long>(), (int)I, 0, 32 + vbits, st));
+    k_low_words<<<ib, kBlock, 0, st>>>(s.vkeys2.as<unsigned long long>(), s.vrank.as<int32_t>(), I);
+  }
   unsigned long long total_cap = 0;
   HM_HIP(hipMemcpyAsync(&total_cap, s.roff.as<unsigned long long>() + E, sizeof(total_cap),
                         hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
   HM_TRY(s.rows.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(total_cap, 1)));
   // 4. upper rows
-  k_proj_fill<MT><<<E, kBlock, lds_cnt, st>>>(
-      E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(),
-      c.vedges.as<int32_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), s.deg.as<uint32_t>(),
-      s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
-      s.orient.as<unsigned long long>());
+  if (tiled)
+    k_proj_fill_tiled<MT><<<E, kBlock, lds_cnt, st>>>(
+        E, T, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(),
+        s.vrank.as<int32_t>(), s.rank.as<int32_t>(), s.roff.as<unsigned long long>(),
+        s.rlen.as<uint32_t>(), s.rows.as<uint32_t>());
+  else
+    k_proj_fill<MT><<<E, kBlock, lds_cnt, st>>>(
+        E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(),
+        c.vedges.as<int32_t>(), s.rank.as<int32_t>(), s.roff.as<unsigned long long>(),
+        s.rlen.as<uint32_t>(), s.rows.as<uint32_t>());
   HM_HIP(hipGetLastError());
   // 5. centre order by work, most first
   k_work<<<(unsigned)((E + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, st>>>(
@@ -427,23 +762,41 @@ static hm_status count_static_dense(hm_graph* g, unsigned long long* h_acc) {
                                                       s.work2.as<unsigned long long>(),
                                                       s.ranks.as<int32_t>(), s.order.as<int32_t>(),
                                                       E, 0, 64, st));
+  if (tiled) {  // per (a, b) state of the tiled triangle kernel, at the scan of the upper lengths
+    k_widen<<<(unsigned)((E + 1 + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+        s.rlen.as<uint32_t>(), s.degr.as<unsigned long long>(), E);
+    tb = s.tmp.bytes;
+    HM_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp.p, tb, s.degr.as<unsigned long long>(),
+                                            s.uoff.as<unsigned long long>(), E + 1, st));
+    unsigned long long U = 0;
+    HM_HIP(hipMemcpyAsync(&U, s.uoff.as<unsigned long long>() + E, sizeof(U), hipMemcpyDeviceToHost, st));
+    HM_HIP(hipStreamSynchronize(st));
+    HM_TRY(s.cur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(U, 1)));
+    HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
+  }
   HM_HIP(hipEventRecord(g->ev[1], st));
   // 6. triangles
-#define HM_TRI_ARGS                                                                           \
-  E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(),                        \
-      c.vedges.as<int32_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), s.szr.as<uint8_t>(),   \
-      s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),             \
-      s.order.as<int32_t>(), acc
-  k_tri<MT, kTriWaves><<<E, kTriWaves * kWave, lds_tri, st>>>(HM_TRI_ARGS);
-#undef HM_TRI_ARGS
+  if (tiled)
+    k_tri_tiled<MT, kTriWaves><<<E, kTriWaves * kWave, lds_tri, st>>>(
+        E, T, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(),
+        s.vrank.as<int32_t>(), s.perm.as<int32_t>(), s.szr.as<uint8_t>(),
+        s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
+        s.uoff.as<unsigned long long>(), s.cur.as<uint32_t>(), s.pm.as<MT>(), s.order.as<int32_t>(), acc);
+  else
+    k_tri<MT, kTriWaves><<<E, kTriWaves * kWave, lds_tri, st>>>(
+        E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(),
+        c.vedges.as<int32_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), s.szr.as<uint8_t>(),
+        s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
+        s.order.as<int32_t>(), acc);
   HM_HIP(hipGetLastError());
   HM_HIP(hipEventRecord(g->ev[2], st));
-  k_stats<<<64, kBlock, 0, st>>>(E, c.V, c.voff.as<uint32_t>(), s.deg.as<uint32_t>(), s.orient.as<unsigned long long>(),
-                                 s.rlen.as<uint32_t>(), acc);
+  k_stats<<<64, kBlock, 0, st>>>(E, c.V, c.voff.as<uint32_t>(), s.deg.as<uint32_t>(),
+                                 s.dlow.as<uint32_t>(), s.rlen.as<uint32_t>(), acc);
   HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
   HM_HIP(hipEventElapsedTime(&g->ms_project, g->ev[0], g->ev[1]));
   HM_HIP(hipEventElapsedTime(&g->ms_triangles, g->ev[1], g->ev[2]));
+  g->last_tile = T;
   return HM_OK;
 }
 
@@ -529,14 +882,22 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
     if ((int64_t)E >= kMaxPackedId) return fail(HM_ERR_UNSUPPORTED, "more than 2^25 hyperedges");
     HM_HIP(hipSetDevice(g->device));
     const int lim = lds_limit_bytes(g->device);
-    if (g->csr.max_size <= 32 && dense_lds_bytes<uint32_t>(E, true) <= (size_t)lim) {
-      HM_TRY(count_static_dense<uint32_t>(g, acc));
+    const bool wide = g->csr.max_size > 32;
+    if (g->tile > 0) {
+      const size_t need = (size_t)g->tile * ((wide ? 8 : 4) + 1) + kTiledStaticLds;
+      if (need > (size_t)lim) {
+        set_error("static count: a tile of %d ids exceeds the LDS limit (%d bytes)", g->tile, lim);
+        return HM_ERR_BAD_INPUT;
+      }
+      HM_TRY(wide ? count_static_impl<mask64_t>(g, acc, g->tile)
+                  : count_static_impl<uint32_t>(g, acc, g->tile));
+    } else if (!wide && dense_lds_bytes<uint32_t>(E, true) <= (size_t)lim) {
+      HM_TRY(count_static_impl<uint32_t>(g, acc, 0));
     } else if (dense_lds_bytes<mask64_t>(E, true) <= (size_t)lim) {
-      HM_TRY(count_static_dense<mask64_t>(g, acc));
+      HM_TRY(count_static_impl<mask64_t>(g, acc, 0));
     } else {
-      set_error("static count: %d hyperedges exceed the dense LDS table (%d bytes per workgroup)",
-                E, lim);
-      return HM_ERR_UNSUPPORTED;
+      HM_TRY(wide ? count_static_impl<mask64_t>(g, acc, auto_tile<mask64_t>(lim))
+                  : count_static_impl<uint32_t>(g, acc, auto_tile<uint32_t>(lim)));
     }
   }
   for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = acc[kAccHist + k];
@@ -610,6 +971,14 @@ hm_status hm_count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
   hm::clear_error();
   if (!g || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_count_static: null argument");
   return hm::count_static(g, out);
+}
+
+hm_status hm_graph_set_tile(hm_graph* g, int32_t tile_ids) {
+  hm::clear_error();
+  if (!g) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_tile: null handle");
+  if (tile_ids < 0) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_tile: negative tile");
+  g->tile = tile_ids;
+  return HM_OK;
 }
 
 hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out) {
