@@ -1,8 +1,8 @@
 // Serving: a request loop over many device-resident hypergraphs.
 //
-// The reference has no serving layer (single host thread, default stream, a
-// synchronize after every launch). This one keeps many graphs resident in HBM and runs their
-// requests concurrently:
+// The reference has no serving layer: one host thread on the default stream with a synchronize
+// after every launch (dynamicHyperGraph/structure/operations.cu:24-29, src/main.cu:238-250). This
+// one keeps many graphs resident in HBM and runs their requests concurrently:
 //   * admission — a graph is admitted only if its estimated device footprint (CSR, projection
 //     rows bounded by sum_v deg_v (deg_v - 1), per-hyperedge scratch) fits the remaining HBM
 //     budget; otherwise hm_server_add_graph returns HM_ERR_OVER_BUDGET;
