@@ -349,7 +349,8 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
 static bool dense_fits(hm_graph* g, int32_t E, bool wide) {
   int lim = 65536;
   (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
-  return (size_t)E * ((wide ? 8 : 4) + 2) + 64 <= (size_t)lim;
+  // k_delta: mask, size and flag per id, plus about 2 KB of static arrays (4 KB reserved)
+  return (size_t)E * ((wide ? 8 : 4) + 2) + 16 + 4096 <= (size_t)lim;
 }
 
 static hm_status local_count_any(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,

@@ -5,7 +5,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/hmotif.h"
+#include "hmotif.h"
 
 namespace hm {
 

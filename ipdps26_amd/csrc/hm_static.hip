@@ -184,7 +184,9 @@ __global__ __launch_bounds__(kBlock) void k_work(int32_t E, const unsigned long 
   if (lane_id() == 0) work[r] = s + n;
 }
 
-// 8 waves per centre workgroup (c1: 4 waves 9.03 ms, 8 waves 7.85 ms, 16 waves 8.09 ms).
+// 8 waves per centre workgroup (c1: 4 waves 9.03 ms, 8 waves 7.85 ms, 16 waves 8.09 ms), each wave
+// with 16 interleaved int32 histogram copies (copy = lane & 15) for its LDS atomics (c1: 2 copies
+// 26.6 ms, 4: 14.2 ms, 8: 7.75 ms, 16: 6.91 ms, 32: 8.06 ms, 64: 13.7 ms).
 constexpr int kTriWaves = 8;
 
 template <typename MT, int NW>
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tri(
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
   uint8_t* sz = reinterpret_cast<uint8_t*>(M + E);
-  constexpr int kCopies = 8;
+  constexpr int kCopies = 16;
   __shared__ int32_t h[NW][32][kCopies];
   __shared__ int8_t tbl[128];
   const int32_t r = order[blockIdx.x];
@@ -395,7 +397,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tri_tiled(
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
   uint8_t* szt = reinterpret_cast<uint8_t*>(M + T);
-  constexpr int kCopies = 8;
+  constexpr int kCopies = 16;
   __shared__ int32_t h[NW][32][kCopies];
   __shared__ int8_t tbl[128];
   __shared__ int32_t s_an[HM_MAX_EDGE_SIZE];
@@ -628,8 +630,9 @@ static hm_status set_lds_attr(const void* fn, size_t bytes) {
   return allow_lds(fn, bytes);
 }
 
-// LDS reserved for the static arrays of the tiled kernels (k_tri_tiled: 8 KB histogram + tables).
-constexpr size_t kTiledStaticLds = 12 * 1024;
+// LDS reserved for the static arrays of the triangle kernels (16 KB histogram + tables), next to
+// the dynamic mask table.
+constexpr size_t kTiledStaticLds = 20 * 1024;
 
 // Largest tile (multiple of 64 ids) whose table and size bytes fit the LDS limit.
 template <typename MT>
@@ -891,9 +894,9 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
       }
       HM_TRY(wide ? count_static_impl<mask64_t>(g, acc, g->tile)
                   : count_static_impl<uint32_t>(g, acc, g->tile));
-    } else if (!wide && dense_lds_bytes<uint32_t>(E, true) <= (size_t)lim) {
+    } else if (!wide && dense_lds_bytes<uint32_t>(E, true) + kTiledStaticLds <= (size_t)lim) {
       HM_TRY(count_static_impl<uint32_t>(g, acc, 0));
-    } else if (dense_lds_bytes<mask64_t>(E, true) <= (size_t)lim) {
+    } else if (dense_lds_bytes<mask64_t>(E, true) + kTiledStaticLds <= (size_t)lim) {
       HM_TRY(count_static_impl<mask64_t>(g, acc, 0));
     } else {
       HM_TRY(wide ? count_static_impl<mask64_t>(g, acc, auto_tile<mask64_t>(lim))

@@ -88,6 +88,16 @@ def test_beyond_one_table_wide_masks(gpu_ready):
     assert closed == st["triangles"] and st["proj_pairs"] == int(deg.sum())
 
 
+@pytest.mark.parametrize("E", [28_672, 28_673])
+def test_single_table_boundary(gpu_ready, E):
+    # 28,672 ids (u32 masks) is the largest single LDS table next to the 20 KB of histograms; one
+    # more id switches to tiles
+    p = hm.GenParams(num_nodes=300_000, num_edges=E, alpha=2.8, kmin=2, kmax=12, zipf_s=0.7, seed=13)
+    off, nodes = hm.generate_static(p)
+    c, _ = count(off, nodes, 0)
+    assert np.array_equal(c, O.static_mochy(off, nodes))
+
+
 def test_tile_argument_checks(gpu_ready):
     off, nodes = random_hypergraph(np.random.default_rng(1), 20, 30, 5)
     with hm.Graph(off, nodes) as g:
