@@ -9,6 +9,11 @@ Workloads (inputs are built and copied to HBM before the timed region):
                          hm_count_temporal call (window bounds, temporal projection, counting).
   static_c1              MoCHy's 30 static bins on 5,000 hyperedges over 2,000 nodes (sizes
                          ~k^-2.5 on [2,25], Zipf s=1.2, seed 1). A step is one hm_count_static call.
+  dynamic_c3             the dynamic workload's generator and batch recipe at a reduced base size
+                         (20,000 hyperedges over 8,000 nodes: c3's V/E and batch/E ratios). A step
+                         is one batch: hm_graph_update with its exact delta (store update + delta).
+                         value = triples touched by the batches (removed + added) per second.
+  serving_c5             plain versus served request streams over 64 reduced graphs (p50/p99).
 The full static 1M-node / 2M-hyperedge graph is out of reach of exact per-triple counting (its
 largest node sits in ~780k hyperedges, i.e. >7e16 triples); the temporal workload is the
 largest configuration the engine counts at full size on one GPU.
@@ -49,13 +54,18 @@ DESC = {
                    "[2,20], Zipf s=1.2, seed 4, int64 us timestamps, delta=3600 s",
     "static_c1": "static_c1: 5,000 hyperedges over 2,000 nodes, sizes ~k^-2.5 on [2,25], "
                  "Zipf s=1.2, seed 1",
+    "dynamic_c3": "dynamic_c3 (reduced): base graph of the dynamic generator family (sizes ~k^-2.2 "
+                  "on [2,30], Zipf s=1.1, seed 3) with 20,000 hyperedges over 8,000 nodes; batch b: "
+                  "280 fresh generator hyperedges (seed 3000+b) and 120 deletes of uniformly chosen "
+                  "live hyperedges (c3's 7,000 / 3,000 per 500,000 scaled to E); steps = batches",
     "serving_c5": "serving_c5 (reduced): 64 graphs of the static_c2 family, E log-uniform on "
                   "[1k, 6k], V = E/2, seed 5; 400 requests, half count queries, half update "
                   "batches of 0.5-5% of E (70/30 insert/delete), seed 5000",
 }
 # Bounded CPU samples: a prefix of the same input sized for ~10-30 s of reference-program time.
-CPU_SAMPLE_EDGES = {"temporal_c4": 600, "static_c1": 3500}
-DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20, "serving_c5": 1}
+CPU_SAMPLE_EDGES = {"temporal_c4": 600, "static_c1": 3500, "dynamic_c3": 4000}
+DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20, "dynamic_c3": 10, "serving_c5": 1}
+DYN_BASE_EDGES, DYN_BASE_NODES = 20_000, 8_000
 
 
 def cpu_threads() -> int:
@@ -90,6 +100,40 @@ def cpu_baseline_static(off, nodes) -> dict:
     return {"value": triples / secs, "unit": "triples/s", "cores": threads, "kind": kind,
             "sample": f"first {k} of {off.size - 1} hyperedges of static_c1 ({triples} triples); "
                       f"{prog} TOTAL_RUNTIME_SECONDS={secs:.2f} (wall {wall:.2f} s)"}
+
+
+def cpu_baseline_recount(off, nodes) -> dict:
+    """The reference's per-batch CPU path for dynamic graphs: a full MoCHy recount (timed on a
+    bounded prefix of the updated graph, as triples recounted per second)."""
+    k = CPU_SAMPLE_EDGES["dynamic_c3"]
+    sizes = np.diff(off)
+    keep = np.nonzero(sizes > 0)[0][:k]  # deleted rows are empty; MoCHy's reader takes none
+    soff = np.concatenate([[0], np.cumsum(sizes[keep])]).astype(np.int64)
+    snodes = np.concatenate([nodes[off[e]:off[e + 1]] for e in keep]).astype(np.int32)
+    with hm.Graph(soff, snodes) as g:
+        triples = int(g.count_static().astype(np.int64).sum())
+    threads = cpu_threads()
+    ref = os.path.join(ROOT, "oracle", "_ref", "mochy_exact_par")
+    with tempfile.TemporaryDirectory() as td:
+        csv = os.path.join(td, "sample.csv")
+        W.write_mochy_csv(csv, soff, snodes)
+        if os.path.exists(ref):
+            out = subprocess.run([ref, str(threads), csv], capture_output=True, text=True,
+                                 check=True, timeout=900).stdout
+            secs = float(re.search(r"TOTAL_RUNTIME_SECONDS ([0-9.eE+-]+)", out).group(1))
+            kind, prog = "reference", "MoCHy main_exact_par"
+        else:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_lib
+
+            t0 = time.perf_counter()
+            oracle_lib.static_mochy(soff, snodes, threads)
+            secs = time.perf_counter() - t0
+            kind, prog = "port", "oracle restatement of MoCHy-E"
+    return {"value": triples / secs, "unit": "triples/s", "cores": threads, "kind": kind,
+            "sample": f"full recount (the reference's per-batch path) of the first {k} live of "
+                      f"{int((np.diff(off) > 0).sum())} hyperedges of the updated dynamic_c3 graph ({triples} "
+                      f"triples); {prog} TOTAL_RUNTIME_SECONDS={secs:.2f}"}
 
 
 def cpu_baseline_temporal(off, nodes, times, delta) -> dict:
@@ -178,6 +222,56 @@ def run_static(steps: int, warmup: int, with_cpu: bool) -> dict:
             "cpu": cpu_baseline_static(off, nodes) if with_cpu else None}
 
 
+def run_dynamic(steps: int, warmup: int, with_cpu: bool) -> dict:
+    p = W.scaled(W.DYNAMIC_C3, DYN_BASE_EDGES, DYN_BASE_NODES)
+    scale = DYN_BASE_EDGES / W.DYNAMIC_C3.num_edges
+    n_ins, n_del = round(W.DYNAMIC_C3_INSERTS * scale), round(W.DYNAMIC_C3_DELETES * scale)
+    off, nodes = hm.generate_static(p)
+    g = hm.Graph(off, nodes)
+    # batches are built up front (the recipe depends on the live set only); the host keeps the
+    # rows too, for the CPU recount of the final graph
+    live = np.arange(DYN_BASE_EDGES, dtype=np.int64)
+    rows = [nodes[off[e]:off[e + 1]] for e in range(DYN_BASE_EDGES)]
+    batches = []
+    for b in range(1, warmup + steps + 1):
+        dels, ioff, inodes = W.dynamic_batch(live, b, p, n_ins, n_del)
+        batches.append((dels, ioff, inodes))
+        nxt = len(rows)
+        for x in dels:
+            rows[int(x)] = nodes[:0]
+        rows.extend(inodes[ioff[k]:ioff[k + 1]] for k in range(ioff.size - 1))
+        live = np.concatenate([np.setdiff1d(live, dels), np.arange(nxt, len(rows))])
+    for dels, ioff, inodes in batches[:warmup]:
+        g.update(dels, ioff, inodes)
+    touched, rows_ms, delta_ms, store_ms, entries = 0, 0.0, 0.0, 0.0, 0
+    t0 = time.perf_counter()
+    for dels, ioff, inodes in batches[warmup:]:
+        g.update(dels, ioff, inodes)
+        us = g.update_stats()
+        touched += us["removed"] + us["added"]
+        rows_ms += us["ms_rows"]
+        delta_ms += us["ms_delta"]
+        store_ms += us["ms_store"]
+        entries += us["row_entries"]
+    t1 = time.perf_counter()
+    t2 = time.perf_counter()
+    total = int(g.count_static().astype(np.int64).sum())  # the recount a batch would need instead
+    recount_ms = (time.perf_counter() - t2) * 1e3
+    g.close()
+    cur_off = np.concatenate([[0], np.cumsum([r.size for r in rows])]).astype(np.int64)
+    cur_nodes = np.concatenate(rows).astype(np.int32)
+    # delta kernel bytes: 4 B per scanned packed row entry + 8 B per histogram bin, per launch
+    kbytes = (4 * entries + 8 * hm.STATIC_BINS * 2 * steps) / (2 * steps)
+    return {"ms": (t1 - t0) * 1e3 / steps, "total": touched, "unit": "triples/s",
+            "kernel": "k_delta", "kernel_ms": delta_ms / (2 * steps), "kernel_bytes": kbytes,
+            "extra": {"base_hyperedges": DYN_BASE_EDGES, "inserts_per_batch": n_ins,
+                      "deletes_per_batch": n_del, "touched_triples": touched,
+                      "ms_per_batch": {"rows": rows_ms / steps, "delta": delta_ms / steps,
+                                       "store": store_ms / steps},
+                      "full_recount_ms": recount_ms, "triples_after": total},
+            "cpu": cpu_baseline_recount(cur_off, cur_nodes) if with_cpu else None}
+
+
 def copy_probe(mib: int) -> None:
     """One device-to-device copy of `mib` MiB through the HIP runtime: the reference kernel whose
     rocprofv3 FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters."""
@@ -262,7 +356,7 @@ def main() -> int:
                 "roofline": None, "cpu_baseline": None}
         print(json.dumps(line), flush=True)
         return 0
-    run = run_temporal if args.workload == "temporal_c4" else run_static
+    run = {"temporal_c4": run_temporal, "static_c1": run_static, "dynamic_c3": run_dynamic}[args.workload]
     r = run(steps, args.warmup, not args.no_cpu)
     achieved = r["kernel_bytes"] / (r["kernel_ms"] * 1e-3) / 1e9
     line = {

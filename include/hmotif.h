@@ -144,6 +144,19 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
                           const int64_t* ins_offsets, const int32_t* ins_nodes, int32_t n_ins,
                           int32_t* assigned_ids, int64_t delta_out[HM_STATIC_BINS]);
 
+/* What the last hm_graph_update on g that computed a delta touched, and where its time went.
+ * HM_ERR_BAD_INPUT if the last update had no delta. No counterpart in the reference, whose anchor
+ * rule does not separate removed and added triples (dynamicHyperGraph/src/HMotifCountUpdate.cu:151-226). */
+typedef struct hm_update_stats {
+  int64_t removed;      /* triples that contained a deleted hyperedge before the batch */
+  int64_t added;        /* triples that contain an inserted hyperedge after it (delta sums to added - removed) */
+  int64_t row_entries;  /* projection-row entries scanned by the delta kernel, both phases */
+  double ms_rows;       /* projection rows of the graph before and after (HIP events) */
+  double ms_delta;      /* delta kernel, both phases (HIP events) */
+  double ms_store;      /* store update: H2V / V2H rebuild, host wall time including its syncs */
+} hm_update_stats;
+hm_status hm_graph_update_stats(hm_graph* g, hm_update_stats* out);
+
 /* Exact delta between two snapshots of one id space, counts(new_g) - counts(old_g), given the
  * ids deleted from old_g and the ids inserted into new_g (an id may be deleted and re-inserted;
  * all other hyperedges must be identical in both). Same arguments as computeMotifCountsDelta

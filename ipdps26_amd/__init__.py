@@ -94,6 +94,7 @@ _SIGNATURES = [
     ("hm_graph_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(_GraphStats)]),
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
     ("hm_graph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("hm_graph_update_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("hm_graph_update", ctypes.c_int,
      [ctypes.c_void_p, _i32p, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _i32p, _i64p]),
     ("hm_count_delta", ctypes.c_int,
@@ -282,6 +283,20 @@ class Graph:
             _ptr(ioff, ctypes.c_int64), _ptr(inodes if inodes.size else np.zeros(1, np.int32), ctypes.c_int32),
             n_ins, _ptr(assigned, ctypes.c_int32), _ptr(out, ctypes.c_int64) if delta else None))
         return assigned[:n_ins], (out if delta else None)
+
+    def update_stats(self) -> dict:
+        """Of the last update with a delta: removed / added triples (held a deleted hyperedge
+        before, hold an inserted one after), row entries the delta kernel scanned, and the
+        milliseconds of the row build, the delta kernel and the store update."""
+        st = _UpdateStats()
+        _check(lib().hm_graph_update_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _UpdateStats._fields_}
+
+
+class _UpdateStats(ctypes.Structure):
+    _fields_ = [("removed", ctypes.c_int64), ("added", ctypes.c_int64),
+                ("row_entries", ctypes.c_int64), ("ms_rows", ctypes.c_double),
+                ("ms_delta", ctypes.c_double), ("ms_store", ctypes.c_double)]
 
 
 def count_delta(old: Graph, new: Graph, deleted, inserted) -> np.ndarray:

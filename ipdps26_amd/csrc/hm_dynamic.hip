@@ -27,6 +27,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -38,6 +39,14 @@
 namespace hm {
 
 __constant__ int8_t c_delta_bin[128] = HM_STATIC_BIN_TABLE;
+
+// Accumulator of a delta count (40 x uint64): bins 0..29 hold the signed delta (two's complement),
+// slot 30 the triples charged to deleted hyperedges (before the batch), slot 31 the triples
+// charged to inserted ones (after it), slot 32 the row entries the delta kernel scanned.
+constexpr int kDynAccRemoved = 30;
+constexpr int kDynAccAdded = 31;
+constexpr int kDynAccEntries = 32;  // projection-row entries scanned by k_delta
+constexpr int kDynAccCount = 40;
 
 // projected degree d_a (all neighbours, by id)
 template <typename MT>
@@ -104,45 +113,59 @@ __global__ __launch_bounds__(kBlock) void k_full_fill(int32_t E, const uint32_t*
 }
 
 // L(S, G) for the changed hyperedges in `changed` (flag[x] = 1 iff x in S); adds sign * L to acc.
-template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_delta(
+// Each changed hyperedge f is shared by `parts` workgroups of NW waves (workgroup (f, p) takes the
+// neighbours b of f with index = p mod parts), so a batch of a few hundred hyperedges still fills
+// the GPU; every workgroup builds f's mask table, part 0 also counts the pairs open at f.
+// Histogram: per wave, kDeltaCopies interleaved int32 copies updated with per-lane LDS atomics.
+constexpr int kDeltaWaves = 16;
+constexpr int kDeltaCopies = 8;
+
+template <typename MT, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_delta(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
     const int32_t* __restrict__ vedges, const unsigned long long* __restrict__ roff,
     const unsigned long long* __restrict__ rlen, const uint32_t* __restrict__ rows,
-    const int32_t* __restrict__ changed, const uint8_t* __restrict__ flag, long long sign,
-    unsigned long long* __restrict__ acc) {
+    const int32_t* __restrict__ changed, const uint8_t* __restrict__ flag, int32_t parts,
+    long long sign, unsigned long long* __restrict__ acc) {
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
   uint8_t* sz = reinterpret_cast<uint8_t*>(M + E);
   uint8_t* fl = sz + E;
-  __shared__ long long h[kWavesPerBlock][32];
+  __shared__ int32_t h[NW][32][kDeltaCopies];
   __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
-  __shared__ unsigned long long red[6][kWavesPerBlock];
+  __shared__ unsigned long long red[6][NW];
   __shared__ int8_t tbl[128];
   __shared__ uint32_t s_next;
-  const int32_t f = changed[blockIdx.x];
+  __shared__ unsigned long long s_scan;
+  const int32_t f = changed[blockIdx.x / parts];
+  const uint32_t part = blockIdx.x % parts;
   if (eoff[f + 1] == eoff[f]) return;
-  for (int k = threadIdx.x; k < kWavesPerBlock * 32; k += blockDim.x) (&h[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < NW * 32 * kDeltaCopies; k += blockDim.x) (&h[0][0][0])[k] = 0;
   for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_delta_bin[threadIdx.x];
-  if (threadIdx.x == 0) s_next = 0;
+  if (threadIdx.x == 0) {
+    s_next = 0;
+    s_scan = 0;
+  }
   for (int32_t x = threadIdx.x; x < E; x += blockDim.x) {
     sz[x] = esize[x];
     fl[x] = flag[x];
   }
   build_masks<MT>(M, E, f, eoff, enodes, voff, vedges, nullptr);
   const int sf = (int)(eoff[f + 1] - eoff[f]);
-  // classes of the allowed neighbours
-  for (int32_t x = threadIdx.x; x < E; x += blockDim.x) {
-    const MT m = M[x];
-    if (m && (!fl[x] || x > f)) {
-      const int C = popc(m);
-      atomicAdd(&cls[2 * (C - 1) + (sz[x] > C ? 1 : 0)], 1u);
+  const int lane = lane_id(), w = wave_id();
+  if (part == 0) {
+    // classes of the allowed neighbours, then every pair of them as if open at f (64-bit, straight
+    // to the global accumulator: C(d_f, 2) can exceed the int32 LDS counters)
+    for (int32_t x = threadIdx.x; x < E; x += blockDim.x) {
+      const MT m = M[x];
+      if (m && (!fl[x] || x > f)) {
+        const int C = popc(m);
+        atomicAdd(&cls[2 * (C - 1) + (sz[x] > C ? 1 : 0)], 1u);
+      }
     }
-  }
-  __syncthreads();
-  {  // every pair of allowed neighbours as if open at f
+    __syncthreads();
     const int K = 2 * sf;
     unsigned long long o[6] = {0, 0, 0, 0, 0, 0};
     for (int p = threadIdx.x; p < K * K; p += blockDim.x) {
@@ -157,25 +180,28 @@ __global__ __launch_bounds__(kBlock) void k_delta(
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       const unsigned long long v = wave_sum(o[q]);
-      if (lane_id() == 0) red[q][wave_id()] = v;
+      if (lane == 0) red[q][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      unsigned long long t = 0;
+      for (int q = 0; q < NW; ++q) t += red[threadIdx.x][q];
+      if (t) {
+        atomicAdd(&acc[kOpenBin0 + threadIdx.x], (unsigned long long)((long long)t * sign));
+        atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], t);
+      }
     }
   }
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    unsigned long long t = 0;
-    for (int q = 0; q < kWavesPerBlock; ++q) t += red[threadIdx.x][q];
-    h[0][kOpenBin0 + threadIdx.x] += (long long)t;
-  }
-  __syncthreads();
-  // neighbours b of f: closed pairs (re-booked) and triples open at b
-  const int lane = lane_id(), w = wave_id();
-  long long* hw = h[w];
+  // neighbours b of f (this part's share): closed pairs (re-booked) and triples open at b
+  const int copy = lane & (kDeltaCopies - 1);
+  int32_t* hc = &h[w][0][copy];
   const uint32_t* rowF = rows + roff[f];
   const uint32_t nF = (uint32_t)rlen[f];
   for (;;) {
-    uint32_t ib = 0;
-    if (lane == 0) ib = atomicAdd(&s_next, 1u);
-    ib = __shfl(ib, 0);
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(&s_next, 1u);
+    k = __shfl(k, 0);
+    const uint32_t ib = part + k * (uint32_t)parts;
     if (ib >= nF) break;
     const int32_t b = (int32_t)(rowF[ib] >> kCntBits);
     if (fl[b] && b < f) continue;
@@ -184,40 +210,36 @@ __global__ __launch_bounds__(kBlock) void k_delta(
     const int sb = sz[b];
     const uint32_t* rowB = rows + roff[b];
     const uint32_t nB = (uint32_t)rlen[b];
-    for (uint32_t j0 = 0; j0 < nB; j0 += kWave) {
-      const uint32_t j = j0 + lane;
-      bool plus = false, minus = false;
-      int bp = 0, bm = 0;
-      if (j < nB) {
-        const uint32_t e = rowB[j];
-        const int32_t c = (int32_t)(e >> kCntBits);
-        const int cbc = (int)(e & kCntMask);
-        if (c != f && (!fl[c] || c > f)) {
-          const MT mc = M[c];
-          const int sc = sz[c];
-          if (mc) {
-            if (c > b) {
-              const int cfc = popc(mc);
-              plus = minus = true;
-              bp = tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))];
-              bm = open_bin(sf, sb, sc, cfb, cfc);
-            }
-          } else {
-            plus = true;
-            bp = open_bin(sb, sf, sc, cfb, cbc);
-          }
+    if (lane == 0) atomicAdd(&s_scan, (unsigned long long)nB);
+    for (uint32_t j = lane; j < nB; j += kWave) {
+      const uint32_t e = rowB[j];
+      const int32_t c = (int32_t)(e >> kCntBits);
+      const int cbc = (int)(e & kCntMask);
+      if (c == f || (fl[c] && c < f)) continue;
+      const MT mc = M[c];
+      const int sc = sz[c];
+      if (mc) {
+        if (c > b) {
+          const int cfc = popc(mc);
+          atomicAdd(&hc[tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))] * kDeltaCopies], 1);
+          atomicSub(&hc[open_bin(sf, sb, sc, cfb, cfc) * kDeltaCopies], 1);
         }
+      } else {
+        atomicAdd(&hc[open_bin(sb, sf, sc, cfb, cbc) * kDeltaCopies], 1);
       }
-      wave_hist_add<long long>(hw, bp, plus, 1);
-      wave_hist_add<long long>(hw, bm, minus, -1);
     }
   }
   __syncthreads();
   if (threadIdx.x < kStaticBins) {
     long long s = 0;
-    for (int q = 0; q < kWavesPerBlock; ++q) s += h[q][threadIdx.x];
-    if (s) atomicAdd(&acc[threadIdx.x], (unsigned long long)(s * sign));
+    for (int q = 0; q < NW; ++q)
+      for (int c = 0; c < kDeltaCopies; ++c) s += h[q][threadIdx.x][c];
+    if (s) {
+      atomicAdd(&acc[threadIdx.x], (unsigned long long)(s * sign));
+      atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], (unsigned long long)s);
+    }
   }
+  if (threadIdx.x == 0 && s_scan) atomicAdd(&acc[kDynAccEntries], s_scan);
 }
 
 // ---- store update kernels -------------------------------------------------------------------
@@ -320,7 +342,8 @@ static hm_status build_full_rows(hm_graph* g, DynScratch& d) {
   return HM_OK;
 }
 
-// acc (device, 32 x u64, two's complement) += sign * L(ids, current graph).
+// acc (device, 40 x u64, two's complement) += sign * L(ids, current graph). Adds the durations of
+// the row build and of the delta kernel (HIP events on the handle's stream) to g->ms_upd_*.
 template <typename MT>
 static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,
                              long long sign) {
@@ -328,29 +351,39 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
   DeviceCsr& c = g->csr;
   const int32_t E = c.E;
   hipStream_t st = g->stream;
+  HM_HIP(hipEventRecord(g->ev[0], st));
   HM_TRY(build_full_rows<MT>(g, d));
+  HM_HIP(hipEventRecord(g->ev[1], st));
   HM_TRY(d.flag.ensure((size_t)std::max(E, 1)));
   HM_TRY(d.ids.ensure(sizeof(int32_t) * n));
   HM_HIP(hipMemcpyAsync(d.ids.p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
   k_fill_u8<<<grid_for(E), kBlock, 0, st>>>(d.flag.as<uint8_t>(), E, 0);
   k_set_flags<<<grid_for(n), kBlock, 0, st>>>(d.ids.as<int32_t>(), n, d.flag.as<uint8_t>());
   const size_t lds = (((size_t)E * sizeof(MT) + 2 * (size_t)E) + 15) & ~(size_t)15;
-  HM_TRY(allow_lds((const void*)k_delta<MT>, lds));
-  k_delta<MT><<<n, kBlock, lds, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
-                                      c.esize.as<uint8_t>(), c.voff.as<uint32_t>(),
-                                      c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
-                                      d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(),
-                                      d.ids.as<int32_t>(), d.flag.as<uint8_t>(), sign,
-                                      d.acc.as<unsigned long long>());
+  HM_TRY(allow_lds((const void*)k_delta<MT, kDeltaWaves>, lds));
+  // enough workgroups to give every CU several: parts per changed hyperedge
+  const int32_t parts = std::max<int32_t>(1, std::min<int32_t>(64, (2048 + n - 1) / n));
+  k_delta<MT, kDeltaWaves><<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
+      E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
+      c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
+      d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ids.as<int32_t>(),
+      d.flag.as<uint8_t>(), parts, sign, d.acc.as<unsigned long long>());
   HM_HIP(hipGetLastError());
+  HM_HIP(hipEventRecord(g->ev[2], st));
+  HM_HIP(hipEventSynchronize(g->ev[2]));
+  float a = 0.f, b = 0.f;
+  HM_HIP(hipEventElapsedTime(&a, g->ev[0], g->ev[1]));
+  HM_HIP(hipEventElapsedTime(&b, g->ev[1], g->ev[2]));
+  g->ms_upd_rows += a;
+  g->ms_upd_delta += b;
   return HM_OK;
 }
 
 static bool dense_fits(hm_graph* g, int32_t E, bool wide) {
   int lim = 65536;
   (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
-  // k_delta: mask, size and flag per id, plus about 2 KB of static arrays (4 KB reserved)
-  return (size_t)E * ((wide ? 8 : 4) + 2) + 16 + 4096 <= (size_t)lim;
+  // k_delta: mask, size and flag per id, plus about 18 KB of static arrays (20 KB reserved)
+  return (size_t)E * ((wide ? 8 : 4) + 2) + 16 + 20 * 1024 <= (size_t)lim;
 }
 
 static hm_status local_count_any(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,
@@ -483,14 +516,18 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
   HM_HIP(hipSetDevice(g->device));
   DynScratch& d = dyn_scratch(g);
   const bool want = delta_out != nullptr;
+  g->ms_upd_rows = g->ms_upd_delta = 0.f;
+  g->upd_removed = g->upd_added = g->upd_entries = -1;
   if (want) {
-    HM_TRY(d.acc.ensure(sizeof(unsigned long long) * 32));
-    k_zero_acc<<<1, 32, 0, g->stream>>>(d.acc.as<unsigned long long>(), 32);
+    HM_TRY(d.acc.ensure(sizeof(unsigned long long) * kDynAccCount));
+    k_zero_acc<<<1, kDynAccCount, 0, g->stream>>>(d.acc.as<unsigned long long>(), kDynAccCount);
     std::vector<int32_t> dels(del_ids, del_ids + n_del);
     std::sort(dels.begin(), dels.end());
     HM_TRY(local_count_any(g, d, dels.data(), n_del, -1));
   }
+  const auto t_store = std::chrono::steady_clock::now();
   HM_TRY(apply_batch(g, d, delmask, ins_off32, inodes, n_ins, newV));
+  g->ms_upd_store = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_store).count();
   g->csr.max_size = maxsz;
   for (int32_t k = 0; k < n_del; ++k) g->live[(size_t)del_ids[k]] = 0;
   g->live.resize((size_t)E + n_ins, 1);
@@ -499,12 +536,28 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
     std::vector<int32_t> ins((size_t)n_ins);
     for (int32_t k = 0; k < n_ins; ++k) ins[(size_t)k] = E + k;
     HM_TRY(local_count_any(g, d, ins.data(), n_ins, +1));
-    unsigned long long h[32];
+    unsigned long long h[kDynAccCount];
     HM_HIP(hipMemcpyAsync(h, d.acc.p, sizeof(h), hipMemcpyDeviceToHost, g->stream));
     HM_HIP(hipStreamSynchronize(g->stream));
     for (int k = 0; k < HM_STATIC_BINS; ++k) delta_out[k] = (int64_t)h[k];
+    g->upd_removed = (int64_t)h[kDynAccRemoved];
+    g->upd_added = (int64_t)h[kDynAccAdded];
+    g->upd_entries = (int64_t)h[kDynAccEntries];
   }
   g->have_stats = false;
+  return HM_OK;
+}
+
+hm_status hm_graph_update_stats(hm_graph* g, hm_update_stats* out) {
+  clear_error();
+  if (!g || !out) return fail(HM_ERR_BAD_INPUT, "hm_graph_update_stats: null argument");
+  if (g->upd_removed < 0) return fail(HM_ERR_BAD_INPUT, "hm_graph_update_stats: no update with a delta has run");
+  out->removed = g->upd_removed;
+  out->added = g->upd_added;
+  out->row_entries = g->upd_entries;
+  out->ms_rows = g->ms_upd_rows;
+  out->ms_delta = g->ms_upd_delta;
+  out->ms_store = g->ms_upd_store;
   return HM_OK;
 }
 
@@ -529,8 +582,8 @@ hm_status hm_count_delta(hm_graph* old_g, hm_graph* new_g, const int32_t* del_id
   for (hm_graph* g : {old_g, new_g}) {
     HM_HIP(hipSetDevice(g->device));
     DynScratch& d = dyn_scratch(g);
-    HM_TRY(d.acc.ensure(sizeof(unsigned long long) * 32));
-    k_zero_acc<<<1, 32, 0, g->stream>>>(d.acc.as<unsigned long long>(), 32);
+    HM_TRY(d.acc.ensure(sizeof(unsigned long long) * kDynAccCount));
+    k_zero_acc<<<1, kDynAccCount, 0, g->stream>>>(d.acc.as<unsigned long long>(), kDynAccCount);
     const bool is_old = g == old_g;
     HM_TRY(local_count_any(g, d, is_old ? dels.data() : ins.data(), is_old ? n_del : n_ins,
                            is_old ? -1 : +1));

@@ -122,4 +122,8 @@ struct hm_graph {
   // id tile of the static count: 0 = automatic (one LDS table when E fits), > 0 = forced tiles
   int32_t tile = 0;
   int32_t last_tile = 0;  // tile used by the last count (0 = single table)
+  // triples that held a deleted hyperedge before / hold an inserted one after the last update
+  // with a delta (-1 when there was none)
+  int64_t upd_removed = -1, upd_added = -1, upd_entries = -1;
+  float ms_upd_rows = 0.f, ms_upd_delta = 0.f, ms_upd_store = 0.f;
 };

@@ -48,11 +48,14 @@ def test_batches_match_oracle_recount(gpu_ready, seed):
         for x in dels:
             rows[x] = []
             live.discard(x)
+        mid = int(O.static_mochy(*csr(rows)).sum())  # deleted rows gone, nothing inserted yet
         rows.extend(new_rows)
         live.update(ids.tolist())
         after = O.static_mochy(*csr(rows))
         print(f"seed={seed} batch={b} E={len(rows)}", flush=True)
         assert np.array_equal(delta, after - before)
+        us = g.update_stats()
+        assert (us["removed"], us["added"]) == (int(before.sum()) - mid, int(after.sum()) - mid)
         assert np.array_equal(g.count_static().astype(np.int64), after)
         before = after
     g.close()
