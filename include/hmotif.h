@@ -167,6 +167,19 @@ hm_status hm_count_delta(hm_graph* old_g, hm_graph* new_g, const int32_t* del_id
                          const int32_t* ins_ids, int32_t n_ins, int64_t delta_out[HM_STATIC_BINS]);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Three-snapshot strict-increasing count (the reference's temporal-window interface)         */
+/* ------------------------------------------------------------------------------------------ */
+/* Replaces computeTemporalMotifCountsStrictInc (temporalDynamicHMotif/include/temporal_count.hpp:8-9,
+ * temporalDynamicHMotif/src/temporal_count.cu:91-217). Layer 0 = older, 1 = middle, 2 = newest
+ * snapshot, each a CSR over its row ids (row r is the reference's hyperedge id r + 1; an empty row
+ * is an absent id). Counts the triples (i, j, k), i < j < k, of older row i, middle row j and newest
+ * row k that pairwise overlap, into the 30 static bins computed from the three rows. Unlike the
+ * reference kernel, which keeps at most 1,024 candidates on each side of j, no candidate is dropped.
+ * This is not THyMe's temporal h-motif count (hm_count_temporal). */
+hm_status hm_count_strict_inc(const int64_t* const offsets[3], const int32_t* const nodes[3],
+                              const int32_t num_rows[3], uint64_t out[HM_STATIC_BINS]);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Temporal hypergraph handle: temporal hyperedges (node set, int64 timestamp) in time order  */
 /* (ties by input order), resident in HBM.                                                    */
 /* Replaces the 3-layer ring buffer TemporalHypergraphIndex /                                 */

@@ -95,6 +95,8 @@ _SIGNATURES = [
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
     ("hm_graph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_update_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("hm_count_strict_inc", ctypes.c_int,
+     [ctypes.c_void_p * 3, ctypes.c_void_p * 3, ctypes.c_int32 * 3, _P(ctypes.c_uint64)]),
     ("hm_graph_update", ctypes.c_int,
      [ctypes.c_void_p, _i32p, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _i32p, _i64p]),
     ("hm_count_delta", ctypes.c_int,
@@ -307,6 +309,21 @@ def count_delta(old: Graph, new: Graph, deleted, inserted) -> np.ndarray:
     _check(lib().hm_count_delta(old._h, new._h, _ptr(d if d.size else np.zeros(1, np.int32), ctypes.c_int32),
                                 d.size, _ptr(i if i.size else np.zeros(1, np.int32), ctypes.c_int32),
                                 i.size, _ptr(out, ctypes.c_int64)))
+    return out
+
+
+def count_strict_inc(older, middle, newest) -> np.ndarray:
+    """The reference's three-snapshot strict-increasing count (30 static bins, uint64): triples
+    (i, j, k), i < j < k, of older row i, middle row j, newest row k, pairwise overlapping. Each
+    snapshot is (offsets int64[n+1], nodes int32[...]) indexed by row id."""
+    offs = [np.ascontiguousarray(l[0], dtype=np.int64) for l in (older, middle, newest)]
+    nods = [np.ascontiguousarray(l[1], dtype=np.int32) for l in (older, middle, newest)]
+    nods = [n if n.size else np.zeros(1, dtype=np.int32) for n in nods]
+    po = (ctypes.c_void_p * 3)(*[o.ctypes.data for o in offs])
+    pn = (ctypes.c_void_p * 3)(*[n.ctypes.data for n in nods])
+    nr = (ctypes.c_int32 * 3)(*[o.size - 1 for o in offs])
+    out = np.zeros(STATIC_BINS, dtype=np.uint64)
+    _check(lib().hm_count_strict_inc(po, pn, nr, _ptr(out, ctypes.c_uint64)))
     return out
 
 

@@ -26,6 +26,16 @@
  *                          <= delta and with at least two overlapping pairs (copies of one node set
  *                          overlap), classified by the Venn pattern written in time order.
  *   oracle_projection_degrees  d_a = number of other hyperedges sharing a node with a.
+ *   oracle_strict_inc      the reference's three-snapshot strict-increasing count,
+ *                          temporalDynamicHMotif/src/temporal_count.cu:91-171 (candidates i < j from
+ *                          the older snapshot's node lists of j's nodes :117-131, k > j from the
+ *                          newest's :133-147, pair checks and classification :149-170 with
+ *                          kernel/motif_utils.cuh:80-100), without the kernel's 1,024-candidate cap
+ *                          (:79-80). Pinned by: the reference driver's synthetic window
+ *                          (temporalDynamicHMotif/src/main.cu:267-305: bin 26 = 1, then delta
+ *                          bin 10 +1 / bin 26 -1) and oracle_strict_inc_brute.
+ *   oracle_strict_inc_brute  definition: every (i, j, k), i < j < k, older row i, middle row j,
+ *                          newest row k, all three pairs overlapping, classified by its Venn pattern.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -642,5 +652,107 @@ int oracle_temporal_brute(int32_t E, const int64_t* off, const int32_t* nodes,
   free(ab);
   free(order);
   csr_free(&g);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Three-snapshot strict-increasing count (the reference's temporal-window kernel).
+ * ------------------------------------------------------------------------------------------- */
+static int strict_inc_build(const int64_t* const off[3], const int32_t* const nodes[3],
+                            const int32_t n[3], csr_t g[3]) {
+  for (int l = 0; l < 3; ++l)
+    if (csr_build(n[l], off[l], nodes[l], &g[l])) return -1;
+  return 0;
+}
+
+int oracle_strict_inc(const int64_t* const off[3], const int32_t* const nodes[3], const int32_t n[3],
+                      int64_t out[30]) {
+  csr_t g[3];
+  for (int i = 0; i < 30; ++i) out[i] = 0;
+  if (strict_inc_build(off, nodes, n, g)) return -1;
+  const csr_t *O = &g[0], *M = &g[1], *N = &g[2];
+  int32_t* candI = (int32_t*)malloc(sizeof(int32_t) * (size_t)(O->E > 0 ? O->E : 1));
+  int32_t* candK = (int32_t*)malloc(sizeof(int32_t) * (size_t)(N->E > 0 ? N->E : 1));
+  uint8_t* seenI = (uint8_t*)calloc((size_t)(O->E > 0 ? O->E : 1), 1);
+  uint8_t* seenK = (uint8_t*)calloc((size_t)(N->E > 0 ? N->E : 1), 1);
+  int32_t* jk = (int32_t*)malloc(sizeof(int32_t) * 4096);
+  for (int32_t j = 0; j < M->E; ++j) {
+    const int32_t* nj = M->nodes + M->off[j];
+    int sj = (int)(M->off[j + 1] - M->off[j]);
+    int ni = 0, nk = 0;
+    for (int p = 0; p < sj; ++p) {  /* older rows through j's nodes, i < j */
+      int32_t v = nj[p];
+      if (v < O->V)
+        for (int64_t q = O->voff[v]; q < O->voff[v + 1]; ++q) {
+          int32_t i = O->vedges[q];
+          if (i < j && !seenI[i]) { seenI[i] = 1; candI[ni++] = i; }
+        }
+      if (v < N->V)
+        for (int64_t q = N->voff[v]; q < N->voff[v + 1]; ++q) {
+          int32_t k = N->vedges[q];
+          if (k > j && !seenK[k]) { seenK[k] = 1; candK[nk++] = k; }
+        }
+    }
+    for (int a = 0; a < ni; ++a) {
+      int32_t i = candI[a];
+      const int32_t* ni_ = O->nodes + O->off[i];
+      int si = (int)(O->off[i + 1] - O->off[i]);
+      int cij = inter_size(ni_, si, nj, sj);
+      if (cij <= 0) continue;
+      for (int b = 0; b < nk; ++b) {
+        int32_t k = candK[b];
+        const int32_t* nk_ = N->nodes + N->off[k];
+        int sk = (int)(N->off[k + 1] - N->off[k]);
+        int cjk = 0, x = 0, y = 0;
+        while (x < sj && y < sk) {  /* j ∩ k, kept for the common part */
+          if (nj[x] < nk_[y]) ++x;
+          else if (nj[x] > nk_[y]) ++y;
+          else { jk[cjk++] = nj[x]; ++x; ++y; }
+        }
+        if (cjk <= 0) continue;
+        int cik = inter_size(ni_, si, nk_, sk);
+        if (cik <= 0) continue;
+        int gijk = inter_size(ni_, si, jk, cjk);
+        out[mochy_bin(si, sj, sk, cij, cjk, cik, gijk)]++;
+      }
+    }
+    for (int a = 0; a < ni; ++a) seenI[candI[a]] = 0;
+    for (int b = 0; b < nk; ++b) seenK[candK[b]] = 0;
+  }
+  free(candI); free(candK); free(seenI); free(seenK); free(jk);
+  for (int l = 0; l < 3; ++l) csr_free(&g[l]);
+  return 0;
+}
+
+int oracle_strict_inc_brute(const int64_t* const off[3], const int32_t* const nodes[3],
+                            const int32_t n[3], int64_t out[30]) {
+  csr_t g[3];
+  for (int i = 0; i < 30; ++i) out[i] = 0;
+  if (strict_inc_build(off, nodes, n, g)) return -1;
+  int32_t* ij = (int32_t*)malloc(sizeof(int32_t) * 4096);
+  for (int32_t i = 0; i < g[0].E; ++i) {
+    const int32_t* a = g[0].nodes + g[0].off[i];
+    int sa = (int)(g[0].off[i + 1] - g[0].off[i]);
+    for (int32_t j = i + 1; j < g[1].E; ++j) {
+      const int32_t* b = g[1].nodes + g[1].off[j];
+      int sb = (int)(g[1].off[j + 1] - g[1].off[j]);
+      int cab = 0, x = 0, y = 0;
+      while (x < sa && y < sb) {
+        if (a[x] < b[y]) ++x;
+        else if (a[x] > b[y]) ++y;
+        else { ij[cab++] = a[x]; ++x; ++y; }
+      }
+      if (!cab) continue;
+      for (int32_t k = j + 1; k < g[2].E; ++k) {
+        const int32_t* c = g[2].nodes + g[2].off[k];
+        int sc = (int)(g[2].off[k + 1] - g[2].off[k]);
+        int cbc = inter_size(b, sb, c, sc), cca = inter_size(c, sc, a, sa);
+        if (!cbc || !cca) continue;
+        out[mochy_bin(sa, sb, sc, cab, cbc, cca, inter_size(ij, cab, c, sc))]++;
+      }
+    }
+  }
+  free(ij);
+  for (int l = 0; l < 3; ++l) csr_free(&g[l]);
   return 0;
 }

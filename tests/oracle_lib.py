@@ -92,3 +92,27 @@ def mochy_table() -> list[int]:
 def thyme_temporal_table() -> list[int]:
     t = lib().oracle_thyme_temporal_table()
     return [t[i] for i in range(128)]
+
+
+def _layers(layers):
+    offs = [np.ascontiguousarray(o, dtype=np.int64) for o, _ in layers]
+    nods = [np.ascontiguousarray(n if len(n) else np.zeros(1), dtype=np.int32) for _, n in layers]
+    po = (ctypes.c_void_p * 3)(*[o.ctypes.data for o in offs])
+    pn = (ctypes.c_void_p * 3)(*[n.ctypes.data for n in nods])
+    nr = (ctypes.c_int32 * 3)(*[o.size - 1 for o in offs])
+    return offs, nods, po, pn, nr
+
+
+def strict_inc(older, middle, newest) -> np.ndarray:
+    """The reference's three-snapshot strict-increasing count (restatement of its kernel)."""
+    keep = _layers([older, middle, newest])
+    out = np.zeros(30, dtype=np.int64)
+    assert lib().oracle_strict_inc(keep[2], keep[3], keep[4], out.ctypes.data_as(_P(ctypes.c_int64))) == 0
+    return out
+
+
+def strict_inc_brute(older, middle, newest) -> np.ndarray:
+    keep = _layers([older, middle, newest])
+    out = np.zeros(30, dtype=np.int64)
+    assert lib().oracle_strict_inc_brute(keep[2], keep[3], keep[4], out.ctypes.data_as(_P(ctypes.c_int64))) == 0
+    return out

@@ -186,3 +186,35 @@ def test_temporal_window_is_inclusive():
     for fn in (O.thyme_temporal, O.temporal_brute):
         assert fn(off, nodes, times, 10).sum() == 1
         assert fn(off, nodes, times, 9).sum() == 0
+
+
+# ---- three-snapshot strict-increasing count (the reference's temporal-window kernel) ----------
+def _window(rows):
+    return csr(rows)
+
+
+def test_strict_inc_reference_synthetic_window():
+    # temporalDynamicHMotif/src/main.cu:267-305: older = middle = newest = {1,2},{2,3},{1,3};
+    # the new window changes the newest row 3 to {1,2,3}
+    L = _window([[1, 2], [2, 3], [1, 3]])
+    L2 = _window([[1, 2], [2, 3], [1, 2, 3]])
+    old = O.strict_inc(L, L, L)
+    assert old[26] == 1 and old.sum() == 1
+    delta = O.strict_inc(L, L, L2) - old
+    assert delta[10] == 1 and delta[26] == -1 and np.abs(delta).sum() == 2
+    assert np.array_equal(O.strict_inc_brute(L, L, L), old)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_strict_inc_matches_brute_force(seed):
+    rng = np.random.default_rng(8000 + seed)
+    base = random_hypergraph(rng, num_edges=40, num_nodes=25, kmax=6, dup_frac=0.2)
+    rows = [list(base[1][base[0][e]:base[0][e + 1]]) for e in range(base[0].size - 1)]
+    layers = []
+    for _ in range(3):  # three snapshots: some rows edited, some emptied, some appended
+        snap = [list(r) for r in rows]
+        for e in rng.choice(len(snap), size=8, replace=False):
+            snap[e] = [] if rng.random() < 0.3 else list(rng.choice(25, size=int(rng.integers(1, 6)), replace=False))
+        snap += [list(rng.choice(25, size=3, replace=False)) for _ in range(int(rng.integers(0, 5)))]
+        layers.append(csr(snap))
+    assert np.array_equal(O.strict_inc(*layers), O.strict_inc_brute(*layers))
