@@ -104,9 +104,10 @@ hm_status hm_graph_destroy(hm_graph* g);
  * and printed them; the count here follows MoCHy-E (MoCHy-master/main_exact_par.cpp:122-165). */
 hm_status hm_count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]);
 
-/* Key tile of hm_count_static on g. 0 (the default): one LDS mask table over all hyperedge ids
- * when they fit (about 30k ids), tiles as large as LDS allows otherwise; > 0: tiles of exactly
- * tile_ids ids forced at any size (tests and tuning). The counts do not depend on the tile.
+/* Key tile of hm_count_static and of the delta of hm_graph_update / hm_count_delta on g.
+ * 0 (the default): one LDS mask table over all hyperedge ids when they fit (about 28k ids for
+ * the static count, 24k for the delta), tiles as large as LDS allows otherwise; > 0: tiles of
+ * exactly tile_ids ids forced at any size (tests and tuning). Results do not depend on the tile.
  * No counterpart in the reference, whose kernel keeps no per-hyperedge table
  * (dynamicHyperGraph/src/HMotifCount.cu:16-95). */
 hm_status hm_graph_set_tile(hm_graph* g, int32_t tile_ids);

@@ -163,4 +163,22 @@ __device__ inline void for_tile_entries(const TileRanges& tr, int sa, uint32_t t
   }
 }
 
+// Mask of a's nodes (s_an[0 .. sa), ascending) contained in hyperedge b.
+template <typename MT>
+__device__ inline MT pair_mask(const int32_t* s_an, int sa, const uint32_t* __restrict__ eoff,
+                               const int32_t* __restrict__ enodes, int32_t b) {
+  MT m = (MT)0;
+  for (uint32_t q = eoff[b], qe = eoff[b + 1]; q < qe; ++q) {
+    const int32_t v = enodes[q];
+    int lo = 0, hi = sa;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_an[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < sa && s_an[lo] == v) m |= (MT)1 << lo;
+  }
+  return m;
+}
+
 }  // namespace hm

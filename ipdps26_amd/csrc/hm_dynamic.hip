@@ -119,6 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_full_fill(int32_t E, const uint32_t*
 // Histogram: per wave, kDeltaCopies interleaved int32 copies updated with per-lane LDS atomics.
 constexpr int kDeltaWaves = 16;
 constexpr int kDeltaCopies = 8;
+constexpr size_t kDeltaTiledStaticLds = 20 * 1024;  // k_delta_tiled: histogram, classes, tables
 
 template <typename MT, int NW>
 __global__ __launch_bounds__(NW * kWave) void k_delta(
@@ -242,6 +243,255 @@ __global__ __launch_bounds__(NW * kWave) void k_delta(
   if (threadIdx.x == 0 && s_scan) atomicAdd(&acc[kDynAccEntries], s_scan);
 }
 
+// ---- tiled delta: hyperedge ids beyond one LDS table ------------------------------------------
+// The id space is cut into tiles of T ids (the walk of hm_static's tiled kernels: the part of each
+// node's V2H row inside the tile, found by binary search, and visit-once ownership by the lane of
+// the lowest node). Full rows come out grouped by tile, so the entries of a row inside a tile are
+// contiguous and a read position per (f, b) pair walks them tile after tile.
+
+// Projected degree d_a over id tiles.
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_deg_tiled(int32_t E, int32_t T, const uint32_t* __restrict__ eoff,
+                                                      const int32_t* __restrict__ enodes,
+                                                      const uint32_t* __restrict__ voff,
+                                                      const int32_t* __restrict__ vedges,
+                                                      unsigned long long* __restrict__ deg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ TileRanges tr;
+  __shared__ uint32_t red[kWavesPerBlock];
+  const int32_t a = blockIdx.x;
+  const uint32_t s = eoff[a];
+  const int sa = (int)(eoff[a + 1] - s);
+  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+  uint32_t d = 0;
+  for (int32_t t0 = 0; t0 < E; t0 += T) {
+    const int32_t t1 = min(E, t0 + T);
+    const uint32_t total = tile_ranges(tr, sa, enodes + s, voff, vedges, t0, t1);
+    if (total == 0) continue;
+    for_tile_entries(tr, sa, total, vedges, [&](int i, int32_t x) {
+      if (x != a) atomicOr(&M[x - t0], (MT)1 << i);
+    });
+    __syncthreads();
+    for_tile_entries(tr, sa, total, vedges, [&](int i, int32_t x) {
+      if (x == a) return;
+      const MT m = M[x - t0];
+      if (m && low_bit(m) == i) {
+        ++d;
+        M[x - t0] = (MT)0;
+      }
+    });
+  }
+  d = wave_sum(d);
+  if (lane_id() == 0) red[wave_id()] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int q = 0; q < kWavesPerBlock; ++q) t += red[q];
+    deg[a] = t;
+  }
+}
+
+// Full row of a, (x << 7 | C_ax) grouped by id tile in ascending tile order.
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_full_fill_tiled(
+    int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges,
+    const unsigned long long* __restrict__ roff, uint32_t* __restrict__ rows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ TileRanges tr;
+  __shared__ uint32_t s_cnt;
+  const int32_t a = blockIdx.x;
+  const uint32_t s = eoff[a];
+  const int sa = (int)(eoff[a + 1] - s);
+  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+  if (threadIdx.x == 0) s_cnt = 0;
+  uint32_t* out = rows + roff[a];
+  for (int32_t t0 = 0; t0 < E; t0 += T) {
+    const int32_t t1 = min(E, t0 + T);
+    const uint32_t total = tile_ranges(tr, sa, enodes + s, voff, vedges, t0, t1);
+    if (total == 0) continue;
+    for_tile_entries(tr, sa, total, vedges, [&](int i, int32_t x) {
+      if (x != a) atomicOr(&M[x - t0], (MT)1 << i);
+    });
+    __syncthreads();
+    for_tile_entries(tr, sa, total, vedges, [&](int i, int32_t x) {
+      if (x == a) return;
+      const MT m = M[x - t0];
+      if (m && low_bit(m) == i) {
+        out[atomicAdd(&s_cnt, 1u)] = ((uint32_t)x << kCntBits) | (uint32_t)popc(m);
+        M[x - t0] = (MT)0;
+      }
+    });
+  }
+}
+
+// rlen of every changed hyperedge (state sizes of k_delta_tiled).
+__global__ void k_gather_len(const int32_t* __restrict__ changed, int32_t n,
+                             const unsigned long long* __restrict__ rlen,
+                             unsigned long long* __restrict__ out) {
+  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = rlen[changed[q]];
+  else if (q == n) out[n] = 0;
+}
+
+// L(S, G) over id tiles: the k_delta decomposition with f's mask table built one tile at a time.
+// Per neighbour b of f (this part's share) the state at soff[q] + ib keeps f's mask in b and the
+// read position in b's tile-grouped row. Flags and sizes are read from HBM (they do not fit LDS).
+template <typename MT, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
+    int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
+    const int32_t* __restrict__ vedges, const unsigned long long* __restrict__ roff,
+    const unsigned long long* __restrict__ rlen, const uint32_t* __restrict__ rows,
+    const int32_t* __restrict__ changed, const uint8_t* __restrict__ flag, int32_t parts,
+    long long sign, const unsigned long long* __restrict__ soff, uint32_t* __restrict__ cur,
+    MT* __restrict__ pm, unsigned long long* __restrict__ acc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ int32_t h[NW][32][kDeltaCopies];
+  __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
+  __shared__ unsigned long long red[6][NW];
+  __shared__ int8_t tbl[128];
+  __shared__ int32_t s_an[HM_MAX_EDGE_SIZE];
+  __shared__ TileRanges tr;
+  __shared__ unsigned long long s_scan;
+  const int32_t q = blockIdx.x / parts;
+  const int32_t f = changed[q];
+  const uint32_t part = blockIdx.x % parts;
+  const uint32_t fs = eoff[f];
+  const int sf = (int)(eoff[f + 1] - fs);
+  if (sf == 0) return;
+  for (int k = threadIdx.x; k < NW * 32 * kDeltaCopies; k += blockDim.x) (&h[0][0][0])[k] = 0;
+  for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
+  if (threadIdx.x < 128) tbl[threadIdx.x] = c_delta_bin[threadIdx.x];
+  if ((int)threadIdx.x < sf) s_an[threadIdx.x] = enodes[fs + threadIdx.x];
+  if (threadIdx.x == 0) s_scan = 0;
+  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+  const int lane = lane_id(), w = wave_id();
+  int32_t* hc = &h[w][0][lane & (kDeltaCopies - 1)];
+  const uint32_t* rowF = rows + roff[f];
+  const uint32_t nF = (uint32_t)rlen[f];
+  const unsigned long long st = soff[q];
+  for (int32_t t0 = 0; t0 < E; t0 += T) {
+    const int32_t t1 = min(E, t0 + T);
+    const uint32_t total = tile_ranges(tr, sf, enodes + fs, voff, vedges, t0, t1);
+    for_tile_entries(tr, sf, total, vedges, [&](int i, int32_t x) {
+      if (x != f) atomicOr(&M[x - t0], (MT)1 << i);
+    });
+    __syncthreads();
+    if (part == 0)  // classes of the allowed neighbours inside this tile (each counted once)
+      for_tile_entries(tr, sf, total, vedges, [&](int i, int32_t x) {
+        if (x == f) return;
+        const MT m = M[x - t0];
+        if (m && low_bit(m) == i && (!flag[x] || x > f)) {
+          const int C = popc(m);
+          atomicAdd(&cls[2 * (C - 1) + (esize[x] > C ? 1 : 0)], 1u);
+        }
+      });
+    // neighbours b of f (this part's share): closed pairs (re-booked) and triples open at b
+    for (uint32_t k = (uint32_t)w;; k += NW) {
+      const uint32_t ib = part + k * (uint32_t)parts;
+      if (ib >= nF) break;
+      const int32_t b = (int32_t)(rowF[ib] >> kCntBits);
+      if (flag[b] && b < f) continue;
+      MT mb;
+      uint32_t pos;
+      if (t0 == 0) {
+        mb = pair_mask<MT>(s_an, sf, eoff, enodes, b);
+        pos = 0;
+        if (lane == 0) pm[st + ib] = mb;
+      } else {
+        mb = pm[st + ib];
+        pos = cur[st + ib];
+      }
+      const int cfb = popc(mb);
+      const int sb = esize[b];
+      const uint32_t* rowB = rows + roff[b];
+      const uint32_t nB = (uint32_t)rlen[b];
+      const uint32_t pos0 = pos;
+      for (;;) {
+        const uint32_t j = pos + lane;
+        bool v = false;
+        uint32_t e = 0;
+        if (j < nB) {
+          e = rowB[j];
+          v = (int32_t)(e >> kCntBits) < t1;
+        }
+        if (v) {
+          const int32_t c = (int32_t)(e >> kCntBits);
+          if (c != f && !(flag[c] && c < f)) {
+            const int cbc = (int)(e & kCntMask);
+            const MT mc = M[c - t0];
+            const int sc = esize[c];
+            if (mc) {
+              if (c > b) {
+                const int cfc = popc(mc);
+                atomicAdd(&hc[tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))] * kDeltaCopies], 1);
+                atomicSub(&hc[open_bin(sf, sb, sc, cfb, cfc) * kDeltaCopies], 1);
+              }
+            } else {
+              atomicAdd(&hc[open_bin(sb, sf, sc, cfb, cbc) * kDeltaCopies], 1);
+            }
+          }
+        }
+        const uint32_t nv = (uint32_t)__popcll(__ballot(v));
+        pos += nv;
+        if (nv < (uint32_t)kWave) break;
+      }
+      if (lane == 0) {
+        cur[st + ib] = pos;
+        if (pos > pos0) atomicAdd(&s_scan, (unsigned long long)(pos - pos0));
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < kStaticBins) {  // flush this tile's histogram (bounds the int32 counters)
+      long long sum = 0;
+      for (int w2 = 0; w2 < NW; ++w2)
+        for (int c = 0; c < kDeltaCopies; ++c) {
+          sum += h[w2][threadIdx.x][c];
+          h[w2][threadIdx.x][c] = 0;
+        }
+      if (sum) {
+        atomicAdd(&acc[threadIdx.x], (unsigned long long)(sum * sign));
+        atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], (unsigned long long)sum);
+      }
+    }
+    for_tile_entries(tr, sf, total, vedges, [&](int, int32_t x) { M[x - t0] = (MT)0; });
+  }
+  if (part == 0) {  // every pair of allowed neighbours as if open at f (64-bit)
+    __syncthreads();
+    const int K = 2 * sf;
+    unsigned long long o[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = threadIdx.x; p < K * K; p += blockDim.x) {
+      const int k1 = p / K, k2 = p - k1 * K;
+      if (k1 > k2) continue;
+      const unsigned long long n1 = cls[k1], n2 = cls[k2];
+      const unsigned long long pairs = (k1 == k2) ? n1 * (n1 - (n1 > 0)) / 2 : n1 * n2;
+      const int b6 = (int)((k1 >> 1) + (k2 >> 1) + 2 < sf) + 2 * ((k1 & 1) + (k2 & 1));
+#pragma unroll
+      for (int qq = 0; qq < 6; ++qq) o[qq] += b6 == qq ? pairs : 0;
+    }
+#pragma unroll
+    for (int qq = 0; qq < 6; ++qq) {
+      const unsigned long long v = wave_sum(o[qq]);
+      if (lane == 0) red[qq][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      unsigned long long t = 0;
+      for (int w2 = 0; w2 < NW; ++w2) t += red[threadIdx.x][w2];
+      if (t) {
+        atomicAdd(&acc[kOpenBin0 + threadIdx.x], (unsigned long long)((long long)t * sign));
+        atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], t);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_scan) atomicAdd(&acc[kDynAccEntries], s_scan);
+}
+
 // ---- store update kernels -------------------------------------------------------------------
 __global__ void k_new_sizes(int32_t E, int32_t nins, const uint32_t* __restrict__ eoff,
                             const uint8_t* __restrict__ del, const uint32_t* __restrict__ ins_off,
@@ -308,21 +558,28 @@ static unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBloc
 
 static DynScratch& dyn_scratch(hm_graph* g) { return g->dyn; }
 
-// Full projection rows of the graph currently on the handle.
+// Full projection rows of the graph currently on the handle (T > 0: tiled kernels, rows grouped
+// by id tile; T == 0: one LDS table, rows ascending).
 template <typename MT>
-static hm_status build_full_rows(hm_graph* g, DynScratch& d) {
+static hm_status build_full_rows(hm_graph* g, DynScratch& d, int32_t T) {
   DeviceCsr& c = g->csr;
   const int32_t E = c.E;
   hipStream_t st = g->stream;
-  const size_t lds = ((size_t)E * sizeof(MT) + 15) & ~(size_t)15;
-  HM_TRY(allow_lds((const void*)k_deg<MT>, lds));
-  HM_TRY(allow_lds((const void*)k_full_fill<MT>, lds));
+  const size_t lds = ((size_t)(T > 0 ? T : E) * sizeof(MT) + 15) & ~(size_t)15;
   HM_TRY(d.deg.ensure(sizeof(unsigned long long) * (E + 1)));
   HM_TRY(d.roff.ensure(sizeof(unsigned long long) * (E + 1)));
   k_zero_acc<<<1, 1, 0, st>>>(d.deg.as<unsigned long long>() + E, 1);
-  k_deg<MT><<<E, kBlock, lds, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
-                                    c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
-                                    d.deg.as<unsigned long long>());
+  if (T > 0) {
+    HM_TRY(allow_lds((const void*)k_deg_tiled<MT>, lds));
+    k_deg_tiled<MT><<<E, kBlock, lds, st>>>(E, T, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                            c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+                                            d.deg.as<unsigned long long>());
+  } else {
+    HM_TRY(allow_lds((const void*)k_deg<MT>, lds));
+    k_deg<MT><<<E, kBlock, lds, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                      c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+                                      d.deg.as<unsigned long long>());
+  }
   HM_HIP(hipGetLastError());
   size_t tb = 0;
   HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d.deg.as<unsigned long long>(),
@@ -335,9 +592,17 @@ static hm_status build_full_rows(hm_graph* g, DynScratch& d) {
   HM_HIP(hipMemcpyAsync(&P, d.roff.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
   HM_TRY(d.rows.ensure(sizeof(uint32_t) * std::max<unsigned long long>(P, 1)));
-  k_full_fill<MT><<<E, kBlock, lds, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
-                                          c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
-                                          d.roff.as<unsigned long long>(), d.rows.as<uint32_t>());
+  if (T > 0) {
+    HM_TRY(allow_lds((const void*)k_full_fill_tiled<MT>, lds));
+    k_full_fill_tiled<MT><<<E, kBlock, lds, st>>>(E, T, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                                  c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+                                                  d.roff.as<unsigned long long>(), d.rows.as<uint32_t>());
+  } else {
+    HM_TRY(allow_lds((const void*)k_full_fill<MT>, lds));
+    k_full_fill<MT><<<E, kBlock, lds, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                            c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+                                            d.roff.as<unsigned long long>(), d.rows.as<uint32_t>());
+  }
   HM_HIP(hipGetLastError());
   return HM_OK;
 }
@@ -346,28 +611,56 @@ static hm_status build_full_rows(hm_graph* g, DynScratch& d) {
 // the row build and of the delta kernel (HIP events on the handle's stream) to g->ms_upd_*.
 template <typename MT>
 static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,
-                             long long sign) {
+                             long long sign, int32_t T) {
   if (n == 0) return HM_OK;
   DeviceCsr& c = g->csr;
   const int32_t E = c.E;
   hipStream_t st = g->stream;
   HM_HIP(hipEventRecord(g->ev[0], st));
-  HM_TRY(build_full_rows<MT>(g, d));
+  HM_TRY(build_full_rows<MT>(g, d, T));
   HM_HIP(hipEventRecord(g->ev[1], st));
   HM_TRY(d.flag.ensure((size_t)std::max(E, 1)));
   HM_TRY(d.ids.ensure(sizeof(int32_t) * n));
   HM_HIP(hipMemcpyAsync(d.ids.p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
   k_fill_u8<<<grid_for(E), kBlock, 0, st>>>(d.flag.as<uint8_t>(), E, 0);
   k_set_flags<<<grid_for(n), kBlock, 0, st>>>(d.ids.as<int32_t>(), n, d.flag.as<uint8_t>());
-  const size_t lds = (((size_t)E * sizeof(MT) + 2 * (size_t)E) + 15) & ~(size_t)15;
-  HM_TRY(allow_lds((const void*)k_delta<MT, kDeltaWaves>, lds));
   // enough workgroups to give every CU several: parts per changed hyperedge
   const int32_t parts = std::max<int32_t>(1, std::min<int32_t>(64, (2048 + n - 1) / n));
-  k_delta<MT, kDeltaWaves><<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
-      E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
-      c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
-      d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ids.as<int32_t>(),
-      d.flag.as<uint8_t>(), parts, sign, d.acc.as<unsigned long long>());
+  if (T > 0) {
+    // state per (changed f, neighbour b) at the exclusive scan of the changed rows' lengths
+    HM_TRY(d.slen.ensure(sizeof(unsigned long long) * ((size_t)n + 1)));
+    HM_TRY(d.soff.ensure(sizeof(unsigned long long) * ((size_t)n + 1)));
+    k_gather_len<<<grid_for((int64_t)n + 1), kBlock, 0, st>>>(d.ids.as<int32_t>(), n, d.deg.as<unsigned long long>(),
+                                                        d.slen.as<unsigned long long>());
+    size_t tb = 0;
+    HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d.slen.as<unsigned long long>(),
+                                            d.soff.as<unsigned long long>(), n + 1, st));
+    HM_TRY(d.tmp.ensure(tb));
+    tb = d.tmp.bytes;
+    HM_HIP(hipcub::DeviceScan::ExclusiveSum(d.tmp.p, tb, d.slen.as<unsigned long long>(),
+                                            d.soff.as<unsigned long long>(), n + 1, st));
+    unsigned long long U = 0;
+    HM_HIP(hipMemcpyAsync(&U, d.soff.as<unsigned long long>() + n, 8, hipMemcpyDeviceToHost, st));
+    HM_HIP(hipStreamSynchronize(st));
+    HM_TRY(d.scur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(U, 1)));
+    HM_TRY(d.spm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
+    const size_t lds = ((size_t)T * sizeof(MT) + 15) & ~(size_t)15;
+    HM_TRY(allow_lds((const void*)k_delta_tiled<MT, kDeltaWaves>, lds));
+    k_delta_tiled<MT, kDeltaWaves><<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
+        E, T, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
+        c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
+        d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ids.as<int32_t>(),
+        d.flag.as<uint8_t>(), parts, sign, d.soff.as<unsigned long long>(), d.scur.as<uint32_t>(),
+        d.spm.as<MT>(), d.acc.as<unsigned long long>());
+  } else {
+    const size_t lds = (((size_t)E * sizeof(MT) + 2 * (size_t)E) + 15) & ~(size_t)15;
+    HM_TRY(allow_lds((const void*)k_delta<MT, kDeltaWaves>, lds));
+    k_delta<MT, kDeltaWaves><<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
+        E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
+        c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
+        d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ids.as<int32_t>(),
+        d.flag.as<uint8_t>(), parts, sign, d.acc.as<unsigned long long>());
+  }
   HM_HIP(hipGetLastError());
   HM_HIP(hipEventRecord(g->ev[2], st));
   HM_HIP(hipEventSynchronize(g->ev[2]));
@@ -391,11 +684,18 @@ static hm_status local_count_any(hm_graph* g, DynScratch& d, const int32_t* ids,
   if (g->csr.max_size > HM_MAX_EDGE_SIZE)
     return fail(HM_ERR_UNSUPPORTED, "delta count: hyperedge larger than 64 nodes");
   const bool wide = g->csr.max_size > 32;
-  if (!dense_fits(g, g->csr.E, wide)) {
-    set_error("delta count: %d hyperedge ids exceed the dense LDS table", g->csr.E);
-    return HM_ERR_UNSUPPORTED;
+  int32_t T = 0;  // one LDS table over all ids when it fits, else tiles as large as LDS allows
+  if (g->tile > 0 || !dense_fits(g, g->csr.E, wide)) {
+    int lim = 65536;
+    (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
+    const int32_t most = (int32_t)((((size_t)lim - kDeltaTiledStaticLds) / (wide ? 8 : 4)) & ~(size_t)63);
+    T = g->tile > 0 ? g->tile : most;
+    if (T > most) {
+      set_error("delta count: a tile of %d ids exceeds the LDS limit", T);
+      return HM_ERR_BAD_INPUT;
+    }
   }
-  return wide ? local_count<mask64_t>(g, d, ids, n, sign) : local_count<uint32_t>(g, d, ids, n, sign);
+  return wide ? local_count<mask64_t>(g, d, ids, n, sign, T) : local_count<uint32_t>(g, d, ids, n, sign, T);
 }
 
 // Applies the batch to the device arrays (deleted rows emptied, inserted rows appended).

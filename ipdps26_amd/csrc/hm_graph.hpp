@@ -97,10 +97,11 @@ struct DynScratch {
   DevBuf deg, roff, rows, flag, ids, acc, tmp;
   DevBuf ins_off, ins_nodes, sizes, noff, nnodes, neoff, nsize, inc_node, inc_edge, srt_node,
       srt_edge, vcnt, nvoff;
+  DevBuf slen, soff, scur, spm;  // tiled delta: per (changed f, neighbour b) read position and mask
   void release() {
     for (DevBuf* b : {&deg, &roff, &rows, &flag, &ids, &acc, &tmp, &ins_off, &ins_nodes, &sizes,
                       &noff, &nnodes, &neoff, &nsize, &inc_node, &inc_edge, &srt_node, &srt_edge,
-                      &vcnt, &nvoff})
+                      &vcnt, &nvoff, &slen, &soff, &scur, &spm})
       b->release();
   }
 };

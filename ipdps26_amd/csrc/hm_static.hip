@@ -363,24 +363,6 @@ __global__ __launch_bounds__(kBlock) void k_proj_fill_tiled(
   if (threadIdx.x == 0) rlen[ra] = s_cnt;
 }
 
-// Mask of a's nodes (s_an[0 .. sa), ascending) contained in hyperedge b.
-template <typename MT>
-__device__ inline MT pair_mask(const int32_t* s_an, int sa, const uint32_t* __restrict__ eoff,
-                               const int32_t* __restrict__ enodes, int32_t b) {
-  MT m = (MT)0;
-  for (uint32_t q = eoff[b], qe = eoff[b + 1]; q < qe; ++q) {
-    const int32_t v = enodes[q];
-    int lo = 0, hi = sa;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (s_an[mid] < v) lo = mid + 1;
-      else hi = mid;
-    }
-    if (lo < sa && s_an[lo] == v) m |= (MT)1 << lo;
-  }
-  return m;
-}
-
 // Triangles of centre rank r over rank tiles of c. For each b in r's upper row, state at
 // uoff[r] + ib keeps a's mask in b (computed once) and the read position in b's row, which is
 // grouped by tile, so the entries of b's row in the current tile are the next ones from there.

@@ -119,3 +119,79 @@ def test_update_rejects_bad_ids(gpu_ready):
             g.update([1])  # already deleted
         ids, delta = g.update([], *csr([[3, 4]]))
         assert ids.tolist() == [3]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tiled_delta_matches_oracle(gpu_ready, seed):
+    """The tiled delta (forced tiles on small graphs): every batch against oracle recounts."""
+    rng = np.random.default_rng(5600 + seed)
+    nv, kmax = int(rng.integers(10, 80)), [4, 8, 40][seed % 3]
+    tile = [1, 64, 97][seed % 3]
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(20, 250)), num_nodes=nv, kmax=kmax)
+    rows = rows_of(off, nodes)
+    live = set(range(len(rows)))
+    g = hm.Graph(off, nodes)
+    g.set_tile(tile)
+    before = O.static_mochy(*csr(rows))
+    for b in range(3):
+        dels, new_rows = random_batch(rng, rows, live, int(rng.integers(0, 15)),
+                                      int(rng.integers(0, 15)), nv, kmax)
+        ids, delta = g.update(dels, *csr(new_rows))
+        for x in dels:
+            rows[x] = []
+            live.discard(x)
+        mid = int(O.static_mochy(*csr(rows)).sum())
+        rows.extend(new_rows)
+        live.update(ids.tolist())
+        after = O.static_mochy(*csr(rows))
+        print(f"seed={seed} tile={tile} batch={b} E={len(rows)}", flush=True)
+        assert np.array_equal(delta, after - before)
+        us = g.update_stats()
+        assert (us["removed"], us["added"]) == (int(before.sum()) - mid, int(after.sum()) - mid)
+        before = after
+    g.close()
+
+
+def test_tiled_snapshot_delta(gpu_ready):
+    rng = np.random.default_rng(78)
+    off, nodes = random_hypergraph(rng, num_edges=300, num_nodes=50, kmax=7)
+    old_rows = rows_of(off, nodes)
+    dels = sorted(rng.choice(len(old_rows), size=30, replace=False).tolist())
+    new_rows = [list(r) for r in old_rows]
+    ins = dels[:10] + [len(old_rows) + k for k in range(12)]
+    for x in dels:
+        new_rows[x] = []
+    new_rows.extend([[] for _ in range(12)])
+    for x in ins:
+        new_rows[x] = list(rng.choice(50, size=int(rng.integers(1, 7)), replace=False))
+    old_rows.extend([[] for _ in range(12)])
+    with hm.Graph(*csr(old_rows)) as go, hm.Graph(*csr(new_rows)) as gn:
+        go.set_tile(50)
+        gn.set_tile(50)
+        delta = hm.count_delta(go, gn, dels, ins)
+    assert np.array_equal(delta, O.static_mochy(*csr(new_rows)) - O.static_mochy(*csr(old_rows)))
+
+
+def test_delta_beyond_one_table(gpu_ready):
+    """40,000 hyperedge ids (more than one LDS table): the automatic path tiles; batches of the
+    dynamic recipe, each delta against two recounts (the static count, itself checked against the
+    oracle at this size in test_static_tiled_gpu) and the final graph against the oracle."""
+    p = hm.GenParams(num_nodes=400_000, num_edges=40_000, alpha=2.8, kmin=2, kmax=12, zipf_s=0.7,
+                     seed=14)
+    off, nodes = hm.generate_static(p)
+    g = hm.Graph(off, nodes)
+    live = np.arange(off.size - 1, dtype=np.int64)
+    rows = rows_of(off, nodes)
+    counts = g.count_static().astype(np.int64)
+    for b in range(1, 4):
+        dels, ioff, inodes = W.dynamic_batch(live, b, p, inserts=560, deletes=240)
+        ids, delta = g.update(dels, ioff, inodes)
+        after = g.count_static().astype(np.int64)
+        assert np.array_equal(counts + delta, after), f"batch {b}"
+        counts = after
+        live = np.concatenate([np.setdiff1d(live, dels.astype(np.int64)), ids.astype(np.int64)])
+        for x in dels:
+            rows[x] = []
+        rows.extend(rows_of(ioff, inodes))
+    assert np.array_equal(counts, O.static_mochy(*csr(rows)))
+    g.close()
