@@ -376,7 +376,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": r["kernel"], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload), "kernel_ms": r["kernel_ms"],
-                     "algorithmic_bytes": r["kernel_bytes"]},
+                     "algorithmic_bytes": r["kernel_bytes"],
+                     "note": "integer kernel: the algorithmic bytes are mostly served from LDS, L2 "
+                             "and the Infinity Cache (traffic = HBM bytes actually fetched); the "
+                             "binding resource is VALU issue (profiles/counters_*.md)"},
         "cpu_baseline": r["cpu"],
     }
     print(json.dumps(line), flush=True)

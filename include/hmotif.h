@@ -262,8 +262,13 @@ hm_status hm_server_submit_count(hm_server* s, int32_t graph_id, uint64_t* ticke
 hm_status hm_server_submit_update(hm_server* s, int32_t graph_id, const int32_t* del_ids,
                                   int32_t n_del, const int64_t* ins_offsets,
                                   const int32_t* ins_nodes, int32_t n_ins, uint64_t* ticket);
-/* Blocks until the request is done; each ticket can be waited for once. */
+/* Blocks until the request is done; each ticket can be waited for (or polled to completion) once. */
 hm_status hm_server_wait(hm_server* s, uint64_t ticket, hm_result* out);
+/* Non-blocking: *done = 0 while the request runs (out untouched, HM_OK); once it is done,
+ * *done = 1 and out receives the result exactly as hm_server_wait would return it (the ticket
+ * is then consumed). The reference has no request interface; this is the hm_poll of the
+ * serving contract next to hm_server_submit_* (submit) and hm_server_wait. */
+hm_status hm_server_poll(hm_server* s, uint64_t ticket, int32_t* done, hm_result* out);
 hm_status hm_server_get_stats(hm_server* s, hm_server_stats* out);
 
 #ifdef __cplusplus

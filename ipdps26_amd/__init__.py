@@ -109,6 +109,8 @@ _SIGNATURES = [
     ("hm_server_submit_update", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, _i32p, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _u64p]),
     ("hm_server_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+    ("hm_server_poll", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint64, _P(ctypes.c_int32), ctypes.c_void_p]),
     ("hm_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("hm_tgraph_create", ctypes.c_int,
      [_i64p, _i32p, _i64p, ctypes.c_int32, _P(ctypes.c_void_p)]),
@@ -460,13 +462,22 @@ class Server:
             _ptr(o, ctypes.c_int64), _ptr(n, ctypes.c_int32), o.size - 1, ctypes.byref(t)))
         return t.value
 
-    def wait(self, ticket: int) -> dict:
-        r = _Result()
-        st = lib().hm_server_wait(self._h, ticket, ctypes.byref(r))
-        _check(st)
+    @staticmethod
+    def _result(r) -> dict:
         return {"kind": r.kind, "counts": np.array(r.counts[:], dtype=np.int64),
                 "queued_ms": r.queued_ms, "run_ms": r.run_ms, "latency_ms": r.latency_ms,
                 "first_new_id": r.first_new_id, "n_new": r.n_new}
+
+    def wait(self, ticket: int) -> dict:
+        r = _Result()
+        _check(lib().hm_server_wait(self._h, ticket, ctypes.byref(r)))
+        return self._result(r)
+
+    def poll(self, ticket: int):
+        """None while the request runs, else its result (the ticket is then consumed)."""
+        r, done = _Result(), ctypes.c_int32(0)
+        _check(lib().hm_server_poll(self._h, ticket, ctypes.byref(done), ctypes.byref(r)))
+        return self._result(r) if done.value else None
 
     def stats(self) -> dict:
         s = _ServerStats()

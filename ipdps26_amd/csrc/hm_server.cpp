@@ -268,14 +268,10 @@ hm_status hm_server_submit_update(hm_server* s, int32_t graph_id, const int32_t*
   return submit(s, r, ticket);
 }
 
-hm_status hm_server_wait(hm_server* s, uint64_t ticket, hm_result* out) {
-  hm::clear_error();
-  if (!s || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_wait: null argument");
-  std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->requests.find(ticket);
-  if (it == s->requests.end()) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_wait: unknown ticket");
+// Copies a finished request into out and forgets its ticket (s->mu held by the caller).
+static hm_status take_result(hm_server* s, std::unordered_map<uint64_t, std::shared_ptr<Request>>::iterator it,
+                             hm_result* out) {
   std::shared_ptr<Request> r = it->second;
-  s->done_cv.wait(lk, [&] { return r->done; });
   s->requests.erase(it);
   using ms = std::chrono::duration<double, std::milli>;
   out->status = r->status;
@@ -291,6 +287,28 @@ hm_status hm_server_wait(hm_server* s, uint64_t ticket, hm_result* out) {
     return r->status;
   }
   return HM_OK;
+}
+
+hm_status hm_server_wait(hm_server* s, uint64_t ticket, hm_result* out) {
+  hm::clear_error();
+  if (!s || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_wait: null argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->requests.find(ticket);
+  if (it == s->requests.end()) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_wait: unknown ticket");
+  std::shared_ptr<Request> r = it->second;
+  s->done_cv.wait(lk, [&] { return r->done; });
+  return take_result(s, s->requests.find(ticket), out);
+}
+
+hm_status hm_server_poll(hm_server* s, uint64_t ticket, int32_t* done, hm_result* out) {
+  hm::clear_error();
+  if (!s || !done || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_poll: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->requests.find(ticket);
+  if (it == s->requests.end()) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_poll: unknown ticket");
+  *done = it->second->done ? 1 : 0;
+  if (!*done) return HM_OK;
+  return take_result(s, it, out);
 }
 
 hm_status hm_server_get_stats(hm_server* s, hm_server_stats* out) {

@@ -16,6 +16,8 @@ File formats (the reference programs' own):
   MoCHy CSV  one hyperedge per line, comma-separated 0-based node ids (MoCHy-master/README.md;
              read by MoCHy-master/read_data.cpp:8-45)
   THyMe TSV  "n1,n2,...<TAB>timestamp" per line (THyMe-main/code/read_data.cpp:22-58)
+  ESCHER     whitespace-separated 1-based node ids per line, '#' comments and empty lines
+             skipped, a non-positive id is an error (dynamicHyperGraph/utils/utils.cpp:66-130)
 """
 from __future__ import annotations
 
@@ -184,3 +186,33 @@ def read_thyme_tsv(path: str) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
             times.append(int(float(t)))
     return (np.asarray(offs, dtype=np.int64), np.asarray(flat, dtype=np.int32),
             np.asarray(times, dtype=np.int64))
+
+
+def write_escher(path: str, offsets: np.ndarray, nodes: np.ndarray) -> None:
+    """ESCHER's input format: 1-based node ids, space-separated (saveHypergraphToFile)."""
+    with open(path, "w") as f:
+        for e in range(offsets.size - 1):
+            f.write(" ".join(str(int(v) + 1) for v in nodes[offsets[e]:offsets[e + 1]]))
+            f.write("\n")
+
+
+def read_escher(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """ESCHER's input format (loadHypergraphFromFile) as 0-based node ids; repeated ids inside a
+    line are kept once (the counting semantics treat hyperedges as sets)."""
+    offs = [0]
+    flat: list[int] = []
+    with open(path) as f:
+        for lineno, line in enumerate(f, 1):
+            if not line.strip() or line.startswith("#"):
+                continue
+            seen: dict[int, None] = {}
+            for tok in line.split():
+                v = int(tok)
+                if v <= 0:
+                    raise ValueError(f"{path}:{lineno}: non-positive vertex id {v}")
+                seen.setdefault(v - 1, None)
+            flat.extend(seen)
+            offs.append(len(flat))
+    if len(offs) == 1:
+        raise ValueError(f"{path}: no hyperedges")
+    return np.asarray(offs, dtype=np.int64), np.asarray(flat, dtype=np.int32)

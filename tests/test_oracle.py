@@ -218,3 +218,16 @@ def test_strict_inc_matches_brute_force(seed):
         snap += [list(rng.choice(25, size=3, replace=False)) for _ in range(int(rng.integers(0, 5)))]
         layers.append(csr(snap))
     assert np.array_equal(O.strict_inc(*layers), O.strict_inc_brute(*layers))
+
+
+def test_escher_format_round_trip(tmp_path):
+    off, nodes = csr([[0, 4, 2], [3], [1, 2]])
+    path = tmp_path / "g.txt"
+    W.write_escher(str(path), off, nodes)
+    assert path.read_text() == "1 5 3\n4\n2 3\n"
+    path.write_text("# comment\n1 5 3 5\n\n4\n2 3\n")
+    o2, n2 = W.read_escher(str(path))
+    assert o2.tolist() == [0, 3, 4, 6] and n2.tolist() == [0, 4, 2, 3, 1, 2]
+    path.write_text("1 0 2\n")
+    with pytest.raises(ValueError):
+        W.read_escher(str(path))

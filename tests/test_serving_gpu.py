@@ -60,3 +60,26 @@ def test_admission_budget(gpu_ready):
         assert srv.stats()["reserved_bytes"] == 0
         with pytest.raises(hm.HMotifError):
             srv.submit_count(gid)
+
+
+def test_poll_until_done(gpu_ready):
+    graphs = W.serving_graphs(3, lo=300, hi=600, seed=8)
+    with hm.Server(1 << 30, 2) as srv:
+        gids = [srv.add_graph(off, nodes) for off, nodes in graphs]
+        tickets = [srv.submit_count(g) for g in gids]
+        got = {}
+        import time
+        t_end = time.time() + 60
+        while len(got) < len(tickets) and time.time() < t_end:
+            for t in tickets:
+                if t not in got:
+                    r = srv.poll(t)
+                    if r is not None:
+                        got[t] = r
+            time.sleep(0.001)
+        assert len(got) == len(tickets)
+        for (off, nodes), t in zip(graphs, tickets):
+            with hm.Graph(off, nodes) as g:
+                assert np.array_equal(got[t]["counts"], g.count_static().astype(np.int64))
+        with pytest.raises(hm.HMotifError):
+            srv.poll(tickets[0])  # consumed
