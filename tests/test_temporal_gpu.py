@@ -84,3 +84,43 @@ def test_stats_and_repeat(gpu_ready):
         b = g.count(600_000_000)
     assert np.array_equal(a, b)
     assert st["instances"] == int(a.astype(np.int64).sum())
+
+
+def static_bin_of_temporal_bin():
+    """Temporal bin -> static bin: the temporal Venn id written in time order is also a static
+    Venn id, and MoCHy's numbering is invariant under relabelling the three hyperedges."""
+    ts, ss = hm.bin_table("temporal"), hm.bin_table("static")
+    out = {}
+    for vid in range(128):
+        if ts[vid] >= 0:
+            assert out.setdefault(int(ts[vid]), int(ss[vid])) == int(ss[vid])
+    assert sorted(out) == list(range(96))
+    return out
+
+
+@pytest.mark.parametrize("edges", [1500, 6000])
+def test_unbounded_window_equals_static_count(gpu_ready, edges):
+    """With a window longer than the whole time span every triple of temporal hyperedges is one
+    temporal instance, so the 96 temporal bins folded onto their static patterns must equal the
+    static count of the same hyperedge list — two independent kernels, at a size (the first 6,000
+    temporal hyperedges of the temporal workload) the CPU oracle does not finish in a test."""
+    off, nodes, times = W.temporal_c4()
+    off, nodes, times = off[: edges + 1], nodes[: off[edges]], times[:edges]
+    t = tcount(off, nodes, times, 1 << 62)
+    with hm.Graph(off, nodes) as g:
+        s = g.count_static().astype(np.int64)
+    folded = np.zeros(30, dtype=np.int64)
+    for tb, sb in static_bin_of_temporal_bin().items():
+        folded[sb] += t[tb]
+    assert np.array_equal(folded, s)
+
+
+def test_full_workload_window_monotone(gpu_ready):
+    """Full temporal workload: an instance inside a window is inside every longer window and its
+    bin does not depend on the window, so the bins grow elementwise with delta."""
+    off, nodes, times = W.temporal_c4()
+    with hm.TemporalGraph(off, nodes, times) as g:
+        short = g.count(W.TEMPORAL_C4_DELTA_US // 2).astype(np.int64)
+        full = g.count(W.TEMPORAL_C4_DELTA_US).astype(np.int64)
+    assert np.all(short <= full) and short.sum() < full.sum()
+    print(f"instances: delta/2 {short.sum()}, delta {full.sum()}", flush=True)
