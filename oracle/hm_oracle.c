@@ -22,6 +22,7 @@
  *                          here ties are broken by input order. Pinned by: golden vectors from the
  *                          reference THyMe+ binary on distinct-timestamp inputs, oracle_temporal_brute,
  *                          and closed forms (m copies of one edge inside delta -> C(m,3) in motif 96).
+ *                          An empty hyperedge is in no instance (THyMe's reader cannot read one).
  *   oracle_temporal_brute  definition: every triple of temporal hyperedges whose time span is
  *                          <= delta and with at least two overlapping pairs (copies of one node set
  *                          overlap), classified by the Venn pattern written in time order.
@@ -595,8 +596,9 @@ int oracle_thyme_temporal(int32_t E, const int64_t* off, const int32_t* nodes,
       out[temporal_bin(sb, sa, sa, cab, sa, cab, cab)] += ba;
       out[temporal_bin(sb, sb, sa, sb, cab, cab, cab)] += bb;
     }
-    /* motif 96 (main_thymeP.cpp:366-367) */
-    if (na_t >= 2) out[95] += (int64_t)(na_t - 1) * (na_t - 2) / 2;
+    /* motif 96 (main_thymeP.cpp:366-367); copies of an empty hyperedge share no node, so they
+       form no instance (THyMe's reader cannot read an empty hyperedge; here it is defined) */
+    if (sa > 0 && na_t >= 2) out[95] += (int64_t)(na_t - 1) * (na_t - 2) / 2;
   }
 #undef SNODES
 #undef SSIZE

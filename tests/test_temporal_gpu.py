@@ -124,3 +124,16 @@ def test_full_workload_window_monotone(gpu_ready):
         full = g.count(W.TEMPORAL_C4_DELTA_US).astype(np.int64)
     assert np.all(short <= full) and short.sum() < full.sum()
     print(f"instances: delta/2 {short.sum()}, delta {full.sum()}", flush=True)
+
+
+def test_empty_hyperedges_are_in_no_instance(gpu_ready):
+    rng = np.random.default_rng(4242)
+    off, nodes = random_hypergraph(rng, num_edges=120, num_nodes=30, kmax=5, dup_frac=0.3)
+    rows = [list(nodes[off[e]:off[e + 1]]) for e in range(off.size - 1)]
+    for e in rng.choice(len(rows), size=25, replace=False):
+        rows[e] = []
+    off, nodes = csr(rows)
+    times = np.cumsum(rng.integers(1, 20, size=len(rows))).astype(np.int64)
+    got = tcount(off, nodes, times, 400)
+    assert np.array_equal(got, O.temporal_brute(off, nodes, times, 400))
+    assert np.array_equal(got, O.thyme_temporal(off, nodes, times, 400))
