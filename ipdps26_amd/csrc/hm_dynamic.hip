@@ -813,9 +813,12 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
   for (size_t k = 0; k < ioff.size(); ++k) ins_off32[k] = (uint32_t)ioff[k];
   for (int32_t k = 0; k < n_ins; ++k) maxsz = std::max<int32_t>(maxsz, (int32_t)(ioff[(size_t)k + 1] - ioff[(size_t)k]));
   const int32_t newV = std::max(g->csr.V, vmax + 1);
+  const bool want = delta_out != nullptr;
+  // a delta needs masks of every changed hyperedge: refuse before the store changes
+  if (want && maxsz > HM_MAX_EDGE_SIZE)
+    return fail(HM_ERR_UNSUPPORTED, "hm_graph_update: a delta needs hyperedges of at most 64 nodes");
   HM_HIP(hipSetDevice(g->device));
   DynScratch& d = dyn_scratch(g);
-  const bool want = delta_out != nullptr;
   g->ms_upd_rows = g->ms_upd_delta = 0.f;
   g->upd_removed = g->upd_added = g->upd_entries = -1;
   if (want) {

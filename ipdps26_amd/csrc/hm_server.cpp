@@ -74,7 +74,9 @@ struct hm_server {
   int device = 0;
   std::mutex mu;
   std::condition_variable work_cv, done_cv;
-  std::vector<GraphSlot> graphs;
+  // a deque: workers and hm_server_remove_graph keep references to a slot while the lock is
+  // released, and adding a graph must not move the others
+  std::deque<GraphSlot> graphs;
   std::deque<int32_t> ready;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> requests;
   std::vector<std::thread> workers;

@@ -83,3 +83,25 @@ def test_poll_until_done(gpu_ready):
                 assert np.array_equal(got[t]["counts"], g.count_static().astype(np.int64))
         with pytest.raises(hm.HMotifError):
             srv.poll(tickets[0])  # consumed
+
+
+def test_add_and_remove_graphs_while_requests_run(gpu_ready):
+    graphs = W.serving_graphs(12, lo=300, hi=1200, seed=9)
+    with hm.Server(1 << 30, 4) as srv:
+        first = srv.add_graph(*graphs[0])
+        tickets = [srv.submit_count(first) for _ in range(6)]
+        more = [srv.add_graph(off, nodes) for off, nodes in graphs[1:]]  # while those run
+        tickets += [srv.submit_count(g) for g in more]
+        results = [srv.wait(t)["counts"] for t in tickets]
+        for g in more[:5]:
+            srv.remove_graph(g)
+        extra = srv.wait(srv.submit_count(more[5]))["counts"]
+    expect = []
+    for off, nodes in graphs:
+        with hm.Graph(off, nodes) as g:
+            expect.append(g.count_static().astype(np.int64))
+    for r in results[:6]:
+        assert np.array_equal(r, expect[0])
+    for k, r in enumerate(results[6:]):
+        assert np.array_equal(r, expect[k + 1])
+    assert np.array_equal(extra, expect[6])
