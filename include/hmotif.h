@@ -203,6 +203,13 @@ hm_status hm_tgraph_destroy(hm_tgraph* g);
  * distinct (with ties THyMe's result depends on random tie-breaking; here ties follow input order). */
 hm_status hm_count_temporal(hm_tgraph* g, int64_t delta, uint64_t out[HM_TEMPORAL_BINS]);
 
+/* Position tile of hm_count_temporal on g. 0 (the default): one LDS table per δ-window when it
+ * fits (windows up to ~19k temporal hyperedges), position tiles as large as LDS allows otherwise;
+ * > 0: tiles of exactly tile_positions positions forced at any window size (tests and tuning).
+ * The counts do not depend on the tile. No counterpart in the reference (THyMe+ keeps the window
+ * in host hash maps, THyMe-main/code/main_thymeP.cpp:54-115). */
+hm_status hm_tgraph_set_tile(hm_tgraph* g, int32_t tile_positions);
+
 typedef struct hm_temporal_stats {
   int64_t num_edges;        /* temporal hyperedges */
   int64_t num_incidences;

@@ -94,6 +94,7 @@ _SIGNATURES = [
     ("hm_graph_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(_GraphStats)]),
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
     ("hm_graph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("hm_tgraph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_update_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("hm_count_strict_inc", ctypes.c_int,
      [ctypes.c_void_p * 3, ctypes.c_void_p * 3, ctypes.c_int32 * 3, _P(ctypes.c_uint64)]),
@@ -385,6 +386,10 @@ class TemporalGraph:
         st = _TemporalStats()
         _check(lib().hm_tgraph_get_stats(self._h, ctypes.byref(st)))
         return {name: getattr(st, name) for name, _ in _TemporalStats._fields_}
+
+    def set_tile(self, tile_positions: int) -> None:
+        """Position tile of count: 0 = automatic, > 0 = forced tiles of that many positions."""
+        _check(lib().hm_tgraph_set_tile(self._h, int(tile_positions)))
 
     def phase_ms(self, phase: str) -> float:
         ms = ctypes.c_double(0.0)

@@ -205,13 +205,14 @@ __device__ inline int as_open_id(int fx, int fy, int cxi, int cyi, int si) {
   return 48 + fx + 2 * fy + 4 * (int)(cxi + cyi < si);
 }
 
-// Counts, for every ordered pair (x earlier, y later) of entries of NL, the as-if-open id
-// (8 cases: fx, fy, [C_xi + C_yi < si]) with one wave. The running class histogram of earlier
-// entries is kept across lanes (lane c holds the number of earlier entries with C = c), and
-// earlier entries of the same 64-entry chunk are counted with one ballot per (f, C) class.
-template <typename MT>
-__device__ inline void ordered_open_pairs(const MT* M, const uint16_t* NL, const uint8_t* sz,
-                                          uint32_t nN, int si, unsigned long long cnt[8]) {
+// Counts, for every ordered pair (x earlier, y later) of the nN neighbours of the anchor (in time
+// order), the as-if-open id (8 cases: fx, fy, [C_xi + C_yi < si]) with one wave. load(k, C, f)
+// gives neighbour k's overlap C with the anchor and f = [it has a node outside the anchor]. The
+// running class histogram of earlier entries is kept across lanes (lane c holds the number of
+// earlier entries with C = c + 1), and earlier entries of the same 64-entry chunk are counted
+// with one ballot per (f, C) class.
+template <typename Load>
+__device__ inline void ordered_open_pairs(uint32_t nN, int si, Load&& load, unsigned long long cnt[8]) {
   const int lane = lane_id();
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   unsigned long long R0 = 0, R1 = 0;  // earlier entries with f = 0 / 1 and C == lane + 1
@@ -220,11 +221,7 @@ __device__ inline void ordered_open_pairs(const MT* M, const uint16_t* NL, const
     const uint32_t k = c0 + lane;
     const bool valid = k < nN;
     int C = 0, f = 0;
-    if (valid) {
-      const int yr = NL[k];
-      C = popc(M[yr]);
-      f = (int)sz[yr] > C ? 1 : 0;
-    }
+    if (valid) load(k, C, f);
     const int T = si - C;  // earlier x count towards [C_x + C < si] iff C_x < T
     // earlier chunks: prefix over the lane-distributed histograms
     unsigned long long P0 = R0, P1 = R1;  // inclusive prefix over lanes 0..lane
@@ -333,7 +330,11 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   // back below when their triangle is met)
   if (w == NW - 1) {
     unsigned long long cnt[8];
-    ordered_open_pairs<MT>(M, NL, sz, nN, si, cnt);
+    ordered_open_pairs(nN, si, [&](uint32_t k, int& C, int& f) {
+      const int yr = NL[k];
+      C = popc(M[yr]);
+      f = (int)sz[yr] > C ? 1 : 0;
+    }, cnt);
     if (lane == 0)
       for (int k = 0; k < 8; ++k)
         if (cnt[k]) atomicAdd(&acc[kTAccHist + tbl[48 + k]], cnt[k]);
@@ -417,6 +418,281 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Tiled kernels: windows longer than one LDS table. Positions are cut into tiles of T positions
+// (relative to the window start); a table is built per tile from the part of each node's position
+// list inside it (tile_ranges / for_tile_entries, positions ascending per node).
+// ---------------------------------------------------------------------------------------------
+
+// fdeg / bdeg of x over position tiles of [lo_x, hi_x].
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_tproj_count_tiled(
+    int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vpos,
+    const int32_t* __restrict__ lo, const int32_t* __restrict__ hi,
+    unsigned long long* __restrict__ fdeg, unsigned long long* __restrict__ bdeg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ TileRanges tr;
+  __shared__ uint32_t red[2][kWavesPerBlock];
+  const int32_t x = blockIdx.x;
+  const uint32_t s = eoff[x];
+  const int sx = (int)(eoff[x + 1] - s);
+  for (int32_t k = threadIdx.x; k < T; k += blockDim.x) M[k] = (MT)0;
+  uint32_t f = 0, b = 0;
+  for (int32_t t0 = lo[x]; t0 <= hi[x]; t0 += T) {
+    const int32_t t1 = min(hi[x] + 1, t0 + T);
+    const uint32_t total = tile_ranges(tr, sx, enodes + s, voff, vpos, t0, t1);
+    if (total == 0) continue;
+    for_tile_entries(tr, sx, total, vpos, [&](int k, int32_t y) {
+      if (y != x) atomicOr(&M[y - t0], (MT)1 << k);
+    });
+    __syncthreads();
+    for_tile_entries(tr, sx, total, vpos, [&](int k, int32_t y) {
+      if (y == x) return;
+      const MT m = M[y - t0];
+      if (m && low_bit(m) == k) {
+        if (y > x) ++f;
+        else ++b;
+        M[y - t0] = (MT)0;
+      }
+    });
+  }
+  f = wave_sum(f);
+  b = wave_sum(b);
+  if (lane_id() == 0) {
+    red[0][wave_id()] = f;
+    red[1][wave_id()] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ff = 0, bb = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+      ff += red[0][w];
+      bb += red[1][w];
+    }
+    fdeg[x] = ff;
+    bdeg[x] = bb;
+  }
+}
+
+// Compacts positions y in [from, to] with M[y - base] != 0 into out[] (ascending), and their masks
+// into outm[] when it is not null. Returns the number written (uniform).
+template <typename MT>
+__device__ inline uint32_t compact_rows_masked(const MT* M, int32_t base, int32_t from, int32_t to,
+                                               uint32_t* __restrict__ out, MT* __restrict__ outm,
+                                               uint32_t* wcnt) {
+  const int lane = lane_id(), w = wave_id();
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t running = 0;
+  for (int32_t y0 = from; y0 <= to; y0 += blockDim.x) {
+    const int32_t y = y0 + (int32_t)threadIdx.x;
+    const MT m = (y <= to) ? M[y - base] : (MT)0;
+    const unsigned long long bal = __ballot(m != (MT)0);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int q = 0; q < kWavesPerBlock; ++q) {
+      woff += q < w ? wcnt[q] : 0;
+      tot += wcnt[q];
+    }
+    if (m) {
+      const uint32_t pos = running + woff + (uint32_t)__popcll(bal & below);
+      out[pos] = ((uint32_t)y << kCntBits) | (uint32_t)popc(m);
+      if (outm) outm[pos] = m;
+    }
+    running += tot;
+    __syncthreads();
+  }
+  return running;
+}
+
+// F(x) and B(x) over position tiles (ascending), and bmask = x's nodes contained in each B(x)
+// entry (the masks the tiled counting kernel needs for the anchor's neighbours).
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_tproj_fill_tiled(
+    int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vpos,
+    const int32_t* __restrict__ lo, const int32_t* __restrict__ hi,
+    const unsigned long long* __restrict__ foff, const unsigned long long* __restrict__ boff,
+    uint32_t* __restrict__ frows, uint32_t* __restrict__ brows, MT* __restrict__ bmask) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ TileRanges tr;
+  __shared__ uint32_t wcnt[kWavesPerBlock];
+  const int32_t x = blockIdx.x;
+  const uint32_t s = eoff[x];
+  const int sx = (int)(eoff[x + 1] - s);
+  for (int32_t k = threadIdx.x; k < T; k += blockDim.x) M[k] = (MT)0;
+  uint32_t* F = frows + foff[x];
+  uint32_t* B = brows + boff[x];
+  MT* BM = bmask + boff[x];
+  uint32_t nf = 0, nb = 0;
+  for (int32_t t0 = lo[x]; t0 <= hi[x]; t0 += T) {
+    const int32_t t1 = min(hi[x] + 1, t0 + T);
+    const uint32_t total = tile_ranges(tr, sx, enodes + s, voff, vpos, t0, t1);
+    if (total == 0) continue;
+    for_tile_entries(tr, sx, total, vpos, [&](int k, int32_t y) {
+      if (y != x) atomicOr(&M[y - t0], (MT)1 << k);
+    });
+    __syncthreads();
+    nb += compact_rows_masked<MT>(M, t0, t0, min(t1, x) - 1, B + nb, BM + nb, wcnt);
+    nf += compact_rows_masked<MT>(M, t0, max(t0, x + 1), t1 - 1, F + nf, (MT*)nullptr, wcnt);
+    for_tile_entries(tr, sx, total, vpos, [&](int, int32_t y) {
+      if (y != x) M[y - t0] = (MT)0;
+    });
+  }
+}
+
+// Counting over position tiles of the window [lo_i, i). The anchor's neighbours are its
+// backward row B(i) (positions ascending, with C_xi) and their masks bmask; the open-at-anchor
+// pairs come from them directly. For each tile a table of the anchor's masks over the tile's
+// positions answers "is y a neighbour of i" for the entries of F(x) and B(x) inside the tile;
+// per (i, x) read positions (fcur, bcur, aligned with B(i)) carry each row from tile to tile.
+template <typename MT, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
+    int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
+    const int32_t* __restrict__ vpos, const int32_t* __restrict__ lo,
+    const unsigned long long* __restrict__ foff, const unsigned long long* __restrict__ fdeg,
+    const uint32_t* __restrict__ frows, const unsigned long long* __restrict__ boff,
+    const unsigned long long* __restrict__ bdeg, const uint32_t* __restrict__ brows,
+    const MT* __restrict__ bmask, uint32_t* __restrict__ fcur, uint32_t* __restrict__ bcur,
+    const int32_t* __restrict__ order, unsigned long long* __restrict__ acc) {
+  constexpr int kC = 4;
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ unsigned int hist[NW][kTemporalBins * kC];
+  __shared__ int8_t tbl[128];
+  __shared__ TileRanges tr;
+  const int32_t i = order[blockIdx.x];
+  const int32_t base = lo[i];
+  const uint32_t nN = (uint32_t)bdeg[i];
+  if (i - base < 2 || nN == 0) return;
+  for (int k = threadIdx.x; k < NW * kTemporalBins * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
+  if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
+  for (int32_t k = threadIdx.x; k < T; k += blockDim.x) M[k] = (MT)0;
+  const uint32_t s = eoff[i];
+  const int si = (int)(eoff[i + 1] - s);
+  const uint32_t* NL = brows + boff[i];
+  const MT* NM = bmask + boff[i];
+  uint32_t* FC = fcur + boff[i];
+  uint32_t* BC = bcur + boff[i];
+  const int lane = lane_id(), w = wave_id();
+  __syncthreads();  // tbl before the open pairs
+
+  // open at i, counted for all ordered neighbour pairs as if disjoint
+  if (w == NW - 1) {
+    unsigned long long cnt[8];
+    ordered_open_pairs(nN, si, [&](uint32_t k, int& C, int& f) {
+      const uint32_t e = NL[k];
+      C = (int)(e & kCntMask);
+      f = (int)esize[e >> kCntBits] > C ? 1 : 0;
+    }, cnt);
+    if (lane == 0)
+      for (int k = 0; k < 8; ++k)
+        if (cnt[k]) atomicAdd(&acc[kTAccHist + tbl[48 + k]], cnt[k]);
+  }
+
+  unsigned int* hw = hist[w];
+  const int cp = lane & (kC - 1);
+  for (int32_t t0 = base; t0 < i; t0 += T) {
+    const int32_t t1 = min(i, t0 + T);
+    const uint32_t total = tile_ranges(tr, si, enodes + s, voff, vpos, t0, t1);
+    for_tile_entries(tr, si, total, vpos, [&](int k, int32_t y) { atomicOr(&M[y - t0], (MT)1 << k); });
+    __syncthreads();
+    for (uint32_t xi = (uint32_t)w; xi < nN; xi += NW) {
+      const uint32_t ex = NL[xi];
+      const int32_t x = (int32_t)(ex >> kCntBits);
+      const int cxi = (int)(ex & kCntMask);
+      const MT mx = NM[xi];
+      const int sx = esize[x];
+      const int fx = sx > cxi ? 1 : 0;
+      // (b) forward row of x inside [max(x + 1, t0), t1): closed triangles and triples open at x
+      // (the read position starts at 0 in the first tile, also for an x beyond that tile)
+      uint32_t fpos = (t0 == base) ? 0u : FC[xi];
+      if (x + 1 < t1) {
+        const uint32_t* row = frows + foff[x];
+        const uint32_t n = (uint32_t)fdeg[x];
+        uint32_t& pos = fpos;
+        for (;;) {
+          const uint32_t j = pos + lane;
+          bool v = false;
+          uint32_t e = 0;
+          if (j < n) {
+            e = row[j];
+            v = (int32_t)(e >> kCntBits) < t1;
+          }
+          if (v) {
+            const int32_t y = (int32_t)(e >> kCntBits);
+            const int cxy = (int)(e & kCntMask);
+            const MT my = M[y - t0];
+            const int sy = esize[y];
+            if (my) {
+              const int cyi = popc(my);
+              atomicAdd(&hw[tbl[venn_id(sx, sy, si, cxy, cyi, cxi, popc((MT)(mx & my)))] * kC + cp], 1u);
+              atomicAdd(&hw[tbl[as_open_id(fx, sy > cyi ? 1 : 0, cxi, cyi, si)] * kC + cp], ~0u);
+            } else {
+              atomicAdd(&hw[tbl[venn_id(sx, sy, si, cxy, 0, cxi, 0)] * kC + cp], 1u);
+            }
+          }
+          const uint32_t nv = (uint32_t)__popcll(__ballot(v));
+          pos += nv;
+          if (nv < (uint32_t)kWave) break;
+        }
+      }
+      if (lane == 0) FC[xi] = fpos;
+      // (e) backward row of x inside [max(base, t0), min(x, t1)): triples (y, x, i) open at x
+      if (t0 < x) {
+        const uint32_t* row = brows + boff[x];
+        const uint32_t n = (uint32_t)bdeg[x];
+        uint32_t pos;
+        if (t0 == base) {  // first entry inside the window
+          uint32_t l = 0, h = n;
+          while (l < h) {
+            const uint32_t mid = (l + h) >> 1;
+            if ((int32_t)(row[mid] >> kCntBits) < base) l = mid + 1;
+            else h = mid;
+          }
+          pos = l;
+        } else {
+          pos = BC[xi];
+        }
+        const int32_t stop = min(x, t1);
+        for (;;) {
+          const uint32_t j = pos + lane;
+          bool v = false;
+          uint32_t e = 0;
+          if (j < n) {
+            e = row[j];
+            v = (int32_t)(e >> kCntBits) < stop;
+          }
+          if (v) {
+            const int32_t y = (int32_t)(e >> kCntBits);
+            if (M[y - t0] == (MT)0)
+              atomicAdd(&hw[tbl[venn_id(esize[y], sx, si, (int)(e & kCntMask), cxi, 0, 0)] * kC + cp], 1u);
+          }
+          const uint32_t nv = (uint32_t)__popcll(__ballot(v));
+          pos += nv;
+          if (nv < (uint32_t)kWave) break;
+        }
+        if (lane == 0) BC[xi] = pos;
+      }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x) {  // flush this tile
+      long long sum = 0;
+      for (int q = 0; q < NW; ++q)
+        for (int c = 0; c < kC; ++c) {
+          sum += (long long)(int)hist[q][b * kC + c];
+          hist[q][b * kC + c] = 0;
+        }
+      if (sum) atomicAdd(&acc[kTAccHist + b], (unsigned long long)sum);
+    }
+    for_tile_entries(tr, si, total, vpos, [&](int, int32_t y) { M[y - t0] = (MT)0; });
+  }
+}
+
 // Per-anchor work estimate (for launch order) and statistics.
 __global__ void k_tstats(int32_t E, const int32_t* __restrict__ lo,
                          const unsigned long long* __restrict__ fdeg,
@@ -482,6 +758,9 @@ struct hm_tgraph {
   hm::DevBuf eoff, enodes, esize, voff, vpos, times;
   // scratch
   hm::DevBuf lo, hi, fdeg, bdeg, foff, boff, frows, brows, work, work2, ids, order, acc, tmp;
+  hm::DevBuf bmask, fcur, bcur;  // tiled windows: anchor masks and per-(anchor, neighbour) read positions
+  int32_t tile = 0;              // 0 = automatic, > 0 = forced position tiles
+  int32_t last_tile = 0;         // tile of the last count (0 = one table per window)
   std::vector<int32_t> order_to_input;  // position -> input index
   hm_temporal_stats stats{};
   bool have_stats = false;
@@ -498,6 +777,10 @@ static size_t win_lds_bytes(int32_t R, bool wide) {
 // for the histogram. Measured on the 200k-edge temporal workload (c4): 4 waves with wave-aggregated
 // histograms 7.04 s, 8 waves aggregated 3.74 s, 8 waves atomics 2.46 s, 16 waves atomics 2.12 s.
 constexpr int kTCountWaves = 16;
+// Static LDS of the tiled kernels (tile walk ranges, reductions; for counting also the 24.6 KB
+// histogram and the bin table), kept free next to the position table.
+constexpr size_t kTProjTiledStaticLds = 4 * 1024;
+constexpr size_t kTCountTiledStaticLds = 28 * 1024;
 
 template <typename MT>
 static hm_status launch_tcount(hm_tgraph* g, int32_t Wmax, size_t lds, unsigned long long* acc) {
@@ -550,18 +833,33 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
   const size_t lds_proj = win_lds_bytes(Rmax, sizeof(MT) == 8);
   const size_t lds_cnt = TCountLds<MT>::bytes(Wmax);
   const size_t static_cnt = sizeof(uint32_t) * 16 * kTemporalBins * 4 + 256;
-  if (lds_proj > (size_t)lim || lds_cnt + static_cnt > (size_t)lim || Wmax > 65535) {
-    set_error("temporal count: a delta window of %d hyperedges exceeds the LDS tables", Wmax);
-    return HM_ERR_UNSUPPORTED;
+  // one LDS table per window when it fits; position tiles otherwise (or when forced)
+  const bool tiled = g->tile > 0 || lds_proj > (size_t)lim || lds_cnt + static_cnt > (size_t)lim ||
+                     Wmax > 65535;
+  int32_t Tp = 0, Tc = 0;
+  if (tiled) {
+    const int32_t most_p = (int32_t)((((size_t)lim - kTProjTiledStaticLds) / sizeof(MT)) & ~(size_t)63);
+    const int32_t most_c = (int32_t)((((size_t)lim - kTCountTiledStaticLds) / sizeof(MT)) & ~(size_t)63);
+    Tp = g->tile > 0 ? g->tile : most_p;
+    Tc = g->tile > 0 ? g->tile : most_c;
+    if (Tp > most_p || Tc > most_c) {
+      set_error("temporal count: a tile of %d positions exceeds the LDS limit", g->tile);
+      return HM_ERR_BAD_INPUT;
+    }
+    HM_TRY(allow_lds((const void*)k_tproj_count_tiled<MT>, sizeof(MT) * (size_t)Tp));
+    HM_TRY(allow_lds((const void*)k_tproj_fill_tiled<MT>, sizeof(MT) * (size_t)Tp));
+    k_tproj_count_tiled<MT><<<E, kBlock, sizeof(MT) * (size_t)Tp, st>>>(
+        Tp, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
+        g->vpos.as<int32_t>(), g->lo.as<int32_t>(), g->hi.as<int32_t>(),
+        g->fdeg.as<unsigned long long>(), g->bdeg.as<unsigned long long>());
+  } else {
+    HM_TRY(allow_lds((const void*)k_tproj_count<MT>, lds_proj));
+    HM_TRY(allow_lds((const void*)k_tproj_fill<MT>, lds_proj));
+    k_tproj_count<MT><<<E, kBlock, lds_proj, st>>>(
+        E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
+        g->vpos.as<int32_t>(), g->lo.as<int32_t>(), g->hi.as<int32_t>(),
+        g->fdeg.as<unsigned long long>(), g->bdeg.as<unsigned long long>());
   }
-  HM_TRY(allow_lds((const void*)k_tproj_count<MT>, lds_proj));
-  HM_TRY(allow_lds((const void*)k_tproj_fill<MT>, lds_proj));
-
-
-  k_tproj_count<MT><<<E, kBlock, lds_proj, st>>>(
-      E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
-      g->vpos.as<int32_t>(), g->lo.as<int32_t>(), g->hi.as<int32_t>(),
-      g->fdeg.as<unsigned long long>(), g->bdeg.as<unsigned long long>());
   HM_HIP(hipGetLastError());
   size_t tb1 = 0, tb2 = 0;
   HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb1, g->fdeg.as<unsigned long long>(),
@@ -585,11 +883,23 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
   HM_HIP(hipStreamSynchronize(st));
   HM_TRY(g->frows.ensure(sizeof(uint32_t) * std::max<unsigned long long>(tot[0], 1)));
   HM_TRY(g->brows.ensure(sizeof(uint32_t) * std::max<unsigned long long>(tot[1], 1)));
-  k_tproj_fill<MT><<<E, kBlock, lds_proj, st>>>(
-      E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
-      g->vpos.as<int32_t>(), g->lo.as<int32_t>(), g->hi.as<int32_t>(),
-      g->foff.as<unsigned long long>(), g->boff.as<unsigned long long>(),
-      g->frows.as<uint32_t>(), g->brows.as<uint32_t>());
+  if (tiled) {
+    const size_t nb = (size_t)std::max<unsigned long long>(tot[1], 1);
+    HM_TRY(g->bmask.ensure(sizeof(MT) * nb));
+    HM_TRY(g->fcur.ensure(sizeof(uint32_t) * nb));
+    HM_TRY(g->bcur.ensure(sizeof(uint32_t) * nb));
+    k_tproj_fill_tiled<MT><<<E, kBlock, sizeof(MT) * (size_t)Tp, st>>>(
+        Tp, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
+        g->vpos.as<int32_t>(), g->lo.as<int32_t>(), g->hi.as<int32_t>(),
+        g->foff.as<unsigned long long>(), g->boff.as<unsigned long long>(),
+        g->frows.as<uint32_t>(), g->brows.as<uint32_t>(), g->bmask.as<MT>());
+  } else {
+    k_tproj_fill<MT><<<E, kBlock, lds_proj, st>>>(
+        E, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
+        g->vpos.as<int32_t>(), g->lo.as<int32_t>(), g->hi.as<int32_t>(),
+        g->foff.as<unsigned long long>(), g->boff.as<unsigned long long>(),
+        g->frows.as<uint32_t>(), g->brows.as<uint32_t>());
+  }
   HM_HIP(hipGetLastError());
   // anchor order: most work first
   k_tstats<<<64, kBlock, 0, st>>>(E, g->lo.as<int32_t>(), g->fdeg.as<unsigned long long>(),
@@ -604,7 +914,19 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
       E, g->foff.as<unsigned long long>(), g->fdeg.as<unsigned long long>(),
       g->bdeg.as<unsigned long long>(), g->frows.as<uint32_t>(), acc);
   HM_HIP(hipEventRecord(g->ev[1], st));
-  HM_TRY(launch_tcount<MT>(g, Wmax, lds_cnt, acc));
+  if (tiled) {
+    HM_TRY(allow_lds((const void*)k_tcount_tiled<MT, kTCountWaves>, sizeof(MT) * (size_t)Tc));
+    k_tcount_tiled<MT, kTCountWaves><<<E, kTCountWaves * kWave, sizeof(MT) * (size_t)Tc, st>>>(
+        Tc, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->esize.as<uint8_t>(),
+        g->voff.as<uint32_t>(), g->vpos.as<int32_t>(), g->lo.as<int32_t>(),
+        g->foff.as<unsigned long long>(), g->fdeg.as<unsigned long long>(), g->frows.as<uint32_t>(),
+        g->boff.as<unsigned long long>(), g->bdeg.as<unsigned long long>(), g->brows.as<uint32_t>(),
+        g->bmask.as<MT>(), g->fcur.as<uint32_t>(), g->bcur.as<uint32_t>(), g->order.as<int32_t>(), acc);
+    HM_HIP(hipGetLastError());
+  } else {
+    HM_TRY(launch_tcount<MT>(g, Wmax, lds_cnt, acc));
+  }
+  g->last_tile = tiled ? Tc : 0;
   HM_HIP(hipEventRecord(g->ev[2], st));
   HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kTAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
@@ -704,7 +1026,8 @@ hm_status hm_tgraph_destroy(hm_tgraph* g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   for (hm::DevBuf* b : {&g->eoff, &g->enodes, &g->esize, &g->voff, &g->vpos, &g->times, &g->lo,
                         &g->hi, &g->fdeg, &g->bdeg, &g->foff, &g->boff, &g->frows, &g->brows,
-                        &g->work, &g->work2, &g->ids, &g->order, &g->acc, &g->tmp})
+                        &g->work, &g->work2, &g->ids, &g->order, &g->acc, &g->tmp, &g->bmask,
+                        &g->fcur, &g->bcur})
     b->release();
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);
@@ -739,6 +1062,14 @@ hm_status hm_count_temporal(hm_tgraph* g, int64_t delta, uint64_t out[HM_TEMPORA
   for (int k = 0; k < HM_TEMPORAL_BINS; ++k) tot += (int64_t)out[k];
   s.instances = tot;
   g->have_stats = true;
+  return HM_OK;
+}
+
+hm_status hm_tgraph_set_tile(hm_tgraph* g, int32_t tile_positions) {
+  hm::clear_error();
+  if (!g) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_set_tile: null handle");
+  if (tile_positions < 0) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_set_tile: negative tile");
+  g->tile = tile_positions;
   return HM_OK;
 }
 

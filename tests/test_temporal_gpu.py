@@ -46,15 +46,19 @@ def test_tiny_matches_brute_force(gpu_ready, seed):
     assert np.array_equal(tcount(off, nodes, times, 25), O.temporal_brute(off, nodes, times, 25))
 
 
-@pytest.mark.parametrize("name", [g["name"] for g in load_golden("temporal")])
-def test_reference_goldens(gpu_ready, name):
-    g = [x for x in load_golden("temporal") if x["name"] == name][0]
+def golden_temporal_input(g):
     if g["kind"] == "fixture":
         off, nodes, _ = W.read_thyme_tsv(os.path.join(GOLDEN, g["file"]))
         times = (np.arange(off.size - 1, dtype=np.int64) + 1) * 1000
-    else:
-        p = g["params"]
-        off, nodes, times = W.temporal_c4(p["num_edges"], p["num_nodes"])
+        return off, nodes, times
+    p = g["params"]
+    return W.temporal_c4(p["num_edges"], p["num_nodes"])
+
+
+@pytest.mark.parametrize("name", [g["name"] for g in load_golden("temporal")])
+def test_reference_goldens(gpu_ready, name):
+    g = [x for x in load_golden("temporal") if x["name"] == name][0]
+    off, nodes, times = golden_temporal_input(g)
     print(f"golden={name} E={off.size - 1}", flush=True)
     assert tcount(off, nodes, times, g["delta"]).tolist() == g["counts96"]
 
@@ -137,3 +141,60 @@ def test_empty_hyperedges_are_in_no_instance(gpu_ready):
     got = tcount(off, nodes, times, 400)
     assert np.array_equal(got, O.temporal_brute(off, nodes, times, 400))
     assert np.array_equal(got, O.thyme_temporal(off, nodes, times, 400))
+
+
+# ---- position tiles (windows longer than one LDS table) -----------------------------------------
+def tcount_tiled(off, nodes, times, delta, tile):
+    with hm.TemporalGraph(off, nodes, times) as g:
+        g.set_tile(tile)
+        return g.count(delta).astype(np.int64)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_forced_tiles_match_oracle(gpu_ready, seed):
+    rng = np.random.default_rng(3300 + seed)
+    kmax = [3, 6, 12, 40][seed % 4]
+    tile = [1, 7, 64, 97][(seed // 2) % 4]
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(3, 500)),
+                                   num_nodes=int(rng.integers(kmax, 120)), kmax=kmax, dup_frac=0.25)
+    E = off.size - 1
+    times = np.cumsum(rng.integers(1, 50, size=E)).astype(np.int64)
+    perm = rng.permutation(E)
+    off2, nodes2 = csr([nodes[off[e]:off[e + 1]] for e in perm])
+    times2 = times[perm]
+    delta = int(rng.integers(10, 3000))
+    print(f"seed={seed} E={E} delta={delta} tile={tile}", flush=True)
+    assert np.array_equal(tcount_tiled(off2, nodes2, times2, delta, tile),
+                          O.thyme_temporal(off2, nodes2, times2, delta))
+
+
+@pytest.mark.parametrize("name", [g["name"] for g in load_golden("temporal")])
+def test_reference_goldens_with_tiles(gpu_ready, name):
+    g = [x for x in load_golden("temporal") if x["name"] == name][0]
+    off, nodes, times = golden_temporal_input(g)
+    got = tcount_tiled(off, nodes, times, g["delta"], 300)
+    assert got.tolist() == g["counts96"]
+
+
+def test_full_workload_tile_independent(gpu_ready):
+    off, nodes, times = W.temporal_c4()
+    one = tcount(off, nodes, times, W.TEMPORAL_C4_DELTA_US)
+    tiled = tcount_tiled(off, nodes, times, W.TEMPORAL_C4_DELTA_US, 2048)  # ~3 tiles per window
+    assert np.array_equal(one, tiled)
+
+
+def test_window_beyond_one_table(gpu_ready):
+    """40,000 temporal hyperedges with an unbounded window: windows up to 40,000 positions, more
+    than one LDS table holds, so the automatic path tiles; folded onto static patterns the bins
+    must equal the static count of the same hyperedge list."""
+    off, nodes, times = W.temporal_c4(40_000, 100_000)
+    t = tcount(off, nodes, times, 1 << 62)
+    with hm.TemporalGraph(off, nodes, times) as g:
+        g.count(1 << 62)
+        assert g.stats()["max_window"] == off.size - 2
+    with hm.Graph(off, nodes) as g:
+        s = g.count_static().astype(np.int64)
+    folded = np.zeros(30, dtype=np.int64)
+    for tb, sb in static_bin_of_temporal_bin().items():
+        folded[sb] += t[tb]
+    assert np.array_equal(folded, s)
