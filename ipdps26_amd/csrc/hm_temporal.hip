@@ -19,11 +19,14 @@
 // Counting kernel (one workgroup per anchor i; every instance is counted at its newest edge i):
 //   M[y] = bitmask of i's nodes contained in y, for y in the window (LDS, dense over the window);
 //   NL = window positions with M != 0 (neighbours of i), ascending.
-//   For every x in NL (one wave per x):
-//     (b) y in F(x), y < i:   record C_xy in the wave's table; if y is not a neighbour of i the
-//                             triple (x, y, i) is open at x.
-//     (d) y in NL, y > x:     triple (x, y, i) with C_xy from the table (closed, or open at i).
+//   open at i:  every ordered pair (x, y) of NL counted as if x ∩ y = ∅, from class histograms
+//               (ordered_open_pairs: C with i, [node outside i], [C_x + C_y < |i|]);
+//   for every x in NL (one wave per x):
+//     (b) y in F(x), y < i:   y a neighbour of i: the closed triangle (x, y, i) +1 in its bin and
+//                             -1 in the as-if-open bin counted above; else (x, y, i) is open at x;
 //     (e) y in B(x), y >= lo_i, y not a neighbour of i: triple (y, x, i), open at x.
+// Windows longer than one LDS table use the tiled kernels below (position tiles, the anchor's
+// neighbours read from its own backward row).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
