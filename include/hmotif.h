@@ -236,6 +236,12 @@ typedef struct hm_server hm_server;
 typedef struct hm_server_config {
   int64_t hbm_budget_bytes; /* graphs are admitted while their estimated footprint fits */
   int32_t slots;            /* requests running at once (worker threads, one stream per graph) */
+  /* 0: every count request counts the graph on the device.
+   * 1: after a graph's first count, its 30 counts are kept current by adding the exact delta of
+   *    each later update (the delta every update computes anyway), and a count request on a
+   *    graph with no request ahead of it is answered from them without a launch. The counts are
+   *    identical either way; an update that fails drops the kept counts until the next count. */
+  int32_t maintain_counts;
 } hm_server_config;
 
 typedef struct hm_result {
@@ -254,7 +260,8 @@ typedef struct hm_server_stats {
   int64_t reserved_bytes;
   int64_t budget_bytes;
   int64_t completed;
-  int64_t coalesced; /* count requests answered by another request's count */
+  int64_t coalesced;  /* count requests answered by another request's count */
+  int64_t maintained; /* count requests answered from kept counts (maintain_counts = 1) */
 } hm_server_stats;
 
 hm_status hm_server_create(const hm_server_config* cfg, hm_server** out);

@@ -11,6 +11,8 @@
 //     on the device while requests on one graph keep their arrival order (an update is seen by
 //     every later count);
 //   * coalescing — consecutive count queries on an unchanged graph are answered by one count;
+//   * kept counts (maintain_counts) — once a graph has been counted, each update adds its exact
+//     delta to the graph's counts, and count queries are answered from them with no launch;
 //   * latency — each request records submit, start and finish times (steady clock).
 // Counts are exact, so a result is the same alone and in the mix.
 #include <algorithm>
@@ -53,6 +55,9 @@ struct GraphSlot {
   bool busy = false;     // a worker is running one of its requests
   bool queued = false;   // the graph id sits in the ready queue
   bool removed = false;
+  bool have_counts = false;  // maintain_counts: `counts` equal the graph's counts after its last
+                             // finished request
+  uint64_t counts[HM_STATIC_BINS] = {0};
 };
 
 int64_t estimate_bytes(const int64_t* off, const int32_t* nodes, int32_t E) {
@@ -82,7 +87,7 @@ struct hm_server {
   std::vector<std::thread> workers;
   uint64_t next_ticket = 1;
   int64_t used_bytes = 0;
-  int64_t completed = 0, coalesced = 0;
+  int64_t completed = 0, coalesced = 0, maintained = 0;
   bool stop = false;
 
   void enqueue_locked(int32_t gid) {
@@ -92,6 +97,20 @@ struct hm_server {
       ready.push_back(gid);
       work_cv.notify_one();
     }
+  }
+
+  // Answers count requests from the slot's kept counts (s->mu held).
+  void answer_kept_locked(GraphSlot& s, const std::vector<std::shared_ptr<Request>>& batch) {
+    const auto now = Clock::now();
+    for (auto& q : batch) {
+      q->start = q->finish = now;
+      q->status = HM_OK;
+      for (int k = 0; k < HM_STATIC_BINS; ++k) q->counts[k] = (int64_t)s.counts[k];
+      q->done = true;
+    }
+    completed += (int64_t)batch.size();
+    maintained += (int64_t)batch.size();
+    done_cv.notify_all();
   }
 
   void worker_main() {
@@ -114,6 +133,12 @@ struct hm_server {
           batch.push_back(s.pending.front());
           s.pending.pop_front();
         }
+      if (batch[0]->kind == HM_REQ_COUNT && s.have_counts) {
+        answer_kept_locked(s, batch);
+        s.busy = false;
+        enqueue_locked(gid);
+        continue;
+      }
       hm_graph* g = s.g;
       lk.unlock();
       const auto t0 = Clock::now();
@@ -133,6 +158,16 @@ struct hm_server {
       const std::string err = st == HM_OK ? std::string() : std::string(hm_last_error());
       const auto t1 = Clock::now();
       lk.lock();
+      if (cfg.maintain_counts) {
+        if (st != HM_OK) {
+          s.have_counts = false;
+        } else if (r.kind == HM_REQ_COUNT) {
+          for (int k = 0; k < HM_STATIC_BINS; ++k) s.counts[k] = (uint64_t)r.counts[k];
+          s.have_counts = true;
+        } else if (s.have_counts) {
+          for (int k = 0; k < HM_STATIC_BINS; ++k) s.counts[k] += (uint64_t)r.counts[k];
+        }
+      }
       for (auto& q : batch) {
         q->start = t0;
         q->finish = t1;
@@ -155,8 +190,10 @@ extern "C" {
 hm_status hm_server_create(const hm_server_config* cfg, hm_server** out) {
   hm::clear_error();
   if (!cfg || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_create: null argument");
-  if (cfg->slots < 1 || cfg->slots > 64 || cfg->hbm_budget_bytes <= 0)
-    return hm::fail(HM_ERR_BAD_INPUT, "hm_server_create: need 1 <= slots <= 64 and a positive budget");
+  if (cfg->slots < 1 || cfg->slots > 64 || cfg->hbm_budget_bytes <= 0 ||
+      (cfg->maintain_counts != 0 && cfg->maintain_counts != 1))
+    return hm::fail(HM_ERR_BAD_INPUT,
+                    "hm_server_create: need 1 <= slots <= 64, a positive budget, maintain_counts 0 or 1");
   *out = nullptr;
   size_t free_b = 0, total_b = 0;
   if (hipSetDevice(hm::current_device()) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
@@ -238,8 +275,13 @@ static hm_status submit(hm_server* s, std::shared_ptr<Request> r, uint64_t* tick
   r->ticket = s->next_ticket++;
   r->submit = Clock::now();
   s->requests[r->ticket] = r;
-  s->graphs[(size_t)r->graph].pending.push_back(r);
-  s->enqueue_locked(r->graph);
+  GraphSlot& g = s->graphs[(size_t)r->graph];
+  if (r->kind == HM_REQ_COUNT && g.have_counts && !g.busy && g.pending.empty()) {
+    s->answer_kept_locked(g, {r});  // nothing ahead of it: the kept counts are current
+  } else {
+    g.pending.push_back(r);
+    s->enqueue_locked(r->graph);
+  }
   *ticket = r->ticket;
   return HM_OK;
 }
@@ -323,6 +365,7 @@ hm_status hm_server_get_stats(hm_server* s, hm_server_stats* out) {
   out->budget_bytes = s->cfg.hbm_budget_bytes;
   out->completed = s->completed;
   out->coalesced = s->coalesced;
+  out->maintained = s->maintained;
   return HM_OK;
 }
 

@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from conftest import csr
 
 import ipdps26_amd as hm
 from ipdps26_amd import serving, workloads as W
@@ -105,3 +106,52 @@ def test_add_and_remove_graphs_while_requests_run(gpu_ready):
     for k, r in enumerate(results[6:]):
         assert np.array_equal(r, expect[k + 1])
     assert np.array_equal(extra, expect[6])
+
+
+# ---- kept counts (maintain_counts = 1) ----------------------------------------------------------
+def test_maintained_counts_equal_plain(gpu_ready):
+    """Counts answered from kept counts (previous count + exact update deltas) equal the plain
+    path's recount request by request."""
+    graphs, reqs = small_stream(n_graphs=8, n_req=160)
+    plain = serving.run_plain(graphs, reqs, realtime=False)
+    kept = serving.run_served(graphs, reqs, realtime=False, slots=4, maintain=True)
+    for k, (a, b) in enumerate(zip(plain["results"], kept["results"])):
+        assert np.array_equal(a, b), f"request {k} ({reqs[k]['kind']})"
+    n_counts = sum(q["kind"] == "count" for q in reqs)
+    assert 0 < kept["maintained"] <= n_counts
+
+
+def test_maintained_counts_start_and_survive_a_failed_update(gpu_ready):
+    """No kept counts before the first count (an update alone keeps none); a failed update drops
+    them, so the next count runs on the device again; every answer equals a fresh count."""
+    (off, nodes), = W.serving_graphs(1, lo=800, hi=800, seed=11)
+    rng = np.random.default_rng(11)
+    with hm.Server(1 << 30, 2, maintain_counts=True) as srv, hm.Graph(off, nodes) as g:
+        gid = srv.add_graph(off, nodes)
+
+        def both_update(dels, ins):
+            o, n = csr(ins)
+            r = srv.wait(srv.submit_update(gid, dels, o, n))
+            _, d = g.update(dels, o, n)
+            assert np.array_equal(r["counts"], d)
+
+        both_update([0, 5], [[1, 2, 3], [2, 3, 4, 5]])
+        assert srv.stats()["maintained"] == 0
+        c = srv.wait(srv.submit_count(gid))["counts"]
+        assert np.array_equal(c, g.count_static().astype(np.int64))
+        assert srv.stats()["maintained"] == 0  # the first count ran on the device
+        live = [x for x in range(off.size - 1) if x not in (0, 5)]
+        for k in range(3):
+            both_update(sorted(rng.choice(live, size=3, replace=False).tolist()) if k == 0 else [],
+                        [sorted(rng.choice(400, size=4, replace=False).tolist())])
+            c = srv.wait(srv.submit_count(gid))["counts"]
+            assert np.array_equal(c, g.count_static().astype(np.int64))
+        assert srv.stats()["maintained"] == 3
+        with pytest.raises(hm.HMotifError):  # id 0 was deleted: the update fails
+            srv.wait(srv.submit_update(gid, [0], np.zeros(1, np.int64), np.zeros(0, np.int32)))
+        c = srv.wait(srv.submit_count(gid))["counts"]
+        assert np.array_equal(c, g.count_static().astype(np.int64))
+        assert srv.stats()["maintained"] == 3  # recounted after the failure
+        c = srv.wait(srv.submit_count(gid))["counts"]
+        assert srv.stats()["maintained"] == 4
+        assert np.array_equal(c, g.count_static().astype(np.int64))
