@@ -225,6 +225,41 @@ hm_status hm_tgraph_get_stats(hm_tgraph* g, hm_temporal_stats* out);
 hm_status hm_tgraph_phase_ms(hm_tgraph* g, const char* phase, double* ms);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Batched static counts: many graphs in one launch sequence.                                 */
+/* No counterpart in the reference (it counts one graph per call, HMotifCount.cu:97-125).     */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct hm_batch hm_batch; /* scratch + stream + tuned forms; one caller thread at a time */
+
+typedef struct hm_batch_stats {
+  int32_t graphs_batched; /* graphs counted by the batched kernels */
+  int32_t graphs_alone;   /* graphs larger than one LDS table, counted one by one (tiled) */
+  int32_t waves;          /* triangle form used: waves per centre (4 or 8), */
+  int32_t copies;         /* LDS histogram copies per wave (8 or 16), */
+  int32_t persistent;     /* 1 = persistent grid fed by a device work queue, 0 = one group per centre */
+  int32_t order;          /* centre order: 0 graph by graph, 1 chunks of graphs with up to 64 MB of
+                             projection rows, 2 all centres of the batch by work */
+  int32_t tuned;          /* 1 if this call ran the autotuner (each run checked equal) */
+  int64_t hyperedges;     /* hyperedges counted by the batched kernels */
+  double ms_prepare;      /* device time before the triangle kernel */
+  double ms_triangles;    /* triangle kernel (the fastest form when this call tuned) */
+} hm_batch_stats;
+
+hm_status hm_batch_create(hm_batch** out);
+hm_status hm_batch_destroy(hm_batch* b);
+/* Exact static counts of n graphs, out[30 * k + bin] for graphs[k], identical to
+ * hm_count_static on each. The graphs must not be used by another call meanwhile; their
+ * statistics (hm_graph_get_stats) are not refreshed. The triangle kernel's form and centre order
+ * are autotuned per batch shape (mask width, log2 of the total and of the largest hyperedge
+ * count) on the first batch of that shape — the three orders, then the eight forms, every run
+ * checked equal — unless hm_batch_set_form fixed them. */
+hm_status hm_batch_count_static(hm_batch* b, hm_graph* const* graphs, int32_t n, uint64_t* out);
+/* waves = 0: autotuned (default); otherwise waves 4 or 8, copies 8 or 16, persistent 0 or 1,
+ * order 0, 1 or 2 (see hm_batch_stats). */
+hm_status hm_batch_set_form(hm_batch* b, int32_t waves, int32_t copies, int32_t persistent,
+                            int32_t order);
+hm_status hm_batch_get_stats(hm_batch* b, hm_batch_stats* out);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Serving: many device-resident hypergraphs, requests admitted and run as slots free up.     */
 /* No counterpart in the reference (its drivers run one graph, one request, one stream).      */
 /* ------------------------------------------------------------------------------------------ */

@@ -102,6 +102,13 @@ _SIGNATURES = [
      [ctypes.c_void_p, _i32p, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _i32p, _i64p]),
     ("hm_count_delta", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, _i32p, ctypes.c_int32, _i32p, ctypes.c_int32, _i64p]),
+    ("hm_batch_create", ctypes.c_int, [_P(ctypes.c_void_p)]),
+    ("hm_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("hm_batch_count_static", ctypes.c_int,
+     [ctypes.c_void_p, _P(ctypes.c_void_p), ctypes.c_int32, _u64p]),
+    ("hm_batch_set_form", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    ("hm_batch_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("hm_server_create", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_void_p)]),
     ("hm_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("hm_server_add_graph", ctypes.c_int, [ctypes.c_void_p, _i64p, _i32p, ctypes.c_int32, _i32p]),
@@ -395,6 +402,56 @@ class TemporalGraph:
         ms = ctypes.c_double(0.0)
         _check(lib().hm_tgraph_phase_ms(self._h, phase.encode(), ctypes.byref(ms)))
         return ms.value
+
+
+class _BatchStats(ctypes.Structure):
+    _fields_ = [("graphs_batched", ctypes.c_int32), ("graphs_alone", ctypes.c_int32),
+                ("waves", ctypes.c_int32), ("copies", ctypes.c_int32),
+                ("persistent", ctypes.c_int32), ("order", ctypes.c_int32), ("tuned", ctypes.c_int32),
+                ("hyperedges", ctypes.c_int64), ("ms_prepare", ctypes.c_double),
+                ("ms_triangles", ctypes.c_double)]
+
+
+class Batch:
+    """Static counts of many graphs in one launch sequence (hm_batch_* in hmotif.h). The
+    triangle kernel's form (waves per centre, LDS histogram copies, persistent work queue or one
+    workgroup per centre) and centre order are autotuned per batch shape unless set_form fixes
+    them."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        _check(lib().hm_batch_create(ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().hm_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def count_static(self, graphs) -> np.ndarray:
+        """uint64[len(graphs), 30]: the exact static counts of every graph."""
+        n = len(graphs)
+        arr = (ctypes.c_void_p * max(n, 1))(*[g._h.value for g in graphs])
+        out = np.zeros((max(n, 1), STATIC_BINS), dtype=np.uint64)
+        _check(lib().hm_batch_count_static(self._h, arr, n, _ptr(out, ctypes.c_uint64)))
+        return out[:n]
+
+    def set_form(self, waves: int = 0, copies: int = 16, persistent: bool = False,
+                 order: int = 2) -> None:
+        """waves=0: autotuned; else waves 4 or 8, copies 8 or 16, persistent work queue or not,
+        centre order 0 (graph by graph), 1 (chunks of 64 MB of rows) or 2 (whole batch by work)."""
+        _check(lib().hm_batch_set_form(self._h, int(waves), int(copies), int(bool(persistent)),
+                                       int(order)))
+
+    def stats(self) -> dict:
+        st = _BatchStats()
+        _check(lib().hm_batch_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _BatchStats._fields_}
 
 
 class _ServerConfig(ctypes.Structure):

@@ -43,6 +43,31 @@ struct DevBuf {
   }
 };
 
+// Pinned host memory that grows on demand (staging for asynchronous copies).
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+  hm_status ensure(size_t need) {
+    if (need <= bytes && p) return HM_OK;
+    release();
+    const size_t want = std::max(need, (size_t)1 << 16);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      set_error("hipHostMalloc(%zu bytes) failed", want);
+      return HM_ERR_OOM;
+    }
+    bytes = want;
+    return HM_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 // Flat, sorted, HBM-resident incidence arrays of one hypergraph.
 //   H2V: eoff[E+1] (uint32), enodes[I] (int32, ascending per row), esize[E] (uint8, 0 = deleted)
 //   V2H: voff[V+1] (uint32), vedges[I] (int32, ascending per row)
