@@ -289,7 +289,7 @@ def copy_probe(mib: int) -> None:
     print(json.dumps({"copy_probe_bytes": mib << 20}), flush=True)
 
 
-def run_serving(slots: int = 4, n_requests: int = 400) -> dict:
+def run_serving(slots: int = 4, n_requests: int = 400, batch_graphs: int = 16) -> dict:
     """Plain (one request at a time) versus served (hm_server, `slots` concurrent) on one request
     stream: throughput with every request queued at t=0, then p50/p99 latency with Poisson
     arrivals at 70% of the plain path's capacity. Results must agree request by request."""
@@ -307,19 +307,34 @@ def run_serving(slots: int = 4, n_requests: int = 400) -> dict:
     note(f"plain saturated: {plain['requests_per_s']:.1f} req/s")
     served = serving.run_served(graphs, reqs, realtime=False, slots=slots)
     note(f"served saturated: {served['requests_per_s']:.1f} req/s")
+    batched = serving.run_served(graphs, reqs, realtime=False, slots=slots, batch_graphs=batch_graphs)
+    note(f"served with batched counts saturated: {batched['requests_per_s']:.1f} req/s")
     kept = serving.run_served(graphs, reqs, realtime=False, slots=slots, maintain=True)
     note(f"served with kept counts saturated: {kept['requests_per_s']:.1f} req/s")
-    same = same_as(plain, served) and same_as(plain, kept)
+    same = same_as(plain, served) and same_as(plain, batched) and same_as(plain, kept)
     rate = 0.7 * plain["requests_per_s"]
     timed = W.serving_requests(graphs, n_requests, rate_per_s=rate)
     plain_rt = serving.run_plain(graphs, timed, realtime=True)
     note(f"plain at {rate:.1f} req/s: p50 {plain_rt['p50_ms']:.1f} ms p99 {plain_rt['p99_ms']:.1f} ms")
     served_rt = serving.run_served(graphs, timed, realtime=True, slots=slots)
     note(f"served at {rate:.1f} req/s: p50 {served_rt['p50_ms']:.1f} ms p99 {served_rt['p99_ms']:.1f} ms")
+    batched_rt = serving.run_served(graphs, timed, realtime=True, slots=slots, batch_graphs=batch_graphs)
+    note(f"served with batched counts at {rate:.1f} req/s: p50 {batched_rt['p50_ms']:.1f} ms "
+         f"p99 {batched_rt['p99_ms']:.1f} ms")
     kept_rt = serving.run_served(graphs, timed, realtime=True, slots=slots, maintain=True)
     note(f"served with kept counts at {rate:.1f} req/s: p50 {kept_rt['p50_ms']:.1f} ms "
          f"p99 {kept_rt['p99_ms']:.1f} ms")
-    same = same and same_as(plain_rt, served_rt) and same_as(plain_rt, kept_rt)
+    same = (same and same_as(plain_rt, served_rt) and same_as(plain_rt, batched_rt)
+            and same_as(plain_rt, kept_rt))
+    # many small graphs: where batched counts pay (launch-bound counts)
+    sg = W.serving_graphs(256, lo=100, hi=1000, seed=55)
+    sreqs = W.serving_requests(sg, 2000, seed=5500)
+    s_plain = serving.run_plain(sg, sreqs, realtime=False)
+    s_served = serving.run_served(sg, sreqs, realtime=False, slots=slots)
+    s_batched = serving.run_served(sg, sreqs, realtime=False, slots=slots, batch_graphs=64)
+    note(f"small graphs saturated: plain {s_plain['requests_per_s']:.0f} req/s, served "
+         f"{s_served['requests_per_s']:.0f}, served with batched counts {s_batched['requests_per_s']:.0f}")
+    same = same and same_as(s_plain, s_served) and same_as(s_plain, s_batched)
     triples = serving.count_triples(reqs, served["results"])
     drop = ("results",)
 
@@ -327,16 +342,25 @@ def run_serving(slots: int = 4, n_requests: int = 400) -> dict:
         return {k: v for k, v in d.items() if k not in drop}
 
     return {"ms": served["makespan_s"] * 1e3, "total": triples, "unit": "triples/s",
-            "extra": {"requests": n_requests, "slots": slots,
+            "extra": {"requests": n_requests, "slots": slots, "batch_graphs": batch_graphs,
                       "results_identical_plain_vs_served": bool(same),
                       "value_is": "triples classified by the recounting served run / its makespan",
                       "saturated": {"plain": strip(plain), "served": strip(served),
+                                    "served_batched_counts": strip(batched),
                                     "served_kept_counts": strip(kept),
                                     "speedup": served["requests_per_s"] / plain["requests_per_s"],
+                                    "speedup_batched_counts": batched["requests_per_s"] / plain["requests_per_s"],
                                     "speedup_kept_counts": kept["requests_per_s"] / plain["requests_per_s"]},
                       "poisson_at_70pct_plain_capacity": {
                           "rate_per_s": rate, "plain": strip(plain_rt), "served": strip(served_rt),
-                          "served_kept_counts": strip(kept_rt)}}}
+                          "served_batched_counts": strip(batched_rt),
+                          "served_kept_counts": strip(kept_rt)},
+                      "small_graphs_saturated": {
+                          "graphs": "256 graphs of the static_c2 family, E log-uniform on [100, 1000], "
+                                    "seed 55; 2,000 requests (seed 5500), all queued at t=0",
+                          "plain": strip(s_plain), "served": strip(s_served),
+                          "served_batched_counts_64": strip(s_batched),
+                          "speedup_batched_counts": s_batched["requests_per_s"] / s_plain["requests_per_s"]}}}
 
 
 def main() -> int:

@@ -152,9 +152,10 @@ typedef struct hm_update_stats {
   int64_t removed;      /* triples that contained a deleted hyperedge before the batch */
   int64_t added;        /* triples that contain an inserted hyperedge after it (delta sums to added - removed) */
   int64_t row_entries;  /* projection-row entries scanned by the delta kernel, both phases */
-  double ms_rows;       /* projection rows of the graph before and after (HIP events) */
+  double ms_rows;       /* projection rows of the graph after the batch, and before it unless the
+                           rows built by the previous update are still held (HIP events) */
   double ms_delta;      /* delta kernel, both phases (HIP events) */
-  double ms_store;      /* store update: H2V / V2H rebuild, host wall time including its syncs */
+  double ms_store;      /* store update: H2V / V2H rebuild, stream time including its host wait */
 } hm_update_stats;
 hm_status hm_graph_update_stats(hm_graph* g, hm_update_stats* out);
 
@@ -277,6 +278,11 @@ typedef struct hm_server_config {
    *    graph with no request ahead of it is answered from them without a launch. The counts are
    *    identical either way; an update that fails drops the kept counts until the next count. */
   int32_t maintain_counts;
+  /* 0 or 1: each count runs on its own. n > 1: a worker whose next request is a count of a small
+   * graph (its last count took under 1 ms) also takes other ready small graphs whose next request
+   * is a count, up to n graphs and about 4 ms of expected work, and counts them with one
+   * hm_batch_count_static call (counts identical to the one-graph count). */
+  int32_t batch_graphs;
 } hm_server_config;
 
 typedef struct hm_result {
@@ -297,6 +303,8 @@ typedef struct hm_server_stats {
   int64_t completed;
   int64_t coalesced;  /* count requests answered by another request's count */
   int64_t maintained; /* count requests answered from kept counts (maintain_counts = 1) */
+  int64_t batched_counts; /* graph counts run inside batched launches (batch_graphs > 1) */
+  int64_t batch_launches; /* batched count calls with more than one graph */
 } hm_server_stats;
 
 hm_status hm_server_create(const hm_server_config* cfg, hm_server** out);

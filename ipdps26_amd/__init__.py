@@ -456,7 +456,7 @@ class Batch:
 
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("slots", ctypes.c_int32),
-                ("maintain_counts", ctypes.c_int32)]
+                ("maintain_counts", ctypes.c_int32), ("batch_graphs", ctypes.c_int32)]
 
 
 class _Result(ctypes.Structure):
@@ -475,18 +475,21 @@ class _Result(ctypes.Structure):
 class _ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in
                 ("graphs", "reserved_bytes", "budget_bytes", "completed", "coalesced",
-                 "maintained")]
+                 "maintained", "batched_counts", "batch_launches")]
 
 
 class Server:
     """Request loop over many device-resident hypergraphs (see hm_server_* in hmotif.h).
     maintain_counts=True keeps each graph's counts current from the exact update deltas after
-    its first count and answers count queries from them without a launch."""
+    its first count and answers count queries from them without a launch; batch_graphs=n > 1
+    lets a worker count up to n ready graphs in one batched launch sequence."""
 
     COUNT, UPDATE = 0, 1
 
-    def __init__(self, hbm_budget_bytes: int, slots: int = 4, maintain_counts: bool = False):
-        cfg = _ServerConfig(int(hbm_budget_bytes), int(slots), int(bool(maintain_counts)))
+    def __init__(self, hbm_budget_bytes: int, slots: int = 4, maintain_counts: bool = False,
+                 batch_graphs: int = 0):
+        cfg = _ServerConfig(int(hbm_budget_bytes), int(slots), int(bool(maintain_counts)),
+                            int(batch_graphs))
         self._h = ctypes.c_void_p()
         _check(lib().hm_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
 

@@ -607,18 +607,25 @@ static hm_status build_full_rows(hm_graph* g, DynScratch& d, int32_t T) {
   return HM_OK;
 }
 
-// acc (device, 40 x u64, two's complement) += sign * L(ids, current graph). Adds the durations of
-// the row build and of the delta kernel (HIP events on the handle's stream) to g->ms_upd_*.
+// acc (device, 40 x u64, two's complement) += sign * L(ids, current graph). Builds the full rows
+// unless those of the current graph (same tile) are still held. evb >= 0: records the start, the
+// end of the row build and the end of the delta kernel in g->ev[evb..evb+2] (read by
+// add_phase_ms once the stream has drained); nothing waits here.
 template <typename MT>
 static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,
-                             long long sign, int32_t T) {
+                             long long sign, int32_t T, int evb) {
   if (n == 0) return HM_OK;
   DeviceCsr& c = g->csr;
   const int32_t E = c.E;
   hipStream_t st = g->stream;
-  HM_HIP(hipEventRecord(g->ev[0], st));
-  HM_TRY(build_full_rows<MT>(g, d, T));
-  HM_HIP(hipEventRecord(g->ev[1], st));
+  if (evb >= 0) HM_HIP(hipEventRecord(g->ev[evb], st));
+  if (!d.rows_valid || d.rows_tile != T) {  // else: the rows of this graph are already built
+    d.rows_valid = false;
+    HM_TRY(build_full_rows<MT>(g, d, T));
+    d.rows_valid = true;
+    d.rows_tile = T;
+  }
+  if (evb >= 0) HM_HIP(hipEventRecord(g->ev[evb + 1], st));
   HM_TRY(d.flag.ensure((size_t)std::max(E, 1)));
   HM_TRY(d.ids.ensure(sizeof(int32_t) * n));
   HM_HIP(hipMemcpyAsync(d.ids.p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
@@ -662,11 +669,15 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
         d.flag.as<uint8_t>(), parts, sign, d.acc.as<unsigned long long>());
   }
   HM_HIP(hipGetLastError());
-  HM_HIP(hipEventRecord(g->ev[2], st));
-  HM_HIP(hipEventSynchronize(g->ev[2]));
+  if (evb >= 0) HM_HIP(hipEventRecord(g->ev[evb + 2], st));
+  return HM_OK;
+}
+
+// Adds the row-build and delta-kernel times of a local_count recorded at evb (stream drained).
+static hm_status add_phase_ms(hm_graph* g, int evb) {
   float a = 0.f, b = 0.f;
-  HM_HIP(hipEventElapsedTime(&a, g->ev[0], g->ev[1]));
-  HM_HIP(hipEventElapsedTime(&b, g->ev[1], g->ev[2]));
+  HM_HIP(hipEventElapsedTime(&a, g->ev[evb], g->ev[evb + 1]));
+  HM_HIP(hipEventElapsedTime(&b, g->ev[evb + 1], g->ev[evb + 2]));
   g->ms_upd_rows += a;
   g->ms_upd_delta += b;
   return HM_OK;
@@ -680,7 +691,7 @@ static bool dense_fits(hm_graph* g, int32_t E, bool wide) {
 }
 
 static hm_status local_count_any(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,
-                                 long long sign) {
+                                 long long sign, int evb) {
   if (g->csr.max_size > HM_MAX_EDGE_SIZE)
     return fail(HM_ERR_UNSUPPORTED, "delta count: hyperedge larger than 64 nodes");
   const bool wide = g->csr.max_size > 32;
@@ -695,7 +706,8 @@ static hm_status local_count_any(hm_graph* g, DynScratch& d, const int32_t* ids,
       return HM_ERR_BAD_INPUT;
     }
   }
-  return wide ? local_count<mask64_t>(g, d, ids, n, sign, T) : local_count<uint32_t>(g, d, ids, n, sign, T);
+  return wide ? local_count<mask64_t>(g, d, ids, n, sign, T, evb)
+              : local_count<uint32_t>(g, d, ids, n, sign, T, evb);
 }
 
 // Applies the batch to the device arrays (deleted rows emptied, inserted rows appended).
@@ -821,16 +833,25 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
   DynScratch& d = dyn_scratch(g);
   g->ms_upd_rows = g->ms_upd_delta = 0.f;
   g->upd_removed = g->upd_added = g->upd_entries = -1;
+  // host ids copied to the device by the removal phase: kept alive until the stream has drained
+  // (the guard, destroyed first, waits on every return path)
+  std::vector<int32_t> dels(del_ids, del_ids + n_del);
+  struct DrainOnExit {
+    hipStream_t s;
+    ~DrainOnExit() { (void)hipStreamSynchronize(s); }
+  } drain{g->stream};
   if (want) {
     HM_TRY(d.acc.ensure(sizeof(unsigned long long) * kDynAccCount));
     k_zero_acc<<<1, kDynAccCount, 0, g->stream>>>(d.acc.as<unsigned long long>(), kDynAccCount);
-    std::vector<int32_t> dels(del_ids, del_ids + n_del);
     std::sort(dels.begin(), dels.end());
-    HM_TRY(local_count_any(g, d, dels.data(), n_del, -1));
+    HM_TRY(local_count_any(g, d, dels.data(), n_del, -1, 0));
   }
-  const auto t_store = std::chrono::steady_clock::now();
+  d.rows_valid = false;  // the store changes: the held rows describe the old graph
+  HM_HIP(hipEventRecord(g->ev[3], g->stream));
   HM_TRY(apply_batch(g, d, delmask, ins_off32, inodes, n_ins, newV));
-  g->ms_upd_store = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_store).count();
+  HM_HIP(hipEventRecord(g->ev[7], g->stream));
+  HM_HIP(hipEventSynchronize(g->ev[7]));
+  HM_HIP(hipEventElapsedTime(&g->ms_upd_store, g->ev[3], g->ev[7]));
   g->csr.max_size = maxsz;
   for (int32_t k = 0; k < n_del; ++k) g->live[(size_t)del_ids[k]] = 0;
   g->live.resize((size_t)E + n_ins, 1);
@@ -838,10 +859,12 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
   if (want) {
     std::vector<int32_t> ins((size_t)n_ins);
     for (int32_t k = 0; k < n_ins; ++k) ins[(size_t)k] = E + k;
-    HM_TRY(local_count_any(g, d, ins.data(), n_ins, +1));
+    HM_TRY(local_count_any(g, d, ins.data(), n_ins, +1, 4));
     unsigned long long h[kDynAccCount];
     HM_HIP(hipMemcpyAsync(h, d.acc.p, sizeof(h), hipMemcpyDeviceToHost, g->stream));
     HM_HIP(hipStreamSynchronize(g->stream));
+    if (n_del > 0) HM_TRY(add_phase_ms(g, 0));
+    if (n_ins > 0) HM_TRY(add_phase_ms(g, 4));
     for (int k = 0; k < HM_STATIC_BINS; ++k) delta_out[k] = (int64_t)h[k];
     g->upd_removed = (int64_t)h[kDynAccRemoved];
     g->upd_added = (int64_t)h[kDynAccAdded];
@@ -889,7 +912,7 @@ hm_status hm_count_delta(hm_graph* old_g, hm_graph* new_g, const int32_t* del_id
     k_zero_acc<<<1, kDynAccCount, 0, g->stream>>>(d.acc.as<unsigned long long>(), kDynAccCount);
     const bool is_old = g == old_g;
     HM_TRY(local_count_any(g, d, is_old ? dels.data() : ins.data(), is_old ? n_del : n_ins,
-                           is_old ? -1 : +1));
+                           is_old ? -1 : +1, -1));
     unsigned long long part[32];
     HM_HIP(hipMemcpyAsync(part, d.acc.p, sizeof(part), hipMemcpyDeviceToHost, g->stream));
     HM_HIP(hipStreamSynchronize(g->stream));

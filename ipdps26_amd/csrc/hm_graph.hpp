@@ -123,7 +123,12 @@ struct DynScratch {
   DevBuf ins_off, ins_nodes, sizes, noff, nnodes, neoff, nsize, inc_node, inc_edge, srt_node,
       srt_edge, vcnt, nvoff;
   DevBuf slen, soff, scur, spm;  // tiled delta: per (changed f, neighbour b) read position and mask
+  // deg / roff / rows hold the full projection rows of the graph as it is now, built with tile
+  // rows_tile (0 = one table). Cleared whenever the store changes.
+  bool rows_valid = false;
+  int32_t rows_tile = -1;
   void release() {
+    rows_valid = false;
     for (DevBuf* b : {&deg, &roff, &rows, &flag, &ids, &acc, &tmp, &ins_off, &ins_nodes, &sizes,
                       &noff, &nnodes, &neoff, &nsize, &inc_node, &inc_edge, &srt_node, &srt_edge,
                       &vcnt, &nvoff, &slen, &soff, &scur, &spm})
@@ -136,7 +141,7 @@ struct DynScratch {
 struct hm_graph {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   hm::DeviceCsr csr;
   hm::CountScratch scratch;
   hm::DynScratch dyn;

@@ -13,6 +13,10 @@
 //   * coalescing — consecutive count queries on an unchanged graph are answered by one count;
 //   * kept counts (maintain_counts) — once a graph has been counted, each update adds its exact
 //     delta to the graph's counts, and count queries are answered from them with no launch;
+//   * batched counts (batch_graphs) — a worker whose next request is a count of a small graph
+//     (last count under 1 ms) also takes other ready small graphs whose next request is a count
+//     (up to batch_graphs graphs and about 4 ms of expected work) and counts them all with one
+//     hm_batch launch sequence;
 //   * latency — each request records submit, start and finish times (steady clock).
 // Counts are exact, so a result is the same alone and in the mix.
 #include <algorithm>
@@ -55,6 +59,7 @@ struct GraphSlot {
   bool busy = false;     // a worker is running one of its requests
   bool queued = false;   // the graph id sits in the ready queue
   bool removed = false;
+  double count_ms = -1.0;    // duration of the graph's last count (its share of a batched one)
   bool have_counts = false;  // maintain_counts: `counts` equal the graph's counts after its last
                              // finished request
   uint64_t counts[HM_STATIC_BINS] = {0};
@@ -87,7 +92,7 @@ struct hm_server {
   std::vector<std::thread> workers;
   uint64_t next_ticket = 1;
   int64_t used_bytes = 0;
-  int64_t completed = 0, coalesced = 0, maintained = 0;
+  int64_t completed = 0, coalesced = 0, maintained = 0, batched_counts = 0, batch_launches = 0;
   bool stop = false;
 
   void enqueue_locked(int32_t gid) {
@@ -113,75 +118,140 @@ struct hm_server {
     done_cv.notify_all();
   }
 
+  // A count that may join a batch: one whose last count (alone, or its share of a batch) took
+  // under kBatchSmallMs. Larger counts fill the device by themselves, and batching them would only
+  // make every request of the batch wait for the whole batch.
+  static constexpr double kBatchSmallMs = 1.0;
+  static constexpr double kBatchBudgetMs = 4.0;  // expected duration of one batch at most
+  static bool small_count(const GraphSlot& g) { return g.count_ms >= 0.0 && g.count_ms < kBatchSmallMs; }
+
+  // Takes the head request of graph gid, plus every directly following count when the head is a
+  // count (s->mu held; the slot becomes busy).
+  std::vector<std::shared_ptr<Request>> take_head_locked(int32_t gid) {
+    GraphSlot& s = graphs[(size_t)gid];
+    s.queued = false;
+    s.busy = true;
+    std::vector<std::shared_ptr<Request>> batch{s.pending.front()};
+    s.pending.pop_front();
+    if (batch[0]->kind == HM_REQ_COUNT)
+      while (!s.pending.empty() && s.pending.front()->kind == HM_REQ_COUNT) {
+        batch.push_back(s.pending.front());
+        s.pending.pop_front();
+      }
+    return batch;
+  }
+
+  // Records a finished request group of graph gid and releases the slot (s->mu held).
+  void finish_locked(int32_t gid, std::vector<std::shared_ptr<Request>>& batch, hm_status st,
+                     const std::string& err, Clock::time_point t0, Clock::time_point t1) {
+    GraphSlot& s = graphs[(size_t)gid];
+    Request& r = *batch[0];
+    if (cfg.maintain_counts) {
+      if (st != HM_OK) {
+        s.have_counts = false;
+      } else if (r.kind == HM_REQ_COUNT) {
+        for (int k = 0; k < HM_STATIC_BINS; ++k) s.counts[k] = (uint64_t)r.counts[k];
+        s.have_counts = true;
+      } else if (s.have_counts) {
+        for (int k = 0; k < HM_STATIC_BINS; ++k) s.counts[k] += (uint64_t)r.counts[k];
+      }
+    }
+    for (auto& q : batch) {
+      q->start = t0;
+      q->finish = t1;
+      q->status = st;
+      q->error = err;
+      if (q.get() != &r) std::memcpy(q->counts, r.counts, sizeof(r.counts));
+      q->done = true;
+    }
+    completed += (int64_t)batch.size();
+    coalesced += (int64_t)batch.size() - 1;
+    s.busy = false;
+    enqueue_locked(gid);
+  }
+
   void worker_main() {
     (void)hipSetDevice(device);
+    (void)hm_init(device);       // handles made by this thread live on the server's device
+    hm_batch* counter = nullptr;  // batched counts of this worker (made on first use)
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
       work_cv.wait(lk, [&] { return stop || !ready.empty(); });
-      if (stop && ready.empty()) return;
+      if (stop && ready.empty()) break;
       const int32_t gid = ready.front();
       ready.pop_front();
       GraphSlot& s = graphs[(size_t)gid];
-      s.queued = false;
-      if (s.pending.empty()) continue;
-      s.busy = true;
-      // head request, plus every directly following count when the head is a count
-      std::vector<std::shared_ptr<Request>> batch{s.pending.front()};
-      s.pending.pop_front();
-      if (batch[0]->kind == HM_REQ_COUNT)
-        while (!s.pending.empty() && s.pending.front()->kind == HM_REQ_COUNT) {
-          batch.push_back(s.pending.front());
-          s.pending.pop_front();
-        }
-      if (batch[0]->kind == HM_REQ_COUNT && s.have_counts) {
-        answer_kept_locked(s, batch);
+      if (s.pending.empty()) {
+        s.queued = false;
+        continue;
+      }
+      std::vector<int32_t> gids{gid};
+      std::vector<std::vector<std::shared_ptr<Request>>> groups{take_head_locked(gid)};
+      if (groups[0][0]->kind == HM_REQ_COUNT && s.have_counts) {
+        answer_kept_locked(s, groups[0]);
         s.busy = false;
         enqueue_locked(gid);
         continue;
       }
-      hm_graph* g = s.g;
+      // continuous batching: other ready graphs whose next request is a count join this count
+      if (groups[0][0]->kind == HM_REQ_COUNT && cfg.batch_graphs > 1 && small_count(s)) {
+        double expect_ms = s.count_ms;
+        for (auto it = ready.begin(); it != ready.end() && (int32_t)gids.size() < cfg.batch_graphs;) {
+          // (a graph with kept counts is left to its own turn: it needs no launch)
+          GraphSlot& o = graphs[(size_t)*it];
+          if (o.busy || o.have_counts || o.pending.empty() || o.pending.front()->kind != HM_REQ_COUNT ||
+              !small_count(o) || expect_ms + o.count_ms > kBatchBudgetMs) {
+            ++it;
+            continue;
+          }
+          expect_ms += o.count_ms;
+          gids.push_back(*it);
+          groups.push_back(take_head_locked(*it));
+          it = ready.erase(it);
+        }
+      }
+      std::vector<hm_graph*> hs;
+      for (int32_t g : gids) hs.push_back(graphs[(size_t)g].g);
       lk.unlock();
       const auto t0 = Clock::now();
-      Request& r = *batch[0];
+      Request& r = *groups[0][0];
       hm_status st;
-      if (r.kind == HM_REQ_COUNT) {
+      std::string err;
+      if (r.kind == HM_REQ_COUNT && hs.size() > 1) {
+        std::vector<uint64_t> out(hs.size() * HM_STATIC_BINS);
+        st = counter ? HM_OK : hm_batch_create(&counter);
+        if (st == HM_OK) st = hm_batch_count_static(counter, hs.data(), (int32_t)hs.size(), out.data());
+        if (st != HM_OK) err = hm_last_error();
+        for (size_t k = 0; k < hs.size(); ++k)
+          for (int i = 0; i < HM_STATIC_BINS; ++i)
+            groups[k][0]->counts[i] = st == HM_OK ? (int64_t)out[k * HM_STATIC_BINS + i] : 0;
+      } else if (r.kind == HM_REQ_COUNT) {
         uint64_t out[HM_STATIC_BINS];
-        st = hm_count_static(g, out);
+        st = hm_count_static(hs[0], out);
         for (int k = 0; k < HM_STATIC_BINS; ++k) r.counts[k] = (int64_t)out[k];
       } else {
         std::vector<int32_t> ids(r.ins_off.size() - 1);
-        st = hm_graph_update(g, r.dels.data(), (int32_t)r.dels.size(), r.ins_off.data(),
+        st = hm_graph_update(hs[0], r.dels.data(), (int32_t)r.dels.size(), r.ins_off.data(),
                              r.ins_nodes.data(), (int32_t)ids.size(), ids.data(), r.counts);
         r.n_new = (int32_t)ids.size();
         r.first_new_id = ids.empty() ? -1 : ids[0];
       }
-      const std::string err = st == HM_OK ? std::string() : std::string(hm_last_error());
+      if (st != HM_OK && err.empty()) err = hm_last_error();
       const auto t1 = Clock::now();
       lk.lock();
-      if (cfg.maintain_counts) {
-        if (st != HM_OK) {
-          s.have_counts = false;
-        } else if (r.kind == HM_REQ_COUNT) {
-          for (int k = 0; k < HM_STATIC_BINS; ++k) s.counts[k] = (uint64_t)r.counts[k];
-          s.have_counts = true;
-        } else if (s.have_counts) {
-          for (int k = 0; k < HM_STATIC_BINS; ++k) s.counts[k] += (uint64_t)r.counts[k];
-        }
+      if (r.kind == HM_REQ_COUNT && st == HM_OK) {
+        const double share = std::chrono::duration<double, std::milli>(t1 - t0).count() / (double)gids.size();
+        for (int32_t g : gids) graphs[(size_t)g].count_ms = share;
       }
-      for (auto& q : batch) {
-        q->start = t0;
-        q->finish = t1;
-        q->status = st;
-        q->error = err;
-        if (q.get() != &r) std::memcpy(q->counts, r.counts, sizeof(r.counts));
-        q->done = true;
+      for (size_t k = 0; k < gids.size(); ++k) finish_locked(gids[k], groups[k], st, err, t0, t1);
+      if (gids.size() > 1) {
+        batched_counts += (int64_t)gids.size();
+        ++batch_launches;
       }
-      completed += (int64_t)batch.size();
-      coalesced += (int64_t)batch.size() - 1;
-      s.busy = false;
-      enqueue_locked(gid);
       done_cv.notify_all();
     }
+    lk.unlock();
+    if (counter) hm_batch_destroy(counter);
   }
 };
 
@@ -191,9 +261,10 @@ hm_status hm_server_create(const hm_server_config* cfg, hm_server** out) {
   hm::clear_error();
   if (!cfg || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_create: null argument");
   if (cfg->slots < 1 || cfg->slots > 64 || cfg->hbm_budget_bytes <= 0 ||
-      (cfg->maintain_counts != 0 && cfg->maintain_counts != 1))
+      (cfg->maintain_counts != 0 && cfg->maintain_counts != 1) || cfg->batch_graphs < 0)
     return hm::fail(HM_ERR_BAD_INPUT,
-                    "hm_server_create: need 1 <= slots <= 64, a positive budget, maintain_counts 0 or 1");
+                    "hm_server_create: need 1 <= slots <= 64, a positive budget, maintain_counts 0 or 1 "
+                    "and batch_graphs >= 0");
   *out = nullptr;
   size_t free_b = 0, total_b = 0;
   if (hipSetDevice(hm::current_device()) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
@@ -366,6 +437,8 @@ hm_status hm_server_get_stats(hm_server* s, hm_server_stats* out) {
   out->completed = s->completed;
   out->coalesced = s->coalesced;
   out->maintained = s->maintained;
+  out->batched_counts = s->batched_counts;
+  out->batch_launches = s->batch_launches;
   return HM_OK;
 }
 

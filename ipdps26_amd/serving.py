@@ -6,7 +6,8 @@ plain   one request at a time in arrival order, each a synchronous hm_count_stat
 served  the same requests submitted to an hm_server (slots run concurrently, one HIP stream per
         graph, requests on one graph keep their order, consecutive counts are coalesced);
         with maintain=True the server also keeps each graph's counts current from the exact
-        update deltas, so count queries after a graph's first count need no launch.
+        update deltas, so count queries after a graph's first count need no launch; with
+        batch_graphs=n a worker counts up to n ready graphs in one batched launch sequence.
 Both start from identical graphs and apply identical batches, so every result must match.
 """
 from __future__ import annotations
@@ -51,8 +52,8 @@ def run_plain(graphs, reqs, realtime: bool) -> dict:
 
 
 def run_served(graphs, reqs, realtime: bool, slots: int = 4, budget_bytes: int = 64 << 30,
-               maintain: bool = False) -> dict:
-    srv = Server(budget_bytes, slots, maintain_counts=maintain)
+               maintain: bool = False, batch_graphs: int = 0) -> dict:
+    srv = Server(budget_bytes, slots, maintain_counts=maintain, batch_graphs=batch_graphs)
     gids = [srv.add_graph(off, nodes) for off, nodes in graphs]
     for t in [srv.submit_count(g) for g in gids]:  # same warm-up as the plain path
         srv.wait(t)
@@ -73,7 +74,8 @@ def run_served(graphs, reqs, realtime: bool, slots: int = 4, budget_bytes: int =
     srv.close()
     return {"results": [d["counts"] for d in done], "makespan_s": makespan,
             "requests_per_s": len(reqs) / makespan, "coalesced": stats["coalesced"],
-            "maintained": stats["maintained"],
+            "maintained": stats["maintained"], "batched_counts": stats["batched_counts"],
+            "batch_launches": stats["batch_launches"],
             **_latency_stats([d["latency_ms"] for d in done])}
 
 

@@ -155,3 +155,18 @@ def test_maintained_counts_start_and_survive_a_failed_update(gpu_ready):
         c = srv.wait(srv.submit_count(gid))["counts"]
         assert srv.stats()["maintained"] == 4
         assert np.array_equal(c, g.count_static().astype(np.int64))
+
+
+# ---- batched counts (batch_graphs > 1) ----------------------------------------------------------
+@pytest.mark.parametrize("maintain", [False, True])
+def test_batched_counts_equal_plain(gpu_ready, maintain):
+    """Counts of several ready graphs run in one batched launch sequence; every result equals the
+    plain path's, with and without kept counts."""
+    graphs, reqs = small_stream(n_graphs=12, n_req=200)
+    plain = serving.run_plain(graphs, reqs, realtime=False)
+    served = serving.run_served(graphs, reqs, realtime=False, slots=2, maintain=maintain,
+                                batch_graphs=8)
+    for k, (a, b) in enumerate(zip(plain["results"], served["results"])):
+        assert np.array_equal(a, b), f"request {k} ({reqs[k]['kind']})"
+    if not maintain:  # all submitted at once: several small graphs are ready with a count at a time
+        assert served["batched_counts"] > 0 and served["batch_launches"] > 0
