@@ -112,6 +112,16 @@ hm_status hm_count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]);
  * (dynamicHyperGraph/src/HMotifCount.cu:16-95). */
 hm_status hm_graph_set_tile(hm_graph* g, int32_t tile_ids);
 
+/* on != 0: after an eager hm_count_static, the whole count of g (every launch, no host wait) is
+ * captured as a hipGraph, and later counts of the unchanged graph replay it — one graph launch
+ * and one wait instead of ~15 launches and two waits. Any update or tile change drops the capture
+ * (the next count runs eagerly and captures again). A capture that contains a memset node is
+ * never replayed. Counts are identical either way. No counterpart in the reference, which
+ * launches and synchronizes per kernel (dynamicHyperGraph/src/HMotifCount.cu:97-125). */
+hm_status hm_graph_set_replay(hm_graph* g, int32_t on);
+/* captured: a capture of the current graph is held; last_replayed: the last count replayed it. */
+hm_status hm_graph_replay_state(hm_graph* g, int32_t* captured, int32_t* last_replayed);
+
 /* Graph statistics computed on the device by the last hm_count_static call on g. */
 typedef struct hm_graph_stats {
   int64_t num_edges;          /* E */

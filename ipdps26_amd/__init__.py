@@ -94,6 +94,8 @@ _SIGNATURES = [
     ("hm_graph_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(_GraphStats)]),
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
     ("hm_graph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("hm_graph_set_replay", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("hm_graph_replay_state", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p]),
     ("hm_tgraph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_update_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("hm_count_strict_inc", ctypes.c_int,
@@ -295,6 +297,16 @@ class Graph:
             _ptr(ioff, ctypes.c_int64), _ptr(inodes if inodes.size else np.zeros(1, np.int32), ctypes.c_int32),
             n_ins, _ptr(assigned, ctypes.c_int32), _ptr(out, ctypes.c_int64) if delta else None))
         return assigned[:n_ins], (out if delta else None)
+
+    def set_replay(self, on: bool = True) -> None:
+        """Capture the count of the unchanged graph as a hipGraph and replay it on later counts."""
+        _check(lib().hm_graph_set_replay(self._h, int(bool(on))))
+
+    def replay_state(self) -> tuple[bool, bool]:
+        """(a capture of the current graph is held, the last count replayed it)."""
+        cap, last = ctypes.c_int32(0), ctypes.c_int32(0)
+        _check(lib().hm_graph_replay_state(self._h, ctypes.byref(cap), ctypes.byref(last)))
+        return bool(cap.value), bool(last.value)
 
     def update_stats(self) -> dict:
         """Of the last update with a delta: removed / added triples (held a deleted hyperedge

@@ -831,6 +831,7 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
     return fail(HM_ERR_UNSUPPORTED, "hm_graph_update: a delta needs hyperedges of at most 64 nodes");
   HM_HIP(hipSetDevice(g->device));
   DynScratch& d = dyn_scratch(g);
+  ++g->version;  // a captured count of the old graph is never replayed
   g->ms_upd_rows = g->ms_upd_delta = 0.f;
   g->upd_removed = g->upd_added = g->upd_entries = -1;
   // host ids copied to the device by the removal phase: kept alive until the stream has drained

@@ -157,4 +157,10 @@ struct hm_graph {
   // with a delta (-1 when there was none)
   int64_t upd_removed = -1, upd_added = -1, upd_entries = -1;
   float ms_upd_rows = 0.f, ms_upd_delta = 0.f, ms_upd_store = 0.f;
+  // captured static count (hm_graph_set_replay): replayed while the graph is unchanged
+  uint64_t version = 0;  // bumped by every store update and tile change
+  bool replay_on = false;
+  hipGraphExec_t replay_exec = nullptr;
+  uint64_t replay_version = 0;
+  bool last_replayed = false;
 };

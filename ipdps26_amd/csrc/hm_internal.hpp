@@ -35,6 +35,10 @@ hm_status check_hip(hipError_t e, const char* what, const char* file, int line);
 // Raises a kernel's dynamic-LDS limit to at least `bytes` (once per kernel and size; the
 // attribute call is not repeated on every launch).
 hm_status allow_lds(const void* kernel, size_t bytes);
+// Ends the stream capture on st and instantiates the captured graph into *exec. *exec stays null
+// (HM_OK) when the capture holds a memset node: replays of such nodes are not trusted, so such a
+// sequence always runs eagerly.
+hm_status end_capture(hipStream_t st, hipGraphExec_t* exec);
 }
 // Evaluates a HIP call; on failure records the message and returns from the enclosing function.
 #define HM_HIP(call)                                                          \

@@ -3,6 +3,7 @@
 
 #include <cstring>
 #include <mutex>
+#include <vector>
 #include <string>
 #include <unordered_map>
 
@@ -43,6 +44,30 @@ hm_status allow_lds(const void* kernel, size_t bytes) {
   hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return check_hip(e, "hipFuncSetAttribute", __FILE__, __LINE__);
   g = bytes;
+  return HM_OK;
+}
+
+hm_status end_capture(hipStream_t st, hipGraphExec_t* exec) {
+  *exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &graph);
+  if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  size_t n = 0;
+  e = hipGraphGetNodes(graph, nullptr, &n);
+  std::vector<hipGraphNode_t> nodes(n);
+  if (e == hipSuccess && n) e = hipGraphGetNodes(graph, nodes.data(), &n);
+  bool memset_node = false;
+  for (size_t k = 0; e == hipSuccess && k < n; ++k) {
+    hipGraphNodeType t;
+    e = hipGraphNodeGetType(nodes[k], &t);
+    memset_node = memset_node || (e == hipSuccess && t == hipGraphNodeTypeMemset);
+  }
+  if (e == hipSuccess && !memset_node) e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    *exec = nullptr;
+    return check_hip(e, "captured graph", __FILE__, __LINE__);
+  }
   return HM_OK;
 }
 

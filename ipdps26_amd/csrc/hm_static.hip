@@ -458,8 +458,10 @@ static int32_t auto_tile(int lim) {
 }
 
 // Static count. T == 0: one LDS table over all ids (dense kernels); T > 0: tiles of T ids/ranks.
+// capture: only enqueue the launches (the stream is being captured, the graph is unchanged since
+// the eager run that sized every buffer): no host waits, no events, no result copy.
 template <typename MT>
-static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32_t T) {
+static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32_t T, bool capture) {
   DeviceCsr& c = g->csr;
   CountScratch& s = g->scratch;
   const int32_t E = c.E;
@@ -504,7 +506,7 @@ static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32
   unsigned long long* acc = s.acc.as<unsigned long long>();
   const unsigned eb = (unsigned)((E + kBlock - 1) / kBlock);
 
-  HM_HIP(hipEventRecord(g->ev[0], st));
+  if (!capture) HM_HIP(hipEventRecord(g->ev[0], st));
   k_zero_u64<<<1, 64, 0, st>>>(acc, kAccCount);
   // 1. projected degrees + combinatorial open counts
   if (tiled)
@@ -554,11 +556,13 @@ static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32
                                              s.vkeys2.as<unsigned long long>(), (int)I, 0, 32 + vbits, st));
     k_low_words<<<ib, kBlock, 0, st>>>(s.vkeys2.as<unsigned long long>(), s.vrank.as<int32_t>(), I);
   }
-  unsigned long long total_cap = 0;
-  HM_HIP(hipMemcpyAsync(&total_cap, s.roff.as<unsigned long long>() + E, sizeof(total_cap),
-                        hipMemcpyDeviceToHost, st));
-  HM_HIP(hipStreamSynchronize(st));
-  HM_TRY(s.rows.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(total_cap, 1)));
+  if (!capture) {
+    unsigned long long total_cap = 0;
+    HM_HIP(hipMemcpyAsync(&total_cap, s.roff.as<unsigned long long>() + E, sizeof(total_cap),
+                          hipMemcpyDeviceToHost, st));
+    HM_HIP(hipStreamSynchronize(st));
+    HM_TRY(s.rows.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(total_cap, 1)));
+  }
   // 4. upper rows
   if (tiled)
     k_proj_fill_tiled<MT><<<E, kBlock, lds_cnt, st>>>(
@@ -587,13 +591,15 @@ static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32
     tb = s.tmp.bytes;
     HM_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp.p, tb, s.degr.as<unsigned long long>(),
                                             s.uoff.as<unsigned long long>(), E + 1, st));
-    unsigned long long U = 0;
-    HM_HIP(hipMemcpyAsync(&U, s.uoff.as<unsigned long long>() + E, sizeof(U), hipMemcpyDeviceToHost, st));
-    HM_HIP(hipStreamSynchronize(st));
-    HM_TRY(s.cur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(U, 1)));
-    HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
+    if (!capture) {
+      unsigned long long U = 0;
+      HM_HIP(hipMemcpyAsync(&U, s.uoff.as<unsigned long long>() + E, sizeof(U), hipMemcpyDeviceToHost, st));
+      HM_HIP(hipStreamSynchronize(st));
+      HM_TRY(s.cur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(U, 1)));
+      HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
+    }
   }
-  HM_HIP(hipEventRecord(g->ev[1], st));
+  if (!capture) HM_HIP(hipEventRecord(g->ev[1], st));
   // 6. triangles
   if (tiled)
     k_tri_tiled<MT, kTriWaves><<<E, kTriWaves * kWave, lds_tri, st>>>(
@@ -608,14 +614,49 @@ static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32
         s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
         s.order.as<int32_t>(), acc);
   HM_HIP(hipGetLastError());
-  HM_HIP(hipEventRecord(g->ev[2], st));
+  if (!capture) HM_HIP(hipEventRecord(g->ev[2], st));
   k_stats<<<64, kBlock, 0, st>>>(E, c.V, c.voff.as<uint32_t>(), s.deg.as<uint32_t>(),
                                  s.dlow.as<uint32_t>(), s.rlen.as<uint32_t>(), acc);
+  g->last_tile = T;
+  if (capture) return HM_OK;
   HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
   HM_HIP(hipEventElapsedTime(&g->ms_project, g->ev[0], g->ev[1]));
   HM_HIP(hipEventElapsedTime(&g->ms_triangles, g->ev[1], g->ev[2]));
-  g->last_tile = T;
+  return HM_OK;
+}
+
+using CountImpl = hm_status (*)(hm_graph*, unsigned long long*, int32_t, bool);
+
+// Replays the captured count of g: one graph launch, one wait. The phases are not separated:
+// ms_triangles receives the whole count, ms_project 0.
+static hm_status replay_count(hm_graph* g, unsigned long long* h_acc) {
+  hipStream_t st = g->stream;
+  HM_HIP(hipEventRecord(g->ev[0], st));
+  HM_HIP(hipGraphLaunch(g->replay_exec, st));
+  HM_HIP(hipEventRecord(g->ev[2], st));
+  HM_HIP(hipMemcpyAsync(h_acc, g->scratch.acc.p, sizeof(unsigned long long) * kAccCount,
+                        hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  g->ms_project = 0.f;
+  HM_HIP(hipEventElapsedTime(&g->ms_triangles, g->ev[0], g->ev[2]));
+  return HM_OK;
+}
+
+// Captures the count of g (just counted eagerly, so every buffer is sized) for later replays.
+static hm_status capture_count(hm_graph* g, CountImpl impl, int32_t T) {
+  if (g->replay_exec) (void)hipGraphExecDestroy(g->replay_exec);
+  g->replay_exec = nullptr;
+  HM_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+  const hm_status st = impl(g, nullptr, T, true);
+  hipGraphExec_t exec = nullptr;
+  const hm_status st2 = end_capture(g->stream, &exec);  // ends the capture on every path
+  if (st != HM_OK || st2 != HM_OK) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    return st != HM_OK ? st : st2;
+  }
+  g->replay_exec = exec;  // null when the capture holds a memset node: counts stay eager
+  g->replay_version = g->version;
   return HM_OK;
 }
 
@@ -695,6 +736,7 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
   const int32_t E = g->csr.E;
   unsigned long long acc[kAccCount] = {0};
   g->ms_project = g->ms_triangles = 0.f;
+  g->last_replayed = false;
   if (E >= 3) {
     if (g->csr.max_size > HM_MAX_EDGE_SIZE)
       return fail(HM_ERR_UNSUPPORTED, "hyperedge larger than 64 nodes");
@@ -702,21 +744,31 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
     HM_HIP(hipSetDevice(g->device));
     const int lim = lds_limit_bytes(g->device);
     const bool wide = g->csr.max_size > 32;
+    CountImpl impl;
+    int32_t T = 0;
     if (g->tile > 0) {
       const size_t need = (size_t)g->tile * ((wide ? 8 : 4) + 1) + kTiledStaticLds;
       if (need > (size_t)lim) {
         set_error("static count: a tile of %d ids exceeds the LDS limit (%d bytes)", g->tile, lim);
         return HM_ERR_BAD_INPUT;
       }
-      HM_TRY(wide ? count_static_impl<mask64_t>(g, acc, g->tile)
-                  : count_static_impl<uint32_t>(g, acc, g->tile));
+      impl = wide ? count_static_impl<mask64_t> : count_static_impl<uint32_t>;
+      T = g->tile;
     } else if (!wide && dense_lds_bytes<uint32_t>(E, true) + kTiledStaticLds <= (size_t)lim) {
-      HM_TRY(count_static_impl<uint32_t>(g, acc, 0));
+      impl = count_static_impl<uint32_t>;
     } else if (dense_lds_bytes<mask64_t>(E, true) + kTiledStaticLds <= (size_t)lim) {
-      HM_TRY(count_static_impl<mask64_t>(g, acc, 0));
+      impl = count_static_impl<mask64_t>;
     } else {
-      HM_TRY(wide ? count_static_impl<mask64_t>(g, acc, auto_tile<mask64_t>(lim))
-                  : count_static_impl<uint32_t>(g, acc, auto_tile<uint32_t>(lim)));
+      impl = wide ? count_static_impl<mask64_t> : count_static_impl<uint32_t>;
+      T = wide ? auto_tile<mask64_t>(lim) : auto_tile<uint32_t>(lim);
+    }
+    if (g->replay_on && g->replay_exec && g->replay_version == g->version) {
+      HM_TRY(replay_count(g, acc));
+      g->last_replayed = true;
+    } else {
+      HM_TRY(impl(g, acc, T, false));
+      // the counts are in hand: a capture that fails only leaves later counts eager
+      if (g->replay_on && capture_count(g, impl, T) != HM_OK) clear_error();
     }
   }
   for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = acc[kAccHist + k];
@@ -776,6 +828,7 @@ hm_status hm_graph_create(const int64_t* offsets, const int32_t* nodes, int32_t 
 hm_status hm_graph_destroy(hm_graph* g) {
   if (!g) return HM_OK;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->replay_exec) (void)hipGraphExecDestroy(g->replay_exec);
   g->scratch.release();
   g->dyn.release();
   g->csr.release();
@@ -796,7 +849,27 @@ hm_status hm_graph_set_tile(hm_graph* g, int32_t tile_ids) {
   hm::clear_error();
   if (!g) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_tile: null handle");
   if (tile_ids < 0) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_tile: negative tile");
+  if (g->tile != tile_ids) ++g->version;
   g->tile = tile_ids;
+  return HM_OK;
+}
+
+hm_status hm_graph_set_replay(hm_graph* g, int32_t on) {
+  hm::clear_error();
+  if (!g) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_replay: null handle");
+  g->replay_on = on != 0;
+  if (!g->replay_on && g->replay_exec) {
+    (void)hipGraphExecDestroy(g->replay_exec);
+    g->replay_exec = nullptr;
+  }
+  return HM_OK;
+}
+
+hm_status hm_graph_replay_state(hm_graph* g, int32_t* captured, int32_t* last_replayed) {
+  hm::clear_error();
+  if (!g || !captured || !last_replayed) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_replay_state: null argument");
+  *captured = (g->replay_exec && g->replay_version == g->version) ? 1 : 0;
+  *last_replayed = g->last_replayed ? 1 : 0;
   return HM_OK;
 }
 
