@@ -343,6 +343,12 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
         if (cnt[k]) atomicAdd(&acc[kTAccHist + tbl[48 + k]], cnt[k]);
   }
 
+  // Triples open at a neighbour x (y overlaps x, not i) fall in one of 16 bins, fixed by the
+  // side of y (before x, or between x and i), I = [i ⊄ x], X = [x has a node in neither y nor i]
+  // and Y = [y ⊄ x]; each lane counts them in registers: per x in a 64-bit word of four 16-bit
+  // fields (X + 2Y; a lane meets at most W / 64 + 1 < 2^16 entries of one row), folded into
+  // open_cnt[side][I][X + 2Y] after each x and into the histogram once per anchor.
+  uint32_t open_cnt[2][2][4] = {};
   for (;;) {
     uint32_t xi = 0;
     if (lane == 0) xi = atomicAdd(&s_next, 1u);
@@ -354,6 +360,8 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     const int sx = sz[xr];
     const int cxi = popc(mx);
     const int fx = sx > cxi ? 1 : 0;
+    const int xfree = sx - cxi;  // X = [C_xy < xfree]
+    unsigned long long fields_b = 0, fields_e = 0;
     // (b) forward row of x inside (x, i): closed triangles and triples open at x. Two row loads
     // are in flight per lane; the row is ascending, so the scan stops at the first entry >= i.
     {
@@ -371,7 +379,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
           atomicAdd(&hw[tbl[venn_id(sx, sy, si, cxy, cyi, cxi, popc((MT)(mx & my)))] * kC + cp], 1u);
           atomicAdd(&hw[tbl[as_open_id(fx, sy > cyi ? 1 : 0, cxi, cyi, si)] * kC + cp], ~0u);  // -1
         } else {
-          atomicAdd(&hw[tbl[venn_id(sx, sy, si, cxy, 0, cxi, 0)] * kC + cp], 1u);
+          fields_b += 1ull << (16 * ((cxy < xfree ? 1 : 0) + (sy > cxy ? 2 : 0)));
         }
       };
       for (uint32_t j0 = 0; j0 < n; j0 += 2 * kWave) {
@@ -391,22 +399,47 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
       const int32_t n = (int32_t)bdeg[x];
       for (int32_t jend = n; jend > 0; jend -= kWave) {
         const int32_t j = jend - kWave + lane;
-        bool ok = false, inside = false;
-        int b = 0;
+        bool inside = false;
         if (j >= 0) {
           const uint32_t e = row[j];
           const int32_t yr = (int32_t)(e >> kCntBits) - base;
           inside = yr >= 0;
           if (inside && M[yr] == (MT)0) {
-            ok = true;
-            b = tbl[venn_id(sz[yr], sx, si, (int)(e & kCntMask), cxi, 0, 0)];
+            const int cxy = (int)(e & kCntMask);
+            fields_e += 1ull << (16 * ((cxy < xfree ? 1 : 0) + ((int)sz[yr] > cxy ? 2 : 0)));
           }
         }
-        if (ok) atomicAdd(&hw[b * kC + cp], 1u);
         if (__ballot(j >= 0 && !inside)) break;
       }
     }
+    // (I is uniform in the wave; both branches index the counters with constants)
+    if (si > cxi) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        open_cnt[0][1][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
+        open_cnt[1][1][k] += (uint32_t)(fields_e >> (16 * k)) & 0xFFFFu;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        open_cnt[0][0][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
+        open_cnt[1][0][k] += (uint32_t)(fields_e >> (16 * k)) & 0xFFFFu;
+      }
+    }
   }
+  // open-at-x bins: (b) time order (x, y, i): Venn id X | Y << 1 | I << 2 | 8 (xy) | 32 (ix);
+  //                 (e) time order (y, x, i): Venn id Y | X << 1 | I << 2 | 8 (yx) | 16 (xi)
+#pragma unroll
+  for (int side = 0; side < 2; ++side)
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t c = wave_sum(open_cnt[side][I][k]);
+        const int X = k & 1, Y = k >> 1;
+        const int id = side == 0 ? (X | Y << 1 | I << 2 | 8 | 32) : (Y | X << 1 | I << 2 | 8 | 16);
+        if (lane == 0 && c) atomicAdd(&hw[tbl[id] * kC], c);
+      }
   __syncthreads();
   for (int b = lane; b < kTemporalBins; b += kWave) {  // fold each wave's copies into copy 0
     int s = 0;
