@@ -106,7 +106,7 @@ hm_status hm_count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]);
 
 /* Key tile of hm_count_static and of the delta of hm_graph_update / hm_count_delta on g.
  * 0 (the default): one LDS mask table over all hyperedge ids when they fit (about 28k ids for
- * the static count, 24k for the delta), tiles as large as LDS allows otherwise; > 0: tiles of
+ * the static count and the delta with 32-bit masks), tiles as large as LDS allows otherwise; > 0: tiles of
  * exactly tile_ids ids forced at any size (tests and tuning). Results do not depend on the tile.
  * No counterpart in the reference, whose kernel keeps no per-hyperedge table
  * (dynamicHyperGraph/src/HMotifCount.cu:16-95). */

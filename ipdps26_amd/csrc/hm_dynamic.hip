@@ -119,6 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_full_fill(int32_t E, const uint32_t*
 // Histogram: per wave, kDeltaCopies interleaved int32 copies updated with per-lane LDS atomics.
 constexpr int kDeltaWaves = 16;
 constexpr int kDeltaCopies = 8;
+constexpr int kDeltaLoads = 8;
 constexpr size_t kDeltaTiledStaticLds = 20 * 1024;  // k_delta_tiled: histogram, classes, tables
 
 template <typename MT, int NW>
@@ -131,8 +132,8 @@ __global__ __launch_bounds__(NW * kWave) void k_delta(
     long long sign, unsigned long long* __restrict__ acc) {
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
-  uint8_t* sz = reinterpret_cast<uint8_t*>(M + E);
-  uint8_t* fl = sz + E;
+  // size of x (at most 64) in bits 0..6, [x in S] in bit 7: one LDS read per row entry
+  uint8_t* szf = reinterpret_cast<uint8_t*>(M + E);
   __shared__ int32_t h[NW][32][kDeltaCopies];
   __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
   __shared__ unsigned long long red[6][NW];
@@ -149,10 +150,7 @@ __global__ __launch_bounds__(NW * kWave) void k_delta(
     s_next = 0;
     s_scan = 0;
   }
-  for (int32_t x = threadIdx.x; x < E; x += blockDim.x) {
-    sz[x] = esize[x];
-    fl[x] = flag[x];
-  }
+  for (int32_t x = threadIdx.x; x < E; x += blockDim.x) szf[x] = (uint8_t)(esize[x] | (flag[x] ? 0x80 : 0));
   build_masks<MT>(M, E, f, eoff, enodes, voff, vedges, nullptr);
   const int sf = (int)(eoff[f + 1] - eoff[f]);
   const int lane = lane_id(), w = wave_id();
@@ -161,9 +159,10 @@ __global__ __launch_bounds__(NW * kWave) void k_delta(
     // to the global accumulator: C(d_f, 2) can exceed the int32 LDS counters)
     for (int32_t x = threadIdx.x; x < E; x += blockDim.x) {
       const MT m = M[x];
-      if (m && (!fl[x] || x > f)) {
+      const int zx = szf[x];
+      if (m && (!(zx & 0x80) || x > f)) {
         const int C = popc(m);
-        atomicAdd(&cls[2 * (C - 1) + (sz[x] > C ? 1 : 0)], 1u);
+        atomicAdd(&cls[2 * (C - 1) + ((zx & 0x7F) > C ? 1 : 0)], 1u);
       }
     }
     __syncthreads();
@@ -193,11 +192,19 @@ __global__ __launch_bounds__(NW * kWave) void k_delta(
       }
     }
   }
-  // neighbours b of f (this part's share): closed pairs (re-booked) and triples open at b
+  // neighbours b of f (this part's share): closed pairs (re-booked) and triples open at b.
+  // A triple open at b (c overlaps b, not f) has bin 20 + X + 2 (F + Y) with F = [f ⊄ b] (fixed
+  // per b), X = [b has a node in neither f nor c] and Y = [c ⊄ b]; each lane counts them per b in
+  // four 16-bit fields of one word (a lane meets at most d_b / 64 + 1 < 2^16 entries of a row) and
+  // folds them into open_cnt[F][X + 2Y] after each b.
   const int copy = lane & (kDeltaCopies - 1);
   int32_t* hc = &h[w][0][copy];
   const uint32_t* rowF = rows + roff[f];
   const uint32_t nF = (uint32_t)rlen[f];
+  uint32_t open_cnt[2][4] = {};
+  // the -1 of a closed pair's as-if-open triple at f: bin 20 + A + 2 ([b ⊄ f] + B) with
+  // A = [f has a node in neither b nor c], B = [c ⊄ f]; counted like the open triples
+  uint32_t neg_cnt[2][4] = {};
   for (;;) {
     uint32_t k = 0;
     if (lane == 0) k = atomicAdd(&s_next, 1u);
@@ -205,31 +212,69 @@ __global__ __launch_bounds__(NW * kWave) void k_delta(
     const uint32_t ib = part + k * (uint32_t)parts;
     if (ib >= nF) break;
     const int32_t b = (int32_t)(rowF[ib] >> kCntBits);
-    if (fl[b] && b < f) continue;
+    const int zb = szf[b];
+    if ((zb & 0x80) && b < f) continue;
     const MT mb = M[b];
     const int cfb = popc(mb);
-    const int sb = sz[b];
+    const int sb = zb & 0x7F;
+    const int bfree = sb - cfb;  // X = [C_bc < bfree]
+    const int ffree = sf - cfb;  // A = [C_fc < ffree]
     const uint32_t* rowB = rows + roff[b];
     const uint32_t nB = (uint32_t)rlen[b];
     if (lane == 0) atomicAdd(&s_scan, (unsigned long long)nB);
-    for (uint32_t j = lane; j < nB; j += kWave) {
-      const uint32_t e = rowB[j];
+    unsigned long long fields = 0, negs = 0;
+    auto visit = [&](uint32_t e) {
       const int32_t c = (int32_t)(e >> kCntBits);
       const int cbc = (int)(e & kCntMask);
-      if (c == f || (fl[c] && c < f)) continue;
+      const int zc = szf[c];
+      if (c == f || ((zc & 0x80) && c < f)) return;
       const MT mc = M[c];
-      const int sc = sz[c];
-      if (mc) {
-        if (c > b) {
-          const int cfc = popc(mc);
-          atomicAdd(&hc[tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))] * kDeltaCopies], 1);
-          atomicSub(&hc[open_bin(sf, sb, sc, cfb, cfc) * kDeltaCopies], 1);
-        }
-      } else {
-        atomicAdd(&hc[open_bin(sb, sf, sc, cfb, cbc) * kDeltaCopies], 1);
-      }
+      const int sc = zc & 0x7F;
+      const int cfc = popc(mc);
+      const bool closed = mc && c > b;
+      if (closed)
+        atomicAdd(&hc[tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))] * kDeltaCopies], 1);
+      // both words take a (possibly zero) increment: no branch selects between them
+      const unsigned long long one_n = 1ull << (16 * ((cfc < ffree ? 1 : 0) + (sc > cfc ? 2 : 0)));
+      const unsigned long long one_o = 1ull << (16 * ((cbc < bfree ? 1 : 0) + (sc > cbc ? 2 : 0)));
+      negs += closed ? one_n : 0ull;
+      fields += mc ? 0ull : one_o;
+    };
+    // kDeltaLoads row loads in flight per lane before any is used (lane l reads l, l + 64, ...)
+    uint32_t j = lane;
+    for (; j + (kDeltaLoads - 1) * kWave < nB; j += kDeltaLoads * kWave) {
+      uint32_t e[kDeltaLoads];
+#pragma unroll
+      for (int u = 0; u < kDeltaLoads; ++u) e[u] = rowB[j + u * kWave];
+#pragma unroll
+      for (int u = 0; u < kDeltaLoads; ++u) visit(e[u]);
+    }
+    for (; j < nB; j += kWave) visit(rowB[j]);
+    // (F is uniform in the wave; both branches index the counters with constants)
+    if (sf > cfb) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) open_cnt[1][q] += (uint32_t)(fields >> (16 * q)) & 0xFFFFu;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) open_cnt[0][q] += (uint32_t)(fields >> (16 * q)) & 0xFFFFu;
+    }
+    if (sb > cfb) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) neg_cnt[1][q] += (uint32_t)(negs >> (16 * q)) & 0xFFFFu;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) neg_cnt[0][q] += (uint32_t)(negs >> (16 * q)) & 0xFFFFu;
     }
   }
+#pragma unroll
+  for (int F = 0; F < 2; ++F)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t c = wave_sum(open_cnt[F][q]);
+      if (lane == 0 && c) atomicAdd(&h[w][kOpenBin0 + (q & 1) + 2 * (F + (q >> 1))][0], (int32_t)c);
+      const uint32_t n = wave_sum(neg_cnt[F][q]);
+      if (lane == 0 && n) atomicSub(&h[w][kOpenBin0 + (q & 1) + 2 * (F + (q >> 1))][0], (int32_t)n);
+    }
   __syncthreads();
   if (threadIdx.x < kStaticBins) {
     long long s = 0;
@@ -660,7 +705,7 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
         d.flag.as<uint8_t>(), parts, sign, d.soff.as<unsigned long long>(), d.scur.as<uint32_t>(),
         d.spm.as<MT>(), d.acc.as<unsigned long long>());
   } else {
-    const size_t lds = (((size_t)E * sizeof(MT) + 2 * (size_t)E) + 15) & ~(size_t)15;
+    const size_t lds = (((size_t)E * sizeof(MT) + (size_t)E) + 15) & ~(size_t)15;
     HM_TRY(allow_lds((const void*)k_delta<MT, kDeltaWaves>, lds));
     k_delta<MT, kDeltaWaves><<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
         E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
@@ -686,8 +731,8 @@ static hm_status add_phase_ms(hm_graph* g, int evb) {
 static bool dense_fits(hm_graph* g, int32_t E, bool wide) {
   int lim = 65536;
   (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
-  // k_delta: mask, size and flag per id, plus about 18 KB of static arrays (20 KB reserved)
-  return (size_t)E * ((wide ? 8 : 4) + 2) + 16 + 20 * 1024 <= (size_t)lim;
+  // k_delta: mask and size-and-flag byte per id, plus about 18 KB of static arrays (20 KB reserved)
+  return (size_t)E * ((wide ? 8 : 4) + 1) + 16 + 20 * 1024 <= (size_t)lim;
 }
 
 static hm_status local_count_any(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,
