@@ -154,6 +154,8 @@ __device__ inline void proj_fill_item(MT* M, int32_t E, int32_t a, const uint32_
   if (threadIdx.x == 0) rlen[ra] = running;
 }
 
+constexpr int kTriLoads = 4;
+
 // Centre rank r (NW waves, each with NC interleaved int32 histogram copies, copy = lane % NC, for
 // its LDS atomics): every triangle r < rb < rc with rb in r's upper row and rc in rb's upper row.
 // M (mask table by rank) spans E ranks; the E size bytes follow it in the same dynamic LDS.
@@ -206,14 +208,16 @@ __device__ inline void tri_item(MT* M, int32_t E, int32_t r, const uint32_t* __r
         atomicSub(&hc[open_bin(sc, sa, sb, cac, cbc) * NC], 1);
       }
     };
-    // two row loads in flight per lane before either is used (lane l reads l, l + 64, ...)
+    // kTriLoads row loads in flight per lane before any is used (lane l reads l, l + 64, ...)
     uint32_t j = lane;
-    for (; j + kWave < nB; j += 2 * kWave) {
-      const uint32_t e0 = rowB[j], e1 = rowB[j + kWave];
-      visit(e0);
-      visit(e1);
+    for (; j + (kTriLoads - 1) * kWave < nB; j += kTriLoads * kWave) {
+      uint32_t e[kTriLoads];
+#pragma unroll
+      for (int u = 0; u < kTriLoads; ++u) e[u] = rowB[j + u * kWave];
+#pragma unroll
+      for (int u = 0; u < kTriLoads; ++u) visit(e[u]);
     }
-    if (j < nB) visit(rowB[j]);
+    for (; j < nB; j += kWave) visit(rowB[j]);
   }
   __syncthreads();
   if (threadIdx.x < kStaticBins) {

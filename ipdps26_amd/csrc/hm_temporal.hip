@@ -349,6 +349,8 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   // fields (X + 2Y; a lane meets at most W / 64 + 1 < 2^16 entries of one row), folded into
   // open_cnt[side][I][X + 2Y] after each x and into the histogram once per anchor.
   uint32_t open_cnt[2][2][4] = {};
+  constexpr int kTLoads = 8;      // forward-row loads in flight per lane
+  constexpr int kTBackLoads = 4;  // backward-row loads in flight per lane
   for (;;) {
     uint32_t xi = 0;
     if (lane == 0) xi = atomicAdd(&s_next, 1u);
@@ -382,14 +384,18 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
           fields_b += 1ull << (16 * ((cxy < xfree ? 1 : 0) + (sy > cxy ? 2 : 0)));
         }
       };
-      for (uint32_t j0 = 0; j0 < n; j0 += 2 * kWave) {
-        const uint32_t ja = j0 + lane, jb = ja + kWave;
-        const uint32_t ea = ja < n ? row[ja] : 0xFFFFFFFFu;
-        const uint32_t eb = jb < n ? row[jb] : 0xFFFFFFFFu;
-        visit(ja, ea);
-        visit(jb, eb);
-        // stop once this pair of chunks reached past the anchor (or the row's end)
-        if (__ballot(jb >= n || (int32_t)(eb >> kCntBits) >= i) != 0ull) break;
+      for (uint32_t j0 = 0; j0 < n; j0 += kTLoads * kWave) {
+        uint32_t e[kTLoads];
+#pragma unroll
+        for (int u = 0; u < kTLoads; ++u) {
+          const uint32_t ju = j0 + lane + u * kWave;
+          e[u] = ju < n ? row[ju] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < kTLoads; ++u) visit(j0 + lane + u * kWave, e[u]);
+        // stop once these chunks reached past the anchor (or the row's end)
+        const uint32_t jl = j0 + lane + (kTLoads - 1) * kWave;
+        if (__ballot(jl >= n || (int32_t)(e[kTLoads - 1] >> kCntBits) >= i) != 0ull) break;
       }
     }
     // (e) backward row of x inside [lo_i, x): triples (y, x, i) open at x. The row is ascending,
@@ -397,19 +403,27 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     {
       const uint32_t* row = brows + boff[x];
       const int32_t n = (int32_t)bdeg[x];
-      for (int32_t jend = n; jend > 0; jend -= kWave) {
-        const int32_t j = jend - kWave + lane;
-        bool inside = false;
-        if (j >= 0) {
-          const uint32_t e = row[j];
-          const int32_t yr = (int32_t)(e >> kCntBits) - base;
-          inside = yr >= 0;
-          if (inside && M[yr] == (MT)0) {
-            const int cxy = (int)(e & kCntMask);
+      // kTBackLoads chunks per step, the latest first; an entry before the window ends the scan
+      for (int32_t jend = n; jend > 0; jend -= kTBackLoads * kWave) {
+        uint32_t e[kTBackLoads];
+#pragma unroll
+        for (int u = 0; u < kTBackLoads; ++u) {
+          const int32_t ju = jend - (u + 1) * kWave + lane;
+          e[u] = ju >= 0 ? row[ju] : 0u;
+        }
+        bool before = false;
+#pragma unroll
+        for (int u = 0; u < kTBackLoads; ++u) {
+          if (jend - (u + 1) * kWave + lane < 0) continue;
+          const int32_t yr = (int32_t)(e[u] >> kCntBits) - base;
+          if (yr < 0) {
+            before = true;
+          } else if (M[yr] == (MT)0) {
+            const int cxy = (int)(e[u] & kCntMask);
             fields_e += 1ull << (16 * ((cxy < xfree ? 1 : 0) + ((int)sz[yr] > cxy ? 2 : 0)));
           }
         }
-        if (__ballot(j >= 0 && !inside)) break;
+        if (__ballot(before)) break;
       }
     }
     // (I is uniform in the wave; both branches index the counters with constants)
