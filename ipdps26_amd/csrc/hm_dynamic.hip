@@ -119,10 +119,14 @@ __global__ __launch_bounds__(kBlock) void k_full_fill(int32_t E, const uint32_t*
 // Histogram: per wave, kDeltaCopies interleaved int32 copies updated with per-lane LDS atomics.
 constexpr int kDeltaWaves = 16;
 constexpr int kDeltaCopies = 8;
-constexpr int kDeltaLoads = 8;
+// Row loads in flight per lane: many when the LDS table already leaves one workgroup per CU
+// (the registers they take cost no occupancy), few when several workgroups fit per CU.
+constexpr int kDeltaLoadsWide = 16;
+constexpr int kDeltaLoadsNarrow = 2;
+constexpr size_t kDeltaOneGroupLds = 80 * 1024;  // above this, one workgroup per CU (160 KB LDS)
 constexpr size_t kDeltaTiledStaticLds = 20 * 1024;  // k_delta_tiled: histogram, classes, tables
 
-template <typename MT, int NW>
+template <typename MT, int NW, int kDeltaLoads>
 __global__ __launch_bounds__(NW * kWave) void k_delta(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
@@ -706,8 +710,10 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
         d.spm.as<MT>(), d.acc.as<unsigned long long>());
   } else {
     const size_t lds = (((size_t)E * sizeof(MT) + (size_t)E) + 15) & ~(size_t)15;
-    HM_TRY(allow_lds((const void*)k_delta<MT, kDeltaWaves>, lds));
-    k_delta<MT, kDeltaWaves><<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
+    const bool one_group = lds + kDeltaTiledStaticLds > kDeltaOneGroupLds;
+    auto fn = one_group ? k_delta<MT, kDeltaWaves, kDeltaLoadsWide> : k_delta<MT, kDeltaWaves, kDeltaLoadsNarrow>;
+    HM_TRY(allow_lds((const void*)fn, lds));
+    fn<<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
         E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
         c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
         d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ids.as<int32_t>(),
