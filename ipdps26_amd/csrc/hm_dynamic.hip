@@ -838,7 +838,7 @@ static hm_status apply_batch(hm_graph* g, DynScratch& d, const std::vector<uint8
   c.E = E2;
   c.V = newV;
   c.I = (int64_t)I2;
-  HM_HIP(hipStreamSynchronize(st));
+  // no wait here: the caller's host arrays stay alive until its stream has drained
   return HM_OK;
 }
 
@@ -901,9 +901,7 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
   d.rows_valid = false;  // the store changes: the held rows describe the old graph
   HM_HIP(hipEventRecord(g->ev[3], g->stream));
   HM_TRY(apply_batch(g, d, delmask, ins_off32, inodes, n_ins, newV));
-  HM_HIP(hipEventRecord(g->ev[7], g->stream));
-  HM_HIP(hipEventSynchronize(g->ev[7]));
-  HM_HIP(hipEventElapsedTime(&g->ms_upd_store, g->ev[3], g->ev[7]));
+  HM_HIP(hipEventRecord(g->ev[7], g->stream));  // read once the stream has drained
   g->csr.max_size = maxsz;
   for (int32_t k = 0; k < n_del; ++k) g->live[(size_t)del_ids[k]] = 0;
   g->live.resize((size_t)E + n_ins, 1);
@@ -922,6 +920,8 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
     g->upd_added = (int64_t)h[kDynAccAdded];
     g->upd_entries = (int64_t)h[kDynAccEntries];
   }
+  HM_HIP(hipEventSynchronize(g->ev[7]));
+  HM_HIP(hipEventElapsedTime(&g->ms_upd_store, g->ev[3], g->ev[7]));
   g->have_stats = false;
   return HM_OK;
 }
