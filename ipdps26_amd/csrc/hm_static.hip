@@ -293,31 +293,38 @@ __global__ __launch_bounds__(NW * kWave) void k_tri_tiled(
         uint32_t pos = __shfl(cb, leader);
         const int sbL = szr[rbL];
         const uint32_t* rowB = rows + obL;
+        // kTriLoads chunks of b's row per step; the row is ascending, so the entries inside the
+        // tile are a prefix and each chunk's valid entries follow the previous chunk's
         for (;;) {
-          const uint32_t j = pos + lane;
-          bool v = false;
-          uint32_t e = 0;
-          if (j < nBL) {
-            e = rowB[j];
-            v = (int32_t)(e >> kCntBits) < t1;
+          uint32_t e[kTriLoads];
+          bool v[kTriLoads];
+#pragma unroll
+          for (int u = 0; u < kTriLoads; ++u) {
+            const uint32_t j = pos + (uint32_t)(u * kWave) + lane;
+            e[u] = j < nBL ? rowB[j] : 0xFFFFFFFFu;
+            v[u] = j < nBL && (int32_t)(e[u] >> kCntBits) < t1;
           }
-          if (v) {
-            const int32_t rc = (int32_t)(e >> kCntBits);
-            const MT mc = M[rc - t0];
-            if (mc) {
-              const int cbc = (int)(e & kCntMask);
-              const int cac = popc(mc);
-              const int g = popc((MT)(mabL & mc));
-              const int sc = szt[rc - t0];
-              atomicAdd(&hc[tbl[venn_id(sa, sbL, sc, cabL, cbc, cac, g)] * kCopies], 1);
-              atomicSub(&hc[open_bin(sa, sbL, sc, cabL, cac) * kCopies], 1);
-              atomicSub(&hc[open_bin(sbL, sa, sc, cabL, cbc) * kCopies], 1);
-              atomicSub(&hc[open_bin(sc, sa, sbL, cac, cbc) * kCopies], 1);
+          uint32_t nv = 0;
+#pragma unroll
+          for (int u = 0; u < kTriLoads; ++u) {
+            if (v[u]) {
+              const int32_t rc = (int32_t)(e[u] >> kCntBits);
+              const MT mc = M[rc - t0];
+              if (mc) {
+                const int cbc = (int)(e[u] & kCntMask);
+                const int cac = popc(mc);
+                const int g = popc((MT)(mabL & mc));
+                const int sc = szt[rc - t0];
+                atomicAdd(&hc[tbl[venn_id(sa, sbL, sc, cabL, cbc, cac, g)] * kCopies], 1);
+                atomicSub(&hc[open_bin(sa, sbL, sc, cabL, cac) * kCopies], 1);
+                atomicSub(&hc[open_bin(sbL, sa, sc, cabL, cbc) * kCopies], 1);
+                atomicSub(&hc[open_bin(sc, sa, sbL, cac, cbc) * kCopies], 1);
+              }
             }
+            nv += (uint32_t)__popcll(__ballot(v[u]));
           }
-          const uint32_t nv = (uint32_t)__popcll(__ballot(v));
           pos += nv;
-          if (nv < (uint32_t)kWave) break;
+          if (nv < (uint32_t)(kTriLoads * kWave)) break;
         }
         if (lane == leader) cb = pos;
       }
