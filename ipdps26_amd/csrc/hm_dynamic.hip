@@ -124,6 +124,7 @@ constexpr int kDeltaCopies = 8;
 constexpr int kDeltaLoadsWide = 16;
 constexpr int kDeltaLoadsNarrow = 2;
 constexpr size_t kDeltaOneGroupLds = 80 * 1024;  // above this, one workgroup per CU (160 KB LDS)
+constexpr int kDeltaTiledLoads = 4;  // row chunks per step of the tiled delta
 constexpr size_t kDeltaTiledStaticLds = 20 * 1024;  // k_delta_tiled: histogram, classes, tables
 
 template <typename MT, int NW, int kDeltaLoads>
@@ -460,34 +461,41 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
       const uint32_t* rowB = rows + roff[b];
       const uint32_t nB = (uint32_t)rlen[b];
       const uint32_t pos0 = pos;
+      // kDeltaTiledLoads chunks of b's row per step; the row's entries inside the tile are a
+      // prefix, so each chunk's valid entries follow the previous chunk's
       for (;;) {
-        const uint32_t j = pos + lane;
-        bool v = false;
-        uint32_t e = 0;
-        if (j < nB) {
-          e = rowB[j];
-          v = (int32_t)(e >> kCntBits) < t1;
+        uint32_t e[kDeltaTiledLoads];
+        bool v[kDeltaTiledLoads];
+#pragma unroll
+        for (int u = 0; u < kDeltaTiledLoads; ++u) {
+          const uint32_t j = pos + (uint32_t)(u * kWave) + lane;
+          e[u] = j < nB ? rowB[j] : 0xFFFFFFFFu;
+          v[u] = j < nB && (int32_t)(e[u] >> kCntBits) < t1;
         }
-        if (v) {
-          const int32_t c = (int32_t)(e >> kCntBits);
-          if (c != f && !(flag[c] && c < f)) {
-            const int cbc = (int)(e & kCntMask);
-            const MT mc = M[c - t0];
-            const int sc = esize[c];
-            if (mc) {
-              if (c > b) {
-                const int cfc = popc(mc);
-                atomicAdd(&hc[tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))] * kDeltaCopies], 1);
-                atomicSub(&hc[open_bin(sf, sb, sc, cfb, cfc) * kDeltaCopies], 1);
+        uint32_t nv = 0;
+#pragma unroll
+        for (int u = 0; u < kDeltaTiledLoads; ++u) {
+          if (v[u]) {
+            const int32_t c = (int32_t)(e[u] >> kCntBits);
+            if (c != f && !(flag[c] && c < f)) {
+              const int cbc = (int)(e[u] & kCntMask);
+              const MT mc = M[c - t0];
+              const int sc = esize[c];
+              if (mc) {
+                if (c > b) {
+                  const int cfc = popc(mc);
+                  atomicAdd(&hc[tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))] * kDeltaCopies], 1);
+                  atomicSub(&hc[open_bin(sf, sb, sc, cfb, cfc) * kDeltaCopies], 1);
+                }
+              } else {
+                atomicAdd(&hc[open_bin(sb, sf, sc, cfb, cbc) * kDeltaCopies], 1);
               }
-            } else {
-              atomicAdd(&hc[open_bin(sb, sf, sc, cfb, cbc) * kDeltaCopies], 1);
             }
           }
+          nv += (uint32_t)__popcll(__ballot(v[u]));
         }
-        const uint32_t nv = (uint32_t)__popcll(__ballot(v));
         pos += nv;
-        if (nv < (uint32_t)kWave) break;
+        if (nv < (uint32_t)(kDeltaTiledLoads * kWave)) break;
       }
       if (lane == 0) {
         cur[st + ib] = pos;
