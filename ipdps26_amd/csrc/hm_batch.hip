@@ -25,7 +25,6 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
-#include <cmath>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -225,20 +224,6 @@ constexpr unsigned long long kOrderChunkBytes[kNumOrders] = {0ull, 64ull << 20, 
 // plus the small tables.
 constexpr size_t kBatchStaticLds = 20 * 1024;
 
-template <typename MT>
-const void* tri_kernel(int form) {
-  switch (form) {
-    case 0: return (const void*)k_batch_tri<MT, 4, 8>;
-    case 1: return (const void*)k_batch_tri<MT, 4, 16>;
-    case 2: return (const void*)k_batch_tri<MT, 8, 8>;
-    case 3: return (const void*)k_batch_tri<MT, 8, 16>;
-    case 4: return (const void*)k_batch_tri<MT, 4, 8>;
-    case 5: return (const void*)k_batch_tri<MT, 4, 16>;
-    case 6: return (const void*)k_batch_tri<MT, 8, 8>;
-    default: return (const void*)k_batch_tri<MT, 8, 16>;
-  }
-}
-
 }  // namespace hm
 
 struct hm_batch {
@@ -272,42 +257,42 @@ size_t table_bytes(int32_t E, bool with_sizes) {
   return (v + 15) & ~(size_t)15;
 }
 
-template <typename MT>
-hm_status launch_tri(hm_batch* b, int form, int32_t Et, const BatchView* views, size_t lds,
-                     unsigned long long* acc_tri) {
-  const TriForm f = kTriForms[form];
-  const void* fn = tri_kernel<MT>(form);
+// One triangle form: NW waves per centre, NC histogram copies per wave; persistent: a grid of
+// (resident workgroups per CU) x CUs fed by the work counter, else one workgroup per centre.
+template <typename MT, int NW, int NC>
+hm_status launch_tri_form(hm_batch* b, bool persistent, int32_t Et, const BatchView* views, size_t lds,
+                          unsigned long long* acc_tri) {
+  const void* fn = (const void*)k_batch_tri<MT, NW, NC>;
   HM_TRY(allow_lds(fn, lds));
   int32_t* queue = nullptr;
   unsigned grid = (unsigned)Et;
-  if (f.persistent) {
+  if (persistent) {
     int per_cu = 0;
-    HM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, f.waves * kWave, lds));
+    HM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NW * kWave, lds));
     const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(b->cu_count, 1);
     grid = (unsigned)std::min<int64_t>(Et, resident);
     HM_TRY(b->queue.ensure(sizeof(int32_t)));
     queue = b->queue.as<int32_t>();
     k_batch_zero_i32<<<1, 1, 0, b->stream>>>(queue);
   }
-  const int32_t* order = b->order.as<int32_t>();
-  const int32_t* gid = b->gid.as<int32_t>();
-  const int32_t* rank = b->rank.as<int32_t>();
-  const int32_t* perm = b->perm.as<int32_t>();
-  const uint8_t* szr = b->szr.as<uint8_t>();
-  const unsigned long long* roff = b->roff.as<unsigned long long>();
-  const uint32_t* rlen = b->rlen.as<uint32_t>();
-  const uint32_t* rows = b->rows.as<uint32_t>();
-  const unsigned threads = (unsigned)(f.waves * kWave);
-#define HM_BATCH_TRI(NW, NC)                                                                      \
-  k_batch_tri<MT, NW, NC><<<grid, threads, lds, b->stream>>>(Et, order, views, gid, rank, perm,   \
-                                                             szr, roff, rlen, rows, acc_tri, queue)
-  if (f.waves == 4 && f.copies == 8) HM_BATCH_TRI(4, 8);
-  else if (f.waves == 4) HM_BATCH_TRI(4, 16);
-  else if (f.copies == 8) HM_BATCH_TRI(8, 8);
-  else HM_BATCH_TRI(8, 16);
-#undef HM_BATCH_TRI
+  k_batch_tri<MT, NW, NC><<<grid, NW * kWave, lds, b->stream>>>(
+      Et, b->order.as<int32_t>(), views, b->gid.as<int32_t>(), b->rank.as<int32_t>(), b->perm.as<int32_t>(),
+      b->szr.as<uint8_t>(), b->roff.as<unsigned long long>(), b->rlen.as<uint32_t>(), b->rows.as<uint32_t>(),
+      acc_tri, queue);
   HM_HIP(hipGetLastError());
   return HM_OK;
+}
+
+template <typename MT>
+hm_status launch_tri(hm_batch* b, int form, int32_t Et, const BatchView* views, size_t lds,
+                     unsigned long long* acc_tri) {
+  const TriForm f = kTriForms[form];
+  const bool p = f.persistent != 0;
+  if (f.waves == 4)
+    return f.copies == 8 ? launch_tri_form<MT, 4, 8>(b, p, Et, views, lds, acc_tri)
+                         : launch_tri_form<MT, 4, 16>(b, p, Et, views, lds, acc_tri);
+  return f.copies == 8 ? launch_tri_form<MT, 8, 8>(b, p, Et, views, lds, acc_tri)
+                       : launch_tri_form<MT, 8, 16>(b, p, Et, views, lds, acc_tri);
 }
 
 // Counts the graphs gs (all dense-eligible for MT) into out[k * 30 + bin].

@@ -27,7 +27,6 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
-#include <chrono>
 #include <cstring>
 #include <vector>
 
