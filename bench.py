@@ -63,7 +63,7 @@ DESC = {
                   "batches of 0.5-5% of E (70/30 insert/delete), seed 5000",
 }
 # Bounded CPU samples: a prefix of the same input sized for ~10-30 s of reference-program time.
-CPU_SAMPLE_EDGES = {"temporal_c4": 600, "static_c1": 3500, "dynamic_c3": 4000}
+CPU_SAMPLE_EDGES = {"temporal_c4": 700, "static_c1": 4000, "dynamic_c3": 4600}
 DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20, "dynamic_c3": 10, "serving_c5": 1}
 DYN_BASE_EDGES, DYN_BASE_NODES = 20_000, 8_000
 
