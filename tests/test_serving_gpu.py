@@ -170,3 +170,27 @@ def test_batched_counts_equal_plain(gpu_ready, maintain):
         assert np.array_equal(a, b), f"request {k} ({reqs[k]['kind']})"
     if not maintain:  # all submitted at once: several small graphs are ready with a count at a time
         assert served["batched_counts"] > 0 and served["batch_launches"] > 0
+
+
+def test_batched_counts_with_graph_removal(gpu_ready):
+    """Graphs removed while batched counts of other graphs are queued: the remaining graphs'
+    counts stay exact, removal waits for a graph's own requests."""
+    graphs = W.serving_graphs(10, lo=150, hi=600, seed=31)
+    expect = []
+    for off, nodes in graphs:
+        with hm.Graph(off, nodes) as g:
+            expect.append(g.count_static().astype(np.int64))
+    with hm.Server(1 << 30, 2, batch_graphs=8) as srv:
+        gids = [srv.add_graph(off, nodes) for off, nodes in graphs]
+        for t in [srv.submit_count(g) for g in gids]:  # first counts: alone, they time the graphs
+            srv.wait(t)
+        tickets = [(k, srv.submit_count(g)) for _ in range(4) for k, g in enumerate(gids)]
+        for k in (1, 4, 7):
+            srv.remove_graph(gids[k])  # waits for graph k's queued counts
+        for k, t in tickets:
+            assert np.array_equal(srv.wait(t)["counts"], expect[k]), k
+        again = [(k, srv.submit_count(gids[k])) for k in range(10) if k not in (1, 4, 7)]
+        for k, t in again:
+            assert np.array_equal(srv.wait(t)["counts"], expect[k]), k
+        st = srv.stats()
+        assert st["graphs"] == 7 and st["batch_launches"] > 0
