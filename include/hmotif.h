@@ -112,6 +112,18 @@ hm_status hm_count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]);
  * (dynamicHyperGraph/src/HMotifCount.cu:16-95). */
 hm_status hm_graph_set_tile(hm_graph* g, int32_t tile_ids);
 
+/* Form of the one-table triangle kernel of hm_count_static on g (waves per centre workgroup and
+ * int32 LDS histogram copies per wave). waves = 0 (default): the form autotuned per shape (mask
+ * width, log2 of the hyperedge count) on the first count of that shape in the process — each of
+ * the 4 forms (4 or 8 waves x 8 or 16 copies) run once, their bins checked equal, the fastest
+ * kept; waves = -1: as 0, but tune the shape again on the next count; otherwise waves 4 or 8 and
+ * copies 8 or 16. Counts are identical in every form. No counterpart in the reference, whose
+ * kernel runs one thread per hyperedge (dynamicHyperGraph/src/HMotifCount.cu:16-95). */
+hm_status hm_graph_set_tri_form(hm_graph* g, int32_t waves, int32_t copies);
+/* Form used by the last one-table count of g (0, 0 after a tiled count) and whether that count
+ * ran the autotuner. */
+hm_status hm_graph_get_tri_form(hm_graph* g, int32_t* waves, int32_t* copies, int32_t* tuned);
+
 /* on != 0: after an eager hm_count_static, the whole count of g (every launch, no host wait) is
  * captured as a hipGraph, and later counts of the unchanged graph replay it — one graph launch
  * and one wait instead of ~15 launches and two waits. Any update or tile change drops the capture

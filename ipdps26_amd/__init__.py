@@ -95,6 +95,8 @@ _SIGNATURES = [
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
     ("hm_graph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_set_replay", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("hm_graph_set_tri_form", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    ("hm_graph_get_tri_form", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
     ("hm_graph_replay_state", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p]),
     ("hm_tgraph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_update_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -297,6 +299,17 @@ class Graph:
             _ptr(ioff, ctypes.c_int64), _ptr(inodes if inodes.size else np.zeros(1, np.int32), ctypes.c_int32),
             n_ins, _ptr(assigned, ctypes.c_int32), _ptr(out, ctypes.c_int64) if delta else None))
         return assigned[:n_ins], (out if delta else None)
+
+    def set_tri_form(self, waves: int = 0, copies: int = 16) -> None:
+        """waves=0: autotuned per shape; -1: tune again on the next count; else 4 or 8 waves with
+        8 or 16 histogram copies."""
+        _check(lib().hm_graph_set_tri_form(self._h, int(waves), int(copies)))
+
+    def tri_form(self) -> tuple[int, int, bool]:
+        """(waves, copies, tuned) of the last one-table count."""
+        w, c, t = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        _check(lib().hm_graph_get_tri_form(self._h, ctypes.byref(w), ctypes.byref(c), ctypes.byref(t)))
+        return w.value, c.value, bool(t.value)
 
     def set_replay(self, on: bool = True) -> None:
         """Capture the count of the unchanged graph as a hipGraph and replay it on later counts."""

@@ -102,6 +102,7 @@ struct CountScratch {
   DevBuf ranks;    // int32[E]    iota for the work sort
   DevBuf dlow;     // uint32[E]   neighbours with a smaller id, per a
   DevBuf acc;      // uint64[64]  histogram (30 bins) + statistics
+  DevBuf acc2;     // uint64[64]  triangle-form autotuning runs
   DevBuf tmp;      // hipcub temporary storage
   // tiled path only
   DevBuf vkeys;    // uint64[I]   (node << 32 | rank) of every incidence, then sorted
@@ -112,7 +113,7 @@ struct CountScratch {
   DevBuf pm;       // mask[U]     a's nodes contained in b, per upper-row entry (a, b)
   void release() {
     for (DevBuf* b : {&deg, &ids, &perm, &rank, &degr, &roff, &rlen, &rows, &szr, &work, &work2,
-                      &order, &ranks, &dlow, &acc, &tmp, &vkeys, &vkeys2, &vrank, &uoff, &cur, &pm})
+                      &order, &ranks, &dlow, &acc, &acc2, &tmp, &vkeys, &vkeys2, &vrank, &uoff, &cur, &pm})
       b->release();
   }
 };
@@ -163,4 +164,9 @@ struct hm_graph {
   hipGraphExec_t replay_exec = nullptr;
   uint64_t replay_version = 0;
   bool last_replayed = false;
+  // one-table triangle form: -1 = autotuned per shape (tri_retune: tune again on the next count)
+  int32_t tri_form = -1;
+  bool tri_retune = false;
+  int32_t last_tri_form = -1;  // form of the last one-table count (-1: none or tiled)
+  bool last_tri_tuned = false;
 };

@@ -25,6 +25,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstring>
 #include <vector>
 
@@ -74,13 +77,19 @@ __global__ __launch_bounds__(kBlock) void k_work(int32_t E, const unsigned long 
   if (lane_id() == 0) work[r] = s + n;
 }
 
-// 8 waves per centre workgroup (c1: 4 waves 9.03 ms, 8 waves 7.85 ms, 16 waves 8.09 ms), each wave
-// with 16 interleaved int32 histogram copies (copy = lane & 15) for its LDS atomics (c1: 2 copies
-// 26.6 ms, 4: 14.2 ms, 8: 7.75 ms, 16: 6.91 ms, 32: 8.06 ms, 64: 13.7 ms).
+// Forms of the one-table triangle kernel: waves per centre workgroup x int32 histogram copies
+// per wave (copy = lane % copies). Measured on c1: 4 waves 9.03 ms, 8 waves 7.85 ms, 16 waves
+// 8.09 ms; 2 copies 26.6 ms, 4: 14.2 ms, 8: 7.75 ms, 16: 6.91 ms, 32: 8.06 ms, 64: 13.7 ms. Small
+// graphs prefer 4 waves; the form is autotuned per shape (below). The tiled kernel keeps 8 waves.
 constexpr int kTriWaves = 8;
-constexpr int kTriCopies = 16;
+struct TriFormSpec {
+  int waves, copies;
+};
+constexpr TriFormSpec kTriFormSpecs[] = {{4, 8}, {4, 16}, {8, 8}, {8, 16}};
+constexpr int kNumTriFormSpecs = 4;
+constexpr int kDefaultTriForm = 3;  // 8 waves, 16 copies
 
-template <typename MT, int NW>
+template <typename MT, int NW, int NC>
 __global__ __launch_bounds__(NW * kWave) void k_tri(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges,
@@ -89,8 +98,8 @@ __global__ __launch_bounds__(NW * kWave) void k_tri(
     const uint32_t* __restrict__ rlen, const uint32_t* __restrict__ rows,
     const int32_t* __restrict__ order, unsigned long long* __restrict__ acc) {
   extern __shared__ __align__(16) unsigned char smem[];
-  tri_item<MT, NW, kTriCopies>(reinterpret_cast<MT*>(smem), E, order[blockIdx.x], eoff, enodes, voff,
-                               vedges, rank, perm, szr, roff, rlen, rows, acc);
+  tri_item<MT, NW, NC>(reinterpret_cast<MT*>(smem), E, order[blockIdx.x], eoff, enodes, voff, vedges,
+                       rank, perm, szr, roff, rlen, rows, acc);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -464,6 +473,92 @@ static int32_t auto_tile(int lim) {
   return (int32_t)((avail / (sizeof(MT) + 1)) & ~(size_t)63);
 }
 
+// Autotuned one-table triangle forms, per process: (device, mask bytes, log2 E) -> form index.
+static std::mutex g_tri_mu;
+static std::map<std::tuple<int, int, int>, int> g_tri_tuned;
+
+static int ceil_log2_i(int64_t x) {
+  int k = 0;
+  while (((int64_t)1 << k) < x) ++k;
+  return k;
+}
+
+template <typename MT, int NW, int NC>
+static hm_status launch_tri_one(hm_graph* g, size_t lds, unsigned long long* acc) {
+  HM_TRY(set_lds_attr<MT>((const void*)k_tri<MT, NW, NC>, lds));
+  DeviceCsr& c = g->csr;
+  CountScratch& s = g->scratch;
+  k_tri<MT, NW, NC><<<c.E, NW * kWave, lds, g->stream>>>(
+      c.E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+      s.rank.as<int32_t>(), s.perm.as<int32_t>(), s.szr.as<uint8_t>(), s.roff.as<unsigned long long>(),
+      s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), s.order.as<int32_t>(), acc);
+  HM_HIP(hipGetLastError());
+  return HM_OK;
+}
+
+template <typename MT>
+static hm_status launch_tri_form(hm_graph* g, int form, size_t lds, unsigned long long* acc) {
+  switch (form) {
+    case 0: return launch_tri_one<MT, 4, 8>(g, lds, acc);
+    case 1: return launch_tri_one<MT, 4, 16>(g, lds, acc);
+    case 2: return launch_tri_one<MT, 8, 8>(g, lds, acc);
+    default: return launch_tri_one<MT, 8, 16>(g, lds, acc);
+  }
+}
+
+// Form of the one-table triangle kernel for g: forced, or tuned for g's shape, or tuned now (every
+// form once into scratch, the bins checked equal, the fastest kept for the process). capture:
+// the form of the eager count just made.
+template <typename MT>
+static hm_status pick_tri_form(hm_graph* g, size_t lds, bool capture, int* form_out) {
+  if (capture) {
+    *form_out = g->last_tri_form >= 0 ? g->last_tri_form : kDefaultTriForm;
+    return HM_OK;
+  }
+  g->last_tri_tuned = false;
+  int form = g->tri_form;
+  const auto key = std::make_tuple(g->device, (int)sizeof(MT), ceil_log2_i(g->csr.E));
+  if (form < 0) {
+    std::lock_guard<std::mutex> lk(g_tri_mu);
+    if (g->tri_retune) g_tri_tuned.erase(key);
+    auto it = g_tri_tuned.find(key);
+    if (it != g_tri_tuned.end()) form = it->second;
+  }
+  g->tri_retune = false;
+  if (form < 0) {
+    CountScratch& s = g->scratch;
+    HM_TRY(s.acc2.ensure(sizeof(unsigned long long) * kAccCount));
+    unsigned long long* acc2 = s.acc2.as<unsigned long long>();
+    unsigned long long first[kStaticBins], cur[kStaticBins];
+    float best = 0.f;
+    for (int f = 0; f < kNumTriFormSpecs; ++f) {
+      k_zero_u64<<<1, 64, 0, g->stream>>>(acc2, kAccCount);
+      HM_HIP(hipEventRecord(g->ev[4], g->stream));
+      HM_TRY(launch_tri_form<MT>(g, f, lds, acc2));
+      HM_HIP(hipEventRecord(g->ev[5], g->stream));
+      HM_HIP(hipMemcpyAsync(cur, acc2, sizeof(cur), hipMemcpyDeviceToHost, g->stream));
+      HM_HIP(hipStreamSynchronize(g->stream));
+      float ms = 0.f;
+      HM_HIP(hipEventElapsedTime(&ms, g->ev[4], g->ev[5]));
+      if (f == 0) {
+        std::memcpy(first, cur, sizeof(cur));
+      } else if (std::memcmp(first, cur, sizeof(cur)) != 0) {
+        set_error("static count: triangle forms %d and 0 disagree", f);
+        return HM_ERR_DEVICE;
+      }
+      if (f == 0 || ms < best) {
+        best = ms;
+        form = f;
+      }
+    }
+    std::lock_guard<std::mutex> lk(g_tri_mu);
+    g_tri_tuned[key] = form;
+    g->last_tri_tuned = true;
+  }
+  *form_out = form;
+  return HM_OK;
+}
+
 // Static count. T == 0: one LDS table over all ids (dense kernels); T > 0: tiles of T ids/ranks.
 // capture: only enqueue the launches (the stream is being captured, the graph is unchanged since
 // the eager run that sized every buffer): no host waits, no events, no result copy.
@@ -484,7 +579,6 @@ static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32
   } else {
     HM_TRY(set_lds_attr<MT>((const void*)k_proj_count<MT>, lds_cnt));
     HM_TRY(set_lds_attr<MT>((const void*)k_proj_fill<MT>, lds_cnt));
-    HM_TRY(set_lds_attr<MT>((const void*)k_tri<MT, kTriWaves>, lds_tri));
   }
 
   HM_TRY(s.deg.ensure(sizeof(uint32_t) * E));
@@ -606,6 +700,9 @@ static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32
       HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
     }
   }
+  int form = -1;
+  if (!tiled) HM_TRY(pick_tri_form<MT>(g, lds_tri, capture, &form));
+  g->last_tri_form = form;
   if (!capture) HM_HIP(hipEventRecord(g->ev[1], st));
   // 6. triangles
   if (tiled)
@@ -615,11 +712,7 @@ static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32
         s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
         s.uoff.as<unsigned long long>(), s.cur.as<uint32_t>(), s.pm.as<MT>(), s.order.as<int32_t>(), acc);
   else
-    k_tri<MT, kTriWaves><<<E, kTriWaves * kWave, lds_tri, st>>>(
-        E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(),
-        c.vedges.as<int32_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), s.szr.as<uint8_t>(),
-        s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
-        s.order.as<int32_t>(), acc);
+    HM_TRY(launch_tri_form<MT>(g, form, lds_tri, acc));
   HM_HIP(hipGetLastError());
   if (!capture) HM_HIP(hipEventRecord(g->ev[2], st));
   k_stats<<<64, kBlock, 0, st>>>(E, c.V, c.voff.as<uint32_t>(), s.deg.as<uint32_t>(),
@@ -858,6 +951,34 @@ hm_status hm_graph_set_tile(hm_graph* g, int32_t tile_ids) {
   if (tile_ids < 0) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_tile: negative tile");
   if (g->tile != tile_ids) ++g->version;
   g->tile = tile_ids;
+  return HM_OK;
+}
+
+hm_status hm_graph_set_tri_form(hm_graph* g, int32_t waves, int32_t copies) {
+  hm::clear_error();
+  if (!g) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_tri_form: null handle");
+  int form = -2;
+  if (waves == 0 || waves == -1) {
+    form = -1;
+  } else {
+    for (int f = 0; f < hm::kNumTriFormSpecs; ++f)
+      if (hm::kTriFormSpecs[f].waves == waves && hm::kTriFormSpecs[f].copies == copies) form = f;
+  }
+  if (form == -2)
+    return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_tri_form: waves must be 0, -1, 4 or 8 and copies 8 or 16");
+  g->tri_form = form;
+  g->tri_retune = waves == -1;
+  ++g->version;  // a captured count holds the previous form
+  return HM_OK;
+}
+
+hm_status hm_graph_get_tri_form(hm_graph* g, int32_t* waves, int32_t* copies, int32_t* tuned) {
+  hm::clear_error();
+  if (!g || !waves || !copies || !tuned) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_get_tri_form: null argument");
+  const int f = g->last_tri_form;
+  *waves = f >= 0 ? hm::kTriFormSpecs[f].waves : 0;
+  *copies = f >= 0 ? hm::kTriFormSpecs[f].copies : 0;
+  *tuned = g->last_tri_tuned ? 1 : 0;
   return HM_OK;
 }
 

@@ -94,3 +94,40 @@ def test_oversized_hyperedge_is_rejected(gpu_ready):
         with pytest.raises(hm.HMotifError) as ei:
             g.count_static()
     assert ei.value.status == 5
+
+
+# ---- triangle forms of the one-table count (hm_graph_set_tri_form) ------------------------------
+TRI_FORMS = [(4, 8), (4, 16), (8, 8), (8, 16)]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_every_tri_form_gives_the_same_counts(gpu_ready, seed):
+    rng = np.random.default_rng(7100 + seed)
+    kmax = [6, 20, 50][seed]
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(200, 900)),
+                                   num_nodes=int(rng.integers(kmax, 300)), kmax=kmax, dup_frac=0.2)
+    with hm.Graph(off, nodes) as g:
+        got = []
+        for w, c in TRI_FORMS:
+            g.set_tri_form(w, c)
+            got.append(g.count_static())
+            assert g.tri_form() == (w, c, False)
+    for x in got[1:]:
+        assert np.array_equal(x, got[0])
+    if seed == 0:
+        assert np.array_equal(got[0].astype(np.int64), O.static_mochy(off, nodes))
+
+
+def test_tri_form_autotuner(gpu_ready):
+    off, nodes = hm.generate_static(W.STATIC_C1)
+    with hm.Graph(off, nodes) as g, hm.Graph(off, nodes) as h:
+        g.set_tri_form(-1)  # tune this shape again
+        a = g.count_static()
+        w, c, tuned = g.tri_form()
+        assert tuned and (w, c) in TRI_FORMS
+        b = h.count_static()  # same shape: the tuned form, no tuning
+        assert h.tri_form() == (w, c, False)
+        g.set_replay(True)
+        g.count_static()
+        assert np.array_equal(g.count_static(), a) and g.replay_state() == (True, True)
+    assert np.array_equal(a, b)
