@@ -350,7 +350,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   // open_cnt[side][I][X + 2Y] after each x and into the histogram once per anchor.
   uint32_t open_cnt[2][2][4] = {};
   constexpr int kTLoads = 8;      // forward-row loads in flight per lane
-  constexpr int kTBackLoads = 4;  // backward-row loads in flight per lane
+  constexpr int kTBackLoads = 8;  // backward-row loads in flight per lane
   for (;;) {
     uint32_t xi = 0;
     if (lane == 0) xi = atomicAdd(&s_next, 1u);
