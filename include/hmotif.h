@@ -178,8 +178,43 @@ typedef struct hm_update_stats {
                            rows built by the previous update are still held (HIP events) */
   double ms_delta;      /* delta kernel, both phases (HIP events) */
   double ms_store;      /* store update: H2V / V2H rebuild, stream time including its host wait */
+  int32_t captured;     /* 1: the update ran as one captured graph (hm_graph_set_update_capture);
+                           ms_delta then holds the whole launch, ms_rows and ms_store 0 */
+  int32_t rows_overflow; /* 1: its rows after the batch outgrew the plan and the addition phase
+                           ran eagerly (its row build and delta added to ms_rows / ms_delta) */
 } hm_update_stats;
 hm_status hm_graph_update_stats(hm_graph* g, hm_update_stats* out);
+
+/* on != 0: every later hm_graph_update of g that asks for a delta (one id table: no forced tile,
+ * and the ids after the batch fit one delta table; hyperedges of at most 64 nodes) runs as ONE
+ * captured hipGraph — removal delta over the rows held from the previous update, store update
+ * (H2V / V2H rebuild), projection rows of the graph after the batch, addition delta — with the
+ * batch staged in pinned memory: one copy in, one graph launch, one copy out, one wait, instead
+ * of ~30 launches and three host waits. Every buffer is sized to a plan with headroom (ids,
+ * nodes, incidences, batch sizes, row entries); a batch that does not fit grows the plan and
+ * captures again (the store and the held rows are kept). Rows after the batch that outgrow the
+ * plan are caught on the device: the addition phase is skipped and rerun eagerly with larger
+ * buffers. A plan that cannot be made (or a capture holding a memset node) leaves g's updates
+ * eager. Deltas, stores and ids are identical either way. on = 0 drops the captured graph. No
+ * counterpart in the reference, whose update rebuilds both snapshots on the host and launches
+ * per step (dynamicHyperGraph/src/main.cu:332-353). */
+hm_status hm_graph_set_update_capture(hm_graph* g, int32_t on);
+typedef struct hm_update_capture_state {
+  int32_t on;               /* capture requested on g */
+  int32_t disabled;         /* a plan could not be made or replayed: g's updates run eagerly */
+  int32_t captured;         /* an instantiated update graph is held */
+  int32_t captures;         /* update graphs captured on g so far (first plan and each regrowth) */
+  int32_t last_captured;    /* the last update with a delta ran as the captured graph */
+  int32_t last_overflow;    /* ... and its rows outgrew the plan (addition phase rerun eagerly) */
+  int64_t id_capacity;      /* hyperedge ids the plan holds (deleted ids keep theirs) */
+  int64_t node_capacity;    /* node ids */
+  int64_t incidence_capacity;
+  int64_t row_capacity;     /* full projection-row entries */
+  int32_t batch_deletes;    /* deleted ids per batch */
+  int32_t batch_inserts;    /* inserted hyperedges per batch */
+  int64_t batch_incidences; /* inserted incidences per batch */
+} hm_update_capture_state;
+hm_status hm_graph_update_capture_state(hm_graph* g, hm_update_capture_state* out);
 
 /* Exact delta between two snapshots of one id space, counts(new_g) - counts(old_g), given the
  * ids deleted from old_g and the ids inserted into new_g (an id may be deleted and re-inserted;

@@ -98,6 +98,8 @@ _SIGNATURES = [
     ("hm_graph_set_tri_form", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("hm_graph_get_tri_form", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
     ("hm_graph_replay_state", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p]),
+    ("hm_graph_set_update_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("hm_graph_update_capture_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("hm_tgraph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_update_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("hm_count_strict_inc", ctypes.c_int,
@@ -321,10 +323,23 @@ class Graph:
         _check(lib().hm_graph_replay_state(self._h, ctypes.byref(cap), ctypes.byref(last)))
         return bool(cap.value), bool(last.value)
 
+    def set_update_capture(self, on: bool = True) -> None:
+        """Run every later update with a delta as one captured hipGraph (one launch, one wait),
+        its buffers sized to a plan with headroom that grows when a batch does not fit."""
+        _check(lib().hm_graph_set_update_capture(self._h, int(bool(on))))
+
+    def update_capture_state(self) -> dict:
+        """Plan of the captured update: on / disabled / captured, captures made, whether the
+        last update ran captured (and overflowed its rows), and the plan's capacities."""
+        st = _UpdateCaptureState()
+        _check(lib().hm_graph_update_capture_state(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _UpdateCaptureState._fields_}
+
     def update_stats(self) -> dict:
         """Of the last update with a delta: removed / added triples (held a deleted hyperedge
-        before, hold an inserted one after), row entries the delta kernel scanned, and the
-        milliseconds of the row build, the delta kernel and the store update."""
+        before, hold an inserted one after), row entries the delta kernel scanned, the
+        milliseconds of the row build, the delta kernel and the store update, and whether it
+        ran as the captured graph (then ms_delta holds the whole launch)."""
         st = _UpdateStats()
         _check(lib().hm_graph_update_stats(self._h, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in _UpdateStats._fields_}
@@ -333,7 +348,18 @@ class Graph:
 class _UpdateStats(ctypes.Structure):
     _fields_ = [("removed", ctypes.c_int64), ("added", ctypes.c_int64),
                 ("row_entries", ctypes.c_int64), ("ms_rows", ctypes.c_double),
-                ("ms_delta", ctypes.c_double), ("ms_store", ctypes.c_double)]
+                ("ms_delta", ctypes.c_double), ("ms_store", ctypes.c_double),
+                ("captured", ctypes.c_int32), ("rows_overflow", ctypes.c_int32)]
+
+
+class _UpdateCaptureState(ctypes.Structure):
+    _fields_ = [("on", ctypes.c_int32), ("disabled", ctypes.c_int32),
+                ("captured", ctypes.c_int32), ("captures", ctypes.c_int32),
+                ("last_captured", ctypes.c_int32), ("last_overflow", ctypes.c_int32),
+                ("id_capacity", ctypes.c_int64), ("node_capacity", ctypes.c_int64),
+                ("incidence_capacity", ctypes.c_int64), ("row_capacity", ctypes.c_int64),
+                ("batch_deletes", ctypes.c_int32), ("batch_inserts", ctypes.c_int32),
+                ("batch_incidences", ctypes.c_int64)]
 
 
 def count_delta(old: Graph, new: Graph, deleted, inserted) -> np.ndarray:

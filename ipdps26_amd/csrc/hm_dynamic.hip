@@ -49,15 +49,12 @@ constexpr int kDynAccCount = 40;
 
 // projected degree d_a (all neighbours, by id)
 template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_deg(int32_t E, const uint32_t* __restrict__ eoff,
-                                                const int32_t* __restrict__ enodes,
-                                                const uint32_t* __restrict__ voff,
-                                                const int32_t* __restrict__ vedges,
-                                                unsigned long long* __restrict__ deg) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  MT* M = reinterpret_cast<MT*>(smem);
+__device__ inline void deg_body(MT* M, int32_t E, int32_t a, const uint32_t* __restrict__ eoff,
+                                const int32_t* __restrict__ enodes,
+                                const uint32_t* __restrict__ voff,
+                                const int32_t* __restrict__ vedges,
+                                unsigned long long* __restrict__ deg) {
   __shared__ uint32_t red[kWavesPerBlock];
-  const int32_t a = blockIdx.x;
   if (eoff[a + 1] == eoff[a]) {  // empty (deleted) hyperedge
     if (threadIdx.x == 0) deg[a] = 0;
     return;
@@ -75,18 +72,25 @@ __global__ __launch_bounds__(kBlock) void k_deg(int32_t E, const uint32_t* __res
   }
 }
 
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_deg(int32_t E, const uint32_t* __restrict__ eoff,
+                                                const int32_t* __restrict__ enodes,
+                                                const uint32_t* __restrict__ voff,
+                                                const int32_t* __restrict__ vedges,
+                                                unsigned long long* __restrict__ deg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  deg_body<MT>(reinterpret_cast<MT*>(smem), E, (int32_t)blockIdx.x, eoff, enodes, voff, vedges, deg);
+}
+
 // full row of a by id: (x << 7 | C_ax) for every x != a with C_ax > 0, ascending
 template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_full_fill(int32_t E, const uint32_t* __restrict__ eoff,
-                                                      const int32_t* __restrict__ enodes,
-                                                      const uint32_t* __restrict__ voff,
-                                                      const int32_t* __restrict__ vedges,
-                                                      const unsigned long long* __restrict__ roff,
-                                                      uint32_t* __restrict__ rows) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  MT* M = reinterpret_cast<MT*>(smem);
+__device__ inline void full_fill_body(MT* M, int32_t E, int32_t a, const uint32_t* __restrict__ eoff,
+                                      const int32_t* __restrict__ enodes,
+                                      const uint32_t* __restrict__ voff,
+                                      const int32_t* __restrict__ vedges,
+                                      const unsigned long long* __restrict__ roff,
+                                      uint32_t* __restrict__ rows) {
   __shared__ uint32_t wcnt[kWavesPerBlock];
-  const int32_t a = blockIdx.x;
   if (eoff[a + 1] == eoff[a]) return;
   build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, nullptr);
   const int lane = lane_id(), w = wave_id();
@@ -111,6 +115,18 @@ __global__ __launch_bounds__(kBlock) void k_full_fill(int32_t E, const uint32_t*
   }
 }
 
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_full_fill(int32_t E, const uint32_t* __restrict__ eoff,
+                                                      const int32_t* __restrict__ enodes,
+                                                      const uint32_t* __restrict__ voff,
+                                                      const int32_t* __restrict__ vedges,
+                                                      const unsigned long long* __restrict__ roff,
+                                                      uint32_t* __restrict__ rows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  full_fill_body<MT>(reinterpret_cast<MT*>(smem), E, (int32_t)blockIdx.x, eoff, enodes, voff, vedges,
+                     roff, rows);
+}
+
 // L(S, G) for the changed hyperedges in `changed` (flag[x] = 1 iff x in S); adds sign * L to acc.
 // Each changed hyperedge f is shared by `parts` workgroups of NW waves (workgroup (f, p) takes the
 // neighbours b of f with index = p mod parts), so a batch of a few hundred hyperedges still fills
@@ -127,15 +143,13 @@ constexpr int kDeltaTiledLoads = 4;  // row chunks per step of the tiled delta
 constexpr size_t kDeltaTiledStaticLds = 20 * 1024;  // k_delta_tiled: histogram, classes, tables
 
 template <typename MT, int NW, int kDeltaLoads>
-__global__ __launch_bounds__(NW * kWave) void k_delta(
-    int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
-    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
-    const int32_t* __restrict__ vedges, const unsigned long long* __restrict__ roff,
-    const unsigned long long* __restrict__ rlen, const uint32_t* __restrict__ rows,
-    const int32_t* __restrict__ changed, const uint8_t* __restrict__ flag, int32_t parts,
-    long long sign, unsigned long long* __restrict__ acc) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  MT* M = reinterpret_cast<MT*>(smem);
+__device__ inline void delta_body(
+    MT* M, int32_t E, int32_t f, uint32_t part, int32_t parts, const uint32_t* __restrict__ eoff,
+    const int32_t* __restrict__ enodes, const uint8_t* __restrict__ esize,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges,
+    const unsigned long long* __restrict__ roff, const unsigned long long* __restrict__ rlen,
+    const uint32_t* __restrict__ rows, const uint8_t* __restrict__ flag, long long sign,
+    unsigned long long* __restrict__ acc) {
   // size of x (at most 64) in bits 0..6, [x in S] in bit 7: one LDS read per row entry
   uint8_t* szf = reinterpret_cast<uint8_t*>(M + E);
   __shared__ int32_t h[NW][32][kDeltaCopies];
@@ -144,8 +158,6 @@ __global__ __launch_bounds__(NW * kWave) void k_delta(
   __shared__ int8_t tbl[128];
   __shared__ uint32_t s_next;
   __shared__ unsigned long long s_scan;
-  const int32_t f = changed[blockIdx.x / parts];
-  const uint32_t part = blockIdx.x % parts;
   if (eoff[f + 1] == eoff[f]) return;
   for (int k = threadIdx.x; k < NW * 32 * kDeltaCopies; k += blockDim.x) (&h[0][0][0])[k] = 0;
   for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
@@ -290,6 +302,20 @@ __global__ __launch_bounds__(NW * kWave) void k_delta(
     }
   }
   if (threadIdx.x == 0 && s_scan) atomicAdd(&acc[kDynAccEntries], s_scan);
+}
+
+template <typename MT, int NW, int kDeltaLoads>
+__global__ __launch_bounds__(NW * kWave) void k_delta(
+    int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
+    const int32_t* __restrict__ vedges, const unsigned long long* __restrict__ roff,
+    const unsigned long long* __restrict__ rlen, const uint32_t* __restrict__ rows,
+    const int32_t* __restrict__ changed, const uint8_t* __restrict__ flag, int32_t parts,
+    long long sign, unsigned long long* __restrict__ acc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  delta_body<MT, NW, kDeltaLoads>(reinterpret_cast<MT*>(smem), E, changed[blockIdx.x / parts],
+                                  blockIdx.x % parts, parts, eoff, enodes, esize, voff, vedges,
+                                  roff, rlen, rows, flag, sign, acc);
 }
 
 // ---- tiled delta: hyperedge ids beyond one LDS table ------------------------------------------
@@ -610,9 +636,160 @@ __global__ void k_zero_acc(unsigned long long* p, int n) {
   for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0;
 }
 
+// ---- captured update: the same steps with the batch's sizes read from the device header ------
+// The grids span the plan's capacities; workgroups and threads past the batch's sizes leave at
+// once, so one instantiated graph serves every batch that fits the plan.
+
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_deg_cap(const UpdHdr* __restrict__ hd,
+                                                    const uint32_t* __restrict__ eoff,
+                                                    const int32_t* __restrict__ enodes,
+                                                    const uint32_t* __restrict__ voff,
+                                                    const int32_t* __restrict__ vedges,
+                                                    unsigned long long* __restrict__ deg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int32_t E = hd->E_new, a = (int32_t)blockIdx.x;
+  if (a >= E) {  // ids past the graph (up to the capacity) scan as empty rows
+    if (threadIdx.x == 0) deg[a] = 0;
+    return;
+  }
+  deg_body<MT>(reinterpret_cast<MT*>(smem), E, a, eoff, enodes, voff, vedges, deg);
+}
+
+// rows after the batch over capacity: flag it, so the fill and the addition phase are skipped
+__global__ void k_check_rows_cap(UpdHdr* __restrict__ hd, const unsigned long long* __restrict__ roff,
+                                 unsigned long long Pcap) {
+  if (threadIdx.x == 0) {
+    const unsigned long long P = roff[hd->E_new];
+    hd->P_new = P;
+    if (P > Pcap) hd->overflow = 1;
+  }
+}
+
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_full_fill_cap(const UpdHdr* __restrict__ hd,
+                                                          const uint32_t* __restrict__ eoff,
+                                                          const int32_t* __restrict__ enodes,
+                                                          const uint32_t* __restrict__ voff,
+                                                          const int32_t* __restrict__ vedges,
+                                                          const unsigned long long* __restrict__ roff,
+                                                          uint32_t* __restrict__ rows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int32_t E = hd->E_new, a = (int32_t)blockIdx.x;
+  if (hd->overflow || a >= E) return;
+  full_fill_body<MT>(reinterpret_cast<MT*>(smem), E, a, eoff, enodes, voff, vedges, roff, rows);
+}
+
+// phase 0: L(deleted ids, graph before the batch), subtracted; phase 1: L(inserted ids, graph
+// after it), added (skipped when the rows overflowed)
+template <typename MT, int NW, int kDeltaLoads>
+__global__ __launch_bounds__(NW * kWave) void k_delta_cap(
+    const UpdHdr* __restrict__ hd, int32_t phase, const int32_t* __restrict__ dels,
+    const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
+    const int32_t* __restrict__ vedges, const unsigned long long* __restrict__ roff,
+    const unsigned long long* __restrict__ rlen, const uint32_t* __restrict__ rows,
+    const uint8_t* __restrict__ flag, unsigned long long* __restrict__ acc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (phase && hd->overflow) return;
+  const int32_t n = phase ? hd->n_ins : hd->n_del;
+  const int32_t parts = phase ? hd->parts_ins : hd->parts_del;
+  const int32_t q = (int32_t)(blockIdx.x / (uint32_t)parts);
+  if (q >= n) return;
+  const int32_t f = phase ? hd->E_old + q : dels[q];
+  delta_body<MT, NW, kDeltaLoads>(reinterpret_cast<MT*>(smem), phase ? hd->E_new : hd->E_old, f,
+                                  blockIdx.x % (uint32_t)parts, parts, eoff, enodes, esize, voff,
+                                  vedges, roff, rlen, rows, flag, phase ? 1ll : -1ll, acc);
+}
+
+__global__ void k_set_flags_cap(const UpdHdr* __restrict__ hd, int32_t phase,
+                                const int32_t* __restrict__ dels, uint8_t* __restrict__ flag) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < (phase ? hd->n_ins : hd->n_del)) flag[phase ? hd->E_old + k : dels[k]] = 1;
+}
+
+__global__ void k_new_sizes_cap(const UpdHdr* __restrict__ hd, int32_t Ecap,
+                                const uint32_t* __restrict__ eoff, const uint8_t* __restrict__ del,
+                                const uint32_t* __restrict__ ins_off,
+                                unsigned long long* __restrict__ sizes) {
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e > Ecap) return;
+  const int32_t E = hd->E_old, nins = hd->n_ins;
+  if (e < E) sizes[e] = del[e] ? 0 : (eoff[e + 1] - eoff[e]);
+  else if (e < E + nins) sizes[e] = ins_off[e - E + 1] - ins_off[e - E];
+  else sizes[e] = 0;
+}
+
+__global__ void k_copy_rows_cap(UpdHdr* __restrict__ hd, int32_t Ecap, const uint32_t* __restrict__ eoff,
+                                const int32_t* __restrict__ enodes, const uint8_t* __restrict__ del,
+                                const uint32_t* __restrict__ ins_off,
+                                const int32_t* __restrict__ ins_nodes,
+                                const unsigned long long* __restrict__ noff,
+                                int32_t* __restrict__ nnodes, uint32_t* __restrict__ neoff,
+                                uint8_t* __restrict__ nsize, int32_t* __restrict__ inc_node,
+                                int32_t* __restrict__ inc_edge) {
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e > Ecap) return;
+  const int32_t E = hd->E_old, E2 = E + hd->n_ins;
+  neoff[e] = (uint32_t)noff[e];
+  if (e == E2) hd->I_new = noff[e];
+  if (e >= E2) {
+    if (e < Ecap) nsize[e] = 0;
+    return;
+  }
+  const int32_t* src;
+  uint32_t n;
+  if (e < E) {
+    src = enodes + eoff[e];
+    n = del[e] ? 0 : eoff[e + 1] - eoff[e];
+  } else {
+    src = ins_nodes + ins_off[e - E];
+    n = ins_off[e - E + 1] - ins_off[e - E];
+  }
+  const unsigned long long o = noff[e];
+  nsize[e] = (uint8_t)(n > 255 ? 255 : n);
+  for (uint32_t k = 0; k < n; ++k) {
+    nnodes[o + k] = src[k];
+    inc_node[o + k] = src[k];
+    inc_edge[o + k] = e;
+  }
+}
+
+// incidences past the batch's total sort last (key above every node id)
+__global__ void k_pad_inc_cap(const UpdHdr* __restrict__ hd, int64_t Icap, int32_t* __restrict__ inc_node,
+                              int32_t* __restrict__ inc_edge, int32_t pad) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < Icap && k >= (int64_t)hd->I_new) {
+    inc_node[k] = pad;
+    inc_edge[k] = 0;
+  }
+}
+
+__global__ void k_node_count_cap(const UpdHdr* __restrict__ hd, int64_t Icap,
+                                 const int32_t* __restrict__ node, uint32_t* __restrict__ cnt) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < Icap && k < (int64_t)hd->I_new) atomicAdd(&cnt[node[k]], 1u);
+}
+
+__global__ void k_copy_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[k];
+}
+
+__global__ void k_copy_bytes(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[k];
+}
+
 static unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 static DynScratch& dyn_scratch(hm_graph* g) { return g->dyn; }
+
+// Workgroups per changed hyperedge of a delta launch over n changed ids: enough to give every CU
+// several.
+static int32_t delta_parts(int32_t n) {
+  return n <= 0 ? 1 : std::max<int32_t>(1, std::min<int32_t>(64, (2048 + n - 1) / n));
+}
 
 // Full projection rows of the graph currently on the handle (T > 0: tiled kernels, rows grouped
 // by id tile; T == 0: one LDS table, rows ascending).
@@ -648,6 +825,7 @@ static hm_status build_full_rows(hm_graph* g, DynScratch& d, int32_t T) {
   HM_HIP(hipMemcpyAsync(&P, d.roff.as<unsigned long long>() + E, 8, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
   HM_TRY(d.rows.ensure(sizeof(uint32_t) * std::max<unsigned long long>(P, 1)));
+  d.rows_P = P;
   if (T > 0) {
     HM_TRY(allow_lds((const void*)k_full_fill_tiled<MT>, lds));
     k_full_fill_tiled<MT><<<E, kBlock, lds, st>>>(E, T, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
@@ -687,8 +865,7 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
   HM_HIP(hipMemcpyAsync(d.ids.p, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
   k_fill_u8<<<grid_for(E), kBlock, 0, st>>>(d.flag.as<uint8_t>(), E, 0);
   k_set_flags<<<grid_for(n), kBlock, 0, st>>>(d.ids.as<int32_t>(), n, d.flag.as<uint8_t>());
-  // enough workgroups to give every CU several: parts per changed hyperedge
-  const int32_t parts = std::max<int32_t>(1, std::min<int32_t>(64, (2048 + n - 1) / n));
+  const int32_t parts = delta_parts(n);
   if (T > 0) {
     // state per (changed f, neighbour b) at the exclusive scan of the changed rows' lengths
     HM_TRY(d.slen.ensure(sizeof(unsigned long long) * ((size_t)n + 1)));
@@ -849,6 +1026,340 @@ static hm_status apply_batch(hm_graph* g, DynScratch& d, const std::vector<uint8
   return HM_OK;
 }
 
+// ---- captured update (hm_graph_set_update_capture) -------------------------------------------
+// One hipGraph holds the whole update with a delta: the removal delta over the rows held from the
+// previous update, the store update, the projection rows of the graph after the batch and the
+// addition delta. The batch's sizes travel in a device header; every buffer is sized to the plan's
+// capacities, so the launch has no host wait. Rows after the batch that exceed the plan are
+// caught on the device (k_check_rows_cap): the fill and the addition phase are skipped, and the
+// host reruns the addition phase eagerly with larger buffers.
+
+// Byte offsets of a plan's input block: header, deleted ids, insert offsets, inserted nodes.
+struct InLayout {
+  size_t dels, ins_off, ins_nodes, total;
+};
+
+static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+static InLayout in_layout(const UpdPlan& p) {
+  InLayout L;
+  L.dels = align16(sizeof(UpdHdr));
+  L.ins_off = align16(L.dels + sizeof(int32_t) * (size_t)p.dcap);
+  L.ins_nodes = align16(L.ins_off + sizeof(uint32_t) * ((size_t)p.icap + 1));
+  L.total = align16(L.ins_nodes + sizeof(int32_t) * (size_t)std::max<int64_t>(p.iicap, 1));
+  return L;
+}
+
+constexpr size_t kUpdOutBytes = sizeof(unsigned long long) * kDynAccCount + sizeof(UpdHdr);
+
+// Device addresses a captured update reads or writes; the graph is valid while they are unchanged.
+static std::vector<const void*> plan_sig(hm_graph* g) {
+  DeviceCsr& c = g->csr;
+  DynScratch& d = g->dyn;
+  return {c.eoff.p,    c.enodes.p,   c.esize.p,    c.voff.p,     c.vedges.p, d.deg.p,
+          d.roff.p,    d.rows.p,     d.flag.p,     d.acc.p,      d.tmp.p,    d.sizes.p,
+          d.noff.p,    d.nnodes.p,   d.neoff.p,    d.nsize.p,    d.inc_node.p, d.inc_edge.p,
+          d.srt_node.p, d.srt_edge.p, d.vcnt.p,    d.nvoff.p,    g->uplan.in.p};
+}
+
+// Grows b to at least `need` bytes, keeping its contents.
+static hm_status grow_keep(DevBuf& b, size_t need, hipStream_t st) {
+  if (b.p && b.bytes >= need) return HM_OK;
+  DevBuf nb;
+  HM_TRY(nb.ensure(need));
+  if (b.p && b.bytes) HM_HIP(hipMemcpyAsync(nb.p, b.p, b.bytes, hipMemcpyDeviceToDevice, st));
+  HM_HIP(hipStreamSynchronize(st));
+  b.release();
+  b = nb;
+  return HM_OK;
+}
+
+// Largest id count whose delta table (mask and size-and-flag byte per id) fits one workgroup.
+static int32_t dense_max_ids(hm_graph* g, bool wide) {
+  int lim = 65536;
+  (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
+  const long long room = (long long)lim - 16 - 20 * 1024;
+  return room <= 0 ? 0 : (int32_t)(room / ((wide ? 8 : 4) + 1));
+}
+
+// Largest delta grid over n <= ncap changed ids.
+static unsigned delta_grid_cap(int32_t ncap) {
+  long long best = 1;
+  for (int32_t n = 1; n <= ncap; ++n) best = std::max<long long>(best, (long long)n * delta_parts(n));
+  return (unsigned)best;
+}
+
+template <typename MT>
+static size_t delta_lds(int32_t E) {
+  return (((size_t)E * sizeof(MT) + (size_t)E) + 15) & ~(size_t)15;
+}
+
+template <typename MT>
+static size_t table_lds(int32_t E) {
+  return ((size_t)E * sizeof(MT) + 15) & ~(size_t)15;
+}
+
+template <typename MT>
+static bool delta_one_group(int32_t Ecap) {
+  return delta_lds<MT>(Ecap) + kDeltaTiledStaticLds > kDeltaOneGroupLds;
+}
+
+// Enqueues the whole captured update on g->stream (under capture: nothing waits).
+template <typename MT>
+static hm_status enqueue_update_cap(hm_graph* g) {
+  DeviceCsr& c = g->csr;
+  DynScratch& d = g->dyn;
+  UpdPlan& p = g->uplan;
+  hipStream_t st = g->stream;
+  const InLayout L = in_layout(p);
+  unsigned char* in = p.in.as<unsigned char>();
+  UpdHdr* hd = reinterpret_cast<UpdHdr*>(in);
+  const int32_t* dels = reinterpret_cast<const int32_t*>(in + L.dels);
+  const uint32_t* ins_off = reinterpret_cast<const uint32_t*>(in + L.ins_off);
+  const int32_t* ins_nodes = reinterpret_cast<const int32_t*>(in + L.ins_nodes);
+  unsigned long long* acc = d.acc.as<unsigned long long>();
+  const int32_t Ecap = p.Ecap, Vcap = p.Vcap;
+  const int64_t Icap = p.Icap;
+  const size_t lds_d = delta_lds<MT>(Ecap), lds_t = table_lds<MT>(Ecap);
+  auto kd = delta_one_group<MT>(Ecap) ? k_delta_cap<MT, kDeltaWaves, kDeltaLoadsWide>
+                                      : k_delta_cap<MT, kDeltaWaves, kDeltaLoadsNarrow>;
+  k_zero_acc<<<1, kDynAccCount, 0, st>>>(acc, kDynAccCount);
+  // removal phase: L(deleted ids, graph before the batch) over the rows held from the last update
+  k_fill_u8<<<grid_for(Ecap), kBlock, 0, st>>>(d.flag.as<uint8_t>(), Ecap, 0);
+  k_set_flags_cap<<<grid_for(p.dcap), kBlock, 0, st>>>(hd, 0, dels, d.flag.as<uint8_t>());
+  kd<<<p.grid_del, kDeltaWaves * kWave, lds_d, st>>>(
+      hd, 0, dels, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
+      c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
+      d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.flag.as<uint8_t>(), acc);
+  HM_HIP(hipGetLastError());
+  // store update into the staging arrays (the removal flags are the deletion mask)
+  k_new_sizes_cap<<<grid_for((int64_t)Ecap + 1), kBlock, 0, st>>>(
+      hd, Ecap, c.eoff.as<uint32_t>(), d.flag.as<uint8_t>(), ins_off, d.sizes.as<unsigned long long>());
+  size_t tb = d.tmp.bytes;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(d.tmp.p, tb, d.sizes.as<unsigned long long>(),
+                                          d.noff.as<unsigned long long>(), Ecap + 1, st));
+  k_copy_rows_cap<<<grid_for((int64_t)Ecap + 1), kBlock, 0, st>>>(
+      hd, Ecap, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), d.flag.as<uint8_t>(), ins_off, ins_nodes,
+      d.noff.as<unsigned long long>(), d.nnodes.as<int32_t>(), d.neoff.as<uint32_t>(),
+      d.nsize.as<uint8_t>(), d.inc_node.as<int32_t>(), d.inc_edge.as<int32_t>());
+  k_pad_inc_cap<<<grid_for(Icap), kBlock, 0, st>>>(hd, Icap, d.inc_node.as<int32_t>(),
+                                                    d.inc_edge.as<int32_t>(), (int32_t)((1u << p.bits) - 1u));
+  HM_HIP(hipGetLastError());
+  tb = d.tmp.bytes;
+  HM_HIP(hipcub::DeviceRadixSort::SortPairs(d.tmp.p, tb, d.inc_node.as<uint32_t>(), d.srt_node.as<uint32_t>(),
+                                            d.inc_edge.as<int32_t>(), d.srt_edge.as<int32_t>(),
+                                            (int)Icap, 0, p.bits, st));
+  k_fill_u32<<<grid_for((int64_t)Vcap + 1), kBlock, 0, st>>>(d.vcnt.as<uint32_t>(), (int64_t)Vcap + 1, 0);
+  k_node_count_cap<<<grid_for(Icap), kBlock, 0, st>>>(hd, Icap, d.inc_node.as<int32_t>(), d.vcnt.as<uint32_t>());
+  tb = d.tmp.bytes;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(d.tmp.p, tb, d.vcnt.as<uint32_t>(), d.nvoff.as<uint32_t>(),
+                                          Vcap + 1, st));
+  // the staging arrays become the store (copied: the graph's addresses stay fixed)
+  k_copy_words<<<grid_for(Icap), kBlock, 0, st>>>(c.enodes.as<uint32_t>(), d.nnodes.as<uint32_t>(), Icap);
+  k_copy_words<<<grid_for((int64_t)Ecap + 1), kBlock, 0, st>>>(c.eoff.as<uint32_t>(), d.neoff.as<uint32_t>(),
+                                                                (int64_t)Ecap + 1);
+  k_copy_bytes<<<grid_for(Ecap), kBlock, 0, st>>>(c.esize.as<uint8_t>(), d.nsize.as<uint8_t>(), Ecap);
+  k_copy_words<<<grid_for(Icap), kBlock, 0, st>>>(c.vedges.as<uint32_t>(), d.srt_edge.as<uint32_t>(), Icap);
+  k_copy_words<<<grid_for((int64_t)Vcap + 1), kBlock, 0, st>>>(c.voff.as<uint32_t>(), d.nvoff.as<uint32_t>(),
+                                                                (int64_t)Vcap + 1);
+  HM_HIP(hipGetLastError());
+  // full rows of the graph after the batch (kept for the next update's removal phase)
+  k_deg_cap<MT><<<(unsigned)Ecap + 1, kBlock, lds_t, st>>>(hd, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                                           c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+                                                           d.deg.as<unsigned long long>());
+  tb = d.tmp.bytes;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(d.tmp.p, tb, d.deg.as<unsigned long long>(),
+                                          d.roff.as<unsigned long long>(), Ecap + 1, st));
+  k_check_rows_cap<<<1, kWave, 0, st>>>(hd, d.roff.as<unsigned long long>(), p.Pcap);
+  k_full_fill_cap<MT><<<(unsigned)Ecap, kBlock, lds_t, st>>>(
+      hd, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+      d.roff.as<unsigned long long>(), d.rows.as<uint32_t>());
+  HM_HIP(hipGetLastError());
+  // addition phase: L(inserted ids, graph after the batch)
+  k_fill_u8<<<grid_for(Ecap), kBlock, 0, st>>>(d.flag.as<uint8_t>(), Ecap, 0);
+  k_set_flags_cap<<<grid_for(p.icap), kBlock, 0, st>>>(hd, 1, dels, d.flag.as<uint8_t>());
+  kd<<<p.grid_ins, kDeltaWaves * kWave, lds_d, st>>>(
+      hd, 1, dels, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
+      c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
+      d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.flag.as<uint8_t>(), acc);
+  HM_HIP(hipGetLastError());
+  return HM_OK;
+}
+
+// Sizes a plan for a batch (E2 ids, n_del / n_ins changed, Iins inserted incidences, V2 nodes
+// after it) with headroom, grows the buffers (keeping the store and the held rows) and captures
+// the update. p.exec stays null when the capture cannot be replayed (memset node).
+template <typename MT>
+static hm_status make_plan(hm_graph* g, int32_t E2, int32_t n_del, int32_t n_ins, int64_t Iins, int32_t V2) {
+  DeviceCsr& c = g->csr;
+  DynScratch& d = g->dyn;
+  UpdPlan& p = g->uplan;
+  hipStream_t st = g->stream;
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  p.exec = nullptr;
+  p.sig.clear();
+  p.wide = sizeof(MT) == 8;
+  p.dcap = std::max<int32_t>(64, 2 * n_del);
+  p.icap = std::max<int32_t>(64, 2 * n_ins);
+  p.iicap = std::max<int64_t>(512, 2 * Iins);
+  const int64_t ewant = (int64_t)E2 + std::max<int64_t>({(int64_t)4 * p.icap, (int64_t)E2 / 8, 256});
+  p.Ecap = (int32_t)std::min<int64_t>({ewant, (int64_t)dense_max_ids(g, p.wide), kMaxPackedId - 1});
+  if (p.Ecap < E2) return fail(HM_ERR_UNSUPPORTED, "captured update: ids beyond one delta table");
+  p.Vcap = (int32_t)std::min<int64_t>((int64_t)V2 + std::max<int64_t>((int64_t)V2 / 8, p.iicap), (1ll << 30));
+  if (p.Vcap < V2) return fail(HM_ERR_UNSUPPORTED, "captured update: node ids too large");
+  p.Icap = c.I + Iins + std::max<int64_t>(c.I / 8, 4 * p.iicap);
+  if (p.Icap >= (1ll << 31)) return fail(HM_ERR_UNSUPPORTED, "captured update: too many incidences");
+  p.Pcap = d.rows_P + d.rows_P / 4 + 4096;
+  p.bits = 1;
+  while (p.bits < 31 && ((int64_t)1 << p.bits) <= (int64_t)p.Vcap) ++p.bits;
+  p.grid_del = delta_grid_cap(p.dcap);
+  p.grid_ins = delta_grid_cap(p.icap);
+  const size_t E1 = (size_t)p.Ecap + 1, V1 = (size_t)p.Vcap + 1, Ic = (size_t)p.Icap;
+  // the store and the held rows keep their contents
+  HM_TRY(grow_keep(c.eoff, 4 * E1, st));
+  HM_TRY(grow_keep(c.enodes, 4 * Ic, st));
+  HM_TRY(grow_keep(c.esize, E1, st));
+  HM_TRY(grow_keep(c.voff, 4 * V1, st));
+  HM_TRY(grow_keep(c.vedges, 4 * Ic, st));
+  HM_TRY(grow_keep(d.deg, 8 * E1, st));
+  HM_TRY(grow_keep(d.roff, 8 * E1, st));
+  HM_TRY(grow_keep(d.rows, 4 * (size_t)p.Pcap, st));
+  HM_TRY(d.flag.ensure(E1));
+  HM_TRY(d.acc.ensure(sizeof(unsigned long long) * kDynAccCount));
+  HM_TRY(d.sizes.ensure(8 * E1));
+  HM_TRY(d.noff.ensure(8 * E1));
+  HM_TRY(d.neoff.ensure(4 * E1));
+  HM_TRY(d.nsize.ensure(E1));
+  for (DevBuf* b : {&d.nnodes, &d.inc_node, &d.inc_edge, &d.srt_node, &d.srt_edge}) HM_TRY(b->ensure(4 * Ic));
+  HM_TRY(d.vcnt.ensure(4 * V1));
+  HM_TRY(d.nvoff.ensure(4 * V1));
+  size_t t1 = 0, t2 = 0, t3 = 0;
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, d.sizes.as<unsigned long long>(),
+                                          d.noff.as<unsigned long long>(), (int)E1, st));
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, d.vcnt.as<uint32_t>(), d.nvoff.as<uint32_t>(),
+                                          (int)V1, st));
+  HM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t3, d.inc_node.as<uint32_t>(), d.srt_node.as<uint32_t>(),
+                                            d.inc_edge.as<int32_t>(), d.srt_edge.as<int32_t>(),
+                                            (int)Ic, 0, p.bits, st));
+  HM_TRY(d.tmp.ensure(std::max({t1, t2, t3, (size_t)16})));
+  const InLayout L = in_layout(p);
+  HM_TRY(p.in.ensure(L.total));
+  HM_TRY(p.hin.ensure(L.total));
+  HM_TRY(p.hout.ensure(kUpdOutBytes));
+  // nothing may allocate or set attributes while the stream is captured
+  HM_TRY(allow_lds((const void*)k_delta_cap<MT, kDeltaWaves, kDeltaLoadsWide>, delta_lds<MT>(p.Ecap)));
+  HM_TRY(allow_lds((const void*)k_delta_cap<MT, kDeltaWaves, kDeltaLoadsNarrow>, delta_lds<MT>(p.Ecap)));
+  HM_TRY(allow_lds((const void*)k_deg_cap<MT>, table_lds<MT>(p.Ecap)));
+  HM_TRY(allow_lds((const void*)k_full_fill_cap<MT>, table_lds<MT>(p.Ecap)));
+  HM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const hm_status s1 = enqueue_update_cap<MT>(g);
+  hipGraphExec_t exec = nullptr;
+  const hm_status s2 = end_capture(st, &exec);  // ends the capture on every path
+  if (s1 != HM_OK || s2 != HM_OK) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    return s1 != HM_OK ? s1 : s2;
+  }
+  p.exec = exec;
+  p.sig = plan_sig(g);
+  ++p.captures;
+  return HM_OK;
+}
+
+// The update as one captured graph. *handled = false (and nothing changed but buffer capacities
+// and the held rows) when the plan cannot be made: the caller runs the eager update.
+template <typename MT>
+static hm_status update_captured(hm_graph* g, const std::vector<int32_t>& dels_sorted,
+                                 const std::vector<int64_t>& ioff, const std::vector<int32_t>& inodes,
+                                 int32_t n_ins, int32_t newV, unsigned long long* h, bool* handled) {
+  *handled = false;
+  DeviceCsr& c = g->csr;
+  DynScratch& d = g->dyn;
+  UpdPlan& p = g->uplan;
+  hipStream_t st = g->stream;
+  const int32_t E = c.E, E2 = E + n_ins, n_del = (int32_t)dels_sorted.size();
+  const int64_t Iins = ioff.back();
+  if (!d.rows_valid || d.rows_tile != 0) {  // the removal phase reads the rows of the graph as it is
+    d.rows_valid = false;
+    HM_TRY(build_full_rows<MT>(g, d, 0));
+    d.rows_valid = true;
+    d.rows_tile = 0;
+  }
+  const bool fits = p.exec && p.wide == (sizeof(MT) == 8) && E2 <= p.Ecap && n_del <= p.dcap &&
+                    n_ins <= p.icap && Iins <= p.iicap && newV <= p.Vcap && c.I + Iins <= p.Icap &&
+                    p.sig == plan_sig(g);
+  if (!fits) {
+    if (make_plan<MT>(g, E2, n_del, n_ins, Iins, newV) != HM_OK) {
+      clear_error();
+      if (p.exec) (void)hipGraphExecDestroy(p.exec);
+      p.exec = nullptr;
+      p.disabled = true;  // this graph's updates stay eager
+      return HM_OK;
+    }
+    if (!p.exec) {
+      p.disabled = true;  // the capture held a memset node
+      return HM_OK;
+    }
+  }
+  const InLayout L = in_layout(p);
+  unsigned char* hin = p.hin.as<unsigned char>();
+  UpdHdr hdr{};
+  hdr.E_old = E;
+  hdr.E_new = E2;
+  hdr.n_del = n_del;
+  hdr.n_ins = n_ins;
+  hdr.V_new = newV;
+  hdr.parts_del = delta_parts(n_del);
+  hdr.parts_ins = delta_parts(n_ins);
+  std::memcpy(hin, &hdr, sizeof(hdr));
+  if (n_del) std::memcpy(hin + L.dels, dels_sorted.data(), sizeof(int32_t) * (size_t)n_del);
+  uint32_t* io = reinterpret_cast<uint32_t*>(hin + L.ins_off);
+  for (int32_t k = 0; k <= n_ins; ++k) io[k] = (uint32_t)ioff[(size_t)k];
+  if (Iins) std::memcpy(hin + L.ins_nodes, inodes.data(), sizeof(int32_t) * (size_t)Iins);
+  *handled = true;
+  d.rows_valid = false;  // rebuilt by the launch unless they overflow
+  struct DrainOnExit {
+    hipStream_t s;
+    ~DrainOnExit() { (void)hipStreamSynchronize(s); }
+  } drain{st};
+  HM_HIP(hipMemcpyAsync(p.in.p, hin, L.total, hipMemcpyHostToDevice, st));
+  HM_HIP(hipEventRecord(g->ev[3], st));
+  HM_HIP(hipGraphLaunch(p.exec, st));
+  HM_HIP(hipEventRecord(g->ev[7], st));
+  unsigned char* out = p.hout.as<unsigned char>();
+  HM_HIP(hipMemcpyAsync(out, d.acc.p, sizeof(unsigned long long) * kDynAccCount, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipMemcpyAsync(out + sizeof(unsigned long long) * kDynAccCount, p.in.p, sizeof(UpdHdr),
+                        hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  UpdHdr back;
+  std::memcpy(&back, out + sizeof(unsigned long long) * kDynAccCount, sizeof(back));
+  c.E = E2;
+  c.V = newV;
+  c.I = (int64_t)back.I_new;
+  float ms = 0.f;
+  HM_HIP(hipEventElapsedTime(&ms, g->ev[3], g->ev[7]));
+  g->ms_upd_rows = 0.f;
+  g->ms_upd_store = 0.f;
+  g->ms_upd_delta = ms;
+  p.last_captured = true;
+  p.last_overflow = back.overflow != 0;
+  if (!back.overflow) {
+    d.rows_valid = true;
+    d.rows_tile = 0;
+    d.rows_P = back.P_new;
+    std::memcpy(h, out, sizeof(unsigned long long) * kDynAccCount);
+    return HM_OK;
+  }
+  // rows over the plan: the device acc holds the removal phase; the addition phase runs eagerly
+  // (larger row buffers; the next update captures a larger plan)
+  std::vector<int32_t> ins((size_t)n_ins);
+  for (int32_t k = 0; k < n_ins; ++k) ins[(size_t)k] = E + k;
+  HM_TRY(local_count_any(g, d, ins.data(), n_ins, +1, 4));
+  HM_HIP(hipMemcpyAsync(h, d.acc.p, sizeof(unsigned long long) * kDynAccCount, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  if (n_ins > 0) HM_TRY(add_phase_ms(g, 4));
+  return HM_OK;
+}
+
 }  // namespace hm
 
 using namespace hm;
@@ -892,6 +1403,31 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
   ++g->version;  // a captured count of the old graph is never replayed
   g->ms_upd_rows = g->ms_upd_delta = 0.f;
   g->upd_removed = g->upd_added = g->upd_entries = -1;
+  g->uplan.last_captured = g->uplan.last_overflow = false;
+  if (want && g->uplan.on && !g->uplan.disabled && g->tile == 0 && dense_fits(g, E + n_ins, maxsz > 32)) {
+    std::vector<int32_t> ds(del_ids, del_ids + n_del);
+    std::sort(ds.begin(), ds.end());
+    const int32_t old_max = g->csr.max_size;
+    g->csr.max_size = maxsz;  // one mask width for both phases
+    unsigned long long h[kDynAccCount];
+    bool handled = false;
+    const hm_status r = maxsz > 32
+                            ? update_captured<mask64_t>(g, ds, ioff, inodes, n_ins, newV, h, &handled)
+                            : update_captured<uint32_t>(g, ds, ioff, inodes, n_ins, newV, h, &handled);
+    if (!handled) g->csr.max_size = old_max;
+    if (r != HM_OK) return r;
+    if (handled) {
+      for (int32_t k = 0; k < n_del; ++k) g->live[(size_t)del_ids[k]] = 0;
+      g->live.resize((size_t)E + n_ins, 1);
+      for (int32_t k = 0; k < n_ins; ++k) assigned_ids[k] = E + k;
+      for (int k = 0; k < HM_STATIC_BINS; ++k) delta_out[k] = (int64_t)h[k];
+      g->upd_removed = (int64_t)h[kDynAccRemoved];
+      g->upd_added = (int64_t)h[kDynAccAdded];
+      g->upd_entries = (int64_t)h[kDynAccEntries];
+      g->have_stats = false;
+      return HM_OK;
+    }
+  }
   // host ids copied to the device by the removal phase: kept alive until the stream has drained
   // (the guard, destroyed first, waits on every return path)
   std::vector<int32_t> dels(del_ids, del_ids + n_del);
@@ -943,6 +1479,43 @@ hm_status hm_graph_update_stats(hm_graph* g, hm_update_stats* out) {
   out->ms_rows = g->ms_upd_rows;
   out->ms_delta = g->ms_upd_delta;
   out->ms_store = g->ms_upd_store;
+  out->captured = g->uplan.last_captured ? 1 : 0;
+  out->rows_overflow = g->uplan.last_overflow ? 1 : 0;
+  return HM_OK;
+}
+
+hm_status hm_graph_set_update_capture(hm_graph* g, int32_t on) {
+  clear_error();
+  if (!g) return fail(HM_ERR_BAD_INPUT, "hm_graph_set_update_capture: null handle");
+  UpdPlan& p = g->uplan;
+  p.on = on != 0;
+  if (!p.on) {
+    if (p.exec) (void)hipGraphExecDestroy(p.exec);
+    p.exec = nullptr;
+    p.sig.clear();
+    p.disabled = false;
+  }
+  return HM_OK;
+}
+
+hm_status hm_graph_update_capture_state(hm_graph* g, hm_update_capture_state* out) {
+  clear_error();
+  if (!g || !out) return fail(HM_ERR_BAD_INPUT, "hm_graph_update_capture_state: null argument");
+  const UpdPlan& p = g->uplan;
+  *out = hm_update_capture_state{};
+  out->on = p.on ? 1 : 0;
+  out->disabled = p.disabled ? 1 : 0;
+  out->captured = p.exec ? 1 : 0;
+  out->captures = p.captures;
+  out->last_captured = p.last_captured ? 1 : 0;
+  out->last_overflow = p.last_overflow ? 1 : 0;
+  out->id_capacity = p.Ecap;
+  out->node_capacity = p.Vcap;
+  out->incidence_capacity = p.Icap;
+  out->row_capacity = (int64_t)p.Pcap;
+  out->batch_deletes = p.dcap;
+  out->batch_inserts = p.icap;
+  out->batch_incidences = p.iicap;
   return HM_OK;
 }
 

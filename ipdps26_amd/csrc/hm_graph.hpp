@@ -128,12 +128,50 @@ struct DynScratch {
   // rows_tile (0 = one table). Cleared whenever the store changes.
   bool rows_valid = false;
   int32_t rows_tile = -1;
+  unsigned long long rows_P = 0;  // entries of the held rows
   void release() {
     rows_valid = false;
     for (DevBuf* b : {&deg, &roff, &rows, &flag, &ids, &acc, &tmp, &ins_off, &ins_nodes, &sizes,
                       &noff, &nnodes, &neoff, &nsize, &inc_node, &inc_edge, &srt_node, &srt_edge,
                       &vcnt, &nvoff, &slen, &soff, &scur, &spm})
       b->release();
+  }
+};
+
+// Device-side header of a captured update: the batch's sizes, written by the host before each
+// launch, and what the launch reports back.
+struct UpdHdr {
+  int32_t E_old, E_new, n_del, n_ins;
+  int32_t V_new, parts_del, parts_ins;
+  int32_t overflow;           // set on the device: rows after the batch exceed the plan
+  unsigned long long I_new;   // incidences after the batch
+  unsigned long long P_new;   // projection-row entries after the batch
+};
+
+// Captured update (hm_graph_set_update_capture): capacities, the instantiated graph and the
+// buffer addresses it was captured with (a graph launch is only valid while they are unchanged).
+struct UpdPlan {
+  bool on = false;
+  bool disabled = false;      // the capture held a memset node: updates stay eager
+  hipGraphExec_t exec = nullptr;
+  bool wide = false;          // 64-bit masks
+  int32_t Ecap = 0, Vcap = 0, dcap = 0, icap = 0;
+  int64_t Icap = 0, iicap = 0;
+  unsigned long long Pcap = 0;
+  int bits = 0;               // radix-sort key bits: every node id and the padding key fit
+  unsigned grid_del = 0, grid_ins = 0;
+  std::vector<const void*> sig;
+  DevBuf in;                  // device input block: header, deleted ids, insert offsets, nodes
+  PinnedBuf hin, hout;        // host staging of the input block; acc and header written back
+  int32_t captures = 0;
+  bool last_captured = false, last_overflow = false;
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    in.release();
+    hin.release();
+    hout.release();
+    sig.clear();
   }
 };
 
@@ -169,4 +207,5 @@ struct hm_graph {
   bool tri_retune = false;
   int32_t last_tri_form = -1;  // form of the last one-table count (-1: none or tiled)
   bool last_tri_tuned = false;
+  hm::UpdPlan uplan;  // captured update with a delta
 };

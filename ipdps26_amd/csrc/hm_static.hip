@@ -929,6 +929,7 @@ hm_status hm_graph_destroy(hm_graph* g) {
   if (!g) return HM_OK;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   if (g->replay_exec) (void)hipGraphExecDestroy(g->replay_exec);
+  g->uplan.release();
   g->scratch.release();
   g->dyn.release();
   g->csr.release();
