@@ -312,7 +312,10 @@ def run_serving(slots: int = 4, n_requests: int = 400, batch_graphs: int = 16) -
     note(f"served with batched counts saturated: {batched['requests_per_s']:.1f} req/s")
     kept = serving.run_served(graphs, reqs, realtime=False, slots=slots, maintain=True)
     note(f"served with kept counts saturated: {kept['requests_per_s']:.1f} req/s")
-    same = same_as(plain, served) and same_as(plain, batched) and same_as(plain, kept)
+    kept_cap = serving.run_served(graphs, reqs, realtime=False, slots=slots, maintain=True, capture=True)
+    note(f"served with kept counts and captured updates saturated: {kept_cap['requests_per_s']:.1f} req/s")
+    same = (same_as(plain, served) and same_as(plain, batched) and same_as(plain, kept)
+            and same_as(plain, kept_cap))
     rate = 0.7 * plain["requests_per_s"]
     timed = W.serving_requests(graphs, n_requests, rate_per_s=rate)
     plain_rt = serving.run_plain(graphs, timed, realtime=True)
@@ -325,8 +328,11 @@ def run_serving(slots: int = 4, n_requests: int = 400, batch_graphs: int = 16) -
     kept_rt = serving.run_served(graphs, timed, realtime=True, slots=slots, maintain=True)
     note(f"served with kept counts at {rate:.1f} req/s: p50 {kept_rt['p50_ms']:.1f} ms "
          f"p99 {kept_rt['p99_ms']:.1f} ms")
+    kept_cap_rt = serving.run_served(graphs, timed, realtime=True, slots=slots, maintain=True, capture=True)
+    note(f"served with kept counts and captured updates at {rate:.1f} req/s: "
+         f"p50 {kept_cap_rt['p50_ms']:.1f} ms p99 {kept_cap_rt['p99_ms']:.1f} ms")
     same = (same and same_as(plain_rt, served_rt) and same_as(plain_rt, batched_rt)
-            and same_as(plain_rt, kept_rt))
+            and same_as(plain_rt, kept_rt) and same_as(plain_rt, kept_cap_rt))
     # many small graphs: where batched counts pay (launch-bound counts)
     sg = W.serving_graphs(256, lo=100, hi=1000, seed=55)
     sreqs = W.serving_requests(sg, 2000, seed=5500)
@@ -349,13 +355,17 @@ def run_serving(slots: int = 4, n_requests: int = 400, batch_graphs: int = 16) -
                       "saturated": {"plain": strip(plain), "served": strip(served),
                                     "served_batched_counts": strip(batched),
                                     "served_kept_counts": strip(kept),
+                                    "served_kept_counts_captured_updates": strip(kept_cap),
                                     "speedup": served["requests_per_s"] / plain["requests_per_s"],
                                     "speedup_batched_counts": batched["requests_per_s"] / plain["requests_per_s"],
-                                    "speedup_kept_counts": kept["requests_per_s"] / plain["requests_per_s"]},
+                                    "speedup_kept_counts": kept["requests_per_s"] / plain["requests_per_s"],
+                                    "speedup_kept_counts_captured_updates":
+                                        kept_cap["requests_per_s"] / plain["requests_per_s"]},
                       "poisson_at_70pct_plain_capacity": {
                           "rate_per_s": rate, "plain": strip(plain_rt), "served": strip(served_rt),
                           "served_batched_counts": strip(batched_rt),
-                          "served_kept_counts": strip(kept_rt)},
+                          "served_kept_counts": strip(kept_rt),
+                          "served_kept_counts_captured_updates": strip(kept_cap_rt)},
                       "small_graphs_saturated": {
                           "graphs": "256 graphs of the static_c2 family, E log-uniform on [100, 1000], "
                                     "seed 55; 2,000 requests (seed 5500), all queued at t=0",

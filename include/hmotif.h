@@ -340,6 +340,9 @@ typedef struct hm_server_config {
    * is a count, up to n graphs and about 4 ms of expected work, and counts them with one
    * hm_batch_count_static call (counts identical to the one-graph count). */
   int32_t batch_graphs;
+  /* 1: every graph's updates run as one captured hipGraph each (hm_graph_set_update_capture):
+   * one launch and one wait per update. Deltas identical either way. */
+  int32_t capture_updates;
 } hm_server_config;
 
 typedef struct hm_result {

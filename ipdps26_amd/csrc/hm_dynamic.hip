@@ -1251,6 +1251,7 @@ static hm_status make_plan(hm_graph* g, int32_t E2, int32_t n_del, int32_t n_ins
   HM_TRY(allow_lds((const void*)k_delta_cap<MT, kDeltaWaves, kDeltaLoadsNarrow>, delta_lds<MT>(p.Ecap)));
   HM_TRY(allow_lds((const void*)k_deg_cap<MT>, table_lds<MT>(p.Ecap)));
   HM_TRY(allow_lds((const void*)k_full_fill_cap<MT>, table_lds<MT>(p.Ecap)));
+  CaptureGuard capturing;  // no allocation or free on another thread meanwhile
   HM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   const hm_status s1 = enqueue_update_cap<MT>(g);
   hipGraphExec_t exec = nullptr;

@@ -20,6 +20,7 @@ struct DevBuf {
   // synchronises the whole device, which would serialise concurrent requests.
   hm_status ensure(size_t need) {
     if (need <= bytes && p) return HM_OK;
+    AllocGuard guard;
     size_t want = need ? need : 16;
     if (p) {
       want = std::max(want, bytes + bytes / 2);
@@ -37,7 +38,10 @@ struct DevBuf {
     return HM_OK;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      AllocGuard guard;
+      (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -52,6 +56,7 @@ struct PinnedBuf {
   hm_status ensure(size_t need) {
     if (need <= bytes && p) return HM_OK;
     release();
+    AllocGuard guard;
     const size_t want = std::max(need, (size_t)1 << 16);
     if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
       p = nullptr;
@@ -62,7 +67,10 @@ struct PinnedBuf {
     return HM_OK;
   }
   void release() {
-    if (p) (void)hipHostFree(p);
+    if (p) {
+      AllocGuard guard;
+      (void)hipHostFree(p);
+    }
     p = nullptr;
     bytes = 0;
   }

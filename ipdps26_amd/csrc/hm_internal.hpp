@@ -21,6 +21,27 @@ inline hm_status fail(hm_status st, const char* msg) {
 // Device selected by hm_init on this thread.
 int current_device();
 
+// Device and pinned allocations and frees (AllocGuard, shared) never overlap a stream capture on
+// another thread (CaptureGuard, exclusive): a free synchronizes the device, which a capture in
+// progress must not see. A thread holding CaptureGuard passes its own AllocGuards through.
+class AllocGuard {
+ public:
+  AllocGuard();
+  ~AllocGuard();
+  AllocGuard(const AllocGuard&) = delete;
+  AllocGuard& operator=(const AllocGuard&) = delete;
+
+ private:
+  bool held_;
+};
+class CaptureGuard {
+ public:
+  CaptureGuard();
+  ~CaptureGuard();
+  CaptureGuard(const CaptureGuard&) = delete;
+  CaptureGuard& operator=(const CaptureGuard&) = delete;
+};
+
 // Validates a caller CSR (offsets[0] == 0, non-decreasing, node ids >= 0) and returns its rows
 // sorted with repeated nodes removed; *vmax_out = largest node id (-1 if none).
 hm_status prepare_host_rows(const int64_t* offsets, const int32_t* nodes, int32_t E,

@@ -3,6 +3,7 @@
 
 #include <cstring>
 #include <mutex>
+#include <shared_mutex>
 #include <vector>
 #include <string>
 #include <unordered_map>
@@ -45,6 +46,26 @@ hm_status allow_lds(const void* kernel, size_t bytes) {
   if (e != hipSuccess) return check_hip(e, "hipFuncSetAttribute", __FILE__, __LINE__);
   g = bytes;
   return HM_OK;
+}
+
+namespace {
+std::shared_mutex g_capture_mu;
+thread_local bool t_capturing = false;
+}  // namespace
+
+AllocGuard::AllocGuard() : held_(!t_capturing) {
+  if (held_) g_capture_mu.lock_shared();
+}
+AllocGuard::~AllocGuard() {
+  if (held_) g_capture_mu.unlock_shared();
+}
+CaptureGuard::CaptureGuard() {
+  g_capture_mu.lock();
+  t_capturing = true;
+}
+CaptureGuard::~CaptureGuard() {
+  t_capturing = false;
+  g_capture_mu.unlock();
 }
 
 hm_status end_capture(hipStream_t st, hipGraphExec_t* exec) {

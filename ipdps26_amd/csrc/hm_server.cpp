@@ -17,6 +17,8 @@
 //     (last count under 1 ms) also takes other ready small graphs whose next request is a count
 //     (up to batch_graphs graphs and about 4 ms of expected work) and counts them all with one
 //     hm_batch launch sequence;
+//   * captured updates (capture_updates) — each graph's update with its delta is one hipGraph
+//     launch and one wait (hm_graph_set_update_capture);
 //   * latency — each request records submit, start and finish times (steady clock).
 // Counts are exact, so a result is the same alone and in the mix.
 #include <algorithm>
@@ -261,10 +263,11 @@ hm_status hm_server_create(const hm_server_config* cfg, hm_server** out) {
   hm::clear_error();
   if (!cfg || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_create: null argument");
   if (cfg->slots < 1 || cfg->slots > 64 || cfg->hbm_budget_bytes <= 0 ||
-      (cfg->maintain_counts != 0 && cfg->maintain_counts != 1) || cfg->batch_graphs < 0)
+      (cfg->maintain_counts != 0 && cfg->maintain_counts != 1) || cfg->batch_graphs < 0 ||
+      (cfg->capture_updates != 0 && cfg->capture_updates != 1))
     return hm::fail(HM_ERR_BAD_INPUT,
-                    "hm_server_create: need 1 <= slots <= 64, a positive budget, maintain_counts 0 or 1 "
-                    "and batch_graphs >= 0");
+                    "hm_server_create: need 1 <= slots <= 64, a positive budget, maintain_counts 0 or 1, "
+                    "batch_graphs >= 0 and capture_updates 0 or 1");
   *out = nullptr;
   size_t free_b = 0, total_b = 0;
   if (hipSetDevice(hm::current_device()) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
@@ -311,6 +314,7 @@ hm_status hm_server_add_graph(hm_server* s, const int64_t* offsets, const int32_
   }
   hm_graph* g = nullptr;
   hm_status st = hm_graph_create(offsets, nodes, num_edges, 0, &g);
+  if (st == HM_OK && s->cfg.capture_updates) st = hm_graph_set_update_capture(g, 1);
   std::lock_guard<std::mutex> lk(s->mu);
   if (st != HM_OK) {
     s->used_bytes -= need;

@@ -747,6 +747,7 @@ static hm_status replay_count(hm_graph* g, unsigned long long* h_acc) {
 static hm_status capture_count(hm_graph* g, CountImpl impl, int32_t T) {
   if (g->replay_exec) (void)hipGraphExecDestroy(g->replay_exec);
   g->replay_exec = nullptr;
+  CaptureGuard capturing;  // no allocation or free on another thread meanwhile
   HM_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
   const hm_status st = impl(g, nullptr, T, true);
   hipGraphExec_t exec = nullptr;

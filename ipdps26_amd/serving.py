@@ -52,8 +52,9 @@ def run_plain(graphs, reqs, realtime: bool) -> dict:
 
 
 def run_served(graphs, reqs, realtime: bool, slots: int = 4, budget_bytes: int = 64 << 30,
-               maintain: bool = False, batch_graphs: int = 0) -> dict:
-    srv = Server(budget_bytes, slots, maintain_counts=maintain, batch_graphs=batch_graphs)
+               maintain: bool = False, batch_graphs: int = 0, capture: bool = False) -> dict:
+    srv = Server(budget_bytes, slots, maintain_counts=maintain, batch_graphs=batch_graphs,
+                 capture_updates=capture)
     gids = [srv.add_graph(off, nodes) for off, nodes in graphs]
     for t in [srv.submit_count(g) for g in gids]:  # same warm-up as the plain path
         srv.wait(t)

@@ -194,3 +194,26 @@ def test_batched_counts_with_graph_removal(gpu_ready):
             assert np.array_equal(srv.wait(t)["counts"], expect[k]), k
         st = srv.stats()
         assert st["graphs"] == 7 and st["batch_launches"] > 0
+
+
+# ---- captured updates (capture_updates = 1) -----------------------------------------------------
+@pytest.mark.parametrize("maintain", [False, True])
+def test_captured_updates_equal_plain(gpu_ready, maintain):
+    """Every graph's updates run as one captured hipGraph each while other slots count, capture
+    and allocate concurrently: every result equals the plain path's, request by request."""
+    graphs, reqs = small_stream(n_graphs=10, n_req=200)
+    plain = serving.run_plain(graphs, reqs, realtime=False)
+    served = serving.run_served(graphs, reqs, realtime=False, slots=4, maintain=maintain, capture=True)
+    for k, (a, b) in enumerate(zip(plain["results"], served["results"])):
+        assert np.array_equal(a, b), f"request {k} ({reqs[k]['kind']})"
+
+
+def test_server_config_rejects_bad_capture_flag(gpu_ready):
+    import ctypes
+
+    from ipdps26_amd import _ServerConfig, lib
+
+    h = ctypes.c_void_p()
+    cfg = _ServerConfig(1 << 30, 2, 0, 0, 2)
+    assert lib().hm_server_create(ctypes.byref(cfg), ctypes.byref(h)) != 0
+    assert "capture_updates" in lib().hm_last_error().decode()
