@@ -208,6 +208,26 @@ __device__ inline int as_open_id(int fx, int fy, int cxi, int cyi, int si) {
   return 48 + fx + 2 * fy + 4 * (int)(cxi + cyi < si);
 }
 
+// Slots of closed triangles (x, y, i) in time order: a triangle adds +1 to the bin of its Venn id v
+// and -1 to its as-if-open bin (the pair x, y was counted as open at i), which is fixed by v and
+// b = [C_xi + C_yi < s_i]: x ⊄ i is Venn bit 0 or 3, y ⊄ i bit 1 or 3, and b is Venn bit 2 when
+// x ∩ y ∩ i is empty. So one counter per (v, b) holds both: 128 slots with x ∩ y ∩ i non-empty
+// (6 region bits, b), 8 without (then every pairwise-only region is non-empty: bits 3-5 set).
+constexpr int kTPairSlots = 136;
+
+__device__ inline int tpair_slot(int v, bool b) {
+  return (v & 64) ? (((v & 63) << 1) | (b ? 1 : 0)) : (128 | (v & 7));
+}
+
+// Venn id and as-if-open id of slot p (inverse of tpair_slot).
+__device__ inline void tpair_bins(int p, int* v, int* as_open) {
+  const int vv = p < 128 ? (64 | (p >> 1)) : (56 | (p & 7));
+  const int b = p < 128 ? (p & 1) : ((p >> 2) & 1);
+  const int fx = (vv & 1) | ((vv >> 3) & 1), fy = ((vv >> 1) & 1) | ((vv >> 3) & 1);
+  *v = vv;
+  *as_open = 48 + fx + 2 * fy + 4 * b;  // as_open_id with [C_xi + C_yi < s_i] = b
+}
+
 // Counts, for every ordered pair (x earlier, y later) of the nN neighbours of the anchor (in time
 // order), the as-if-open id (8 cases: fx, fy, [C_xi + C_yi < si]) with one wave. load(k, C, f)
 // gives neighbour k's overlap C with the anchor and f = [it has a node outside the anchor]. The
@@ -281,7 +301,8 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     const int32_t* __restrict__ order, int32_t Wmax, unsigned long long* __restrict__ acc) {
   constexpr int kC = 4;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ unsigned int hist[NW][kTemporalBins * kC];
+  __shared__ unsigned int hist[NW][kTPairSlots * kC];
+  __shared__ unsigned long long bsum[kTemporalBins];
   __shared__ int8_t tbl[128];
   __shared__ uint32_t wcnt[NW];
   __shared__ uint32_t s_nN, s_next;
@@ -295,7 +316,8 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   off += ((size_t)Wmax * sizeof(uint16_t) + 15) & ~(size_t)15;
   uint8_t* sz = reinterpret_cast<uint8_t*>(smem + off);
 
-  for (int k = threadIdx.x; k < NW * kTemporalBins * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < NW * kTPairSlots * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < kTemporalBins; k += blockDim.x) bsum[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
   if (threadIdx.x == 0) s_next = 0;
   for (int k = threadIdx.x; k < W; k += blockDim.x) sz[k] = esize[base + k];
@@ -361,7 +383,6 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     const MT mx = M[xr];
     const int sx = sz[xr];
     const int cxi = popc(mx);
-    const int fx = sx > cxi ? 1 : 0;
     const int xfree = sx - cxi;  // X = [C_xy < xfree]
     unsigned long long fields_b = 0, fields_e = 0;
     // (b) forward row of x inside (x, i): closed triangles and triples open at x. Two row loads
@@ -378,8 +399,8 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
         const int sy = sz[yr];
         if (my) {
           const int cyi = popc(my);
-          atomicAdd(&hw[tbl[venn_id(sx, sy, si, cxy, cyi, cxi, popc((MT)(mx & my)))] * kC + cp], 1u);
-          atomicAdd(&hw[tbl[as_open_id(fx, sy > cyi ? 1 : 0, cxi, cyi, si)] * kC + cp], ~0u);  // -1
+          const int v = venn_id(sx, sy, si, cxy, cyi, cxi, popc((MT)(mx & my)));
+          atomicAdd(&hw[tpair_slot(v, cxi + cyi < si) * kC + cp], 1u);
         } else {
           fields_b += 1ull << (16 * ((cxy < xfree ? 1 : 0) + (sy > cxy ? 2 : 0)));
         }
@@ -452,20 +473,24 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
         const uint32_t c = wave_sum(open_cnt[side][I][k]);
         const int X = k & 1, Y = k >> 1;
         const int id = side == 0 ? (X | Y << 1 | I << 2 | 8 | 32) : (Y | X << 1 | I << 2 | 8 | 16);
-        if (lane == 0 && c) atomicAdd(&hw[tbl[id] * kC], c);
+        if (lane == 0 && c) atomicAdd(&bsum[tbl[id]], (unsigned long long)c);
       }
   __syncthreads();
-  for (int b = lane; b < kTemporalBins; b += kWave) {  // fold each wave's copies into copy 0
-    int s = 0;
-    for (int c = 0; c < kC; ++c) s += (int)hw[b * kC + c];
-    hw[b * kC] = (unsigned)s;
+  // each triangle slot: +count to its closed bin, -count to its as-if-open bin
+  for (int p = threadIdx.x; p < kTPairSlots; p += blockDim.x) {
+    unsigned long long s = 0;
+    for (int q = 0; q < NW; ++q)
+      for (int c = 0; c < kC; ++c) s += hist[q][p * kC + c];
+    if (s) {
+      int v, as_open;
+      tpair_bins(p, &v, &as_open);
+      atomicAdd(&bsum[tbl[v]], s);
+      atomicAdd(&bsum[tbl[as_open]], (unsigned long long)(-(long long)s));
+    }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x) {
-    long long s = 0;
-    for (int q = 0; q < NW; ++q) s += (long long)(int)hist[q][b * kC];
-    if (s) atomicAdd(&acc[kTAccHist + b], (unsigned long long)s);
-  }
+  for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x)
+    if (bsum[b]) atomicAdd(&acc[kTAccHist + b], bsum[b]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -882,7 +907,8 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
   (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
   const size_t lds_proj = win_lds_bytes(Rmax, sizeof(MT) == 8);
   const size_t lds_cnt = TCountLds<MT>::bytes(Wmax);
-  const size_t static_cnt = sizeof(uint32_t) * 16 * kTemporalBins * 4 + 256;
+  const size_t static_cnt = sizeof(uint32_t) * 16 * kTPairSlots * 4 +
+                            sizeof(unsigned long long) * kTemporalBins + 256;
   // one LDS table per window when it fits; position tiles otherwise (or when forced)
   const bool tiled = g->tile > 0 || lds_proj > (size_t)lim || lds_cnt + static_cnt > (size_t)lim ||
                      Wmax > 65535;
