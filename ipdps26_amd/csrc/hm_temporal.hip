@@ -211,18 +211,19 @@ __device__ inline int as_open_id(int fx, int fy, int cxi, int cyi, int si) {
 // Slots of closed triangles (x, y, i) in time order: a triangle adds +1 to the bin of its Venn id v
 // and -1 to its as-if-open bin (the pair x, y was counted as open at i), which is fixed by v and
 // b = [C_xi + C_yi < s_i]: x ⊄ i is Venn bit 0 or 3, y ⊄ i bit 1 or 3, and b is Venn bit 2 when
-// x ∩ y ∩ i is empty. So one counter per (v, b) holds both: 128 slots with x ∩ y ∩ i non-empty
-// (6 region bits, b), 8 without (then every pairwise-only region is non-empty: bits 3-5 set).
-constexpr int kTPairSlots = 136;
+// x ∩ y ∩ i is empty. So one counter per (v, b) holds both. Slot = (low << 1 | b) + 16 mid +
+// 16 [x ∩ y ∩ i = ∅] with low = Venn bits 0-2 (private regions) and mid = bits 3-5 (pairwise-only
+// regions): 0..127 with x ∩ y ∩ i non-empty, 128..143 without (then mid = 7).
+constexpr int kTPairSlots = 144;
 
-__device__ inline int tpair_slot(int v, bool b) {
-  return (v & 64) ? (((v & 63) << 1) | (b ? 1 : 0)) : (128 | (v & 7));
-}
+// [r > 0] of a region count r >= 0, as one unsigned min
+__device__ inline unsigned nz(int r) { return min((unsigned)r, 1u); }
 
-// Venn id and as-if-open id of slot p (inverse of tpair_slot).
+// Venn id and as-if-open id of slot p.
 __device__ inline void tpair_bins(int p, int* v, int* as_open) {
-  const int vv = p < 128 ? (64 | (p >> 1)) : (56 | (p & 7));
-  const int b = p < 128 ? (p & 1) : ((p >> 2) & 1);
+  const int q = p < 128 ? p : p - 128;
+  const int b = q & 1, low = (q >> 1) & 7;
+  const int vv = p < 128 ? (low | (p >> 4) << 3 | 64) : (low | 56);
   const int fx = (vv & 1) | ((vv >> 3) & 1), fy = ((vv >> 1) & 1) | ((vv >> 3) & 1);
   *v = vv;
   *as_open = 48 + fx + 2 * fy + 4 * b;  // as_open_id with [C_xi + C_yi < s_i] = b
@@ -384,6 +385,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     const int sx = sz[xr];
     const int cxi = popc(mx);
     const int xfree = sx - cxi;  // X = [C_xy < xfree]
+    const int ifree = si - cxi;
     unsigned long long fields_b = 0, fields_e = 0;
     // (b) forward row of x inside (x, i): closed triangles and triples open at x. Two row loads
     // are in flight per lane; the row is ascending, so the scan stops at the first entry >= i.
@@ -398,11 +400,17 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
         const MT my = M[yr];
         const int sy = sz[yr];
         if (my) {
+          // Venn regions of (x, y, i) from the overlaps; every count below is >= 0
           const int cyi = popc(my);
-          const int v = venn_id(sx, sy, si, cxy, cyi, cxi, popc((MT)(mx & my)));
-          atomicAdd(&hw[tpair_slot(v, cxi + cyi < si) * kC + cp], 1u);
+          const int g = popc((MT)(mx & my));
+          const unsigned low = nz(xfree - cxy + g) | nz(sy - cxy - cyi + g) << 1 | nz(ifree - cyi + g) << 2;
+          const unsigned mid = nz(cxy - g) | nz(cyi - g) << 1 | nz(cxi - g) << 2;
+          const unsigned b = (unsigned)max(min(ifree - cyi, 1), 0);
+          const unsigned slot = (low << 1 | b) + (mid << 4) + ((1u - nz(g)) << 4);
+          atomicAdd(&hw[slot * kC + cp], 1u);
         } else {
-          fields_b += 1ull << (16 * ((cxy < xfree ? 1 : 0) + (sy > cxy ? 2 : 0)));
+          // x ∩ y and x ∩ i are disjoint here, so C_xy <= xfree
+          fields_b += 1ull << (nz(xfree - cxy) << 4 | nz(sy - cxy) << 5);
         }
       };
       for (uint32_t j0 = 0; j0 < n; j0 += kTLoads * kWave) {
@@ -441,7 +449,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
             before = true;
           } else if (M[yr] == (MT)0) {
             const int cxy = (int)(e[u] & kCntMask);
-            fields_e += 1ull << (16 * ((cxy < xfree ? 1 : 0) + ((int)sz[yr] > cxy ? 2 : 0)));
+            fields_e += 1ull << (nz(xfree - cxy) << 4 | nz((int)sz[yr] - cxy) << 5);
           }
         }
         if (__ballot(before)) break;
