@@ -645,14 +645,16 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
   constexpr int kC = 4;
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
-  __shared__ unsigned int hist[NW][kTemporalBins * kC];
+  __shared__ unsigned int hist[NW][kTPairSlots * kC];
+  __shared__ unsigned long long bsum[kTemporalBins];
   __shared__ int8_t tbl[128];
   __shared__ TileRanges tr;
   const int32_t i = order[blockIdx.x];
   const int32_t base = lo[i];
   const uint32_t nN = (uint32_t)bdeg[i];
   if (i - base < 2 || nN == 0) return;
-  for (int k = threadIdx.x; k < NW * kTemporalBins * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < NW * kTPairSlots * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < kTemporalBins; k += blockDim.x) bsum[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
   for (int32_t k = threadIdx.x; k < T; k += blockDim.x) M[k] = (MT)0;
   const uint32_t s = eoff[i];
@@ -679,6 +681,8 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
 
   unsigned int* hw = hist[w];
   const int cp = lane & (kC - 1);
+  // triples open at x, per lane by (side, I, X + 2Y) as in k_tcount
+  uint32_t open_cnt[2][2][4] = {};
   for (int32_t t0 = base; t0 < i; t0 += T) {
     const int32_t t1 = min(i, t0 + T);
     const uint32_t total = tile_ranges(tr, si, enodes + s, voff, vpos, t0, t1);
@@ -690,7 +694,8 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
       const int cxi = (int)(ex & kCntMask);
       const MT mx = NM[xi];
       const int sx = esize[x];
-      const int fx = sx > cxi ? 1 : 0;
+      const int xfree = sx - cxi, ifree = si - cxi;
+      unsigned long long fields_b = 0, fields_e = 0;
       // (b) forward row of x inside [max(x + 1, t0), t1): closed triangles and triples open at x
       // (the read position starts at 0 in the first tile, also for an x beyond that tile)
       uint32_t fpos = (t0 == base) ? 0u : FC[xi];
@@ -713,10 +718,14 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
             const int sy = esize[y];
             if (my) {
               const int cyi = popc(my);
-              atomicAdd(&hw[tbl[venn_id(sx, sy, si, cxy, cyi, cxi, popc((MT)(mx & my)))] * kC + cp], 1u);
-              atomicAdd(&hw[tbl[as_open_id(fx, sy > cyi ? 1 : 0, cxi, cyi, si)] * kC + cp], ~0u);
+              const int g = popc((MT)(mx & my));
+              const unsigned low = nz(xfree - cxy + g) | nz(sy - cxy - cyi + g) << 1 | nz(ifree - cyi + g) << 2;
+              const unsigned mid = nz(cxy - g) | nz(cyi - g) << 1 | nz(cxi - g) << 2;
+              const unsigned b = (unsigned)max(min(ifree - cyi, 1), 0);
+              const unsigned slot = (low << 1 | b) + (mid << 4) + ((1u - nz(g)) << 4);
+              atomicAdd(&hw[slot * kC + cp], 1u);
             } else {
-              atomicAdd(&hw[tbl[venn_id(sx, sy, si, cxy, 0, cxi, 0)] * kC + cp], 1u);
+              fields_b += 1ull << (nz(xfree - cxy) << 4 | nz(sy - cxy) << 5);
             }
           }
           const uint32_t nv = (uint32_t)__popcll(__ballot(v));
@@ -752,8 +761,10 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
           }
           if (v) {
             const int32_t y = (int32_t)(e >> kCntBits);
-            if (M[y - t0] == (MT)0)
-              atomicAdd(&hw[tbl[venn_id(esize[y], sx, si, (int)(e & kCntMask), cxi, 0, 0)] * kC + cp], 1u);
+            if (M[y - t0] == (MT)0) {
+              const int cxy = (int)(e & kCntMask);
+              fields_e += 1ull << (nz(xfree - cxy) << 4 | nz((int)esize[y] - cxy) << 5);
+            }
           }
           const uint32_t nv = (uint32_t)__popcll(__ballot(v));
           pos += nv;
@@ -761,19 +772,54 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
         }
         if (lane == 0) BC[xi] = pos;
       }
+      // (I is uniform in the wave; both branches index the counters with constants)
+      if (si > cxi) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          open_cnt[0][1][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
+          open_cnt[1][1][k] += (uint32_t)(fields_e >> (16 * k)) & 0xFFFFu;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          open_cnt[0][0][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
+          open_cnt[1][0][k] += (uint32_t)(fields_e >> (16 * k)) & 0xFFFFu;
+        }
+      }
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x) {  // flush this tile
-      long long sum = 0;
+    // this tile's triangle slots into their closed (+) and as-if-open (-) bins (per tile, so a
+    // slot counter stays far below 2^32)
+    for (int p = threadIdx.x; p < kTPairSlots; p += blockDim.x) {
+      unsigned long long sum = 0;
       for (int q = 0; q < NW; ++q)
         for (int c = 0; c < kC; ++c) {
-          sum += (long long)(int)hist[q][b * kC + c];
-          hist[q][b * kC + c] = 0;
+          sum += hist[q][p * kC + c];
+          hist[q][p * kC + c] = 0;
         }
-      if (sum) atomicAdd(&acc[kTAccHist + b], (unsigned long long)sum);
+      if (sum) {
+        int v, as_open;
+        tpair_bins(p, &v, &as_open);
+        atomicAdd(&bsum[tbl[v]], sum);
+        atomicAdd(&bsum[tbl[as_open]], (unsigned long long)(-(long long)sum));
+      }
     }
     for_tile_entries(tr, si, total, vpos, [&](int, int32_t y) { M[y - t0] = (MT)0; });
   }
+#pragma unroll
+  for (int side = 0; side < 2; ++side)
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t c = wave_sum(open_cnt[side][I][k]);
+        const int X = k & 1, Y = k >> 1;
+        const int id = side == 0 ? (X | Y << 1 | I << 2 | 8 | 32) : (Y | X << 1 | I << 2 | 8 | 16);
+        if (lane == 0 && c) atomicAdd(&bsum[tbl[id]], (unsigned long long)c);
+      }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x)
+    if (bsum[b]) atomicAdd(&acc[kTAccHist + b], bsum[b]);
 }
 
 // Per-anchor work estimate (for launch order) and statistics.
@@ -860,10 +906,10 @@ static size_t win_lds_bytes(int32_t R, bool wide) {
 // for the histogram. Measured on the 200k-edge temporal workload (c4): 4 waves with wave-aggregated
 // histograms 7.04 s, 8 waves aggregated 3.74 s, 8 waves atomics 2.46 s, 16 waves atomics 2.12 s.
 constexpr int kTCountWaves = 16;
-// Static LDS of the tiled kernels (tile walk ranges, reductions; for counting also the 24.6 KB
-// histogram and the bin table), kept free next to the position table.
+// Static LDS of the tiled kernels (tile walk ranges, reductions; for counting also the 36.9 KB
+// triangle-slot counters, the bin sums and the bin table), kept free next to the position table.
 constexpr size_t kTProjTiledStaticLds = 4 * 1024;
-constexpr size_t kTCountTiledStaticLds = 28 * 1024;
+constexpr size_t kTCountTiledStaticLds = 40 * 1024;
 
 template <typename MT>
 static hm_status launch_tcount(hm_tgraph* g, int32_t Wmax, size_t lds, unsigned long long* acc) {
