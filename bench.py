@@ -241,12 +241,13 @@ def run_dynamic(steps: int, warmup: int, with_cpu: bool) -> dict:
             rows[int(x)] = nodes[:0]
         rows.extend(inodes[ioff[k]:ioff[k + 1]] for k in range(ioff.size - 1))
         live = np.concatenate([np.setdiff1d(live, dels), np.arange(nxt, len(rows))])
+    deltas = []
     for dels, ioff, inodes in batches[:warmup]:
-        g.update(dels, ioff, inodes)
+        deltas.append(g.update(dels, ioff, inodes)[1])
     touched, rows_ms, delta_ms, store_ms, entries = 0, 0.0, 0.0, 0.0, 0
     t0 = time.perf_counter()
     for dels, ioff, inodes in batches[warmup:]:
-        g.update(dels, ioff, inodes)
+        deltas.append(g.update(dels, ioff, inodes)[1])
         us = g.update_stats()
         touched += us["removed"] + us["added"]
         rows_ms += us["ms_rows"]
@@ -254,6 +255,16 @@ def run_dynamic(steps: int, warmup: int, with_cpu: bool) -> dict:
         store_ms += us["ms_store"]
         entries += us["row_entries"]
     t1 = time.perf_counter()
+    # the same batches as captured updates (one graph launch per update + delta) on a second copy
+    gc = hm.Graph(off, nodes)
+    gc.set_update_capture(True)
+    same_cap = all(np.array_equal(gc.update(*bt)[1], d) for bt, d in zip(batches[:warmup], deltas))
+    tc = time.perf_counter()
+    for bt, d in zip(batches[warmup:], deltas[warmup:]):
+        same_cap = np.array_equal(gc.update(*bt)[1], d) and same_cap
+    cap_ms = (time.perf_counter() - tc) * 1e3 / steps
+    cap_state = gc.update_capture_state()
+    gc.close()
     g.count_static()  # untimed: scratch sized and the triangle form tuned for this shape
     t2 = time.perf_counter()
     total = int(g.count_static().astype(np.int64).sum())  # the recount a batch would need instead
@@ -269,7 +280,10 @@ def run_dynamic(steps: int, warmup: int, with_cpu: bool) -> dict:
                       "deletes_per_batch": n_del, "touched_triples": touched,
                       "ms_per_batch": {"rows": rows_ms / steps, "delta": delta_ms / steps,
                                        "store": store_ms / steps},
-                      "full_recount_ms": recount_ms, "triples_after": total},
+                      "full_recount_ms": recount_ms, "triples_after": total,
+                      "captured_updates": {"ms_per_batch": cap_ms, "deltas_identical": bool(same_cap),
+                                           "captures": cap_state["captures"],
+                                           "last_ran_captured": bool(cap_state["last_captured"])}},
             "cpu": cpu_baseline_recount(cur_off, cur_nodes) if with_cpu else None}
 
 
