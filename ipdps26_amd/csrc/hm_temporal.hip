@@ -208,6 +208,24 @@ __device__ inline int as_open_id(int fx, int fy, int cxi, int cyi, int si) {
   return 48 + fx + 2 * fy + 4 * (int)(cxi + cyi < si);
 }
 
+// Buffer descriptor over n packed row entries at p (raw, no stride; word 3 as for gfx9 raw
+// buffers): a read at entry index >= n returns 0. Indices passed to row_load must be
+// non-negative and small enough that 4 * index does not wrap.
+// p and n must be the same on every active lane (one row per wave); they are read from the
+// first lane so that the descriptor sits in scalar registers.
+__device__ inline __amdgpu_buffer_rsrc_t row_rsrc(const uint32_t* p, uint32_t n) {
+  const unsigned long long a = (unsigned long long)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  const uint32_t nn = __builtin_amdgcn_readfirstlane(n);
+  uint32_t* q = reinterpret_cast<uint32_t*>((unsigned long long)hi << 32 | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(q, (short)0, (int)(nn * 4u), 0x00020000);
+}
+
+__device__ inline uint32_t row_load(__amdgpu_buffer_rsrc_t rs, uint32_t j) {
+  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(j * 4u), 0, 0);
+}
+
 // Slots of closed triangles (x, y, i) in time order: a triangle adds +1 to the bin of its Venn id v
 // and -1 to its as-if-open bin (the pair x, y was counted as open at i), which is fixed by v and
 // b = [C_xi + C_yi < s_i]: x ⊄ i is Venn bit 0 or 3, y ⊄ i bit 1 or 3, and b is Venn bit 2 when
@@ -387,14 +405,16 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     const int xfree = sx - cxi;  // X = [C_xy < xfree]
     const int ifree = si - cxi;
     unsigned long long fields_b = 0, fields_e = 0;
-    // (b) forward row of x inside (x, i): closed triangles and triples open at x. Two row loads
-    // are in flight per lane; the row is ascending, so the scan stops at the first entry >= i.
+    // (b) forward row of x inside (x, i): closed triangles and triples open at x. The row is
+    // ascending, so the scan stops at the first entry >= i. Rows are read through a buffer
+    // descriptor bounded by the row's length: a read past it returns 0, which no entry is (its
+    // overlap is >= 1), and (e - 1) >> kCntBits then lands past every anchor.
     {
-      const uint32_t* row = frows + foff[x];
       const uint32_t n = (uint32_t)fdeg[x];
-      auto visit = [&](uint32_t j, uint32_t e) {
-        const int32_t y = (int32_t)(e >> kCntBits);
-        if (j >= n || y >= i) return;
+      const __amdgpu_buffer_rsrc_t rs = row_rsrc(frows + foff[x], n);
+      auto visit = [&](uint32_t e) {
+        const int32_t y = (int32_t)((e - 1u) >> kCntBits);
+        if (y >= i) return;
         const int32_t yr = y - base;
         const int cxy = (int)(e & kCntMask);
         const MT my = M[yr];
@@ -416,36 +436,35 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
       for (uint32_t j0 = 0; j0 < n; j0 += kTLoads * kWave) {
         uint32_t e[kTLoads];
 #pragma unroll
-        for (int u = 0; u < kTLoads; ++u) {
-          const uint32_t ju = j0 + lane + u * kWave;
-          e[u] = ju < n ? row[ju] : 0xFFFFFFFFu;
-        }
+        for (int u = 0; u < kTLoads; ++u) e[u] = row_load(rs, j0 + lane + u * kWave);
 #pragma unroll
-        for (int u = 0; u < kTLoads; ++u) visit(j0 + lane + u * kWave, e[u]);
+        for (int u = 0; u < kTLoads; ++u) visit(e[u]);
         // stop once these chunks reached past the anchor (or the row's end)
-        const uint32_t jl = j0 + lane + (kTLoads - 1) * kWave;
-        if (__ballot(jl >= n || (int32_t)(e[kTLoads - 1] >> kCntBits) >= i) != 0ull) break;
+        if (__ballot((int32_t)((e[kTLoads - 1] - 1u) >> kCntBits) >= i) != 0ull) break;
       }
     }
     // (e) backward row of x inside [lo_i, x): triples (y, x, i) open at x. The row is ascending,
     // so it is read from its end until it leaves the window.
     {
-      const uint32_t* row = brows + boff[x];
       const int32_t n = (int32_t)bdeg[x];
-      // kTBackLoads chunks per step, the latest first; an entry before the window ends the scan
+      const __amdgpu_buffer_rsrc_t rs = row_rsrc(brows + boff[x], (uint32_t)n);
+      // kTBackLoads chunks per step, the latest first; an entry before the window ends the scan,
+      // and so does a read before the row's start (taken at index n, past the descriptor's range:
+      // it returns 0). The index is never negative: the hardware adds a load's immediate offset
+      // to its register offset without wrapping, so a negative register offset would not come
+      // back into range for the chunks the compiler addresses through the immediate.
       for (int32_t jend = n; jend > 0; jend -= kTBackLoads * kWave) {
         uint32_t e[kTBackLoads];
 #pragma unroll
         for (int u = 0; u < kTBackLoads; ++u) {
           const int32_t ju = jend - (u + 1) * kWave + lane;
-          e[u] = ju >= 0 ? row[ju] : 0u;
+          e[u] = row_load(rs, ju >= 0 ? (uint32_t)ju : (uint32_t)n);
         }
         bool before = false;
 #pragma unroll
         for (int u = 0; u < kTBackLoads; ++u) {
-          if (jend - (u + 1) * kWave + lane < 0) continue;
           const int32_t yr = (int32_t)(e[u] >> kCntBits) - base;
-          if (yr < 0) {
+          if (e[u] == 0u || yr < 0) {
             before = true;
           } else if (M[yr] == (MT)0) {
             const int cxy = (int)(e[u] & kCntMask);
