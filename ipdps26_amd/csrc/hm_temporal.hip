@@ -369,6 +369,9 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   const int si = (int)esize[i];
   unsigned int* hw = hist[w];
   const int cp = lane & (kC - 1);
+  // the window tables addressed by absolute position (the window start folded into the base)
+  const MT* Mabs = M - base;
+  const uint8_t* szabs = sz - base;
 
   // open at i, counted for all ordered neighbour pairs as if disjoint (closed ones are taken
   // back below when their triangle is met)
@@ -415,10 +418,9 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
       auto visit = [&](uint32_t e) {
         const int32_t y = (int32_t)((e - 1u) >> kCntBits);
         if (y >= i) return;
-        const int32_t yr = y - base;
         const int cxy = (int)(e & kCntMask);
-        const MT my = M[yr];
-        const int sy = sz[yr];
+        const MT my = Mabs[y];
+        const int sy = szabs[y];
         if (my) {
           // Venn regions of (x, y, i) from the overlaps; every count below is >= 0
           const int cyi = popc(my);
@@ -463,12 +465,12 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
         bool before = false;
 #pragma unroll
         for (int u = 0; u < kTBackLoads; ++u) {
-          const int32_t yr = (int32_t)(e[u] >> kCntBits) - base;
-          if (e[u] == 0u || yr < 0) {
+          const int32_t y = (int32_t)(e[u] >> kCntBits);
+          if (e[u] == 0u || y < base) {
             before = true;
-          } else if (M[yr] == (MT)0) {
+          } else if (Mabs[y] == (MT)0) {
             const int cxy = (int)(e[u] & kCntMask);
-            fields_e += 1ull << (nz(xfree - cxy) << 4 | nz((int)sz[yr] - cxy) << 5);
+            fields_e += 1ull << (nz(xfree - cxy) << 4 | nz((int)szabs[y] - cxy) << 5);
           }
         }
         if (__ballot(before)) break;
