@@ -279,7 +279,9 @@ typedef struct hm_temporal_stats {
   int64_t num_nodes;        /* max node id + 1 */
 } hm_temporal_stats;
 hm_status hm_tgraph_get_stats(hm_tgraph* g, hm_temporal_stats* out);
-/* Phases "project" (window bounds + temporal projection rows) and "count" (k_tcount). */
+/* Phases "project" (window bounds + temporal projection rows), "count" (k_tcount or
+ * k_tcount_tiled) and "wedge" (k_twedge, one-table counts only: the open triples whose earliest
+ * edge precedes their middle edge, counted per middle edge). */
 hm_status hm_tgraph_phase_ms(hm_tgraph* g, const char* phase, double* ms);
 
 /* ------------------------------------------------------------------------------------------ */

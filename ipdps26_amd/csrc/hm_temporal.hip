@@ -389,9 +389,14 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
 
   // Triples open at a neighbour x (y overlaps x, not i) fall in one of 16 bins, fixed by the
   // side of y (before x, or between x and i), I = [i ⊄ x], X = [x has a node in neither y nor i]
-  // and Y = [y ⊄ x]; each lane counts them in registers: per x in a 64-bit word of four 16-bit
-  // fields (X + 2Y; a lane meets at most W / 64 + 1 < 2^16 entries of one row), folded into
-  // open_cnt[side][I][X + 2Y] after each x and into the histogram once per anchor.
+  // and Y = [y ⊄ x]. Side (b), y between x and i: each lane counts them in registers, per x in
+  // a 64-bit word of four 16-bit fields (X + 2Y; a lane meets at most W / 64 + 1 < 2^16 entries
+  // of one row), folded into open_cnt[0][I][X + 2Y] after each x. Side (e), y before x: k_twedge
+  // counts every pair (y in B(x), i in F(x)) with y inside i's window by class, whether or not
+  // y overlaps i; the pairs that do are the triangles (y, x, i), met here as the forward entries
+  // of their earliest edge, and each subtracts its class from open_cnt[1] (as side (e) counts of
+  // the triple's middle edge: I' = [i ⊄ y], X' = [|y| > C_xy + C_yi], Y' = [x ⊄ y], per lane, in
+  // two words by I').
   uint32_t open_cnt[2][2][4] = {};
   constexpr int kTLoads = 8;      // forward-row loads in flight per lane
   constexpr int kTBackLoads = 8;  // backward-row loads in flight per lane
@@ -407,7 +412,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     const int cxi = popc(mx);
     const int xfree = sx - cxi;  // X = [C_xy < xfree]
     const int ifree = si - cxi;
-    unsigned long long fields_b = 0, fields_e = 0;
+    unsigned long long fields_b = 0, tri_e0 = 0, tri_e1 = 0;
     // (b) forward row of x inside (x, i): closed triangles and triples open at x. The row is
     // ascending, so the scan stops at the first entry >= i. Rows are read through a buffer
     // descriptor bounded by the row's length: a read past it returns 0, which no entry is (its
@@ -430,6 +435,12 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
           const unsigned b = (unsigned)max(min(ifree - cyi, 1), 0);
           const unsigned slot = (low << 1 | b) + (mid << 4) + ((1u - nz(g)) << 4);
           atomicAdd(&hw[slot * kC + cp], 1u);
+          // the triangle as a side (e) pair of its middle edge y (taken back from k_twedge's count)
+          const unsigned long long one_e =
+              1ull << (16 * ((sy - cyi - cxy > 0 ? 1 : 0) + (sx > cxy ? 2 : 0)));
+          const bool ie = si > cyi;
+          tri_e1 += ie ? one_e : 0ull;
+          tri_e0 += ie ? 0ull : one_e;
         } else {
           // x ∩ y and x ∩ i are disjoint here, so C_xy <= xfree
           fields_b += 1ull << (nz(xfree - cxy) << 4 | nz(sy - cxy) << 5);
@@ -445,50 +456,18 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
         if (__ballot((int32_t)((e[kTLoads - 1] - 1u) >> kCntBits) >= i) != 0ull) break;
       }
     }
-    // (e) backward row of x inside [lo_i, x): triples (y, x, i) open at x. The row is ascending,
-    // so it is read from its end until it leaves the window.
-    {
-      const int32_t n = (int32_t)bdeg[x];
-      const __amdgpu_buffer_rsrc_t rs = row_rsrc(brows + boff[x], (uint32_t)n);
-      // kTBackLoads chunks per step, the latest first; an entry before the window ends the scan,
-      // and so does a read before the row's start (taken at index n, past the descriptor's range:
-      // it returns 0). The index is never negative: the hardware adds a load's immediate offset
-      // to its register offset without wrapping, so a negative register offset would not come
-      // back into range for the chunks the compiler addresses through the immediate.
-      for (int32_t jend = n; jend > 0; jend -= kTBackLoads * kWave) {
-        uint32_t e[kTBackLoads];
-#pragma unroll
-        for (int u = 0; u < kTBackLoads; ++u) {
-          const int32_t ju = jend - (u + 1) * kWave + lane;
-          e[u] = row_load(rs, ju >= 0 ? (uint32_t)ju : (uint32_t)n);
-        }
-        bool before = false;
-#pragma unroll
-        for (int u = 0; u < kTBackLoads; ++u) {
-          const int32_t y = (int32_t)(e[u] >> kCntBits);
-          if (e[u] == 0u || y < base) {
-            before = true;
-          } else if (Mabs[y] == (MT)0) {
-            const int cxy = (int)(e[u] & kCntMask);
-            fields_e += 1ull << (nz(xfree - cxy) << 4 | nz((int)szabs[y] - cxy) << 5);
-          }
-        }
-        if (__ballot(before)) break;
-      }
-    }
     // (I is uniform in the wave; both branches index the counters with constants)
     if (si > cxi) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        open_cnt[0][1][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
-        open_cnt[1][1][k] += (uint32_t)(fields_e >> (16 * k)) & 0xFFFFu;
-      }
+      for (int k = 0; k < 4; ++k) open_cnt[0][1][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
     } else {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        open_cnt[0][0][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
-        open_cnt[1][0][k] += (uint32_t)(fields_e >> (16 * k)) & 0xFFFFu;
-      }
+      for (int k = 0; k < 4; ++k) open_cnt[0][0][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      open_cnt[1][0][k] += (uint32_t)(tri_e0 >> (16 * k)) & 0xFFFFu;
+      open_cnt[1][1][k] += (uint32_t)(tri_e1 >> (16 * k)) & 0xFFFFu;
     }
   }
   // open-at-x bins: (b) time order (x, y, i): Venn id X | Y << 1 | I << 2 | 8 (xy) | 32 (ix);
@@ -502,7 +481,9 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
         const uint32_t c = wave_sum(open_cnt[side][I][k]);
         const int X = k & 1, Y = k >> 1;
         const int id = side == 0 ? (X | Y << 1 | I << 2 | 8 | 32) : (Y | X << 1 | I << 2 | 8 | 16);
-        if (lane == 0 && c) atomicAdd(&bsum[tbl[id]], (unsigned long long)c);
+        // side (e) holds triangles to take back from k_twedge's pair counts
+        const unsigned long long v = side == 0 ? (unsigned long long)c : (unsigned long long)(-(long long)c);
+        if (lane == 0 && c) atomicAdd(&bsum[tbl[id]], v);
       }
   __syncthreads();
   // each triangle slot: +count to its closed bin, -count to its as-if-open bin
@@ -520,6 +501,116 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   __syncthreads();
   for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x)
     if (bsum[b]) atomicAdd(&acc[kTAccHist + b], bsum[b]);
+}
+
+// Side (e) of the triples open at a middle edge x: for every pair (y in B(x), i in F(x)) with y
+// inside i's window (lo_i <= y; then also lo_i <= x), the bin of (y, x, i) as if y and i were
+// disjoint — Venn id Y | X << 1 | I << 2 | 8 | 16 with Y = [y ⊄ x], X = [|x| > C_xy + C_xi],
+// I = [i ⊄ x]. k_tcount takes back the pairs with y overlapping i (the triangles (y, x, i)).
+// One wave per x walks F(x) in time order (lo_i is non-decreasing along it) and keeps the
+// entries of B(x) still inside the current window in an LDS histogram by (Y, C_xy); a wave
+// prefix sum over the histogram answers "how many have C_xy < |x| - C_xi" for each i.
+template <int NW>
+__global__ __launch_bounds__(NW * kWave) void k_twedge(
+    int32_t E, const int32_t* __restrict__ lo, const uint8_t* __restrict__ esize,
+    const unsigned long long* __restrict__ foff, const unsigned long long* __restrict__ fdeg,
+    const uint32_t* __restrict__ frows, const unsigned long long* __restrict__ boff,
+    const unsigned long long* __restrict__ bdeg, const uint32_t* __restrict__ brows,
+    unsigned long long* __restrict__ acc) {
+  __shared__ uint32_t H[NW][2][kWave + 1];  // [Y][C_xy], C_xy in 1..64
+  __shared__ int8_t tbl[128];
+  const int lane = lane_id(), w = wave_id();
+  if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
+  for (int k = lane; k < 2 * (kWave + 1); k += kWave) (&H[w][0][0])[k] = 0u;
+  __syncthreads();
+  const int32_t x = (int32_t)blockIdx.x * NW + w;
+  if (x >= E) return;
+  const uint32_t nb = (uint32_t)bdeg[x], nf = (uint32_t)fdeg[x];
+  if (nb == 0 || nf == 0) return;
+  const int sx = esize[x];
+  const uint32_t* B = brows + boff[x];
+  const uint32_t* F = frows + foff[x];
+  auto ybit = [&](uint32_t e) { return (int)esize[e >> kCntBits] > (int)(e & kCntMask) ? 1 : 0; };
+  for (uint32_t j = lane; j < nb; j += kWave) {
+    const uint32_t e = B[j];
+    atomicAdd(&H[w][ybit(e)][e & kCntMask], 1u);
+  }
+  __threadfence_block();
+  // inclusive prefix over C_xy of the histogram (lane c - 1 holds C_xy <= c)
+  uint32_t P0 = 0, P1 = 0;
+  auto rescan = [&]() {
+    P0 = H[w][0][lane + 1];
+    P1 = H[w][1][lane + 1];
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t q0 = __shfl_up(P0, o), q1 = __shfl_up(P1, o);
+      if (lane >= o) {
+        P0 += q0;
+        P1 += q1;
+      }
+    }
+  };
+  rescan();
+  // entries of B(x) from p on are inside the window; the chunk [cb, cb + 64) is held in registers
+  uint32_t p = 0, cb = 0;
+  uint32_t eb = lane < nb ? B[lane] : 0u;
+  unsigned long long cnt[2][2][2] = {};  // [I][X][Y]
+  bool done = false;
+  for (uint32_t j0 = 0; j0 < nf && !done; j0 += kWave) {
+    const uint32_t j = j0 + lane;
+    uint32_t ei = 0;
+    int32_t loi = 0x7FFFFFFF;
+    int si = 0;
+    if (j < nf) {
+      ei = F[j];
+      loi = lo[ei >> kCntBits];
+      si = esize[ei >> kCntBits];
+    }
+    const int kn = (int)min((uint32_t)kWave, nf - j0);
+    for (int k = 0; k < kn; ++k) {
+      const int32_t lok = __builtin_amdgcn_readlane(loi, k);
+      if (lok > x) {  // x is outside this anchor's window, and so for every later one
+        done = true;
+        break;
+      }
+      // drop the entries before the window (ascending: the next ones from p)
+      bool dropped = false;
+      while (p < nb) {
+        const uint32_t idx = cb + (uint32_t)lane;
+        const bool rm = idx >= p && idx < nb && (int32_t)(eb >> kCntBits) < lok;
+        const unsigned long long bal = __ballot(rm);
+        if (rm) atomicSub(&H[w][ybit(eb)][eb & kCntMask], 1u);
+        const uint32_t nrm = (uint32_t)__popcll(bal);
+        dropped = dropped || nrm != 0;
+        p += nrm;
+        if (p < cb + kWave) break;  // the chunk still holds entries inside the window
+        cb += kWave;                // chunk used up: fetch the next one
+        eb = cb + lane < nb ? B[cb + lane] : 0u;
+      }
+      if (dropped) {
+        __threadfence_block();
+        rescan();
+      }
+      const uint32_t ek = __builtin_amdgcn_readlane(ei, k);
+      const int cxi = (int)(ek & kCntMask);
+      const int sik = __builtin_amdgcn_readlane(si, k);
+      const int T = sx - cxi;  // X = [C_xy < T]
+      const uint32_t t0 = __builtin_amdgcn_readlane(P0, kWave - 1);
+      const uint32_t t1 = __builtin_amdgcn_readlane(P1, kWave - 1);
+      const uint32_t a0 = T >= 2 ? __builtin_amdgcn_readlane(P0, min(T - 2, kWave - 1)) : 0u;
+      const uint32_t a1 = T >= 2 ? __builtin_amdgcn_readlane(P1, min(T - 2, kWave - 1)) : 0u;
+      const int I = sik > cxi ? 1 : 0;
+      cnt[I][1][0] += a0;
+      cnt[I][0][0] += t0 - a0;
+      cnt[I][1][1] += a1;
+      cnt[I][0][1] += t1 - a1;
+    }
+  }
+  if (lane == 0) {
+    for (int I = 0; I < 2; ++I)
+      for (int X = 0; X < 2; ++X)
+        for (int Y = 0; Y < 2; ++Y)
+          if (cnt[I][X][Y]) atomicAdd(&acc[kTAccHist + tbl[Y | X << 1 | I << 2 | 8 | 16]], cnt[I][X][Y]);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -914,7 +1005,7 @@ struct hm_tgraph {
   std::vector<int32_t> order_to_input;  // position -> input index
   hm_temporal_stats stats{};
   bool have_stats = false;
-  float ms_project = 0.f, ms_count = 0.f;
+  float ms_project = 0.f, ms_count = 0.f, ms_wedge = 0.f;
 };
 
 namespace hm {
@@ -927,6 +1018,7 @@ static size_t win_lds_bytes(int32_t R, bool wide) {
 // for the histogram. Measured on the 200k-edge temporal workload (c4): 4 waves with wave-aggregated
 // histograms 7.04 s, 8 waves aggregated 3.74 s, 8 waves atomics 2.46 s, 16 waves atomics 2.12 s.
 constexpr int kTCountWaves = 16;
+constexpr int kWedgeWaves = 4;  // k_twedge: one wave per middle edge
 // Static LDS of the tiled kernels (tile walk ranges, reductions; for counting also the 36.9 KB
 // triangle-slot counters, the bin sums and the bin table), kept free next to the position table.
 constexpr size_t kTProjTiledStaticLds = 4 * 1024;
@@ -1079,10 +1171,19 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
   }
   g->last_tile = tiled ? Tc : 0;
   HM_HIP(hipEventRecord(g->ev[2], st));
+  if (!tiled) {  // side (e) pairs of the one-table count (the tiled count scans backward rows)
+    k_twedge<kWedgeWaves><<<(unsigned)((E + kWedgeWaves - 1) / kWedgeWaves), kWedgeWaves * kWave, 0, st>>>(
+        E, g->lo.as<int32_t>(), g->esize.as<uint8_t>(), g->foff.as<unsigned long long>(),
+        g->fdeg.as<unsigned long long>(), g->frows.as<uint32_t>(), g->boff.as<unsigned long long>(),
+        g->bdeg.as<unsigned long long>(), g->brows.as<uint32_t>(), acc);
+    HM_HIP(hipGetLastError());
+  }
+  HM_HIP(hipEventRecord(g->ev[3], st));
   HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kTAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
   HM_HIP(hipEventElapsedTime(&g->ms_project, g->ev[0], g->ev[1]));
   HM_HIP(hipEventElapsedTime(&g->ms_count, g->ev[1], g->ev[2]));
+  HM_HIP(hipEventElapsedTime(&g->ms_wedge, g->ev[2], g->ev[3]));
   return HM_OK;
 }
 
@@ -1237,6 +1338,7 @@ hm_status hm_tgraph_phase_ms(hm_tgraph* g, const char* phase, double* ms) {
   if (!g || !phase || !ms) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_phase_ms: null argument");
   if (std::strcmp(phase, "project") == 0) *ms = g->ms_project;
   else if (std::strcmp(phase, "count") == 0) *ms = g->ms_count;
+  else if (std::strcmp(phase, "wedge") == 0) *ms = g->ms_wedge;
   else return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_phase_ms: unknown phase");
   return HM_OK;
 }
