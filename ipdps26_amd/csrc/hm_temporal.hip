@@ -247,6 +247,33 @@ __device__ inline void tpair_bins(int p, int* v, int* as_open) {
   *as_open = 48 + fx + 2 * fy + 4 * b;  // as_open_id with [C_xi + C_yi < s_i] = b
 }
 
+// One-table count: a triangle (x, y, i) in time order also takes back its side (e) pair of the
+// middle edge y (counted by k_twedge): bin Y' | X' << 1 | I' << 2 | 8 | 16 with Y' = [x ⊄ y]
+// (Venn bit 0 or 5), I' = [i ⊄ y] (bit 2 or 5) and X' = [|y| > C_xy + C_yi], which is Venn bit 1
+// when x ∩ y ∩ i is empty and one more bit b2 otherwise. Slot = 2 tpair + b2 for x ∩ y ∩ i
+// non-empty (0..255), 256 + Venn bits 0-2 without (256..263).
+constexpr int kTSlots = 264;
+
+// Venn id, as-if-open id and side (e) id of the middle edge, of one-table slot p.
+__device__ inline void tslot_bins(int p, int* v, int* as_open, int* side_e) {
+  int vv, b, b2;
+  if (p < 256) {
+    const int q = p >> 1;
+    b2 = p & 1;
+    b = q & 1;
+    vv = ((q >> 1) & 7) | (q >> 4) << 3 | 64;
+  } else {
+    vv = (p & 7) | 56;
+    b = (vv >> 2) & 1;
+    b2 = (vv >> 1) & 1;
+  }
+  const int fx = (vv & 1) | ((vv >> 3) & 1), fy = ((vv >> 1) & 1) | ((vv >> 3) & 1);
+  const int ye = (vv & 1) | ((vv >> 5) & 1), ie = ((vv >> 2) & 1) | ((vv >> 5) & 1);
+  *v = vv;
+  *as_open = 48 + fx + 2 * fy + 4 * b;  // as_open_id with [C_xi + C_yi < s_i] = b
+  *side_e = ye | b2 << 1 | ie << 2 | 8 | 16;
+}
+
 // Counts, for every ordered pair (x earlier, y later) of the nN neighbours of the anchor (in time
 // order), the as-if-open id (8 cases: fx, fy, [C_xi + C_yi < si]) with one wave. load(k, C, f)
 // gives neighbour k's overlap C with the anchor and f = [it has a node outside the anchor]. The
@@ -318,9 +345,9 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     const uint32_t* __restrict__ frows, const unsigned long long* __restrict__ boff,
     const unsigned long long* __restrict__ bdeg, const uint32_t* __restrict__ brows,
     const int32_t* __restrict__ order, int32_t Wmax, unsigned long long* __restrict__ acc) {
-  constexpr int kC = 4;
+  constexpr int kC = 2;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ unsigned int hist[NW][kTPairSlots * kC];
+  __shared__ unsigned int hist[NW][kTSlots * kC];
   __shared__ unsigned long long bsum[kTemporalBins];
   __shared__ int8_t tbl[128];
   __shared__ uint32_t wcnt[NW];
@@ -335,7 +362,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   off += ((size_t)Wmax * sizeof(uint16_t) + 15) & ~(size_t)15;
   uint8_t* sz = reinterpret_cast<uint8_t*>(smem + off);
 
-  for (int k = threadIdx.x; k < NW * kTPairSlots * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < NW * kTSlots * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
   for (int k = threadIdx.x; k < kTemporalBins; k += blockDim.x) bsum[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
   if (threadIdx.x == 0) s_next = 0;
@@ -391,13 +418,11 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   // side of y (before x, or between x and i), I = [i ⊄ x], X = [x has a node in neither y nor i]
   // and Y = [y ⊄ x]. Side (b), y between x and i: each lane counts them in registers, per x in
   // a 64-bit word of four 16-bit fields (X + 2Y; a lane meets at most W / 64 + 1 < 2^16 entries
-  // of one row), folded into open_cnt[0][I][X + 2Y] after each x. Side (e), y before x: k_twedge
+  // of one row), folded into open_cnt[I][X + 2Y] after each x. Side (e), y before x: k_twedge
   // counts every pair (y in B(x), i in F(x)) with y inside i's window by class, whether or not
   // y overlaps i; the pairs that do are the triangles (y, x, i), met here as the forward entries
-  // of their earliest edge, and each subtracts its class from open_cnt[1] (as side (e) counts of
-  // the triple's middle edge: I' = [i ⊄ y], X' = [|y| > C_xy + C_yi], Y' = [x ⊄ y], per lane, in
-  // two words by I').
-  uint32_t open_cnt[2][2][4] = {};
+  // of their earliest edge, and each takes its pair back through its slot (kTSlots).
+  uint32_t open_cnt[2][4] = {};  // [I][X + 2Y], side (b)
   constexpr int kTLoads = 8;      // forward-row loads in flight per lane
   constexpr int kTBackLoads = 8;  // backward-row loads in flight per lane
   for (;;) {
@@ -412,7 +437,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     const int cxi = popc(mx);
     const int xfree = sx - cxi;  // X = [C_xy < xfree]
     const int ifree = si - cxi;
-    unsigned long long fields_b = 0, tri_e0 = 0, tri_e1 = 0;
+    unsigned long long fields_b = 0;
     // (b) forward row of x inside (x, i): closed triangles and triples open at x. The row is
     // ascending, so the scan stops at the first entry >= i. Rows are read through a buffer
     // descriptor bounded by the row's length: a read past it returns 0, which no entry is (its
@@ -433,14 +458,9 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
           const unsigned low = nz(xfree - cxy + g) | nz(sy - cxy - cyi + g) << 1 | nz(ifree - cyi + g) << 2;
           const unsigned mid = nz(cxy - g) | nz(cyi - g) << 1 | nz(cxi - g) << 2;
           const unsigned b = (unsigned)max(min(ifree - cyi, 1), 0);
-          const unsigned slot = (low << 1 | b) + (mid << 4) + ((1u - nz(g)) << 4);
+          const unsigned b2 = (unsigned)max(min(sy - cyi - cxy, 1), 0);
+          const unsigned slot = g ? (((low << 1 | b) + (mid << 4)) << 1 | b2) : (256u | low);
           atomicAdd(&hw[slot * kC + cp], 1u);
-          // the triangle as a side (e) pair of its middle edge y (taken back from k_twedge's count)
-          const unsigned long long one_e =
-              1ull << (16 * ((sy - cyi - cxy > 0 ? 1 : 0) + (sx > cxy ? 2 : 0)));
-          const bool ie = si > cyi;
-          tri_e1 += ie ? one_e : 0ull;
-          tri_e0 += ie ? 0ull : one_e;
         } else {
           // x ∩ y and x ∩ i are disjoint here, so C_xy <= xfree
           fields_b += 1ull << (nz(xfree - cxy) << 4 | nz(sy - cxy) << 5);
@@ -459,43 +479,35 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     // (I is uniform in the wave; both branches index the counters with constants)
     if (si > cxi) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) open_cnt[0][1][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
+      for (int k = 0; k < 4; ++k) open_cnt[1][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
     } else {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) open_cnt[0][0][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      open_cnt[1][0][k] += (uint32_t)(tri_e0 >> (16 * k)) & 0xFFFFu;
-      open_cnt[1][1][k] += (uint32_t)(tri_e1 >> (16 * k)) & 0xFFFFu;
+      for (int k = 0; k < 4; ++k) open_cnt[0][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
     }
   }
-  // open-at-x bins: (b) time order (x, y, i): Venn id X | Y << 1 | I << 2 | 8 (xy) | 32 (ix);
-  //                 (e) time order (y, x, i): Venn id Y | X << 1 | I << 2 | 8 (yx) | 16 (xi)
+  // open-at-x bins, side (b), time order (x, y, i): Venn id X | Y << 1 | I << 2 | 8 (xy) | 32 (ix)
 #pragma unroll
-  for (int side = 0; side < 2; ++side)
+  for (int I = 0; I < 2; ++I)
 #pragma unroll
-    for (int I = 0; I < 2; ++I)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t c = wave_sum(open_cnt[side][I][k]);
-        const int X = k & 1, Y = k >> 1;
-        const int id = side == 0 ? (X | Y << 1 | I << 2 | 8 | 32) : (Y | X << 1 | I << 2 | 8 | 16);
-        // side (e) holds triangles to take back from k_twedge's pair counts
-        const unsigned long long v = side == 0 ? (unsigned long long)c : (unsigned long long)(-(long long)c);
-        if (lane == 0 && c) atomicAdd(&bsum[tbl[id]], v);
-      }
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t c = wave_sum(open_cnt[I][k]);
+      const int X = k & 1, Y = k >> 1;
+      if (lane == 0 && c) atomicAdd(&bsum[tbl[X | Y << 1 | I << 2 | 8 | 32]], (unsigned long long)c);
+    }
   __syncthreads();
-  // each triangle slot: +count to its closed bin, -count to its as-if-open bin
-  for (int p = threadIdx.x; p < kTPairSlots; p += blockDim.x) {
+  // each triangle slot: +count to its closed bin, -count to its as-if-open bin and to its middle
+  // edge's side (e) bin
+  for (int p = threadIdx.x; p < kTSlots; p += blockDim.x) {
     unsigned long long s = 0;
     for (int q = 0; q < NW; ++q)
       for (int c = 0; c < kC; ++c) s += hist[q][p * kC + c];
     if (s) {
-      int v, as_open;
-      tpair_bins(p, &v, &as_open);
+      int v, as_open, side_e;
+      tslot_bins(p, &v, &as_open, &side_e);
+      const unsigned long long neg = (unsigned long long)(-(long long)s);
       atomicAdd(&bsum[tbl[v]], s);
-      atomicAdd(&bsum[tbl[as_open]], (unsigned long long)(-(long long)s));
+      atomicAdd(&bsum[tbl[as_open]], neg);
+      atomicAdd(&bsum[tbl[side_e]], neg);
     }
   }
   __syncthreads();
@@ -1074,7 +1086,7 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
   (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
   const size_t lds_proj = win_lds_bytes(Rmax, sizeof(MT) == 8);
   const size_t lds_cnt = TCountLds<MT>::bytes(Wmax);
-  const size_t static_cnt = sizeof(uint32_t) * 16 * kTPairSlots * 4 +
+  const size_t static_cnt = sizeof(uint32_t) * 16 * kTSlots * 2 +
                             sizeof(unsigned long long) * kTemporalBins + 256;
   // one LDS table per window when it fits; position tiles otherwise (or when forced)
   const bool tiled = g->tile > 0 || lds_proj > (size_t)lim || lds_cnt + static_cnt > (size_t)lim ||
