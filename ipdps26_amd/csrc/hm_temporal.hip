@@ -519,9 +519,11 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
 // inside i's window (lo_i <= y; then also lo_i <= x), the bin of (y, x, i) as if y and i were
 // disjoint — Venn id Y | X << 1 | I << 2 | 8 | 16 with Y = [y ⊄ x], X = [|x| > C_xy + C_xi],
 // I = [i ⊄ x]. k_tcount takes back the pairs with y overlapping i (the triangles (y, x, i)).
-// One wave per x walks F(x) in time order (lo_i is non-decreasing along it) and keeps the
-// entries of B(x) still inside the current window in an LDS histogram by (Y, C_xy); a wave
-// prefix sum over the histogram answers "how many have C_xy < |x| - C_xi" for each i.
+// One wave per x walks F(x) in time order, 64 anchors i per step (one per lane; lo_i is
+// non-decreasing along F(x)), and keeps the entries of B(x) inside the step's first window in an
+// LDS histogram by (Y, C_xy) with its prefix sum over C_xy across the lanes. A lane's count is the
+// histogram's "C_xy < |x| - C_xi" minus the entries leaving the window during the step that lie
+// before its own window start (those are broadcast one by one).
 template <int NW>
 __global__ __launch_bounds__(NW * kWave) void k_twedge(
     int32_t E, const int32_t* __restrict__ lo, const uint8_t* __restrict__ esize,
@@ -548,7 +550,7 @@ __global__ __launch_bounds__(NW * kWave) void k_twedge(
     atomicAdd(&H[w][ybit(e)][e & kCntMask], 1u);
   }
   __threadfence_block();
-  // inclusive prefix over C_xy of the histogram (lane c - 1 holds C_xy <= c)
+  // inclusive prefix over C_xy of the histogram (lane c - 1 holds the count of C_xy <= c)
   uint32_t P0 = 0, P1 = 0;
   auto rescan = [&]() {
     P0 = H[w][0][lane + 1];
@@ -562,12 +564,12 @@ __global__ __launch_bounds__(NW * kWave) void k_twedge(
     }
   };
   rescan();
-  // entries of B(x) from p on are inside the window; the chunk [cb, cb + 64) is held in registers
+  // entries of B(x) from p on are inside the current window; chunk [cb, cb + 64) in registers
   uint32_t p = 0, cb = 0;
   uint32_t eb = lane < nb ? B[lane] : 0u;
-  unsigned long long cnt[2][2][2] = {};  // [I][X][Y]
-  bool done = false;
-  for (uint32_t j0 = 0; j0 < nf && !done; j0 += kWave) {
+  int yb = lane < nb ? ybit(eb) : 0;
+  unsigned long long cnt[2][2][2] = {};  // per lane: [I][X][Y]
+  for (uint32_t j0 = 0; j0 < nf; j0 += kWave) {
     const uint32_t j = j0 + lane;
     uint32_t ei = 0;
     int32_t loi = 0x7FFFFFFF;
@@ -577,52 +579,73 @@ __global__ __launch_bounds__(NW * kWave) void k_twedge(
       loi = lo[ei >> kCntBits];
       si = esize[ei >> kCntBits];
     }
-    const int kn = (int)min((uint32_t)kWave, nf - j0);
-    for (int k = 0; k < kn; ++k) {
-      const int32_t lok = __builtin_amdgcn_readlane(loi, k);
-      if (lok > x) {  // x is outside this anchor's window, and so for every later one
-        done = true;
-        break;
+    const bool valid = loi <= x;  // a prefix of the lanes (lo_i non-decreasing)
+    const unsigned long long vb = __ballot(valid);
+    if (vb == 0ull) break;        // x is outside every later anchor's window
+    const int32_t lo_last = __builtin_amdgcn_readlane(loi, __popcll(vb) - 1);
+    const int cxi = (int)(ei & kCntMask);
+    const int T = sx - cxi;  // X = [C_xy < T]
+    // counts over the window of the step's first anchor (histogram as it stands)
+    uint32_t a0 = __shfl(P0, T >= 2 ? T - 2 : 0), a1 = __shfl(P1, T >= 2 ? T - 2 : 0);
+    if (T < 2) {
+      a0 = 0u;
+      a1 = 0u;
+    }
+    uint32_t t0 = __builtin_amdgcn_readlane(P0, kWave - 1), t1 = __builtin_amdgcn_readlane(P1, kWave - 1);
+    // entries leaving the window during this step (y < lo of the step's last anchor): each lane
+    // takes back those before its own window start, then they leave the histogram
+    bool dropped = false;
+    while (p < nb) {
+      const uint32_t idx = cb + (uint32_t)lane;
+      const bool rm = idx >= p && idx < nb && (int32_t)(eb >> kCntBits) < lo_last;
+      unsigned long long bal = __ballot(rm);
+      const uint32_t nrm = (uint32_t)__popcll(bal);
+      while (bal) {
+        const int sl = __builtin_ctzll(bal);
+        bal &= bal - 1ull;
+        const uint32_t ee = __builtin_amdgcn_readlane(eb, sl);
+        const int ye = __builtin_amdgcn_readlane(yb, sl);
+        if ((int32_t)(ee >> kCntBits) < loi) {
+          const uint32_t lt = (int)(ee & kCntMask) < T ? 1u : 0u;
+          if (ye) {
+            t1 -= 1u;
+            a1 -= lt;
+          } else {
+            t0 -= 1u;
+            a0 -= lt;
+          }
+        }
       }
-      // drop the entries before the window (ascending: the next ones from p)
-      bool dropped = false;
-      while (p < nb) {
-        const uint32_t idx = cb + (uint32_t)lane;
-        const bool rm = idx >= p && idx < nb && (int32_t)(eb >> kCntBits) < lok;
-        const unsigned long long bal = __ballot(rm);
-        if (rm) atomicSub(&H[w][ybit(eb)][eb & kCntMask], 1u);
-        const uint32_t nrm = (uint32_t)__popcll(bal);
-        dropped = dropped || nrm != 0;
-        p += nrm;
-        if (p < cb + kWave) break;  // the chunk still holds entries inside the window
-        cb += kWave;                // chunk used up: fetch the next one
-        eb = cb + lane < nb ? B[cb + lane] : 0u;
-      }
-      if (dropped) {
-        __threadfence_block();
-        rescan();
-      }
-      const uint32_t ek = __builtin_amdgcn_readlane(ei, k);
-      const int cxi = (int)(ek & kCntMask);
-      const int sik = __builtin_amdgcn_readlane(si, k);
-      const int T = sx - cxi;  // X = [C_xy < T]
-      const uint32_t t0 = __builtin_amdgcn_readlane(P0, kWave - 1);
-      const uint32_t t1 = __builtin_amdgcn_readlane(P1, kWave - 1);
-      const uint32_t a0 = T >= 2 ? __builtin_amdgcn_readlane(P0, min(T - 2, kWave - 1)) : 0u;
-      const uint32_t a1 = T >= 2 ? __builtin_amdgcn_readlane(P1, min(T - 2, kWave - 1)) : 0u;
-      const int I = sik > cxi ? 1 : 0;
+      if (rm) atomicSub(&H[w][yb][eb & kCntMask], 1u);
+      dropped = dropped || nrm != 0;
+      p += nrm;
+      if (p < cb + kWave) break;  // the chunk still holds entries inside the window
+      cb += kWave;                // chunk used up: fetch the next one
+      eb = cb + lane < nb ? B[cb + lane] : 0u;
+      yb = cb + lane < nb ? ybit(eb) : 0;
+    }
+    if (valid) {
+      const int I = si > cxi ? 1 : 0;
       cnt[I][1][0] += a0;
       cnt[I][0][0] += t0 - a0;
       cnt[I][1][1] += a1;
       cnt[I][0][1] += t1 - a1;
     }
+    if (dropped) {
+      __threadfence_block();
+      rescan();
+    }
+    if (vb != ~0ull) break;  // some anchor of this step already lies past x's reach
   }
-  if (lane == 0) {
-    for (int I = 0; I < 2; ++I)
-      for (int X = 0; X < 2; ++X)
-        for (int Y = 0; Y < 2; ++Y)
-          if (cnt[I][X][Y]) atomicAdd(&acc[kTAccHist + tbl[Y | X << 1 | I << 2 | 8 | 16]], cnt[I][X][Y]);
-  }
+#pragma unroll
+  for (int I = 0; I < 2; ++I)
+#pragma unroll
+    for (int X = 0; X < 2; ++X)
+#pragma unroll
+      for (int Y = 0; Y < 2; ++Y) {
+        const unsigned long long c = wave_sum(cnt[I][X][Y]);
+        if (lane == 0 && c) atomicAdd(&acc[kTAccHist + tbl[Y | X << 1 | I << 2 | 8 | 16]], c);
+      }
 }
 
 // ---------------------------------------------------------------------------------------------
