@@ -22,11 +22,14 @@
 //   open at i:  every ordered pair (x, y) of NL counted as if x ∩ y = ∅, from class histograms
 //               (ordered_open_pairs: C with i, [node outside i], [C_x + C_y < |i|]);
 //   for every x in NL (one wave per x):
-//     (b) y in F(x), y < i:   y a neighbour of i: the closed triangle (x, y, i) +1 in its bin and
-//                             -1 in the as-if-open bin counted above; else (x, y, i) is open at x;
-//     (e) y in B(x), y >= lo_i, y not a neighbour of i: triple (y, x, i), open at x.
-// Windows longer than one LDS table use the tiled kernels below (position tiles, the anchor's
-// neighbours read from its own backward row).
+//     (b) y in F(x), y < i:   y a neighbour of i: the closed triangle (x, y, i) +1 in its bin,
+//                             -1 in the as-if-open bin counted above and -1 in its middle edge's
+//                             side (e) bin (below); else (x, y, i) is open at x.
+// Side (e), triples (y, x, i) open at x with y earlier (k_twedge, one wave per x): every pair
+// (y in B(x), i in F(x)) with y >= lo_i counted by class whether or not y overlaps i; the pairs
+// where it does are the triangles, taken back by the counting kernel as above.
+// Windows longer than one LDS table use the tiled counting kernel below (position tiles, the
+// anchor's neighbours read from its own backward row) with the same k_twedge.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -226,35 +229,22 @@ __device__ inline uint32_t row_load(__amdgpu_buffer_rsrc_t rs, uint32_t j) {
   return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(j * 4u), 0, 0);
 }
 
-// Slots of closed triangles (x, y, i) in time order: a triangle adds +1 to the bin of its Venn id v
-// and -1 to its as-if-open bin (the pair x, y was counted as open at i), which is fixed by v and
-// b = [C_xi + C_yi < s_i]: x ⊄ i is Venn bit 0 or 3, y ⊄ i bit 1 or 3, and b is Venn bit 2 when
-// x ∩ y ∩ i is empty. So one counter per (v, b) holds both. Slot = (low << 1 | b) + 16 mid +
-// 16 [x ∩ y ∩ i = ∅] with low = Venn bits 0-2 (private regions) and mid = bits 3-5 (pairwise-only
-// regions): 0..127 with x ∩ y ∩ i non-empty, 128..143 without (then mid = 7).
-constexpr int kTPairSlots = 144;
-
 // [r > 0] of a region count r >= 0, as one unsigned min
 __device__ inline unsigned nz(int r) { return min((unsigned)r, 1u); }
 
-// Venn id and as-if-open id of slot p.
-__device__ inline void tpair_bins(int p, int* v, int* as_open) {
-  const int q = p < 128 ? p : p - 128;
-  const int b = q & 1, low = (q >> 1) & 7;
-  const int vv = p < 128 ? (low | (p >> 4) << 3 | 64) : (low | 56);
-  const int fx = (vv & 1) | ((vv >> 3) & 1), fy = ((vv >> 1) & 1) | ((vv >> 3) & 1);
-  *v = vv;
-  *as_open = 48 + fx + 2 * fy + 4 * b;  // as_open_id with [C_xi + C_yi < s_i] = b
-}
-
-// One-table count: a triangle (x, y, i) in time order also takes back its side (e) pair of the
-// middle edge y (counted by k_twedge): bin Y' | X' << 1 | I' << 2 | 8 | 16 with Y' = [x ⊄ y]
-// (Venn bit 0 or 5), I' = [i ⊄ y] (bit 2 or 5) and X' = [|y| > C_xy + C_yi], which is Venn bit 1
-// when x ∩ y ∩ i is empty and one more bit b2 otherwise. Slot = 2 tpair + b2 for x ∩ y ∩ i
-// non-empty (0..255), 256 + Venn bits 0-2 without (256..263).
+// Slots of closed triangles (x, y, i) in time order. A triangle adds +1 to the bin of its Venn id
+// v (bits: 0-2 the private regions of x, y, i; 3-5 the pairwise-only regions xy, yi, xi; 6 the
+// triple region), -1 to its as-if-open bin (the pair x, y was counted as open at i) and -1 to
+// its middle edge's side (e) bin (the pair (x, i) of y, counted by k_twedge). The as-if-open bin
+// is fixed by v and b = [C_xi + C_yi < s_i] (x ⊄ i is Venn bit 0 or 3, y ⊄ i bit 1 or 3), the
+// side (e) bin Y' | X' << 1 | I' << 2 | 8 | 16 by v and b2 = X' = [|y| > C_xy + C_yi]
+// (Y' = [x ⊄ y] is bit 0 or 5, I' = [i ⊄ y] bit 2 or 5). Without a triple region b is bit 2 and
+// b2 bit 1, and every pairwise-only region is non-empty. So one counter per (v, b, b2) holds all
+// three: slot = 2 ((low << 1 | b) + 16 mid) + b2 with x ∩ y ∩ i non-empty (0..255), 256 + low
+// without (256..263), low = bits 0-2 and mid = bits 3-5.
 constexpr int kTSlots = 264;
 
-// Venn id, as-if-open id and side (e) id of the middle edge, of one-table slot p.
+// Venn id, as-if-open id and side (e) id of the middle edge, of slot p.
 __device__ inline void tslot_bins(int p, int* v, int* as_open, int* side_e) {
   int vv, b, b2;
   if (p < 256) {
@@ -424,7 +414,6 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
   // of their earliest edge, and each takes its pair back through its slot (kTSlots).
   uint32_t open_cnt[2][4] = {};  // [I][X + 2Y], side (b)
   constexpr int kTLoads = 8;      // forward-row loads in flight per lane
-  constexpr int kTBackLoads = 8;  // backward-row loads in flight per lane
   for (;;) {
     uint32_t xi = 0;
     if (lane == 0) xi = atomicAdd(&s_next, 1u);
@@ -777,8 +766,9 @@ __global__ __launch_bounds__(kBlock) void k_tproj_fill_tiled(
 // Counting over position tiles of the window [lo_i, i). The anchor's neighbours are its
 // backward row B(i) (positions ascending, with C_xi) and their masks bmask; the open-at-anchor
 // pairs come from them directly. For each tile a table of the anchor's masks over the tile's
-// positions answers "is y a neighbour of i" for the entries of F(x) and B(x) inside the tile;
-// per (i, x) read positions (fcur, bcur, aligned with B(i)) carry each row from tile to tile.
+// positions answers "is y a neighbour of i" for the entries of F(x) inside the tile; per (i, x)
+// read positions (fcur, aligned with B(i)) carry each row from tile to tile. The triples open at
+// x with the other edge earlier come from k_twedge, as for the one-table kernel.
 template <typename MT, int NW>
 __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
     int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
@@ -787,12 +777,12 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
     const unsigned long long* __restrict__ foff, const unsigned long long* __restrict__ fdeg,
     const uint32_t* __restrict__ frows, const unsigned long long* __restrict__ boff,
     const unsigned long long* __restrict__ bdeg, const uint32_t* __restrict__ brows,
-    const MT* __restrict__ bmask, uint32_t* __restrict__ fcur, uint32_t* __restrict__ bcur,
+    const MT* __restrict__ bmask, uint32_t* __restrict__ fcur,
     const int32_t* __restrict__ order, unsigned long long* __restrict__ acc) {
-  constexpr int kC = 4;
+  constexpr int kC = 2;
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
-  __shared__ unsigned int hist[NW][kTPairSlots * kC];
+  __shared__ unsigned int hist[NW][kTSlots * kC];
   __shared__ unsigned long long bsum[kTemporalBins];
   __shared__ int8_t tbl[128];
   __shared__ TileRanges tr;
@@ -800,7 +790,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
   const int32_t base = lo[i];
   const uint32_t nN = (uint32_t)bdeg[i];
   if (i - base < 2 || nN == 0) return;
-  for (int k = threadIdx.x; k < NW * kTPairSlots * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < NW * kTSlots * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
   for (int k = threadIdx.x; k < kTemporalBins; k += blockDim.x) bsum[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
   for (int32_t k = threadIdx.x; k < T; k += blockDim.x) M[k] = (MT)0;
@@ -809,7 +799,6 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
   const uint32_t* NL = brows + boff[i];
   const MT* NM = bmask + boff[i];
   uint32_t* FC = fcur + boff[i];
-  uint32_t* BC = bcur + boff[i];
   const int lane = lane_id(), w = wave_id();
   __syncthreads();  // tbl before the open pairs
 
@@ -828,8 +817,9 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
 
   unsigned int* hw = hist[w];
   const int cp = lane & (kC - 1);
-  // triples open at x, per lane by (side, I, X + 2Y) as in k_tcount
-  uint32_t open_cnt[2][2][4] = {};
+  // triples open at x with the other edge later, per lane by (I, X + 2Y) as in k_tcount (those
+  // with the other edge earlier: k_twedge, less the triangles taken back through their slots)
+  uint32_t open_cnt[2][4] = {};
   for (int32_t t0 = base; t0 < i; t0 += T) {
     const int32_t t1 = min(i, t0 + T);
     const uint32_t total = tile_ranges(tr, si, enodes + s, voff, vpos, t0, t1);
@@ -842,7 +832,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
       const MT mx = NM[xi];
       const int sx = esize[x];
       const int xfree = sx - cxi, ifree = si - cxi;
-      unsigned long long fields_b = 0, fields_e = 0;
+      unsigned long long fields_b = 0;
       // (b) forward row of x inside [max(x + 1, t0), t1): closed triangles and triples open at x
       // (the read position starts at 0 in the first tile, also for an x beyond that tile)
       uint32_t fpos = (t0 == base) ? 0u : FC[xi];
@@ -869,7 +859,8 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
               const unsigned low = nz(xfree - cxy + g) | nz(sy - cxy - cyi + g) << 1 | nz(ifree - cyi + g) << 2;
               const unsigned mid = nz(cxy - g) | nz(cyi - g) << 1 | nz(cxi - g) << 2;
               const unsigned b = (unsigned)max(min(ifree - cyi, 1), 0);
-              const unsigned slot = (low << 1 | b) + (mid << 4) + ((1u - nz(g)) << 4);
+              const unsigned b2 = (unsigned)max(min(sy - cyi - cxy, 1), 0);
+              const unsigned slot = g ? (((low << 1 | b) + (mid << 4)) << 1 | b2) : (256u | low);
               atomicAdd(&hw[slot * kC + cp], 1u);
             } else {
               fields_b += 1ull << (nz(xfree - cxy) << 4 | nz(sy - cxy) << 5);
@@ -881,63 +872,19 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
         }
       }
       if (lane == 0) FC[xi] = fpos;
-      // (e) backward row of x inside [max(base, t0), min(x, t1)): triples (y, x, i) open at x
-      if (t0 < x) {
-        const uint32_t* row = brows + boff[x];
-        const uint32_t n = (uint32_t)bdeg[x];
-        uint32_t pos;
-        if (t0 == base) {  // first entry inside the window
-          uint32_t l = 0, h = n;
-          while (l < h) {
-            const uint32_t mid = (l + h) >> 1;
-            if ((int32_t)(row[mid] >> kCntBits) < base) l = mid + 1;
-            else h = mid;
-          }
-          pos = l;
-        } else {
-          pos = BC[xi];
-        }
-        const int32_t stop = min(x, t1);
-        for (;;) {
-          const uint32_t j = pos + lane;
-          bool v = false;
-          uint32_t e = 0;
-          if (j < n) {
-            e = row[j];
-            v = (int32_t)(e >> kCntBits) < stop;
-          }
-          if (v) {
-            const int32_t y = (int32_t)(e >> kCntBits);
-            if (M[y - t0] == (MT)0) {
-              const int cxy = (int)(e & kCntMask);
-              fields_e += 1ull << (nz(xfree - cxy) << 4 | nz((int)esize[y] - cxy) << 5);
-            }
-          }
-          const uint32_t nv = (uint32_t)__popcll(__ballot(v));
-          pos += nv;
-          if (nv < (uint32_t)kWave) break;
-        }
-        if (lane == 0) BC[xi] = pos;
-      }
       // (I is uniform in the wave; both branches index the counters with constants)
       if (si > cxi) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          open_cnt[0][1][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
-          open_cnt[1][1][k] += (uint32_t)(fields_e >> (16 * k)) & 0xFFFFu;
-        }
+        for (int k = 0; k < 4; ++k) open_cnt[1][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
       } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          open_cnt[0][0][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
-          open_cnt[1][0][k] += (uint32_t)(fields_e >> (16 * k)) & 0xFFFFu;
-        }
+        for (int k = 0; k < 4; ++k) open_cnt[0][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
       }
     }
     __syncthreads();
     // this tile's triangle slots into their closed (+) and as-if-open (-) bins (per tile, so a
     // slot counter stays far below 2^32)
-    for (int p = threadIdx.x; p < kTPairSlots; p += blockDim.x) {
+    for (int p = threadIdx.x; p < kTSlots; p += blockDim.x) {
       unsigned long long sum = 0;
       for (int q = 0; q < NW; ++q)
         for (int c = 0; c < kC; ++c) {
@@ -945,25 +892,24 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
           hist[q][p * kC + c] = 0;
         }
       if (sum) {
-        int v, as_open;
-        tpair_bins(p, &v, &as_open);
+        int v, as_open, side_e;
+        tslot_bins(p, &v, &as_open, &side_e);
+        const unsigned long long neg = (unsigned long long)(-(long long)sum);
         atomicAdd(&bsum[tbl[v]], sum);
-        atomicAdd(&bsum[tbl[as_open]], (unsigned long long)(-(long long)sum));
+        atomicAdd(&bsum[tbl[as_open]], neg);
+        atomicAdd(&bsum[tbl[side_e]], neg);
       }
     }
     for_tile_entries(tr, si, total, vpos, [&](int, int32_t y) { M[y - t0] = (MT)0; });
   }
 #pragma unroll
-  for (int side = 0; side < 2; ++side)
+  for (int I = 0; I < 2; ++I)
 #pragma unroll
-    for (int I = 0; I < 2; ++I)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t c = wave_sum(open_cnt[side][I][k]);
-        const int X = k & 1, Y = k >> 1;
-        const int id = side == 0 ? (X | Y << 1 | I << 2 | 8 | 32) : (Y | X << 1 | I << 2 | 8 | 16);
-        if (lane == 0 && c) atomicAdd(&bsum[tbl[id]], (unsigned long long)c);
-      }
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t c = wave_sum(open_cnt[I][k]);
+      const int X = k & 1, Y = k >> 1;
+      if (lane == 0 && c) atomicAdd(&bsum[tbl[X | Y << 1 | I << 2 | 8 | 32]], (unsigned long long)c);
+    }
   __syncthreads();
   for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x)
     if (bsum[b]) atomicAdd(&acc[kTAccHist + b], bsum[b]);
@@ -1034,7 +980,7 @@ struct hm_tgraph {
   hm::DevBuf eoff, enodes, esize, voff, vpos, times;
   // scratch
   hm::DevBuf lo, hi, fdeg, bdeg, foff, boff, frows, brows, work, work2, ids, order, acc, tmp;
-  hm::DevBuf bmask, fcur, bcur;  // tiled windows: anchor masks and per-(anchor, neighbour) read positions
+  hm::DevBuf bmask, fcur;  // tiled windows: anchor masks and per-(anchor, neighbour) read positions
   int32_t tile = 0;              // 0 = automatic, > 0 = forced position tiles
   int32_t last_tile = 0;         // tile of the last count (0 = one table per window)
   std::vector<int32_t> order_to_input;  // position -> input index
@@ -1054,10 +1000,10 @@ static size_t win_lds_bytes(int32_t R, bool wide) {
 // histograms 7.04 s, 8 waves aggregated 3.74 s, 8 waves atomics 2.46 s, 16 waves atomics 2.12 s.
 constexpr int kTCountWaves = 16;
 constexpr int kWedgeWaves = 4;  // k_twedge: one wave per middle edge
-// Static LDS of the tiled kernels (tile walk ranges, reductions; for counting also the 36.9 KB
+// Static LDS of the tiled kernels (tile walk ranges, reductions; for counting also the 33.8 KB
 // triangle-slot counters, the bin sums and the bin table), kept free next to the position table.
 constexpr size_t kTProjTiledStaticLds = 4 * 1024;
-constexpr size_t kTCountTiledStaticLds = 40 * 1024;
+constexpr size_t kTCountTiledStaticLds = 36 * 1024;
 
 template <typename MT>
 static hm_status launch_tcount(hm_tgraph* g, int32_t Wmax, size_t lds, unsigned long long* acc) {
@@ -1165,7 +1111,6 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
     const size_t nb = (size_t)std::max<unsigned long long>(tot[1], 1);
     HM_TRY(g->bmask.ensure(sizeof(MT) * nb));
     HM_TRY(g->fcur.ensure(sizeof(uint32_t) * nb));
-    HM_TRY(g->bcur.ensure(sizeof(uint32_t) * nb));
     k_tproj_fill_tiled<MT><<<E, kBlock, sizeof(MT) * (size_t)Tp, st>>>(
         Tp, g->eoff.as<uint32_t>(), g->enodes.as<int32_t>(), g->voff.as<uint32_t>(),
         g->vpos.as<int32_t>(), g->lo.as<int32_t>(), g->hi.as<int32_t>(),
@@ -1199,14 +1144,14 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
         g->voff.as<uint32_t>(), g->vpos.as<int32_t>(), g->lo.as<int32_t>(),
         g->foff.as<unsigned long long>(), g->fdeg.as<unsigned long long>(), g->frows.as<uint32_t>(),
         g->boff.as<unsigned long long>(), g->bdeg.as<unsigned long long>(), g->brows.as<uint32_t>(),
-        g->bmask.as<MT>(), g->fcur.as<uint32_t>(), g->bcur.as<uint32_t>(), g->order.as<int32_t>(), acc);
+        g->bmask.as<MT>(), g->fcur.as<uint32_t>(), g->order.as<int32_t>(), acc);
     HM_HIP(hipGetLastError());
   } else {
     HM_TRY(launch_tcount<MT>(g, Wmax, lds_cnt, acc));
   }
   g->last_tile = tiled ? Tc : 0;
   HM_HIP(hipEventRecord(g->ev[2], st));
-  if (!tiled) {  // side (e) pairs of the one-table count (the tiled count scans backward rows)
+  {  // side (e) pairs of the triples open at a middle edge (both kernels take the triangles back)
     k_twedge<kWedgeWaves><<<(unsigned)((E + kWedgeWaves - 1) / kWedgeWaves), kWedgeWaves * kWave, 0, st>>>(
         E, g->lo.as<int32_t>(), g->esize.as<uint8_t>(), g->foff.as<unsigned long long>(),
         g->fdeg.as<unsigned long long>(), g->frows.as<uint32_t>(), g->boff.as<unsigned long long>(),
@@ -1314,7 +1259,7 @@ hm_status hm_tgraph_destroy(hm_tgraph* g) {
   for (hm::DevBuf* b : {&g->eoff, &g->enodes, &g->esize, &g->voff, &g->vpos, &g->times, &g->lo,
                         &g->hi, &g->fdeg, &g->bdeg, &g->foff, &g->boff, &g->frows, &g->brows,
                         &g->work, &g->work2, &g->ids, &g->order, &g->acc, &g->tmp, &g->bmask,
-                        &g->fcur, &g->bcur})
+                        &g->fcur})
     b->release();
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);
