@@ -1274,7 +1274,7 @@ hm_status hm_count_temporal(hm_tgraph* g, int64_t delta, uint64_t out[HM_TEMPORA
   if (delta < 0 || delta > ((int64_t)1 << 62))
     return hm::fail(HM_ERR_BAD_INPUT, "hm_count_temporal: delta must lie in [0, 2^62]");
   unsigned long long acc[hm::kTAccCount] = {0};
-  g->ms_project = g->ms_count = 0.f;
+  g->ms_project = g->ms_count = g->ms_wedge = 0.f;
   if (g->E >= 3) {
     if (hipSetDevice(g->device) != hipSuccess) return hm::fail(HM_ERR_DEVICE, "hipSetDevice failed");
     const hm_status st = g->max_size <= 32 ? hm::count_temporal_impl<uint32_t>(g, delta, acc)
