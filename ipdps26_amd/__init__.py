@@ -450,6 +450,8 @@ class TemporalGraph:
         _check(lib().hm_tgraph_set_tile(self._h, int(tile_positions)))
 
     def phase_ms(self, phase: str) -> float:
+        """Milliseconds of the last count's phase: "project" (window bounds and temporal
+        projection rows), "count" (the counting kernel) or "wedge" (k_twedge)."""
         ms = ctypes.c_double(0.0)
         _check(lib().hm_tgraph_phase_ms(self._h, phase.encode(), ctypes.byref(ms)))
         return ms.value
