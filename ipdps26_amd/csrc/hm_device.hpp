@@ -59,6 +59,9 @@ __device__ inline void wave_hist_add(CT* h, int bin, bool valid, CT delta) {
 
 // Venn-region emptiness id of (x1, x2, x3) (MoCHy-master/motif_id.cpp:24-34):
 // bit0..2 = x1/x2/x3-only regions, bit3 = x1x2-only, bit4 = x2x3-only, bit5 = x3x1-only, bit6 = all.
+// [r > 0] of a region count r >= 0 (one unsigned min)
+__device__ inline unsigned nz(int r) { return min((unsigned)r, 1u); }
+
 __device__ inline int venn_id(int s1, int s2, int s3, int c12, int c23, int c31, int g) {
   return (int)(s1 - c12 - c31 + g > 0) | ((int)(s2 - c12 - c23 + g > 0) << 1) |
          ((int)(s3 - c31 - c23 + g > 0) << 2) | ((int)(c12 > g) << 3) | ((int)(c23 > g) << 4) |

@@ -140,7 +140,26 @@ constexpr int kDeltaLoadsWide = 16;
 constexpr int kDeltaLoadsNarrow = 2;
 constexpr size_t kDeltaOneGroupLds = 80 * 1024;  // above this, one workgroup per CU (160 KB LDS)
 constexpr int kDeltaTiledLoads = 4;  // row chunks per step of the tiled delta
+// Slots of a closed pair (b, c) of a changed hyperedge f (Venn id v over f, b, c in that order:
+// bits 0-2 private regions, 3-5 pairwise-only regions fb, bc, fc, 6 the triple region). Its
+// as-if-open bin at f, 20 + A + 2 ([b ⊄ f] + [c ⊄ f]), is fixed by v ([b ⊄ f] is bit 1 or 4,
+// [c ⊄ f] bit 2 or 4) and A = [|f| > C_fb + C_fc], which is bit 0 without a triple region.
+// Slot = (low << 1 | A) + 16 mid + 16 [no triple region], low = bits 0-2, mid = bits 3-5:
+// 0..127 with a triple region, 128..143 without (mid = 7).
+constexpr int kDSlots = 144;
+constexpr int kDSlotCopies = 2;  // interleaved copies per wave (copy = lane & 1)
+
+// Venn id and as-if-open bin (at f) of slot p.
+__device__ inline void dslot_bins(int p, int* v, int* as_open) {
+  const int q = p < 128 ? p : p - 128;
+  const int a = q & 1, low = (q >> 1) & 7;
+  const int vv = p < 128 ? (low | (p >> 4) << 3 | 64) : (low | 56);
+  const int bf = ((vv >> 1) & 1) | ((vv >> 4) & 1), cf = ((vv >> 2) & 1) | ((vv >> 4) & 1);
+  *v = vv;
+  *as_open = kOpenBin0 + a + 2 * (bf + cf);
+}
 constexpr size_t kDeltaTiledStaticLds = 20 * 1024;  // k_delta_tiled: histogram, classes, tables
+constexpr size_t kDeltaStaticLds = 24 * 1024;       // k_delta: slot counters, bins, classes, tables
 
 template <typename MT, int NW, int kDeltaLoads>
 __device__ inline void delta_body(
@@ -152,14 +171,16 @@ __device__ inline void delta_body(
     unsigned long long* __restrict__ acc) {
   // size of x (at most 64) in bits 0..6, [x in S] in bit 7: one LDS read per row entry
   uint8_t* szf = reinterpret_cast<uint8_t*>(M + E);
-  __shared__ int32_t h[NW][32][kDeltaCopies];
+  __shared__ int32_t h[NW][32];                       // open bins, then every bin
+  __shared__ uint32_t hs[NW][kDSlots * kDSlotCopies];  // closed pairs by slot
   __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
   __shared__ unsigned long long red[6][NW];
   __shared__ int8_t tbl[128];
   __shared__ uint32_t s_next;
   __shared__ unsigned long long s_scan;
   if (eoff[f + 1] == eoff[f]) return;
-  for (int k = threadIdx.x; k < NW * 32 * kDeltaCopies; k += blockDim.x) (&h[0][0][0])[k] = 0;
+  for (int k = threadIdx.x; k < NW * 32; k += blockDim.x) (&h[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < NW * kDSlots * kDSlotCopies; k += blockDim.x) (&hs[0][0])[k] = 0u;
   for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_delta_bin[threadIdx.x];
   if (threadIdx.x == 0) {
@@ -213,14 +234,13 @@ __device__ inline void delta_body(
   // per b), X = [b has a node in neither f nor c] and Y = [c ⊄ b]; each lane counts them per b in
   // four 16-bit fields of one word (a lane meets at most d_b / 64 + 1 < 2^16 entries of a row) and
   // folds them into open_cnt[F][X + 2Y] after each b.
-  const int copy = lane & (kDeltaCopies - 1);
-  int32_t* hc = &h[w][0][copy];
+  // A closed pair (b, c) is one LDS atomic into its slot (kDSlots), which fixes both its closed
+  // bin (+1) and its as-if-open bin at f (-1).
+  uint32_t* hw = hs[w];
+  const int cp = lane & (kDSlotCopies - 1);
   const uint32_t* rowF = rows + roff[f];
   const uint32_t nF = (uint32_t)rlen[f];
   uint32_t open_cnt[2][4] = {};
-  // the -1 of a closed pair's as-if-open triple at f: bin 20 + A + 2 ([b ⊄ f] + B) with
-  // A = [f has a node in neither b nor c], B = [c ⊄ f]; counted like the open triples
-  uint32_t neg_cnt[2][4] = {};
   for (;;) {
     uint32_t k = 0;
     if (lane == 0) k = atomicAdd(&s_next, 1u);
@@ -238,7 +258,7 @@ __device__ inline void delta_body(
     const uint32_t* rowB = rows + roff[b];
     const uint32_t nB = (uint32_t)rlen[b];
     if (lane == 0) atomicAdd(&s_scan, (unsigned long long)nB);
-    unsigned long long fields = 0, negs = 0;
+    unsigned long long fields = 0;
     auto visit = [&](uint32_t e) {
       const int32_t c = (int32_t)(e >> kCntBits);
       const int cbc = (int)(e & kCntMask);
@@ -247,13 +267,15 @@ __device__ inline void delta_body(
       const MT mc = M[c];
       const int sc = zc & 0x7F;
       const int cfc = popc(mc);
-      const bool closed = mc && c > b;
-      if (closed)
-        atomicAdd(&hc[tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))] * kDeltaCopies], 1);
-      // both words take a (possibly zero) increment: no branch selects between them
-      const unsigned long long one_n = 1ull << (16 * ((cfc < ffree ? 1 : 0) + (sc > cfc ? 2 : 0)));
+      if (mc && c > b) {
+        // Venn regions of (f, b, c) from the overlaps; every count below is >= 0
+        const int g = popc((MT)(mb & mc));
+        const unsigned low = nz(ffree - cfc + g) | nz(bfree - cbc + g) << 1 | nz(sc - cfc - cbc + g) << 2;
+        const unsigned mid = nz(cfb - g) | nz(cbc - g) << 1 | nz(cfc - g) << 2;
+        const unsigned a = (unsigned)max(min(ffree - cfc, 1), 0);
+        atomicAdd(&hw[((low << 1 | a) + (mid << 4) + ((1u - nz(g)) << 4)) * kDSlotCopies + cp], 1u);
+      }
       const unsigned long long one_o = 1ull << (16 * ((cbc < bfree ? 1 : 0) + (sc > cbc ? 2 : 0)));
-      negs += closed ? one_n : 0ull;
       fields += mc ? 0ull : one_o;
     };
     // kDeltaLoads row loads in flight per lane before any is used (lane l reads l, l + 64, ...)
@@ -274,28 +296,30 @@ __device__ inline void delta_body(
 #pragma unroll
       for (int q = 0; q < 4; ++q) open_cnt[0][q] += (uint32_t)(fields >> (16 * q)) & 0xFFFFu;
     }
-    if (sb > cfb) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) neg_cnt[1][q] += (uint32_t)(negs >> (16 * q)) & 0xFFFFu;
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) neg_cnt[0][q] += (uint32_t)(negs >> (16 * q)) & 0xFFFFu;
-    }
   }
 #pragma unroll
   for (int F = 0; F < 2; ++F)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const uint32_t c = wave_sum(open_cnt[F][q]);
-      if (lane == 0 && c) atomicAdd(&h[w][kOpenBin0 + (q & 1) + 2 * (F + (q >> 1))][0], (int32_t)c);
-      const uint32_t n = wave_sum(neg_cnt[F][q]);
-      if (lane == 0 && n) atomicSub(&h[w][kOpenBin0 + (q & 1) + 2 * (F + (q >> 1))][0], (int32_t)n);
+      if (lane == 0 && c) atomicAdd(&h[w][kOpenBin0 + (q & 1) + 2 * (F + (q >> 1))], (int32_t)c);
     }
+  __syncthreads();
+  for (int p = threadIdx.x; p < kDSlots; p += blockDim.x) {  // slots into their two bins
+    uint32_t n = 0;
+    for (int q = 0; q < NW; ++q)
+      for (int c = 0; c < kDSlotCopies; ++c) n += hs[q][p * kDSlotCopies + c];
+    if (n) {
+      int v, as_open;
+      dslot_bins(p, &v, &as_open);
+      atomicAdd(&h[0][tbl[v]], (int32_t)n);
+      atomicSub(&h[0][as_open], (int32_t)n);
+    }
+  }
   __syncthreads();
   if (threadIdx.x < kStaticBins) {
     long long s = 0;
-    for (int q = 0; q < NW; ++q)
-      for (int c = 0; c < kDeltaCopies; ++c) s += h[q][threadIdx.x][c];
+    for (int q = 0; q < NW; ++q) s += h[q][threadIdx.x];
     if (s) {
       atomicAdd(&acc[threadIdx.x], (unsigned long long)(s * sign));
       atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], (unsigned long long)s);
@@ -894,7 +918,7 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
         d.spm.as<MT>(), d.acc.as<unsigned long long>());
   } else {
     const size_t lds = (((size_t)E * sizeof(MT) + (size_t)E) + 15) & ~(size_t)15;
-    const bool one_group = lds + kDeltaTiledStaticLds > kDeltaOneGroupLds;
+    const bool one_group = lds + kDeltaStaticLds > kDeltaOneGroupLds;
     auto fn = one_group ? k_delta<MT, kDeltaWaves, kDeltaLoadsWide> : k_delta<MT, kDeltaWaves, kDeltaLoadsNarrow>;
     HM_TRY(allow_lds((const void*)fn, lds));
     fn<<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
@@ -921,8 +945,8 @@ static hm_status add_phase_ms(hm_graph* g, int evb) {
 static bool dense_fits(hm_graph* g, int32_t E, bool wide) {
   int lim = 65536;
   (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
-  // k_delta: mask and size-and-flag byte per id, plus about 18 KB of static arrays (20 KB reserved)
-  return (size_t)E * ((wide ? 8 : 4) + 1) + 16 + 20 * 1024 <= (size_t)lim;
+  // k_delta: mask and size-and-flag byte per id, plus about 22 KB of static arrays (24 KB reserved)
+  return (size_t)E * ((wide ? 8 : 4) + 1) + 16 + kDeltaStaticLds <= (size_t)lim;
 }
 
 static hm_status local_count_any(hm_graph* g, DynScratch& d, const int32_t* ids, int32_t n,
@@ -1078,7 +1102,7 @@ static hm_status grow_keep(DevBuf& b, size_t need, hipStream_t st) {
 static int32_t dense_max_ids(hm_graph* g, bool wide) {
   int lim = 65536;
   (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device);
-  const long long room = (long long)lim - 16 - 20 * 1024;
+  const long long room = (long long)lim - 16 - (long long)kDeltaStaticLds;
   return room <= 0 ? 0 : (int32_t)(room / ((wide ? 8 : 4) + 1));
 }
 
@@ -1101,7 +1125,7 @@ static size_t table_lds(int32_t E) {
 
 template <typename MT>
 static bool delta_one_group(int32_t Ecap) {
-  return delta_lds<MT>(Ecap) + kDeltaTiledStaticLds > kDeltaOneGroupLds;
+  return delta_lds<MT>(Ecap) + kDeltaStaticLds > kDeltaOneGroupLds;
 }
 
 // Enqueues the whole captured update on g->stream (under capture: nothing waits).

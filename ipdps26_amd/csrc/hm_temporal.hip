@@ -229,9 +229,6 @@ __device__ inline uint32_t row_load(__amdgpu_buffer_rsrc_t rs, uint32_t j) {
   return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(j * 4u), 0, 0);
 }
 
-// [r > 0] of a region count r >= 0, as one unsigned min
-__device__ inline unsigned nz(int r) { return min((unsigned)r, 1u); }
-
 // Slots of closed triangles (x, y, i) in time order. A triangle adds +1 to the bin of its Venn id
 // v (bits: 0-2 the private regions of x, y, i; 3-5 the pairwise-only regions xy, yi, xi; 6 the
 // triple region), -1 to its as-if-open bin (the pair x, y was counted as open at i) and -1 to
