@@ -158,7 +158,7 @@ __device__ inline void dslot_bins(int p, int* v, int* as_open) {
   *v = vv;
   *as_open = kOpenBin0 + a + 2 * (bf + cf);
 }
-constexpr size_t kDeltaTiledStaticLds = 20 * 1024;  // k_delta_tiled: histogram, classes, tables
+constexpr size_t kDeltaTiledStaticLds = 40 * 1024;  // k_delta_tiled: histograms, classes, tables
 constexpr size_t kDeltaStaticLds = 24 * 1024;       // k_delta: slot counters, bins, classes, tables
 
 template <typename MT, int NW, int kDeltaLoads>
@@ -450,6 +450,7 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
   __shared__ int32_t h[NW][32][kDeltaCopies];
+  __shared__ uint32_t hs[NW][kDSlots * kDSlotCopies];  // closed pairs by slot, as in k_delta
   __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
   __shared__ unsigned long long red[6][NW];
   __shared__ int8_t tbl[128];
@@ -463,6 +464,7 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
   const int sf = (int)(eoff[f + 1] - fs);
   if (sf == 0) return;
   for (int k = threadIdx.x; k < NW * 32 * kDeltaCopies; k += blockDim.x) (&h[0][0][0])[k] = 0;
+  for (int k = threadIdx.x; k < NW * kDSlots * kDSlotCopies; k += blockDim.x) (&hs[0][0])[k] = 0u;
   for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_delta_bin[threadIdx.x];
   if ((int)threadIdx.x < sf) s_an[threadIdx.x] = enodes[fs + threadIdx.x];
@@ -470,6 +472,8 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
   for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
   const int lane = lane_id(), w = wave_id();
   int32_t* hc = &h[w][0][lane & (kDeltaCopies - 1)];
+  uint32_t* hsw = hs[w];
+  const int cps = lane & (kDSlotCopies - 1);
   const uint32_t* rowF = rows + roff[f];
   const uint32_t nF = (uint32_t)rlen[f];
   const unsigned long long st = soff[q];
@@ -533,8 +537,12 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
               if (mc) {
                 if (c > b) {
                   const int cfc = popc(mc);
-                  atomicAdd(&hc[tbl[venn_id(sf, sb, sc, cfb, cbc, cfc, popc((MT)(mb & mc)))] * kDeltaCopies], 1);
-                  atomicSub(&hc[open_bin(sf, sb, sc, cfb, cfc) * kDeltaCopies], 1);
+                  const int g = popc((MT)(mb & mc));
+                  const unsigned low =
+                      nz(sf - cfb - cfc + g) | nz(sb - cfb - cbc + g) << 1 | nz(sc - cfc - cbc + g) << 2;
+                  const unsigned mid = nz(cfb - g) | nz(cbc - g) << 1 | nz(cfc - g) << 2;
+                  const unsigned a = (unsigned)max(min(sf - cfb - cfc, 1), 0);
+                  atomicAdd(&hsw[((low << 1 | a) + (mid << 4) + ((1u - nz(g)) << 4)) * kDSlotCopies + cps], 1u);
                 }
               } else {
                 atomicAdd(&hc[open_bin(sb, sf, sc, cfb, cbc) * kDeltaCopies], 1);
@@ -549,6 +557,21 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
       if (lane == 0) {
         cur[st + ib] = pos;
         if (pos > pos0) atomicAdd(&s_scan, (unsigned long long)(pos - pos0));
+      }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < kDSlots; p += blockDim.x) {  // this tile's slots into their bins
+      uint32_t n = 0;
+      for (int w2 = 0; w2 < NW; ++w2)
+        for (int c = 0; c < kDSlotCopies; ++c) {
+          n += hs[w2][p * kDSlotCopies + c];
+          hs[w2][p * kDSlotCopies + c] = 0u;
+        }
+      if (n) {
+        int v, as_open;
+        dslot_bins(p, &v, &as_open);
+        atomicAdd(&h[0][tbl[v]][0], (int32_t)n);
+        atomicSub(&h[0][as_open][0], (int32_t)n);
       }
     }
     __syncthreads();
