@@ -220,7 +220,7 @@ constexpr int kNumOrders = 3;
 // which measured within 1% of the best form on graphs of 1k-6k hyperedges.
 constexpr float kTuneBudgetMs = 5.f;
 constexpr unsigned long long kOrderChunkBytes[kNumOrders] = {0ull, 64ull << 20, ~0ull};
-// LDS beside the dynamic table: histogram copies (at most 8 waves x 32 bins x 16 copies x 4 B)
+// LDS beside the dynamic table: triangle slot counters (243 slots x at most 16 copies x 4 B)
 // plus the small tables.
 constexpr size_t kBatchStaticLds = 20 * 1024;
 

@@ -20,7 +20,9 @@
 //   k_proj_fill   same table indexed by rank; writes a's upper row (rank > rank(a)) as packed
 //                 (rank << 7 | C) words in ascending rank.
 //   k_tri         same table by rank; for every b in a's upper row, scans b's upper row:
-//                 M[c] != 0 <=> triangle, C_ac = popc(M[c]), g_abc = popc(M[b] & M[c]).
+//                 M[c] != 0 <=> triangle, C_ac = popc(M[c]), g_abc = popc(M[b] & M[c]); one
+//                 LDS atomic per triangle into the slot that fixes its closed bin and its three
+//                 as-if-open bins, the slots folded into the bins once per centre.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -77,10 +79,10 @@ __global__ __launch_bounds__(kBlock) void k_work(int32_t E, const unsigned long 
   if (lane_id() == 0) work[r] = s + n;
 }
 
-// Forms of the one-table triangle kernel: waves per centre workgroup x int32 histogram copies
-// per wave (copy = lane % copies). Measured on c1: 4 waves 9.03 ms, 8 waves 7.85 ms, 16 waves
-// 8.09 ms; 2 copies 26.6 ms, 4: 14.2 ms, 8: 7.75 ms, 16: 6.91 ms, 32: 8.06 ms, 64: 13.7 ms. Small
-// graphs prefer 4 waves; the form is autotuned per shape (below). The tiled kernel keeps 8 waves.
+// Forms of the one-table triangle kernel: waves per centre workgroup x copies of the 243 uint32
+// slot counters, shared by the workgroup (copy = lane % copies). Measured on the 5,000-hyperedge
+// graph: 8 waves 5.06 ms with 8 or 16 copies, 4 waves 5.92 / 5.89 ms. Small graphs prefer 4
+// waves; the form is autotuned per shape (below). The tiled kernel keeps 8 waves, 16 copies.
 constexpr int kTriWaves = 8;
 struct TriFormSpec {
   int waves, copies;
@@ -222,9 +224,9 @@ __global__ __launch_bounds__(NW * kWave) void k_tri_tiled(
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
   uint8_t* szt = reinterpret_cast<uint8_t*>(M + T);
-  constexpr int kCopies = 16;
-  __shared__ int32_t h[NW][32][kCopies];
-  __shared__ int8_t tbl[128];
+  constexpr int kCopies = 16;  // slot counter copies shared by the workgroup (copy = lane % 16)
+  __shared__ uint32_t hs[kTriSlots * kCopies];
+  __shared__ unsigned long long hb[32];
   __shared__ int32_t s_an[HM_MAX_EDGE_SIZE];
   __shared__ TileRanges tr;
   __shared__ uint32_t s_end;
@@ -235,13 +237,13 @@ __global__ __launch_bounds__(NW * kWave) void k_tri_tiled(
   const int32_t a = perm[r];
   const uint32_t s = eoff[a];
   const int sa = (int)(eoff[a + 1] - s);
-  for (int k = threadIdx.x; k < NW * 32 * kCopies; k += blockDim.x) (&h[0][0][0])[k] = 0;
-  if (threadIdx.x < 128) tbl[threadIdx.x] = c_static_bin[threadIdx.x];
+  for (int k = threadIdx.x; k < kTriSlots * kCopies; k += blockDim.x) hs[k] = 0u;
+  if (threadIdx.x < 32) hb[threadIdx.x] = 0ull;
   if ((int)threadIdx.x < sa) s_an[threadIdx.x] = enodes[s + threadIdx.x];
   for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
 
   const int lane = lane_id(), w = wave_id();
-  const int copy = lane & (kCopies - 1);
+  uint32_t* hc = hs + (lane & (kCopies - 1));
   const uint32_t* rowA = rows + roff[r];
   const unsigned long long st = uoff[r];
   uint32_t prev_end = 0;
@@ -265,7 +267,6 @@ __global__ __launch_bounds__(NW * kWave) void k_tri_tiled(
     });
     __syncthreads();
     const uint32_t end = s_end;
-    int32_t* hc = &h[w][0][copy];
     for (uint32_t base = (uint32_t)w * kWave; base < end; base += NW * kWave) {
       const uint32_t ib = base + lane;
       const bool have = ib < end;
@@ -324,10 +325,13 @@ __global__ __launch_bounds__(NW * kWave) void k_tri_tiled(
                 const int cac = popc(mc);
                 const int g = popc((MT)(mabL & mc));
                 const int sc = szt[rc - t0];
-                atomicAdd(&hc[tbl[venn_id(sa, sbL, sc, cabL, cbc, cac, g)] * kCopies], 1);
-                atomicSub(&hc[open_bin(sa, sbL, sc, cabL, cac) * kCopies], 1);
-                atomicSub(&hc[open_bin(sbL, sa, sc, cabL, cbc) * kCopies], 1);
-                atomicSub(&hc[open_bin(sc, sa, sbL, cac, cbc) * kCopies], 1);
+                const int ao = sa - cabL - cac + g, bo = sbL - cabL - cbc + g, co = sc - cac - cbc + g;
+                const unsigned pairs =
+                    (unsigned)(cabL > g) | (unsigned)(cbc > g) << 1 | (unsigned)(cac > g) << 2;
+                const unsigned t = (unsigned)((ao > 0) + (ao > g)) +
+                                   3u * (unsigned)((bo > 0) + (bo > g)) +
+                                   9u * (unsigned)((co > 0) + (co > g));
+                atomicAdd(&hc[((g ? pairs : 8u) * 27u + t) * kCopies], 1u);
               }
             }
             nv += (uint32_t)__popcll(__ballot(v[u]));
@@ -344,14 +348,26 @@ __global__ __launch_bounds__(NW * kWave) void k_tri_tiled(
     }
     if (threadIdx.x == 0 && end < nA) atomicMin(&s_next, (int32_t)(rowA[end] >> kCntBits) / T);
     __syncthreads();
-    if (threadIdx.x < kStaticBins) {  // flush this tile's histogram (bounds the int32 counters)
-      long long sum = 0;
-      for (int q = 0; q < NW; ++q)
-        for (int c = 0; c < kCopies; ++c) {
-          sum += h[q][threadIdx.x][c];
-          h[q][threadIdx.x][c] = 0;
-        }
-      if (sum) atomicAdd(&acc[kAccHist + threadIdx.x], (unsigned long long)sum);
+    // fold this tile's slots into the bins and flush them (bounds the uint32 counters)
+    for (int p = threadIdx.x; p < kTriSlots; p += blockDim.x) {
+      unsigned long long n = 0;
+      for (int c = 0; c < kCopies; ++c) {
+        n += hs[p * kCopies + c];
+        hs[p * kCopies + c] = 0u;
+      }
+      if (n) {
+        int v, oa, ob, oc;
+        tri_slot_bins(p, &v, &oa, &ob, &oc);
+        atomicAdd(&hb[c_static_bin[v]], n);
+        atomicAdd(&hb[oa], 0ull - n);
+        atomicAdd(&hb[ob], 0ull - n);
+        atomicAdd(&hb[oc], 0ull - n);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < kStaticBins && hb[threadIdx.x]) {
+      atomicAdd(&acc[kAccHist + threadIdx.x], hb[threadIdx.x]);
+      hb[threadIdx.x] = 0ull;
     }
     for_tile_entries(tr, sa, total, vrank, [&](int, int32_t x) { M[x - t0] = (MT)0; });
     const int32_t kn = s_next;

@@ -156,8 +156,32 @@ __device__ inline void proj_fill_item(MT* M, int32_t E, int32_t a, const uint32_
 
 constexpr int kTriLoads = 4;
 
-// Centre rank r (NW waves, each with NC interleaved int32 histogram copies, copy = lane % NC, for
-// its LDS atomics): every triangle r < rb < rc with rb in r's upper row and rc in rb's upper row.
+// Slots of a closed triple (a, b, c): its bin and the three bins it has as if open at a, at b and
+// at c follow from the Venn id and, per private region x_only (x_only = s_x - C_xy - C_xz + g),
+// the bit [x_only > g] = [s_x > C_xy + C_xz] that decides whether x keeps a private node once the
+// triple region is counted as if it were shared by y and z only. slot = pairs * 27 + t_a + 3 t_b
+// + 9 t_c with t_x = [x_only > 0] + [x_only > g] and pairs = [C_ab > g] | [C_bc > g] << 1 |
+// [C_ca > g] << 2 (slot group 8: g = 0, every pair region non-empty, t_x in {0, 2}).
+constexpr int kTriSlots = 243;
+
+// Closed bin's Venn id and the three as-if-open bins (centres a, b, c) of slot p.
+__device__ inline void tri_slot_bins(int p, int* v, int* oa, int* ob, int* oc) {
+  const int grp = p / 27, t = p - grp * 27;
+  const int ta = t % 3, tb = (t / 3) % 3, tc = t / 9;
+  const int pairs = grp < 8 ? grp : 7;
+  const int b0 = ta > 0, b1 = tb > 0, b2 = tc > 0;
+  const int ab = pairs & 1, bc = (pairs >> 1) & 1, ca = (pairs >> 2) & 1;
+  *v = b0 | b1 << 1 | b2 << 2 | pairs << 3 | (grp < 8 ? 64 : 0);
+  // open_bin(s_x, s_y, s_z, C_xy, C_xz) = kOpenBin0 + [x_only > g] + 2 ([y ⊄ x] + [z ⊄ x]), with
+  // [b ⊄ a] = [b_only + bc_only > 0] and likewise for the other five containments
+  *oa = kOpenBin0 + (ta == 2) + 2 * ((b1 | bc) + (b2 | bc));
+  *ob = kOpenBin0 + (tb == 2) + 2 * ((b0 | ca) + (b2 | ca));
+  *oc = kOpenBin0 + (tc == 2) + 2 * ((b0 | ab) + (b1 | ab));
+}
+
+// Centre rank r (NW waves; NC copies of the slot counters shared by the workgroup, copy = lane %
+// NC): every triangle r < rb < rc with rb in r's upper row and rc in rb's upper row, one LDS
+// atomic each into its slot; the slots are folded into the 30 bins once per centre.
 // M (mask table by rank) spans E ranks; the E size bytes follow it in the same dynamic LDS.
 template <typename MT, int NW, int NC>
 __device__ inline void tri_item(MT* M, int32_t E, int32_t r, const uint32_t* __restrict__ eoff,
@@ -169,19 +193,19 @@ __device__ inline void tri_item(MT* M, int32_t E, int32_t r, const uint32_t* __r
                                 const unsigned long long* __restrict__ roff,
                                 const uint32_t* __restrict__ rlen, const uint32_t* __restrict__ rows,
                                 unsigned long long* __restrict__ acc) {
-  __shared__ int32_t h[NW][32][NC];
-  __shared__ int8_t tbl[128];
+  __shared__ uint32_t hs[kTriSlots * NC];
+  __shared__ unsigned long long hb[32];
   uint8_t* sz = reinterpret_cast<uint8_t*>(M + E);
   const uint32_t nA = rlen[r];
   if (nA < 2) return;  // no triangle has r as its lowest-ranked member
   const int32_t a = perm[r];
-  for (int k = threadIdx.x; k < NW * 32 * NC; k += blockDim.x) (&h[0][0][0])[k] = 0;
-  if (threadIdx.x < 128) tbl[threadIdx.x] = c_static_bin[threadIdx.x];
+  for (int k = threadIdx.x; k < kTriSlots * NC; k += blockDim.x) hs[k] = 0u;
+  if (threadIdx.x < 32) hb[threadIdx.x] = 0ull;
   for (int x = threadIdx.x; x < E; x += blockDim.x) sz[x] = szr[x];
   build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, rank);
 
   const int lane = lane_id(), w = wave_id();
-  const int copy = lane & (NC - 1);
+  uint32_t* hc = hs + (lane & (NC - 1));
   const int sa = sz[r];
   const uint32_t* rowA = rows + roff[r];
   for (uint32_t ib = w; ib < nA; ib += NW) {
@@ -192,7 +216,6 @@ __device__ inline void tri_item(MT* M, int32_t E, int32_t r, const uint32_t* __r
     const int sb = sz[rb];
     const uint32_t* rowB = rows + roff[rb];
     const uint32_t nB = rlen[rb];
-    int32_t* hc = &h[w][0][copy];
     auto visit = [&](uint32_t ec) {
       const int32_t rc = (int32_t)(ec >> kCntBits);
       const MT mc = M[rc];
@@ -201,11 +224,11 @@ __device__ inline void tri_item(MT* M, int32_t E, int32_t r, const uint32_t* __r
         const int cac = popc(mc);
         const int g = popc((MT)(mab & mc));
         const int sc = sz[rc];
-        const int closed = tbl[venn_id(sa, sb, sc, cab, cbc, cac, g)];
-        atomicAdd(&hc[closed * NC], 1);
-        atomicSub(&hc[open_bin(sa, sb, sc, cab, cac) * NC], 1);
-        atomicSub(&hc[open_bin(sb, sa, sc, cab, cbc) * NC], 1);
-        atomicSub(&hc[open_bin(sc, sa, sb, cac, cbc) * NC], 1);
+        const int ao = sa - cab - cac + g, bo = sb - cab - cbc + g, co = sc - cac - cbc + g;
+        const unsigned pairs = (unsigned)(cab > g) | (unsigned)(cbc > g) << 1 | (unsigned)(cac > g) << 2;
+        const unsigned t = (unsigned)((ao > 0) + (ao > g)) + 3u * (unsigned)((bo > 0) + (bo > g)) +
+                           9u * (unsigned)((co > 0) + (co > g));
+        atomicAdd(&hc[((g ? pairs : 8u) * 27u + t) * NC], 1u);
       }
     };
     // kTriLoads row loads in flight per lane before any is used (lane l reads l, l + 64, ...)
@@ -220,12 +243,20 @@ __device__ inline void tri_item(MT* M, int32_t E, int32_t r, const uint32_t* __r
     for (; j < nB; j += kWave) visit(rowB[j]);
   }
   __syncthreads();
-  if (threadIdx.x < kStaticBins) {
-    long long s = 0;
-    for (int q = 0; q < NW; ++q)
-      for (int c = 0; c < NC; ++c) s += h[q][threadIdx.x][c];
-    if (s) atomicAdd(&acc[kAccHist + threadIdx.x], (unsigned long long)s);
+  for (int p = threadIdx.x; p < kTriSlots; p += blockDim.x) {
+    unsigned long long n = 0;
+    for (int c = 0; c < NC; ++c) n += hs[p * NC + c];
+    if (n) {
+      int v, oa, ob, oc;
+      tri_slot_bins(p, &v, &oa, &ob, &oc);
+      atomicAdd(&hb[c_static_bin[v]], n);
+      atomicAdd(&hb[oa], 0ull - n);
+      atomicAdd(&hb[ob], 0ull - n);
+      atomicAdd(&hb[oc], 0ull - n);
+    }
   }
+  __syncthreads();
+  if (threadIdx.x < kStaticBins && hb[threadIdx.x]) atomicAdd(&acc[kAccHist + threadIdx.x], hb[threadIdx.x]);
 }
 
 }  // namespace hm
