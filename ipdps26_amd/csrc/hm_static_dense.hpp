@@ -193,6 +193,9 @@ __device__ inline void tri_item(MT* M, int32_t E, int32_t r, const uint32_t* __r
                                 const unsigned long long* __restrict__ roff,
                                 const uint32_t* __restrict__ rlen, const uint32_t* __restrict__ rows,
                                 unsigned long long* __restrict__ acc) {
+  // the hosts reserve 20 KB of LDS beside the dynamic table for these counters
+  static_assert(kTriSlots * NC * sizeof(uint32_t) + 32 * sizeof(unsigned long long) <= 16 * 1024,
+                "slot counters outgrow the static LDS reservation");
   __shared__ uint32_t hs[kTriSlots * NC];
   __shared__ unsigned long long hb[32];
   uint8_t* sz = reinterpret_cast<uint8_t*>(M + E);
