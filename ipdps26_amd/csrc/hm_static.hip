@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void k_work(int32_t E, const unsigned long 
 
 // Forms of the one-table triangle kernel: waves per centre workgroup x copies of the 243 uint32
 // slot counters, shared by the workgroup (copy = lane % copies). Measured on the 5,000-hyperedge
-// graph: 8 waves 5.06 ms with 8 or 16 copies, 4 waves 5.92 / 5.89 ms. Small graphs prefer 4
+// graph: 8 waves 4.45 / 4.46 ms with 8 / 16 copies, 4 waves 5.41 / 5.38 ms. Small graphs prefer 4
 // waves; the form is autotuned per shape (below). The tiled kernel keeps 8 waves, 16 copies.
 constexpr int kTriWaves = 8;
 struct TriFormSpec {
@@ -325,13 +325,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tri_tiled(
                 const int cac = popc(mc);
                 const int g = popc((MT)(mabL & mc));
                 const int sc = szt[rc - t0];
-                const int ao = sa - cabL - cac + g, bo = sbL - cabL - cbc + g, co = sc - cac - cbc + g;
-                const unsigned pairs =
-                    (unsigned)(cabL > g) | (unsigned)(cbc > g) << 1 | (unsigned)(cac > g) << 2;
-                const unsigned t = (unsigned)((ao > 0) + (ao > g)) +
-                                   3u * (unsigned)((bo > 0) + (bo > g)) +
-                                   9u * (unsigned)((co > 0) + (co > g));
-                atomicAdd(&hc[((g ? pairs : 8u) * 27u + t) * kCopies], 1u);
+                atomicAdd(&hc[tri_slot(cabL - sa, cabL - sbL, cabL, sc, cbc, cac, g) * kCopies], 1u);
               }
             }
             nv += (uint32_t)__popcll(__ballot(v[u]));

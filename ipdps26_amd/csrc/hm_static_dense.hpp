@@ -179,6 +179,22 @@ __device__ inline void tri_slot_bins(int p, int* v, int* oa, int* ob, int* oc) {
   *oc = kOpenBin0 + (tc == 2) + 2 * ((b0 | ab) + (b1 | ab));
 }
 
+// Slot of a closed triple, from sign bits ([x < 0] = (unsigned)x >> 31, all terms small) rather
+// than compare-and-select chains: ka = C_ab - s_a and kb = C_ab - s_b are fixed per b, so
+//   [a_only > 0] = [ka + C_ac - g < 0],   [a_only > g] = [ka + C_ac < 0],
+//   [b_only > 0] = [kb + C_bc - g < 0],   [b_only > g] = [kb + C_bc < 0],
+//   [c_only > 0] = [C_ac - g + C_bc - s_c < 0],   [c_only > g] = [C_ac + C_bc - s_c < 0],
+// the pair bits [C_xy > g] = [g - C_xy < 0], and group 8 (g = 0, pairs 7) = pairs + [g - 1 < 0].
+__device__ inline unsigned tri_slot(int ka, int kb, int cab, int sc, int cbc, int cac, int g) {
+  const int d = cac - g, e = cbc - g;
+  const unsigned ta = ((unsigned)(ka + d) >> 31) + ((unsigned)(ka + cac) >> 31);
+  const unsigned tb = ((unsigned)(kb + e) >> 31) + ((unsigned)(kb + cbc) >> 31);
+  const unsigned tc = ((unsigned)(d + cbc - sc) >> 31) + ((unsigned)(cac + cbc - sc) >> 31);
+  const unsigned grp = ((unsigned)(g - cab) >> 31) + ((unsigned)(-e) >> 31 << 1) +
+                       ((unsigned)(-d) >> 31 << 2) + ((unsigned)(g - 1) >> 31);
+  return grp * 27u + ta + 3u * tb + 9u * tc;
+}
+
 // Centre rank r (NW waves; NC copies of the slot counters shared by the workgroup, copy = lane %
 // NC): every triangle r < rb < rc with rb in r's upper row and rc in rb's upper row, one LDS
 // atomic each into its slot; the slots are folded into the 30 bins once per centre.
@@ -216,7 +232,7 @@ __device__ inline void tri_item(MT* M, int32_t E, int32_t r, const uint32_t* __r
     const int32_t rb = (int32_t)(eb >> kCntBits);
     const int cab = (int)(eb & kCntMask);
     const MT mab = M[rb];
-    const int sb = sz[rb];
+    const int ka = cab - sa, kb = cab - sz[rb];
     const uint32_t* rowB = rows + roff[rb];
     const uint32_t nB = rlen[rb];
     auto visit = [&](uint32_t ec) {
@@ -227,11 +243,7 @@ __device__ inline void tri_item(MT* M, int32_t E, int32_t r, const uint32_t* __r
         const int cac = popc(mc);
         const int g = popc((MT)(mab & mc));
         const int sc = sz[rc];
-        const int ao = sa - cab - cac + g, bo = sb - cab - cbc + g, co = sc - cac - cbc + g;
-        const unsigned pairs = (unsigned)(cab > g) | (unsigned)(cbc > g) << 1 | (unsigned)(cac > g) << 2;
-        const unsigned t = (unsigned)((ao > 0) + (ao > g)) + 3u * (unsigned)((bo > 0) + (bo > g)) +
-                           9u * (unsigned)((co > 0) + (co > g));
-        atomicAdd(&hc[((g ? pairs : 8u) * 27u + t) * NC], 1u);
+        atomicAdd(&hc[tri_slot(ka, kb, cab, sc, cbc, cac, g) * NC], 1u);
       }
     };
     // kTriLoads row loads in flight per lane before any is used (lane l reads l, l + 64, ...)
