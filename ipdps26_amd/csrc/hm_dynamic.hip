@@ -138,6 +138,9 @@ constexpr int kDeltaCopies = 8;
 // (the registers they take cost no occupancy), few when several workgroups fit per CU.
 constexpr int kDeltaLoadsWide = 16;
 constexpr int kDeltaLoadsNarrow = 2;
+// The narrow form serves graphs whose table leaves room for two 16-wave workgroups per CU: at
+// most 64 VGPRs (8 waves per SIMD) keeps both resident (at 85 VGPRs only one fits).
+constexpr int kDeltaWavesPerEu(int loads) { return loads <= kDeltaLoadsNarrow ? 8 : 1; }
 constexpr size_t kDeltaOneGroupLds = 80 * 1024;  // above this, one workgroup per CU (160 KB LDS)
 constexpr int kDeltaTiledLoads = 4;  // row chunks per step of the tiled delta
 // Slots of a closed pair (b, c) of a changed hyperedge f (Venn id v over f, b, c in that order:
@@ -329,7 +332,7 @@ __device__ inline void delta_body(
 }
 
 template <typename MT, int NW, int kDeltaLoads>
-__global__ __launch_bounds__(NW * kWave) void k_delta(
+__global__ __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(kDeltaWavesPerEu(kDeltaLoads)))) void k_delta(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
     const int32_t* __restrict__ vedges, const unsigned long long* __restrict__ roff,
@@ -730,7 +733,7 @@ __global__ __launch_bounds__(kBlock) void k_full_fill_cap(const UpdHdr* __restri
 // phase 0: L(deleted ids, graph before the batch), subtracted; phase 1: L(inserted ids, graph
 // after it), added (skipped when the rows overflowed)
 template <typename MT, int NW, int kDeltaLoads>
-__global__ __launch_bounds__(NW * kWave) void k_delta_cap(
+__global__ __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(kDeltaWavesPerEu(kDeltaLoads)))) void k_delta_cap(
     const UpdHdr* __restrict__ hd, int32_t phase, const int32_t* __restrict__ dels,
     const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint8_t* __restrict__ esize, const uint32_t* __restrict__ voff,
