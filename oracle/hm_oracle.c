@@ -23,6 +23,10 @@
  *                          reference THyMe+ binary on distinct-timestamp inputs, oracle_temporal_brute,
  *                          and closed forms (m copies of one edge inside delta -> C(m,3) in motif 96).
  *                          An empty hyperedge is in no instance (THyMe's reader cannot read one).
+ *   oracle_thyme_temporal_range  the same count restricted to the instances whose newest edge has
+ *                          a time-order position in [lo, hi): shards of one count for a process
+ *                          pool. Pinned by: the shards of every temporal golden summing to the
+ *                          whole count (tests/test_oracle.py).
  *   oracle_temporal_brute  definition: every triple of temporal hyperedges whose time span is
  *                          <= delta and with at least two overlapping pairs (copies of one node set
  *                          overlap), classified by the Venn pattern written in time order.
@@ -422,11 +426,16 @@ static int64_t ordered_pairs(const int64_t* X, int nx, const int64_t* Y, int ny)
   return c;
 }
 
-/* THyMe+ (THyMe-main/code/main_thymeP.cpp:54-389) over int64 timestamps. */
-int oracle_thyme_temporal(int32_t E, const int64_t* off, const int32_t* nodes,
-                          const int64_t* times, int64_t delta, int64_t out[96]) {
+/* THyMe+ (THyMe-main/code/main_thymeP.cpp:54-389) over int64 timestamps, counting the instances
+ * whose newest edge sits at a time-order position in [lo, hi). THyMe+ builds the window state edge
+ * by edge; the state at position lo depends only on the edges inside lo's window, so the loop starts
+ * at the window's first position (adding, never counting, until lo). The counts do not depend on
+ * the order of the adjacency lists (closed pairs are taken once by static id, open pairs add both
+ * time orders), so any split of [0, E) into ranges sums to the whole count. */
+static int thyme_range(int32_t E, const int64_t* off, const int32_t* nodes, const int64_t* times,
+                       int64_t delta, int32_t lo, int32_t hi, int64_t out[96]) {
   for (int i = 0; i < 96; ++i) out[i] = 0;
-  if (E <= 0) return 0;
+  if (E <= 0 || lo >= hi) return 0;
   csr_t g;
   if (csr_build(E, off, nodes, &g)) return -1;
   /* static dedupe by node set (main_thymeP.cpp:164-179) */
@@ -478,7 +487,8 @@ int oracle_thyme_temporal(int32_t E, const int64_t* off, const int32_t* nodes,
 #define SSIZE(s) ((int)(g.off[rep[s] + 1] - g.off[rep[s]]))
 
   int32_t start = 0;
-  for (int32_t end = 0; end < E; ++end) {
+  while (times[order[start]] + delta < times[order[lo]]) ++start;
+  for (int32_t end = start; end < hi; ++end) {
     int32_t ea = order[end];
     int32_t a = sid[ea];
     int64_t ta = times[ea];
@@ -536,6 +546,7 @@ int oracle_thyme_temporal(int32_t E, const int64_t* off, const int32_t* nodes,
         iv_push(&adjc[adj[a].v[k]], adjc[a].v[k]);
       }
     }
+    if (end < lo) continue; /* window state only */
     /* (3) instances with a as the newest edge (main_thymeP.cpp:240-335) */
     const int32_t* na = SNODES(a);
     int32_t da = adj[a].n;
@@ -566,8 +577,8 @@ int oracle_thyme_temporal(int32_t E, const int64_t* off, const int32_t* nodes,
         int cbc = adjc[b].v[j];
         int sc = SSIZE(c);
         int cca = (markA[c] == end) ? markC[c] : 0;
-        int gabc = inter_size(inter, cnt, SNODES(c), sc);
         if (cca && !(b < c)) continue;
+        int gabc = inter_size(inter, cnt, SNODES(c), sc);
         const int64_t* Tc = etimes[c] + ehead[c];
         int nc_t = ecnt[c] - ehead[c];
         int64_t bc = ordered_pairs(Tb, nb_t, Tc, nc_t);
@@ -609,6 +620,19 @@ int oracle_thyme_temporal(int32_t E, const int64_t* off, const int32_t* nodes,
   free(byset); free(sid); free(rep); free(order);
   csr_free(&g);
   return 0;
+}
+
+int oracle_thyme_temporal(int32_t E, const int64_t* off, const int32_t* nodes,
+                          const int64_t* times, int64_t delta, int64_t out[96]) {
+  return thyme_range(E, off, nodes, times, delta, 0, E, out);
+}
+
+/* The instances whose newest edge has time-order position in [lo, hi) (ties by input index). */
+int oracle_thyme_temporal_range(int32_t E, const int64_t* off, const int32_t* nodes,
+                                const int64_t* times, int64_t delta, int32_t lo, int32_t hi,
+                                int64_t out[96]) {
+  if (lo < 0 || hi > E || lo > hi) return -1;
+  return thyme_range(E, off, nodes, times, delta, lo, hi, out);
 }
 
 /* Brute force over all triples of temporal hyperedges (definition). */

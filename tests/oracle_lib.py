@@ -26,10 +26,13 @@ def lib() -> ctypes.CDLL:
         so.oracle_static_brute.argtypes = [ctypes.c_int32, i64p, i32p, i64p]
         so.oracle_thyme_temporal.argtypes = [ctypes.c_int32, i64p, i32p, i64p, ctypes.c_int64, i64p]
         so.oracle_temporal_brute.argtypes = [ctypes.c_int32, i64p, i32p, i64p, ctypes.c_int64, i64p]
+        so.oracle_thyme_temporal_range.argtypes = [ctypes.c_int32, i64p, i32p, i64p, ctypes.c_int64,
+                                                   ctypes.c_int32, ctypes.c_int32, i64p]
         so.oracle_projection_degrees.argtypes = [ctypes.c_int32, i64p, i32p, i64p]
         so.oracle_mochy_table.restype = _P(ctypes.c_int)
         so.oracle_thyme_temporal_table.restype = _P(ctypes.c_int)
         for fn in ("oracle_static_mochy", "oracle_static_brute", "oracle_thyme_temporal",
+                   "oracle_thyme_temporal_range",
                    "oracle_temporal_brute", "oracle_projection_degrees"):
             getattr(so, fn).restype = ctypes.c_int
         _lib = so
@@ -65,6 +68,16 @@ def thyme_temporal(off, nodes, times, delta: int) -> np.ndarray:
     out = np.zeros(96, dtype=np.int64)
     assert lib().oracle_thyme_temporal(off.size - 1, po, pn, times.ctypes.data_as(_P(ctypes.c_int64)),
                                        delta, out.ctypes.data_as(_P(ctypes.c_int64))) == 0
+    return out
+
+
+def thyme_temporal_range(off, nodes, times, delta: int, lo: int, hi: int) -> np.ndarray:
+    """THyMe+ counts of the instances whose newest edge has time-order position in [lo, hi)."""
+    off, nodes, po, pn = _arr(off, nodes)
+    times = np.ascontiguousarray(times, dtype=np.int64)
+    out = np.zeros(96, dtype=np.int64)
+    assert lib().oracle_thyme_temporal_range(off.size - 1, po, pn, times.ctypes.data_as(_P(ctypes.c_int64)),
+                                             delta, lo, hi, out.ctypes.data_as(_P(ctypes.c_int64))) == 0
     return out
 
 

@@ -168,6 +168,42 @@ def test_thyme_restatement_equals_brute_force(seed):
                           O.temporal_brute(off, nodes, times, delta))
 
 
+@pytest.mark.parametrize("name", ["enron_distinct_d100000", "c4_family_e3000"])
+def test_temporal_range_shards_sum_to_golden(name):
+    """The anchor-range form (used to shard the full temporal workload over a process pool):
+    shards of uneven sizes sum to the reference THyMe+ golden."""
+    g = [x for x in load_golden("temporal") if x["name"] == name][0]
+    if g["kind"] == "fixture":
+        off, nodes, _ = W.read_thyme_tsv(os.path.join(GOLDEN, g["file"]))
+        times = (np.arange(off.size - 1, dtype=np.int64) + 1) * 1000
+    else:
+        p = g["params"]
+        off, nodes, times = W.temporal_c4(p["num_edges"], p["num_nodes"])
+    E = off.size - 1
+    cuts = [0, 1, E // 7, E // 3, E // 3 + 1, (2 * E) // 3, E]
+    total = np.zeros(96, dtype=np.int64)
+    for lo, hi in zip(cuts[:-1], cuts[1:]):
+        total += O.thyme_temporal_range(off, nodes, times, g["delta"], lo, hi)
+    assert total.tolist() == g["counts96"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_temporal_range_shards_equal_brute_force(seed):
+    """Random inputs with copies, distinct timestamps in shuffled input order (THyMe is defined
+    only for distinct timestamps), random shard cuts: the shards sum to the definition."""
+    rng = np.random.default_rng(300 + seed)
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(3, 60)),
+                                   num_nodes=int(rng.integers(3, 20)), kmax=int(rng.integers(1, 6)),
+                                   dup_frac=0.25)
+    E = off.size - 1
+    times = (rng.permutation(E) * int(rng.integers(1, 9)) + 1).astype(np.int64)  # unsorted, distinct
+    delta = int(rng.integers(5, 200))
+    cuts = sorted({0, E, *rng.integers(0, E + 1, size=4).tolist()})
+    total = sum(O.thyme_temporal_range(off, nodes, times, delta, lo, hi)
+                for lo, hi in zip(cuts[:-1], cuts[1:]))
+    assert np.array_equal(total, O.temporal_brute(off, nodes, times, delta))
+
+
 def test_temporal_copies_of_one_edge():
     m = 7
     off, nodes = csr([[1, 2, 3]] * m)
