@@ -1238,15 +1238,14 @@ static hm_status enqueue_update_cap(hm_graph* g) {
 
 // Sizes a plan for a batch (E2 ids, n_del / n_ins changed, Iins inserted incidences, V2 nodes
 // after it) with headroom, grows the buffers (keeping the store and the held rows) and captures
-// the update. p.exec stays null when the capture cannot be replayed (memset node).
+// the update. p.cap.exec stays null when the capture cannot be replayed (memset node).
 template <typename MT>
 static hm_status make_plan(hm_graph* g, int32_t E2, int32_t n_del, int32_t n_ins, int64_t Iins, int32_t V2) {
   DeviceCsr& c = g->csr;
   DynScratch& d = g->dyn;
   UpdPlan& p = g->uplan;
   hipStream_t st = g->stream;
-  if (p.exec) (void)hipGraphExecDestroy(p.exec);
-  p.exec = nullptr;
+  p.cap.reset();
   p.sig.clear();
   p.wide = sizeof(MT) == 8;
   p.dcap = std::max<int32_t>(64, 2 * n_del);
@@ -1304,13 +1303,13 @@ static hm_status make_plan(hm_graph* g, int32_t E2, int32_t n_del, int32_t n_ins
   CaptureGuard capturing;  // no allocation or free on another thread meanwhile
   HM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   const hm_status s1 = enqueue_update_cap<MT>(g);
-  hipGraphExec_t exec = nullptr;
-  const hm_status s2 = end_capture(st, &exec);  // ends the capture on every path
+  CapturedGraph cap;
+  const hm_status s2 = end_capture(st, &cap);  // ends the capture on every path
   if (s1 != HM_OK || s2 != HM_OK) {
-    if (exec) (void)hipGraphExecDestroy(exec);
+    cap.reset();
     return s1 != HM_OK ? s1 : s2;
   }
-  p.exec = exec;
+  p.cap = cap;
   p.sig = plan_sig(g);
   ++p.captures;
   return HM_OK;
@@ -1335,23 +1334,31 @@ static hm_status update_captured(hm_graph* g, const std::vector<int32_t>& dels_s
     d.rows_valid = true;
     d.rows_tile = 0;
   }
-  const bool fits = p.exec && p.wide == (sizeof(MT) == 8) && E2 <= p.Ecap && n_del <= p.dcap &&
+  const bool fits = p.cap.exec && p.wide == (sizeof(MT) == 8) && E2 <= p.Ecap && n_del <= p.dcap &&
                     n_ins <= p.icap && Iins <= p.iicap && newV <= p.Vcap && c.I + Iins <= p.Icap &&
                     p.sig == plan_sig(g);
   if (!fits) {
     if (make_plan<MT>(g, E2, n_del, n_ins, Iins, newV) != HM_OK) {
       clear_error();
-      if (p.exec) (void)hipGraphExecDestroy(p.exec);
-      p.exec = nullptr;
+      p.cap.reset();
       p.disabled = true;  // this graph's updates stay eager
       return HM_OK;
     }
-    if (!p.exec) {
+    if (!p.cap.exec) {
       p.disabled = true;  // the capture held a memset node
       return HM_OK;
     }
   }
   const InLayout L = in_layout(p);
+  // the staging block, the device copy of it and the write-back must lie inside the buffers the
+  // plan sized (a batch that fits the plan fits these; anything else is an engine bug)
+  if (n_del > p.dcap || n_ins > p.icap || Iins > p.iicap || (int64_t)ioff.size() != (int64_t)n_ins + 1 ||
+      (int64_t)inodes.size() < Iins || L.total > p.hin.bytes || L.total > p.in.bytes ||
+      kUpdOutBytes > p.hout.bytes || !p.hin.p || !p.hout.p || !p.in.p ||
+      L.ins_off + sizeof(uint32_t) * ((size_t)n_ins + 1) > L.ins_nodes ||
+      L.ins_nodes + sizeof(int32_t) * (size_t)Iins > L.total ||
+      L.dels + sizeof(int32_t) * (size_t)n_del > L.ins_off)
+    return fail(HM_ERR_DEVICE, "captured update: staging layout does not hold the batch");
   unsigned char* hin = p.hin.as<unsigned char>();
   UpdHdr hdr{};
   hdr.E_old = E;
@@ -1374,7 +1381,7 @@ static hm_status update_captured(hm_graph* g, const std::vector<int32_t>& dels_s
   } drain{st};
   HM_HIP(hipMemcpyAsync(p.in.p, hin, L.total, hipMemcpyHostToDevice, st));
   HM_HIP(hipEventRecord(g->ev[3], st));
-  HM_HIP(hipGraphLaunch(p.exec, st));
+  HM_HIP(hipGraphLaunch(p.cap.exec, st));
   HM_HIP(hipEventRecord(g->ev[7], st));
   unsigned char* out = p.hout.as<unsigned char>();
   HM_HIP(hipMemcpyAsync(out, d.acc.p, sizeof(unsigned long long) * kDynAccCount, hipMemcpyDeviceToHost, st));
@@ -1541,8 +1548,7 @@ hm_status hm_graph_set_update_capture(hm_graph* g, int32_t on) {
   UpdPlan& p = g->uplan;
   p.on = on != 0;
   if (!p.on) {
-    if (p.exec) (void)hipGraphExecDestroy(p.exec);
-    p.exec = nullptr;
+    p.cap.reset();
     p.sig.clear();
     p.disabled = false;
   }
@@ -1556,7 +1562,7 @@ hm_status hm_graph_update_capture_state(hm_graph* g, hm_update_capture_state* ou
   *out = hm_update_capture_state{};
   out->on = p.on ? 1 : 0;
   out->disabled = p.disabled ? 1 : 0;
-  out->captured = p.exec ? 1 : 0;
+  out->captured = p.cap.exec ? 1 : 0;
   out->captures = p.captures;
   out->last_captured = p.last_captured ? 1 : 0;
   out->last_overflow = p.last_overflow ? 1 : 0;

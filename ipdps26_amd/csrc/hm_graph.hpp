@@ -53,16 +53,19 @@ struct PinnedBuf {
   size_t bytes = 0;
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
+  // The new block is allocated before the old one is freed, so a failed growth leaves the old
+  // block (and its address) in place. Callers drain every copy from the old block first.
   hm_status ensure(size_t need) {
     if (need <= bytes && p) return HM_OK;
-    release();
     AllocGuard guard;
     const size_t want = std::max(need, (size_t)1 << 16);
-    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
-      p = nullptr;
+    void* np = nullptr;
+    if (hipHostMalloc(&np, want, hipHostMallocDefault) != hipSuccess || !np) {
       set_error("hipHostMalloc(%zu bytes) failed", want);
       return HM_ERR_OOM;
     }
+    if (p) (void)hipHostFree(p);
+    p = np;
     bytes = want;
     return HM_OK;
   }
@@ -161,7 +164,7 @@ struct UpdHdr {
 struct UpdPlan {
   bool on = false;
   bool disabled = false;      // the capture held a memset node: updates stay eager
-  hipGraphExec_t exec = nullptr;
+  CapturedGraph cap;          // the instantiated update (cap.exec null: none)
   bool wide = false;          // 64-bit masks
   int32_t Ecap = 0, Vcap = 0, dcap = 0, icap = 0;
   int64_t Icap = 0, iicap = 0;
@@ -174,8 +177,7 @@ struct UpdPlan {
   int32_t captures = 0;
   bool last_captured = false, last_overflow = false;
   void release() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    exec = nullptr;
+    cap.reset();
     in.release();
     hin.release();
     hout.release();
@@ -207,7 +209,7 @@ struct hm_graph {
   // captured static count (hm_graph_set_replay): replayed while the graph is unchanged
   uint64_t version = 0;  // bumped by every store update and tile change
   bool replay_on = false;
-  hipGraphExec_t replay_exec = nullptr;
+  hm::CapturedGraph replay;  // replay.exec null: no captured count
   uint64_t replay_version = 0;
   bool last_replayed = false;
   // one-table triangle form: -1 = autotuned per shape (tri_retune: tune again on the next count)

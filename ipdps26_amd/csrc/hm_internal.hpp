@@ -56,10 +56,24 @@ hm_status check_hip(hipError_t e, const char* what, const char* file, int line);
 // Raises a kernel's dynamic-LDS limit to at least `bytes` (once per kernel and size; the
 // attribute call is not repeated on every launch).
 hm_status allow_lds(const void* kernel, size_t bytes);
-// Ends the stream capture on st and instantiates the captured graph into *exec. *exec stays null
-// (HM_OK) when the capture holds a memset node: replays of such nodes are not trusted, so such a
-// sequence always runs eagerly.
-hm_status end_capture(hipStream_t st, hipGraphExec_t* exec);
+// An instantiated stream capture and the graph it was instantiated from. The template lives as
+// long as the executable graph (reset() destroys both), so nothing that inspects a launch, such
+// as a profiler's graph hooks, can meet a destroyed template.
+struct CapturedGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+// Ends the stream capture on st and instantiates the captured graph into *out (reset first).
+// out->exec stays null (HM_OK) when the capture holds any node other than kernels (memset nodes
+// are not trusted on replays, copy nodes would refer to the capturing call's host memory), so
+// such a sequence always runs eagerly.
+hm_status end_capture(hipStream_t st, CapturedGraph* out);
 }
 // Evaluates a HIP call; on failure records the message and returns from the enclosing function.
 #define HM_HIP(call)                                                          \

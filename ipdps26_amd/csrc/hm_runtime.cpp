@@ -68,8 +68,8 @@ CaptureGuard::~CaptureGuard() {
   g_capture_mu.unlock();
 }
 
-hm_status end_capture(hipStream_t st, hipGraphExec_t* exec) {
-  *exec = nullptr;
+hm_status end_capture(hipStream_t st, CapturedGraph* out) {
+  out->reset();
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(st, &graph);
   if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture", __FILE__, __LINE__);
@@ -77,18 +77,23 @@ hm_status end_capture(hipStream_t st, hipGraphExec_t* exec) {
   e = hipGraphGetNodes(graph, nullptr, &n);
   std::vector<hipGraphNode_t> nodes(n);
   if (e == hipSuccess && n) e = hipGraphGetNodes(graph, nodes.data(), &n);
-  bool memset_node = false;
+  // only kernel nodes are replayed: a memset node has given wrong results on replays, and a copy
+  // node would refer to host memory of the capturing call
+  bool other_node = false;
   for (size_t k = 0; e == hipSuccess && k < n; ++k) {
     hipGraphNodeType t;
     e = hipGraphNodeGetType(nodes[k], &t);
-    memset_node = memset_node || (e == hipSuccess && t == hipGraphNodeTypeMemset);
+    other_node = other_node || (e == hipSuccess && t != hipGraphNodeTypeKernel && t != hipGraphNodeTypeEmpty);
   }
-  if (e == hipSuccess && !memset_node) e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) {
-    *exec = nullptr;
-    return check_hip(e, "captured graph", __FILE__, __LINE__);
+  hipGraphExec_t exec = nullptr;
+  if (e == hipSuccess && !other_node) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess || !exec) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    return e != hipSuccess ? check_hip(e, "captured graph", __FILE__, __LINE__) : HM_OK;
   }
+  out->graph = graph;
+  out->exec = exec;
   return HM_OK;
 }
 

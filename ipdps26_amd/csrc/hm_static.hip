@@ -743,7 +743,7 @@ using CountImpl = hm_status (*)(hm_graph*, unsigned long long*, int32_t, bool);
 static hm_status replay_count(hm_graph* g, unsigned long long* h_acc) {
   hipStream_t st = g->stream;
   HM_HIP(hipEventRecord(g->ev[0], st));
-  HM_HIP(hipGraphLaunch(g->replay_exec, st));
+  HM_HIP(hipGraphLaunch(g->replay.exec, st));
   HM_HIP(hipEventRecord(g->ev[2], st));
   HM_HIP(hipMemcpyAsync(h_acc, g->scratch.acc.p, sizeof(unsigned long long) * kAccCount,
                         hipMemcpyDeviceToHost, st));
@@ -755,18 +755,17 @@ static hm_status replay_count(hm_graph* g, unsigned long long* h_acc) {
 
 // Captures the count of g (just counted eagerly, so every buffer is sized) for later replays.
 static hm_status capture_count(hm_graph* g, CountImpl impl, int32_t T) {
-  if (g->replay_exec) (void)hipGraphExecDestroy(g->replay_exec);
-  g->replay_exec = nullptr;
+  g->replay.reset();
   CaptureGuard capturing;  // no allocation or free on another thread meanwhile
   HM_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
   const hm_status st = impl(g, nullptr, T, true);
-  hipGraphExec_t exec = nullptr;
-  const hm_status st2 = end_capture(g->stream, &exec);  // ends the capture on every path
+  CapturedGraph cap;
+  const hm_status st2 = end_capture(g->stream, &cap);  // ends the capture on every path
   if (st != HM_OK || st2 != HM_OK) {
-    if (exec) (void)hipGraphExecDestroy(exec);
+    cap.reset();
     return st != HM_OK ? st : st2;
   }
-  g->replay_exec = exec;  // null when the capture holds a memset node: counts stay eager
+  g->replay = cap;  // exec null when the capture holds a memset node: counts stay eager
   g->replay_version = g->version;
   return HM_OK;
 }
@@ -873,7 +872,7 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
       impl = wide ? count_static_impl<mask64_t> : count_static_impl<uint32_t>;
       T = wide ? auto_tile<mask64_t>(lim) : auto_tile<uint32_t>(lim);
     }
-    if (g->replay_on && g->replay_exec && g->replay_version == g->version) {
+    if (g->replay_on && g->replay.exec && g->replay_version == g->version) {
       HM_TRY(replay_count(g, acc));
       g->last_replayed = true;
     } else {
@@ -939,7 +938,7 @@ hm_status hm_graph_create(const int64_t* offsets, const int32_t* nodes, int32_t 
 hm_status hm_graph_destroy(hm_graph* g) {
   if (!g) return HM_OK;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
-  if (g->replay_exec) (void)hipGraphExecDestroy(g->replay_exec);
+  g->replay.reset();
   g->uplan.release();
   g->scratch.release();
   g->dyn.release();
@@ -998,17 +997,14 @@ hm_status hm_graph_set_replay(hm_graph* g, int32_t on) {
   hm::clear_error();
   if (!g) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_replay: null handle");
   g->replay_on = on != 0;
-  if (!g->replay_on && g->replay_exec) {
-    (void)hipGraphExecDestroy(g->replay_exec);
-    g->replay_exec = nullptr;
-  }
+  if (!g->replay_on) g->replay.reset();
   return HM_OK;
 }
 
 hm_status hm_graph_replay_state(hm_graph* g, int32_t* captured, int32_t* last_replayed) {
   hm::clear_error();
   if (!g || !captured || !last_replayed) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_replay_state: null argument");
-  *captured = (g->replay_exec && g->replay_version == g->version) ? 1 : 0;
+  *captured = (g->replay.exec && g->replay_version == g->version) ? 1 : 0;
   *last_replayed = g->last_replayed ? 1 : 0;
   return HM_OK;
 }
