@@ -19,15 +19,21 @@ largest node sits in ~780k hyperedges, i.e. >7e16 triples); the temporal workloa
 largest configuration the engine counts at full size on one GPU.
 
 value = h-motif instances classified per second = (sum of the bins) / step time.
-roofline = the dominant kernel (k_tcount or k_tri): algorithmic bytes per launch divided by its
-average duration from HIP events on the stream it runs on. Its algorithmic bytes are
-8 P (projection rows read once, 8 B per directed pair) + 8 * sum over pairs (x, y), y after x,
-of d_y (neighbour rows read for closed triples) + 8 B per histogram bin. `traffic` is the HBM
-bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known
-size) when profiles/traffic_<workload>.txt holds it, else null.
+roofline = the dominant kernel (k_tcount, k_tri or k_delta), bound "hbm": achieved = the HBM bytes
+one launch moves, read by rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a plain 1 GiB
+device copy needs to read its own size; profiles/traffic_<workload>.txt, same build), divided by
+the kernel's average duration from HIP events on its stream in this run; frac = achieved / 8 TB/s.
+The cost model of the counting path (8 B per directed projection pair + 8 B per neighbour-row
+entry read for closed triples + 8 B per bin) is reported beside it as model_bytes: most of those
+bytes come from LDS, L2 and the Infinity Cache, so it is not an HBM lower bound. valu = the
+kernel's SQ_INSTS_VALU per launch (profiles/counters_<kernel>_<workload>.md) per second of its
+duration here, against the issue rate of a VALU-only calibration kernel measured under the same
+counter (profiles/valu_probe.txt; bench.py --valu-probe).
 cpu_baseline = the reference program timed on this host on a bounded prefix of the same input:
 THyMe+ main_thymeP (temporal) or MoCHy main_exact_par (static), built from the reference sources
-into oracle/_ref.
+into oracle/_ref, with `nproc` threads (cores and SMT of the host stated beside it).
+serving (default run) = a shortened plain-versus-served request stream over the reduced serving
+set: requests/s, p50/p99, results identical request by request.
 """
 from __future__ import annotations
 
@@ -62,6 +68,10 @@ DESC = {
                   "[1k, 6k], V = E/2, seed 5; 400 requests, half count queries, half update "
                   "batches of 0.5-5% of E (70/30 insert/delete), seed 5000",
 }
+WHY_DEFAULT = ("the largest configuration the engine counts exactly at full size on one GPU: the "
+               "2M-hyperedge static graph (one node in 780,282 hyperedges, > 7.9e16 triples), the "
+               "500k-hyperedge dynamic base and the [10k, 1M]-hyperedge serving graphs are beyond "
+               "exact per-triple counting, and no exact hub-aware method is built")
 # Bounded CPU samples: a prefix of the same input sized for ~10-30 s of reference-program time.
 CPU_SAMPLE_EDGES = {"temporal_c4": 700, "static_c1": 4000, "dynamic_c3": 4600}
 DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20, "dynamic_c3": 10, "serving_c5": 1}
@@ -69,7 +79,35 @@ DYN_BASE_EDGES, DYN_BASE_NODES = 20_000, 8_000
 
 
 def cpu_threads() -> int:
-    return min(os.cpu_count() or 1, 16)
+    """Threads for the reference CPU programs: `nproc` (the CPUs this process may run on, as
+    limited by its affinity mask and OMP_NUM_THREADS)."""
+    try:
+        return max(1, int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout))
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        return len(os.sched_getaffinity(0)) or 1
+
+
+def host_cpu() -> dict:
+    """Logical CPUs, physical cores and SMT of the CPUs this process may run on (sysfs)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology"
+        try:
+            pkg = open(f"{base}/physical_package_id").read().strip()
+            core = open(f"{base}/core_id").read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add((str(c), str(c)))
+    model = ""
+    try:
+        model = next((l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                      if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"logical_cpus_allowed": len(cpus), "physical_cores_allowed": len(cores),
+            "smt_threads_per_core": round(len(cpus) / max(1, len(cores)), 2),
+            "online_logical_cpus": os.cpu_count(), "model": model}
 
 
 def cpu_baseline_static(off, nodes) -> dict:
@@ -164,16 +202,59 @@ def cpu_baseline_temporal(off, nodes, times, delta) -> dict:
                       f"delta=3600 s ({inst} instances); {prog} wall time {secs:.2f} s"}
 
 
-def measured_traffic(workload: str):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 summary."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+def _profile_kv(name: str) -> dict:
+    """key: value lines of a committed rocprofv3 summary under profiles/ ({} when absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    out = {}
     if os.path.exists(path):
         with open(path) as f:
             for line in f:
-                key, _, val = line.partition(":")
-                if key.strip() == "hbm_bytes_per_launch":
-                    return int(val.strip())
-    return None
+                key, sep, val = line.partition(":")
+                if sep:
+                    out[key.strip()] = val.strip()
+    return out
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 summary."""
+    v = _profile_kv(f"traffic_{workload}.txt").get("hbm_bytes_per_launch")
+    return int(v) if v else None
+
+
+def measured_valu(kernel: str, workload: str):
+    """SQ_INSTS_VALU of one launch of `kernel` (profiles/counters_<kernel>_<workload>.md)."""
+    path = os.path.join(ROOT, "profiles", f"counters_{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^\|\s*SQ_INSTS_VALU\s*\|\s*([0-9.eE+]+)\s*\|", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def roofline(workload: str, r: dict) -> dict:
+    kms = r["kernel_ms"]
+    traffic = measured_traffic(workload)
+    achieved = traffic / (kms * 1e-3) / 1e9 if traffic else None
+    out = {"bound": "hbm", "kernel": r["kernel"], "achieved": achieved, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+           "kernel_ms": kms,
+           "traffic_source": f"profiles/traffic_{workload}.txt: rocprofv3 --pmc FETCH_SIZE and "
+                             "WRITE_SIZE of one launch, scaled x2 and x1 by a 1 GiB device copy",
+           "model_bytes": r["kernel_bytes"],
+           "model_gbs": r["kernel_bytes"] / (kms * 1e-3) / 1e9,
+           "model_note": "cost model of the counting path (projection pairs, neighbour-row entries, "
+                         "bins at 8 B each); mostly served by LDS, L2 and the Infinity Cache, so not "
+                         "an HBM lower bound and not the roofline numerator"}
+    valu = measured_valu(r["kernel"], workload)
+    probe = _profile_kv("valu_probe.txt")
+    if valu and probe.get("valu_wave_instructions_per_s"):
+        peak = float(probe["valu_wave_instructions_per_s"])
+        rate = valu / (kms * 1e-3)
+        out["valu"] = {"issued_per_s": rate, "peak_per_s": peak, "frac": rate / peak,
+                       "unit": "wave64 VALU instructions/s",
+                       "source": f"SQ_INSTS_VALU per launch from profiles/counters_{r['kernel']}_"
+                                 f"{workload}.md over this run's kernel time; peak = the same "
+                                 "counter over the VALU-only calibration kernel (profiles/valu_probe.txt)"}
+    return out
 
 
 def run_temporal(steps: int, warmup: int, with_cpu: bool) -> dict:
@@ -304,6 +385,52 @@ def copy_probe(mib: int) -> None:
     print(json.dumps({"copy_probe_bytes": mib << 20}), flush=True)
 
 
+def valu_probe(iters: int) -> None:
+    """The VALU calibration kernel (hm_probe_valu): its SQ_INSTS_VALU reading under rocprofv3 over
+    its duration is the issue peak the roofline's valu fraction divides by."""
+    ms = hm.probe_valu(iters)
+    print(json.dumps({"valu_probe_iters": iters, "valu_probe_ms": ms}), flush=True)
+
+
+def serving_summary(slots: int = 4, n_requests: int = 400) -> dict:
+    """The serving half of the metric in the default run: one request stream over the reduced
+    serving set, plain (one request at a time) against served (hm_server, `slots` concurrent, with
+    and without kept counts), saturated throughput and p50/p99 under Poisson arrivals at 70% of
+    the plain path's capacity; every result compared request by request."""
+    from ipdps26_amd import serving
+
+    graphs = W.serving_graphs()
+    reqs = W.serving_requests(graphs, n_requests)
+    same = lambda a, b: all(np.array_equal(x, y) for x, y in zip(a["results"], b["results"]))
+    keep = ("requests_per_s", "p50_ms", "p99_ms", "mean_ms")
+    pick = lambda d: {k: d[k] for k in keep}
+    plain = serving.run_plain(graphs, reqs, realtime=False)
+    served = serving.run_served(graphs, reqs, realtime=False, slots=slots)
+    kept = serving.run_served(graphs, reqs, realtime=False, slots=slots, maintain=True)
+    rate = 0.7 * plain["requests_per_s"]
+    timed = W.serving_requests(graphs, n_requests, rate_per_s=rate)
+    plain_rt = serving.run_plain(graphs, timed, realtime=True)
+    served_rt = serving.run_served(graphs, timed, realtime=True, slots=slots)
+    kept_rt = serving.run_served(graphs, timed, realtime=True, slots=slots, maintain=True)
+    identical = (same(plain, served) and same(plain, kept) and same(plain_rt, served_rt)
+                 and same(plain_rt, kept_rt))
+    print(f"[serving] plain {plain['requests_per_s']:.0f} req/s, served {served['requests_per_s']:.0f}, "
+          f"kept counts {kept['requests_per_s']:.0f}; p99 at {rate:.0f} req/s: plain "
+          f"{plain_rt['p99_ms']:.1f} ms, served {served_rt['p99_ms']:.1f}, kept {kept_rt['p99_ms']:.1f}",
+          file=sys.stderr, flush=True)
+    return {"workload": DESC["serving_c5"], "slots": slots, "requests": n_requests,
+            "results_identical_plain_vs_served": bool(identical),
+            "saturated": {"plain": pick(plain), "served": pick(served), "served_kept_counts": pick(kept),
+                          "speedup_served": served["requests_per_s"] / plain["requests_per_s"],
+                          "speedup_kept_counts": kept["requests_per_s"] / plain["requests_per_s"]},
+            "poisson_at_70pct_plain_capacity": {"rate_per_s": rate, "plain": pick(plain_rt),
+                                                "served": pick(served_rt),
+                                                "served_kept_counts": pick(kept_rt)},
+            "note": "kept counts: count queries are answered from each graph's counts kept current "
+                    "by the exact deltas of its update batches (no launch); the recounting served "
+                    "form runs every count query on the device"}
+
+
 def run_serving(slots: int = 4, n_requests: int = 400, batch_graphs: int = 16) -> dict:
     """Plain (one request at a time) versus served (hm_server, `slots` concurrent) on one request
     stream: throughput with every request queued at t=0, then p50/p99 latency with Poisson
@@ -397,9 +524,17 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="temporal_c4", choices=sorted(DESC))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the shortened serving stream of the default run")
+    ap.add_argument("--valu-probe", type=int, default=0, metavar="ITERS",
+                    help="only run the VALU calibration kernel (counter calibration)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return 0
+    if args.valu_probe:
+        hm.init(0)
+        valu_probe(args.valu_probe)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no "
@@ -420,7 +555,13 @@ def main() -> int:
         return 0
     run = {"temporal_c4": run_temporal, "static_c1": run_static, "dynamic_c3": run_dynamic}[args.workload]
     r = run(steps, args.warmup, not args.no_cpu)
-    achieved = r["kernel_bytes"] / (r["kernel_ms"] * 1e-3) / 1e9
+    config = {"workload": DESC[args.workload], **r["extra"]}
+    if args.workload == "temporal_c4":
+        config["why_this_workload"] = WHY_DEFAULT
+        if not args.no_serving:
+            config["serving"] = serving_summary()
+    if r["cpu"] is not None:
+        r["cpu"]["host"] = host_cpu()
     line = {
         "metric": "exact h-motif triples classified/sec",
         "value": r["total"] / (r["ms"] * 1e-3),
@@ -434,14 +575,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": DESC[args.workload], **r["extra"]},
-        "roofline": {"bound": "hbm", "kernel": r["kernel"], "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload), "kernel_ms": r["kernel_ms"],
-                     "algorithmic_bytes": r["kernel_bytes"],
-                     "note": "integer kernel: the algorithmic bytes are mostly served from LDS, L2 "
-                             "and the Infinity Cache (traffic = HBM bytes actually fetched); the "
-                             "binding resource is VALU issue (profiles/counters_*.md)"},
+        "config": config,
+        "roofline": roofline(args.workload, r),
         "cpu_baseline": r["cpu"],
     }
     print(json.dumps(line), flush=True)

@@ -1,5 +1,6 @@
 """GPU parity of the static count (hm_count_static, through the C ABI) against the oracle, the
 reference's golden vectors, brute force and closed forms. Integer counts: bit-exact."""
+import json
 import math
 import os
 
@@ -81,11 +82,23 @@ def test_wedge_invariant_and_stats_c1(gpu_ready):
     assert np.array_equal(c, c2)
     open_ = int(c[20:26].sum())
     closed = int(c.sum()) - open_
+    # wedges from the oracle's projected degrees, independent of the engine's own statistics
+    deg = O.projection_degrees(off, nodes)
+    assert open_ + 3 * closed == sum(int(d) * (int(d) - 1) // 2 for d in deg)
     assert open_ + 3 * closed == st["wedges"]
     assert closed == st["triangles"]
-    deg = O.projection_degrees(off, nodes)
     assert st["proj_pairs"] == int(deg.sum()) and st["max_proj_degree"] == int(deg.max())
     assert st["num_edges"] == 5000 and st["num_incidences"] == nodes.size
+
+
+def test_c1_all_30_bins(gpu_ready):
+    """All 30 bins of the 5,000-hyperedge static workload, including the four MoCHy does not print
+    (0, 1, 4, 6), against the oracle's full-size vector (whose printed bins equal the reference
+    MoCHy golden of the same input)."""
+    path = os.path.join(GOLDEN, "static_c1_30bins.json")
+    g = json.load(open(path))
+    off, nodes = hm.generate_static(hm.GenParams(**g["params"]))
+    assert count(off, nodes).tolist() == g["counts30"]
 
 
 def test_oversized_hyperedge_is_rejected(gpu_ready):

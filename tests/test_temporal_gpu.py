@@ -1,6 +1,7 @@
 """GPU parity of the temporal count (hm_count_temporal, through the C ABI) against the THyMe+
 oracle, the reference THyMe+ golden vectors and the temporal brute force. Integer counts:
 bit-exact."""
+import json
 import math
 import os
 
@@ -128,6 +129,21 @@ def test_full_workload_window_monotone(gpu_ready):
         full = g.count(W.TEMPORAL_C4_DELTA_US).astype(np.int64)
     assert np.all(short <= full) and short.sum() < full.sum()
     print(f"instances: delta/2 {short.sum()}, delta {full.sum()}", flush=True)
+
+
+def test_full_workload_equals_oracle(gpu_ready):
+    """The full temporal workload (200,000 temporal hyperedges, delta = 3,600 s) bin for bin
+    against the oracle's vector (THyMe+ restatement summed over anchor ranges in the build
+    container; tests/golden/make_oracle_full.py)."""
+    path = os.path.join(GOLDEN, "temporal_c4_full.json")
+    if not os.path.exists(path):
+        pytest.skip("tests/golden/temporal_c4_full.json not generated yet")
+    g = json.load(open(path))
+    off, nodes, times = W.temporal_c4()
+    assert g["delta"] == W.TEMPORAL_C4_DELTA_US and g["params"]["num_edges"] == off.size - 1
+    got = tcount(off, nodes, times, g["delta"])
+    print(f"instances {int(got.sum())} (oracle {g['instances']})", flush=True)
+    assert got.tolist() == g["counts96"]
 
 
 def test_empty_hyperedges_are_in_no_instance(gpu_ready):
