@@ -385,11 +385,11 @@ def copy_probe(mib: int) -> None:
     print(json.dumps({"copy_probe_bytes": mib << 20}), flush=True)
 
 
-def valu_probe(iters: int) -> None:
+def valu_probe(iters: int, lanes: int) -> None:
     """The VALU calibration kernel (hm_probe_valu): its SQ_INSTS_VALU reading under rocprofv3 over
     its duration is the issue peak the roofline's valu fraction divides by."""
-    ms = hm.probe_valu(iters)
-    print(json.dumps({"valu_probe_iters": iters, "valu_probe_ms": ms}), flush=True)
+    ms = hm.probe_valu(iters, lanes)
+    print(json.dumps({"valu_probe_iters": iters, "valu_probe_lanes": lanes, "valu_probe_ms": ms}), flush=True)
 
 
 def serving_summary(slots: int = 4, n_requests: int = 400) -> dict:
@@ -528,13 +528,15 @@ def main() -> int:
                     help="skip the shortened serving stream of the default run")
     ap.add_argument("--valu-probe", type=int, default=0, metavar="ITERS",
                     help="only run the VALU calibration kernel (counter calibration)")
+    ap.add_argument("--valu-lanes", type=int, default=64,
+                    help="active lanes per wave of the VALU calibration kernel (1..64)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return 0
     if args.valu_probe:
         hm.init(0)
-        valu_probe(args.valu_probe)
+        valu_probe(args.valu_probe, args.valu_lanes)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no "

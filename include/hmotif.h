@@ -394,10 +394,11 @@ hm_status hm_server_get_stats(hm_server* s, hm_server_stats* out);
 /* Calibration. No counterpart in the reference.                                               */
 /* ------------------------------------------------------------------------------------------ */
 /* Runs an integer-VALU-only kernel (8 workgroups of 4 waves per CU, `iters` loop trips of 32
- * independent xor/shift/add steps per lane) on the current device and returns the duration of
- * its second launch in *ms. Under rocprofv3 counters its SQ_INSTS_VALU per cycle is the VALU
- * issue peak that the counting kernels' own issue rate is compared with (bench.py --valu-probe). */
-hm_status hm_probe_valu(int32_t iters, double* ms);
+ * independent xor/shift/add steps in the first `lanes` (1..64) lanes of every wave) on the current
+ * device and returns the duration of its second launch in *ms. Under rocprofv3 counters its
+ * SQ_INSTS_VALU per cycle is the VALU issue peak that the counting kernels' own issue rate is
+ * compared with (bench.py --valu-probe); lanes < 64 measures issue with part of EXEC cleared. */
+hm_status hm_probe_valu(int32_t iters, int32_t lanes, double* ms);
 
 #ifdef __cplusplus
 }

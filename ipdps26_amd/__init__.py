@@ -126,7 +126,7 @@ _SIGNATURES = [
     ("hm_server_poll", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint64, _P(ctypes.c_int32), ctypes.c_void_p]),
     ("hm_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
-    ("hm_probe_valu", ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    ("hm_probe_valu", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     ("hm_tgraph_create", ctypes.c_int,
      [_i64p, _i32p, _i64p, ctypes.c_int32, _P(ctypes.c_void_p)]),
     ("hm_tgraph_destroy", ctypes.c_int, [ctypes.c_void_p]),
@@ -164,10 +164,10 @@ def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(_P(ctype))
 
 
-def probe_valu(iters: int) -> float:
+def probe_valu(iters: int, lanes: int = 64) -> float:
     """Duration (ms) of the VALU calibration kernel (hm_probe_valu) on the current device."""
     ms = ctypes.c_double(0.0)
-    _check(lib().hm_probe_valu(int(iters), ctypes.byref(ms)))
+    _check(lib().hm_probe_valu(int(iters), int(lanes), ctypes.byref(ms)))
     return ms.value
 
 

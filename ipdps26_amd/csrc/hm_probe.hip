@@ -14,15 +14,19 @@ namespace {
 
 constexpr int kChains = 8;
 
-__global__ __launch_bounds__(256) void k_probe_valu(uint32_t* out, int32_t iters, uint32_t seed) {
+// Lanes at or above `lanes` (of each wave's 64) skip the chains: with lanes < 64 every VALU
+// instruction of the loop issues with part of EXEC cleared.
+__global__ __launch_bounds__(256) void k_probe_valu(uint32_t* out, int32_t iters, uint32_t seed, int32_t lanes) {
   uint32_t x[kChains];
 #pragma unroll
   for (int c = 0; c < kChains; ++c) x[c] = seed + threadIdx.x * 131u + (uint32_t)c * 977u + blockIdx.x;
-  for (int32_t it = 0; it < iters; ++it) {
+  if ((int32_t)(threadIdx.x & 63) < lanes) {
+    for (int32_t it = 0; it < iters; ++it) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 4; ++u) {
 #pragma unroll
-      for (int c = 0; c < kChains; ++c) x[c] = (x[c] ^ (0x9E3779B9u + (uint32_t)c)) + (x[c] >> 5);
+        for (int c = 0; c < kChains; ++c) x[c] = (x[c] ^ (0x9E3779B9u + (uint32_t)c)) + (x[c] >> 5);
+      }
     }
   }
   uint32_t s = 0;
@@ -34,9 +38,10 @@ __global__ __launch_bounds__(256) void k_probe_valu(uint32_t* out, int32_t iters
 }  // namespace
 }  // namespace hm
 
-extern "C" hm_status hm_probe_valu(int32_t iters, double* ms) {
+extern "C" hm_status hm_probe_valu(int32_t iters, int32_t lanes, double* ms) {
   hm::clear_error();
-  if (iters <= 0 || !ms) return hm::fail(HM_ERR_BAD_INPUT, "hm_probe_valu: need iters > 0 and an output");
+  if (iters <= 0 || lanes < 1 || lanes > 64 || !ms)
+    return hm::fail(HM_ERR_BAD_INPUT, "hm_probe_valu: need iters > 0, 1 <= lanes <= 64 and an output");
   int dev = hm::current_device();
   HM_HIP(hipSetDevice(dev));
   int cus = 0;
@@ -52,9 +57,9 @@ extern "C" hm_status hm_probe_valu(int32_t iters, double* ms) {
   hm_status st = HM_OK;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = HM_ERR_DEVICE;
   if (st == HM_OK) {
-    hm::k_probe_valu<<<grid, block>>>(out, iters, 1u);  // warm-up (code object load)
+    hm::k_probe_valu<<<grid, block>>>(out, iters, 1u, lanes);  // warm-up (code object load)
     (void)hipEventRecord(e0, nullptr);
-    hm::k_probe_valu<<<grid, block>>>(out, iters, 2u);
+    hm::k_probe_valu<<<grid, block>>>(out, iters, 2u, lanes);
     (void)hipEventRecord(e1, nullptr);
     if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
       st = hm::fail(HM_ERR_DEVICE, "hm_probe_valu: launch failed");
