@@ -104,6 +104,21 @@ hm_status hm_graph_destroy(hm_graph* g);
  * and printed them; the count here follows MoCHy-E (MoCHy-master/main_exact_par.cpp:122-165). */
 hm_status hm_count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]);
 
+/* Static count that allocates nothing. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same handle, output and counts as hm_count_static; its scratch is a static device
+ * array of HM_NOALLOC_SCRATCH_BYTES per GPU, held from the first call on that GPU until the
+ * process ends, instead of the handle's own buffers. Limits: the hyperedge ids must fit one LDS
+ * table (hm_count_static's automatic one-table path, no forced tile), and ~65 B per hyperedge plus
+ * 4 B per directed projection pair must fit the array (the 5,000-hyperedge static workload needs
+ * ~43 MB); otherwise HM_ERR_OVER_BUDGET (or HM_ERR_UNSUPPORTED for a forced tile) and no count.
+ * The call never calls hipMalloc, hipHostMalloc or hipFree and never autotunes (it uses the
+ * handle's triangle form, else the one tuned for its shape, else 8 waves x 16 copies). Calls on one
+ * GPU share its array: the library runs them one at a time, and each drains its stream before it
+ * returns. */
+#define HM_NOALLOC_SCRATCH_BYTES (64ll << 20)
+hm_status hm_count_static_noalloc(hm_graph* g, uint64_t out[HM_STATIC_BINS]);
+
 /* Key tile of hm_count_static and of the delta of hm_graph_update / hm_count_delta on g.
  * 0 (the default): one LDS mask table over all hyperedge ids when they fit (about 28k ids for
  * the static count and the delta with 32-bit masks), tiles as large as LDS allows otherwise; > 0: tiles of

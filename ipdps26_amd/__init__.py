@@ -91,6 +91,7 @@ _SIGNATURES = [
      [_i64p, _i32p, ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_void_p)]),
     ("hm_graph_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("hm_count_static", ctypes.c_int, [ctypes.c_void_p, _u64p]),
+    ("hm_count_static_noalloc", ctypes.c_int, [ctypes.c_void_p, _u64p]),
     ("hm_graph_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(_GraphStats)]),
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
     ("hm_graph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
@@ -277,6 +278,13 @@ class Graph:
         """Exact static h-motif counts: uint64[30] in MoCHy bin order."""
         out = np.zeros(STATIC_BINS, dtype=np.uint64)
         _check(lib().hm_count_static(self._h, _ptr(out, ctypes.c_uint64)))
+        return out
+
+    def count_static_noalloc(self) -> np.ndarray:
+        """The same counts without any device allocation (hm_count_static_noalloc: the library's
+        static per-GPU scratch; one-table graphs of at most ~64 MiB of count scratch)."""
+        out = np.zeros(STATIC_BINS, dtype=np.uint64)
+        _check(lib().hm_count_static_noalloc(self._h, _ptr(out, ctypes.c_uint64)))
         return out
 
     def stats(self) -> dict:

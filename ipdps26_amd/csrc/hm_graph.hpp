@@ -20,6 +20,11 @@ struct DevBuf {
   // synchronises the whole device, which would serialise concurrent requests.
   hm_status ensure(size_t need) {
     if (need <= bytes && p) return HM_OK;
+    if (alloc_forbidden()) {
+      set_error("a buffer of %zu bytes is needed; this entry point allocates nothing and its "
+                "scratch holds %zu", need, bytes);
+      return HM_ERR_OVER_BUDGET;
+    }
     AllocGuard guard;
     size_t want = need ? need : 16;
     if (p) {
@@ -57,6 +62,10 @@ struct PinnedBuf {
   // block (and its address) in place. Callers drain every copy from the old block first.
   hm_status ensure(size_t need) {
     if (need <= bytes && p) return HM_OK;
+    if (alloc_forbidden()) {
+      set_error("a pinned buffer of %zu bytes is needed; this entry point allocates nothing", need);
+      return HM_ERR_OVER_BUDGET;
+    }
     AllocGuard guard;
     const size_t want = std::max(need, (size_t)1 << 16);
     void* np = nullptr;

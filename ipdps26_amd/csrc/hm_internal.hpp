@@ -42,6 +42,21 @@ class CaptureGuard {
   CaptureGuard& operator=(const CaptureGuard&) = delete;
 };
 
+// While a NoAllocScope lives on a thread, growing a device or pinned buffer on that thread fails
+// with HM_ERR_OVER_BUDGET instead of allocating (the entry points that promise no allocation run
+// the shared pipeline under one).
+bool alloc_forbidden();
+class NoAllocScope {
+ public:
+  NoAllocScope();
+  ~NoAllocScope();
+  NoAllocScope(const NoAllocScope&) = delete;
+  NoAllocScope& operator=(const NoAllocScope&) = delete;
+
+ private:
+  bool prev_;
+};
+
 // Validates a caller CSR (offsets[0] == 0, non-decreasing, node ids >= 0) and returns its rows
 // sorted with repeated nodes removed; *vmax_out = largest node id (-1 if none).
 hm_status prepare_host_rows(const int64_t* offsets, const int32_t* nodes, int32_t E,

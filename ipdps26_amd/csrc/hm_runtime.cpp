@@ -51,7 +51,12 @@ hm_status allow_lds(const void* kernel, size_t bytes) {
 namespace {
 std::shared_mutex g_capture_mu;
 thread_local bool t_capturing = false;
+thread_local bool t_no_alloc = false;
 }  // namespace
+
+bool alloc_forbidden() { return t_no_alloc; }
+NoAllocScope::NoAllocScope() : prev_(t_no_alloc) { t_no_alloc = true; }
+NoAllocScope::~NoAllocScope() { t_no_alloc = prev_; }
 
 AllocGuard::AllocGuard() : held_(!t_capturing) {
   if (held_) g_capture_mu.lock_shared();

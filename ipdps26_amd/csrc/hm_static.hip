@@ -535,6 +535,7 @@ static hm_status pick_tri_form(hm_graph* g, size_t lds, bool capture, int* form_
     if (it != g_tri_tuned.end()) form = it->second;
   }
   g->tri_retune = false;
+  if (form < 0 && alloc_forbidden()) form = kDefaultTriForm;  // tuning needs its own buffer
   if (form < 0) {
     CountScratch& s = g->scratch;
     HM_TRY(s.acc2.ensure(sizeof(unsigned long long) * kAccCount));
@@ -842,6 +843,148 @@ hm_status upload_csr(hm_graph* g, const std::vector<int64_t>& h_off,
   return HM_OK;
 }
 
+// Graph statistics of the count just made (hm_graph_get_stats).
+static void fill_count_stats(hm_graph* g, const uint64_t out[HM_STATIC_BINS], const unsigned long long* acc) {
+  const int32_t E = g->csr.E;
+  hm_graph_stats& st = g->stats;
+  st.num_edges = E;
+  st.num_nodes = g->csr.V;
+  st.num_incidences = g->csr.I;
+  st.proj_pairs = (int64_t)acc[kAccP];
+  st.max_proj_degree = (int64_t)acc[kAccMaxDeg];
+  st.oriented_row_sum = (int64_t)acc[kAccOrient];
+  st.wedges = (int64_t)acc[kAccWedges];
+  int64_t closed = 0;
+  for (int k = 0; k < HM_STATIC_BINS; ++k)
+    if (k < kOpenBin0 || k >= kOpenBin0 + 6) closed += (int64_t)out[k];
+  st.triangles = closed;
+  st.max_node_degree = (int64_t)acc[kAccMaxNodeDeg];
+  g->have_stats = true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Static count without allocation (hm_count_static_noalloc): the one-table pipeline above with its
+// scratch carved from `scratch`, one file-scope device array per GPU. Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol): its device address
+// comes from hipGetSymbolAddress, looked up once per device with that device current.
+constexpr size_t kNoAllocScratchBytes = (size_t)HM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct NoAllocDevice {
+  std::mutex mu;         // serializes the calls on one device (they share its scratch)
+  void* ptr = nullptr;   // device address of that device's copy of `scratch`
+};
+constexpr int kNoAllocMaxDevices = 64;
+static NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+// Device address of `scratch` on device dev (current); dev's mutex is held.
+static hm_status noalloc_base(int dev, unsigned char** base) {
+  NoAllocDevice& d = g_noalloc[dev];
+  if (!d.ptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    if (!p) return fail(HM_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+    d.ptr = p;
+  }
+  *base = static_cast<unsigned char*>(d.ptr);
+  return HM_OK;
+}
+
+// Carves the one-table count's buffers for E hyperedges out of [base, base + cap): the
+// per-hyperedge arrays, the accumulator and the sort/scan storage first, the packed projection
+// rows in the rest. Fails (nothing launched) when the fixed part does not fit.
+static hm_status noalloc_carve(hm_graph* g, unsigned char* base, size_t cap, CountScratch* s) {
+  const int32_t E = g->csr.E;
+  size_t at = 0;
+  auto take = [&](DevBuf& b, size_t bytes) {
+    at = (at + 255) & ~(size_t)255;
+    b.p = at + bytes <= cap ? base + at : nullptr;
+    b.bytes = bytes;
+    at += bytes;
+  };
+  const size_t e4 = sizeof(uint32_t) * (size_t)E, e8 = sizeof(unsigned long long) * (size_t)E;
+  take(s->deg, e4);
+  take(s->dlow, e4);
+  take(s->ids, e4);
+  take(s->perm, e4);
+  take(s->rank, e4);
+  take(s->degr, e8 + 8);
+  take(s->roff, e8 + 8);
+  take(s->rlen, e4);
+  take(s->szr, (size_t)E);
+  take(s->work, e8);
+  take(s->work2, e8);
+  take(s->order, e4);
+  take(s->ranks, e4);
+  take(s->acc, sizeof(unsigned long long) * kAccCount);
+  // sort and scan storage, sized as the pipeline asks (queries only: nothing runs)
+  size_t tb1 = 0, tb2 = 0, tb3 = 0;
+  HM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                            (const int32_t*)nullptr, (int32_t*)nullptr, E, 0, 32, g->stream));
+  HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, (const unsigned long long*)nullptr,
+                                          (unsigned long long*)nullptr, E + 1, g->stream));
+  HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb3, (const unsigned long long*)nullptr,
+                                                      (unsigned long long*)nullptr, (const int32_t*)nullptr,
+                                                      (int32_t*)nullptr, E, 0, 64, g->stream));
+  take(s->tmp, std::max({tb1, tb2, tb3, (size_t)16}));
+  at = (at + 255) & ~(size_t)255;
+  if (at >= cap) {
+    set_error("hm_count_static_noalloc: %d hyperedges need %zu bytes of scratch before the "
+              "projection rows; the static scratch holds %zu", E, at, cap);
+    return HM_ERR_OVER_BUDGET;
+  }
+  s->rows.p = base + at;  // the rest: 4 B per directed projection pair (checked before the fill)
+  s->rows.bytes = cap - at;
+  return HM_OK;
+}
+
+hm_status count_static_noalloc(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
+  const int32_t E = g->csr.E;
+  unsigned long long acc[kAccCount] = {0};
+  g->ms_project = g->ms_triangles = 0.f;
+  g->last_replayed = false;
+  if (E >= 3) {
+    if (g->csr.max_size > HM_MAX_EDGE_SIZE)
+      return fail(HM_ERR_UNSUPPORTED, "hyperedge larger than 64 nodes");
+    if (g->tile > 0)
+      return fail(HM_ERR_UNSUPPORTED, "hm_count_static_noalloc: forced tiles need the allocating count");
+    if (g->device < 0 || g->device >= kNoAllocMaxDevices)
+      return fail(HM_ERR_UNSUPPORTED, "hm_count_static_noalloc: device index beyond the scratch table");
+    HM_HIP(hipSetDevice(g->device));
+    const int lim = lds_limit_bytes(g->device);
+    const bool wide = g->csr.max_size > 32;
+    CountImpl impl;
+    if (!wide && dense_lds_bytes<uint32_t>(E, true) + kTiledStaticLds <= (size_t)lim)
+      impl = count_static_impl<uint32_t>;
+    else if (dense_lds_bytes<mask64_t>(E, true) + kTiledStaticLds <= (size_t)lim)
+      impl = count_static_impl<mask64_t>;
+    else
+      return fail(HM_ERR_OVER_BUDGET, "hm_count_static_noalloc: the hyperedge ids exceed one LDS "
+                                      "table (the tiled count allocates)");
+    NoAllocDevice& dev = g_noalloc[g->device];
+    std::lock_guard<std::mutex> lk(dev.mu);
+    unsigned char* base = nullptr;
+    HM_TRY(noalloc_base(g->device, &base));
+    CountScratch carved;
+    HM_TRY(noalloc_carve(g, base, kNoAllocScratchBytes, &carved));
+    std::swap(g->scratch, carved);  // the handle's own scratch is put back below on every path
+    hm_status st;
+    {
+      NoAllocScope no_alloc;
+      st = impl(g, acc, 0, false);
+    }
+    // the scratch is shared by the device's next call: nothing of this one may still run
+    const hipError_t drained = hipStreamSynchronize(g->stream);
+    std::swap(g->scratch, carved);
+    if (st != HM_OK) return st;
+    HM_HIP(drained);
+  }
+  for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = acc[kAccHist + k];
+  fill_count_stats(g, out, acc);
+  return HM_OK;
+}
+
 hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
   const int32_t E = g->csr.E;
   unsigned long long acc[kAccCount] = {0};
@@ -882,20 +1025,7 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
     }
   }
   for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = acc[kAccHist + k];
-  hm_graph_stats& st = g->stats;
-  st.num_edges = E;
-  st.num_nodes = g->csr.V;
-  st.num_incidences = g->csr.I;
-  st.proj_pairs = (int64_t)acc[kAccP];
-  st.max_proj_degree = (int64_t)acc[kAccMaxDeg];
-  st.oriented_row_sum = (int64_t)acc[kAccOrient];
-  st.wedges = (int64_t)acc[kAccWedges];
-  int64_t closed = 0;
-  for (int k = 0; k < HM_STATIC_BINS; ++k)
-    if (k < kOpenBin0 || k >= kOpenBin0 + 6) closed += (int64_t)out[k];
-  st.triangles = closed;
-  st.max_node_degree = (int64_t)acc[kAccMaxNodeDeg];
-  g->have_stats = true;
+  fill_count_stats(g, out, acc);
   return HM_OK;
 }
 
@@ -954,6 +1084,12 @@ hm_status hm_count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
   hm::clear_error();
   if (!g || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_count_static: null argument");
   return hm::count_static(g, out);
+}
+
+hm_status hm_count_static_noalloc(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
+  hm::clear_error();
+  if (!g || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_count_static_noalloc: null argument");
+  return hm::count_static_noalloc(g, out);
 }
 
 hm_status hm_graph_set_tile(hm_graph* g, int32_t tile_ids) {
