@@ -249,11 +249,14 @@ def roofline(workload: str, r: dict) -> dict:
     if valu and probe.get("valu_wave_instructions_per_s"):
         peak = float(probe["valu_wave_instructions_per_s"])
         rate = valu / (kms * 1e-3)
-        out["valu"] = {"issued_per_s": rate, "peak_per_s": peak, "frac": rate / peak,
+        out["valu"] = {"issued_per_s": rate, "calibration_per_s": peak, "ratio_to_calibration": rate / peak,
                        "unit": "wave64 VALU instructions/s",
                        "source": f"SQ_INSTS_VALU per launch from profiles/counters_{r['kernel']}_"
-                                 f"{workload}.md over this run's kernel time; peak = the same "
-                                 "counter over the VALU-only calibration kernel (profiles/valu_probe.txt)"}
+                                 f"{workload}.md over this run's kernel time; calibration = the same "
+                                 "counter over a VALU-only integer kernel (profiles/valu_probe.txt: "
+                                 "one 4-cycle pass per wave64 instruction per SIMD, 0.99 of it reached)",
+                       "note": "a ratio near or above 1 means the kernel is VALU-issue bound; above 1, "
+                               "some of its counted instructions take less than a full pass"}
     return out
 
 
@@ -351,6 +354,7 @@ def run_dynamic(steps: int, warmup: int, with_cpu: bool) -> dict:
     total = int(g.count_static().astype(np.int64).sum())  # the recount a batch would need instead
     recount_ms = (time.perf_counter() - t2) * 1e3
     g.close()
+    store_full = store_updates_full_c3()
     cur_off = np.concatenate([[0], np.cumsum([r.size for r in rows])]).astype(np.int64)
     cur_nodes = np.concatenate(rows).astype(np.int32)
     # delta kernel bytes: 4 B per scanned packed row entry + 8 B per histogram bin, per launch
@@ -362,10 +366,37 @@ def run_dynamic(steps: int, warmup: int, with_cpu: bool) -> dict:
                       "ms_per_batch": {"rows": rows_ms / steps, "delta": delta_ms / steps,
                                        "store": store_ms / steps},
                       "full_recount_ms": recount_ms, "triples_after": total,
+                      "store_updates_full_c3": store_full,
                       "captured_updates": {"ms_per_batch": cap_ms, "deltas_identical": bool(same_cap),
                                            "captures": cap_state["captures"],
                                            "last_ran_captured": bool(cap_state["last_captured"])}},
             "cpu": cpu_baseline_recount(cur_off, cur_nodes) if with_cpu else None}
+
+
+def store_updates_full_c3() -> dict:
+    """The dynamic workload's store update at full size (500,000-hyperedge base, 100 batches of
+    7,000 inserts and 3,000 deletes), without the delta: device ms of the store update per batch
+    (HIP events) and host wall time per hm_graph_update call."""
+    off, nodes = hm.generate_static(W.DYNAMIC_C3)
+    live = np.arange(off.size - 1, dtype=np.int64)
+    batches = []
+    for b in range(1, W.DYNAMIC_C3_BATCHES + 1):
+        dels, ioff, inodes = W.dynamic_batch(live, b)
+        batches.append((dels, ioff, inodes))
+        live = np.concatenate([np.setdiff1d(live, dels.astype(np.int64)),
+                               np.arange(off.size - 1 + (b - 1) * W.DYNAMIC_C3_INSERTS,
+                                         off.size - 1 + b * W.DYNAMIC_C3_INSERTS, dtype=np.int64)])
+    dev, wall = [], []
+    with hm.Graph(off, nodes) as g:
+        for dels, ioff, inodes in batches:
+            t0 = time.perf_counter()
+            g.update(dels, ioff, inodes, delta=False)
+            wall.append((time.perf_counter() - t0) * 1e3)
+            dev.append(g.phase_ms("store"))
+        ids_after = g.read_csr()[0].size - 1
+    return {"base_hyperedges": int(off.size - 1), "batches": len(batches),
+            "ids_after": int(ids_after), "store_ms_per_batch_mean": float(np.mean(dev)),
+            "store_ms_per_batch_max": float(np.max(dev)), "update_call_wall_ms_mean": float(np.mean(wall))}
 
 
 def copy_probe(mib: int) -> None:
