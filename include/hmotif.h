@@ -164,8 +164,20 @@ typedef struct hm_graph_stats {
 hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out);
 
 /* Kernel timing of the last hm_count_static on g, measured with HIP events on the handle's
- * stream: names are "project" and "triangles"; ms receives the duration of that phase. */
+ * stream: names are "project" and "triangles"; "store" is the store-update part of the last eager
+ * hm_graph_update (0 after a captured one); ms receives the duration of that phase. */
 hm_status hm_graph_phase_ms(hm_graph* g, const char* phase, double* ms);
+
+/* Current sizes of g's store: hyperedge ids (deleted ones included, as empty rows), nodes, and
+ * incidences. No counterpart in the reference, whose store prints its flat arrays instead
+ * (dynamicHyperGraph/utils/utils.cpp:434-435). */
+hm_status hm_graph_get_sizes(hm_graph* g, int32_t* num_edges, int32_t* num_nodes, int64_t* num_incidences);
+/* Copies g's store to host arrays: H2V offsets[num_edges + 1] and nodes[num_incidences] (rows in
+ * hyperedge id order, nodes ascending, a deleted hyperedge an empty row) and V2H
+ * voffsets[num_nodes + 1] and vedges[num_incidences] (hyperedge ids ascending per node). Any
+ * pointer may be null to skip that array; sizes from hm_graph_get_sizes. Waits for g's stream. */
+hm_status hm_graph_read_csr(hm_graph* g, int64_t* offsets, int32_t* nodes, int64_t* voffsets,
+                            int32_t* vedges);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Dynamic hypergraphs                                                                        */

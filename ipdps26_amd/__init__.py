@@ -91,6 +91,8 @@ _SIGNATURES = [
      [_i64p, _i32p, ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_void_p)]),
     ("hm_graph_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("hm_count_static", ctypes.c_int, [ctypes.c_void_p, _u64p]),
+    ("hm_graph_get_sizes", ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_int32), _P(ctypes.c_int64)]),
+    ("hm_graph_read_csr", ctypes.c_int, [ctypes.c_void_p, _i64p, _i32p, _i64p, _i32p]),
     ("hm_count_static_noalloc", ctypes.c_int, [ctypes.c_void_p, _u64p]),
     ("hm_graph_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(_GraphStats)]),
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
@@ -286,6 +288,19 @@ class Graph:
         out = np.zeros(STATIC_BINS, dtype=np.uint64)
         _check(lib().hm_count_static_noalloc(self._h, _ptr(out, ctypes.c_uint64)))
         return out
+
+    def read_csr(self):
+        """The store as it is now: (offsets int64[E+1], nodes int32[I], voffsets int64[V+1],
+        vedges int32[I]); a deleted hyperedge is an empty row."""
+        E, V, I = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int64(0)
+        _check(lib().hm_graph_get_sizes(self._h, ctypes.byref(E), ctypes.byref(V), ctypes.byref(I)))
+        off = np.zeros(E.value + 1, dtype=np.int64)
+        nodes = np.zeros(max(I.value, 1), dtype=np.int32)
+        voff = np.zeros(V.value + 1, dtype=np.int64)
+        vedges = np.zeros(max(I.value, 1), dtype=np.int32)
+        _check(lib().hm_graph_read_csr(self._h, _ptr(off, ctypes.c_int64), _ptr(nodes, ctypes.c_int32),
+                                       _ptr(voff, ctypes.c_int64), _ptr(vedges, ctypes.c_int32)))
+        return off, nodes[: I.value], voff, vedges[: I.value]
 
     def stats(self) -> dict:
         st = _GraphStats()

@@ -669,10 +669,21 @@ static hm_status count_static_impl(hm_graph* g, unsigned long long* h_acc, int32
     k_low_words<<<ib, kBlock, 0, st>>>(s.vkeys2.as<unsigned long long>(), s.vrank.as<int32_t>(), I);
   }
   if (!capture) {
-    unsigned long long total_cap = 0;
+    unsigned long long total_cap = 0, dmax = 0;
     HM_HIP(hipMemcpyAsync(&total_cap, s.roff.as<unsigned long long>() + E, sizeof(total_cap),
                           hipMemcpyDeviceToHost, st));
+    // degr holds the degrees in ascending order (the sorted keys): its last entry is d_max
+    HM_HIP(hipMemcpyAsync(&dmax, s.degr.as<unsigned long long>() + (E - 1), sizeof(dmax),
+                          hipMemcpyDeviceToHost, st));
     HM_HIP(hipStreamSynchronize(st));
+    // k_tri_tiled's uint32 slot counters are flushed per tile; copy (lane % 16) of a centre's
+    // workgroup takes at most a 16th of every neighbour row's entries inside the tile, plus one
+    // partial step per neighbour: at most d_max * (T / 16 + 4 * kTriLoads + 1) per tile
+    if (tiled && dmax * ((unsigned long long)T / 16 + 4 * kTriLoads + 1) >= (1ull << 32)) {
+      set_error("static count: projected degree %llu x tile of %d ids exceeds the bound of the "
+                "32-bit slot counters (2^32)", dmax, T);
+      return HM_ERR_UNSUPPORTED;
+    }
     HM_TRY(s.rows.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(total_cap, 1)));
   }
   // 4. upper rows
@@ -1153,11 +1164,42 @@ hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out) {
   return HM_OK;
 }
 
+hm_status hm_graph_get_sizes(hm_graph* g, int32_t* num_edges, int32_t* num_nodes, int64_t* num_incidences) {
+  hm::clear_error();
+  if (!g || !num_edges || !num_nodes || !num_incidences)
+    return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_get_sizes: null argument");
+  *num_edges = g->csr.E;
+  *num_nodes = g->csr.V;
+  *num_incidences = g->csr.I;
+  return HM_OK;
+}
+
+hm_status hm_graph_read_csr(hm_graph* g, int64_t* offsets, int32_t* nodes, int64_t* voffsets,
+                            int32_t* vedges) {
+  hm::clear_error();
+  if (!g) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_read_csr: null handle");
+  const hm::DeviceCsr& c = g->csr;
+  HM_HIP(hipSetDevice(g->device));
+  HM_HIP(hipStreamSynchronize(g->stream));
+  auto read_offsets = [&](const hm::DevBuf& src, int64_t n, int64_t* dst) -> hm_status {
+    std::vector<uint32_t> tmp((size_t)n + 1);
+    if (src.p) HM_HIP(hipMemcpy(tmp.data(), src.p, sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k <= n; ++k) dst[k] = src.p ? (int64_t)tmp[(size_t)k] : 0;
+    return HM_OK;
+  };
+  if (offsets) HM_TRY(read_offsets(c.eoff, c.E, offsets));
+  if (voffsets) HM_TRY(read_offsets(c.voff, c.V, voffsets));
+  if (nodes && c.I > 0) HM_HIP(hipMemcpy(nodes, c.enodes.p, sizeof(int32_t) * (size_t)c.I, hipMemcpyDeviceToHost));
+  if (vedges && c.I > 0) HM_HIP(hipMemcpy(vedges, c.vedges.p, sizeof(int32_t) * (size_t)c.I, hipMemcpyDeviceToHost));
+  return HM_OK;
+}
+
 hm_status hm_graph_phase_ms(hm_graph* g, const char* phase, double* ms) {
   hm::clear_error();
   if (!g || !phase || !ms) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_phase_ms: null argument");
   if (std::strcmp(phase, "project") == 0) *ms = g->ms_project;
   else if (std::strcmp(phase, "triangles") == 0) *ms = g->ms_triangles;
+  else if (std::strcmp(phase, "store") == 0) *ms = g->ms_upd_store;
   else return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_phase_ms: unknown phase");
   return HM_OK;
 }

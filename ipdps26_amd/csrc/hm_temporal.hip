@@ -1067,6 +1067,13 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
       set_error("temporal count: a tile of %d positions exceeds the LDS limit", g->tile);
       return HM_ERR_BAD_INPUT;
     }
+    // k_tcount_tiled's slot counters are 32-bit and flushed per tile: within one tile an anchor
+    // meets at most Wmax window neighbours, each with at most Tc forward entries in the tile
+    if ((unsigned long long)Wmax * (unsigned long long)Tc >= (1ull << 32)) {
+      set_error("temporal count: window of %d x tile of %d positions exceeds the bound of the "
+                "32-bit slot counters (2^32)", Wmax, Tc);
+      return HM_ERR_UNSUPPORTED;
+    }
     HM_TRY(allow_lds((const void*)k_tproj_count_tiled<MT>, sizeof(MT) * (size_t)Tp));
     HM_TRY(allow_lds((const void*)k_tproj_fill_tiled<MT>, sizeof(MT) * (size_t)Tp));
     k_tproj_count_tiled<MT><<<E, kBlock, sizeof(MT) * (size_t)Tp, st>>>(

@@ -195,3 +195,48 @@ def test_delta_beyond_one_table(gpu_ready):
         rows.extend(rows_of(ioff, inodes))
     assert np.array_equal(counts, O.static_mochy(*csr(rows)))
     g.close()
+
+
+def expected_store(rows, num_nodes):
+    """H2V (rows sorted) and V2H (hyperedge ids ascending per node) of a list of rows."""
+    sizes = np.array([r.size for r in rows], dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    nodes = np.concatenate([np.sort(r) for r in rows]).astype(np.int32)
+    eid = np.repeat(np.arange(len(rows), dtype=np.int32), sizes)
+    order = np.argsort(nodes, kind="stable")  # edge ids stay ascending within a node
+    vcnt = np.bincount(nodes, minlength=num_nodes)
+    voff = np.concatenate([[0], np.cumsum(vcnt)]).astype(np.int64)
+    return off, nodes, voff, eid[order]
+
+
+def test_full_c3_store_updates(gpu_ready):
+    """The dynamic workload at full size, store only: the 500,000-hyperedge base and its 100
+    batches of 7,000 fresh generator hyperedges and 3,000 deletes of uniformly chosen live ones.
+    (Its deltas are not counted here: the base's largest node sits in ~199k hyperedges.) After
+    batches 1, 50 and 100 the device store, H2V and V2H, equals the rows the host keeps."""
+    p = W.DYNAMIC_C3
+    off, nodes = hm.generate_static(p)
+    E = off.size - 1
+    rows = [nodes[off[e]:off[e + 1]] for e in range(E)]
+    live = np.arange(E, dtype=np.int64)
+    empty = np.zeros(0, dtype=np.int32)
+    ms_store = []
+    with hm.Graph(off, nodes) as g:
+        for b in range(1, W.DYNAMIC_C3_BATCHES + 1):
+            dels, ioff, inodes = W.dynamic_batch(live, b)
+            ids, _ = g.update(dels, ioff, inodes, delta=False)
+            ms_store.append(g.phase_ms("store"))
+            assert ids.tolist() == list(range(len(rows), len(rows) + ioff.size - 1))
+            for x in dels:
+                rows[int(x)] = empty
+            rows.extend(inodes[ioff[k]:ioff[k + 1]] for k in range(ioff.size - 1))
+            live = np.concatenate([np.setdiff1d(live, dels.astype(np.int64)), ids.astype(np.int64)])
+            if b in (1, 50, W.DYNAMIC_C3_BATCHES):
+                got = g.read_csr()
+                want = expected_store(rows, got[2].size - 1)
+                for name, x, y in zip(("offsets", "nodes", "voffsets", "vedges"), got, want):
+                    assert np.array_equal(x, y), f"batch {b}: {name}"
+                print(f"batch {b}: {len(rows)} ids, {int(got[0][-1])} incidences, store update "
+                      f"{ms_store[-1]:.3f} ms", flush=True)
+    print(f"store update per batch: mean {np.mean(ms_store):.3f} ms, max {np.max(ms_store):.3f} ms",
+          flush=True)
