@@ -399,7 +399,11 @@ typedef struct hm_server_stats {
 hm_status hm_server_create(const hm_server_config* cfg, hm_server** out);
 /* Waits for the workers to finish the queued requests and frees every graph. */
 hm_status hm_server_destroy(hm_server* s);
-/* HM_ERR_OVER_BUDGET if the graph's estimated device footprint does not fit the budget. */
+/* HM_ERR_OVER_BUDGET if the graph's estimated device footprint does not fit the budget. After every
+ * request the server charges what each graph's buffers hold (they grow with updates), so
+ * reserved_bytes follows the device memory in use; while it exceeds the budget, new graphs are
+ * refused and update requests fail with HM_ERR_OVER_BUDGET (the graph unchanged; counts still
+ * run) until a graph is removed. */
 hm_status hm_server_add_graph(hm_server* s, const int64_t* offsets, const int32_t* nodes,
                               int32_t num_edges, int32_t* graph_id);
 hm_status hm_server_remove_graph(hm_server* s, int32_t graph_id);

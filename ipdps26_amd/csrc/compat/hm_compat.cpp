@@ -273,6 +273,11 @@ void computeMotifCountsDelta(const CBSTContext& oldH2vCtx, const CBSTContext& ol
   std::vector<long long> d;
   computeMotifCountsDelta(oldH2vCtx, oldH2hCtx, newH2vCtx, newH2hCtx, deletedHyperedgeIds,
                           insertedHyperedgeIds, d);
+  for (size_t k = 0; k < d.size(); ++k)
+    if (d[k] < (long long)INT_MIN || d[k] > (long long)INT_MAX)
+      throw hmotif_compat_error("computeMotifCountsDelta: delta of bin " + std::to_string(k) + " (" +
+                                std::to_string(d[k]) +
+                                ") does not fit an int; use the std::vector<long long> overload");
   outDeltaCounts.assign(d.begin(), d.end());
 }
 
@@ -389,6 +394,10 @@ void computeTemporalMotifCountsStrictInc(const TemporalHypergraphIndex& thg, std
   }
   uint64_t out[HM_STATIC_BINS];
   check(hm_count_strict_inc(po, pn, nr, out), "hm_count_strict_inc");
+  for (int k = 0; k < HM_STATIC_BINS; ++k)
+    if (out[k] > (uint64_t)INT_MAX)
+      throw hmotif_compat_error("computeTemporalMotifCountsStrictInc: count of bin " + std::to_string(k) +
+                                " (" + std::to_string(out[k]) + ") does not fit an int");
   outCounts30.assign(out, out + HM_STATIC_BINS);
 }
 

@@ -57,6 +57,10 @@ class NoAllocScope {
   bool prev_;
 };
 
+// Device bytes g holds now: its store, count scratch, dynamic scratch and captured-update input
+// (the capacities of its buffers, headroom included). g must not be in use by another thread.
+int64_t graph_device_bytes(const hm_graph* g);
+
 // Validates a caller CSR (offsets[0] == 0, non-decreasing, node ids >= 0) and returns its rows
 // sorted with repeated nodes removed; *vmax_out = largest node id (-1 if none).
 hm_status prepare_host_rows(const int64_t* offsets, const int32_t* nodes, int32_t E,

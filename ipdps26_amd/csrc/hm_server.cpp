@@ -214,6 +214,7 @@ struct hm_server {
       }
       std::vector<hm_graph*> hs;
       for (int32_t g : gids) hs.push_back(graphs[(size_t)g].g);
+      const bool over_budget = used_bytes > cfg.hbm_budget_bytes;
       lk.unlock();
       const auto t0 = Clock::now();
       Request& r = *groups[0][0];
@@ -231,6 +232,10 @@ struct hm_server {
         uint64_t out[HM_STATIC_BINS];
         st = hm_count_static(hs[0], out);
         for (int k = 0; k < HM_STATIC_BINS; ++k) r.counts[k] = (int64_t)out[k];
+      } else if (over_budget) {
+        // the graphs' buffers have grown past the budget: no update runs until a graph is removed
+        st = HM_ERR_OVER_BUDGET;
+        err = "hm_server: device memory held by the graphs exceeds the HBM budget; remove a graph";
       } else {
         std::vector<int32_t> ids(r.ins_off.size() - 1);
         st = hm_graph_update(hs[0], r.dels.data(), (int32_t)r.dels.size(), r.ins_off.data(),
@@ -239,8 +244,18 @@ struct hm_server {
         r.first_new_id = ids.empty() ? -1 : ids[0];
       }
       if (st != HM_OK && err.empty()) err = hm_last_error();
+      // what the graphs hold now (buffers grow with updates and first counts), charged below
+      std::vector<int64_t> held;
+      for (hm_graph* h : hs) held.push_back(hm::graph_device_bytes(h));
       const auto t1 = Clock::now();
       lk.lock();
+      for (size_t k = 0; k < gids.size(); ++k) {
+        GraphSlot& gs = graphs[(size_t)gids[k]];
+        if (held[k] > gs.bytes) {
+          used_bytes += held[k] - gs.bytes;
+          gs.bytes = held[k];
+        }
+      }
       if (r.kind == HM_REQ_COUNT && st == HM_OK) {
         const double share = std::chrono::duration<double, std::milli>(t1 - t0).count() / (double)gids.size();
         for (int32_t g : gids) graphs[(size_t)g].count_ms = share;

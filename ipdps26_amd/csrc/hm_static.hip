@@ -854,6 +854,25 @@ hm_status upload_csr(hm_graph* g, const std::vector<int64_t>& h_off,
   return HM_OK;
 }
 
+int64_t graph_device_bytes(const hm_graph* g) {
+  int64_t n = 0;
+  auto add = [&](const DevBuf& b) { n += b.p ? (int64_t)b.bytes : 0; };
+  const DeviceCsr& c = g->csr;
+  for (const DevBuf* b : {&c.eoff, &c.enodes, &c.esize, &c.voff, &c.vedges}) add(*b);
+  const CountScratch& s = g->scratch;
+  for (const DevBuf* b : {&s.deg, &s.ids, &s.perm, &s.rank, &s.degr, &s.roff, &s.rlen, &s.rows, &s.szr,
+                          &s.work, &s.work2, &s.order, &s.ranks, &s.dlow, &s.acc, &s.acc2, &s.tmp,
+                          &s.vkeys, &s.vkeys2, &s.vrank, &s.uoff, &s.cur, &s.pm})
+    add(*b);
+  const DynScratch& d = g->dyn;
+  for (const DevBuf* b : {&d.deg, &d.roff, &d.rows, &d.flag, &d.ids, &d.acc, &d.tmp, &d.ins_off, &d.ins_nodes,
+                          &d.sizes, &d.noff, &d.nnodes, &d.neoff, &d.nsize, &d.inc_node, &d.inc_edge,
+                          &d.srt_node, &d.srt_edge, &d.vcnt, &d.nvoff, &d.slen, &d.soff, &d.scur, &d.spm})
+    add(*b);
+  add(g->uplan.in);
+  return n;
+}
+
 // Graph statistics of the count just made (hm_graph_get_stats).
 static void fill_count_stats(hm_graph* g, const uint64_t out[HM_STATIC_BINS], const unsigned long long* acc) {
   const int32_t E = g->csr.E;

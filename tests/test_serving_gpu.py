@@ -63,6 +63,34 @@ def test_admission_budget(gpu_ready):
             srv.submit_count(gid)
 
 
+def test_budget_charges_growth(gpu_ready):
+    """Buffers a graph grows after its admission (the store, the update's projection rows, count
+    scratch) are charged: reserved_bytes rises to what the graph holds; once that is over the
+    budget, an update fails with HM_ERR_OVER_BUDGET and leaves the graph unchanged, counts still
+    run, and removing the graph releases everything."""
+    off, nodes = hm.generate_static(W.scaled(W.STATIC_C2, 2000, 1000))
+    with hm.Server(1 << 30, 1) as probe:
+        probe.add_graph(off, nodes)
+        est = probe.stats()["reserved_bytes"]
+    budget = est + (1 << 20)
+    ins_off, ins_nodes = hm.generate_static(W.scaled(W.STATIC_C2, 6000, 1000, seed=99))
+    with hm.Server(budget, 1) as srv, hm.Graph(off, nodes) as g:
+        gid = srv.add_graph(off, nodes)
+        r = srv.wait(srv.submit_update(gid, [], ins_off, ins_nodes))
+        _, d = g.update([], ins_off, ins_nodes)
+        assert np.array_equal(r["counts"], d)
+        held = srv.stats()["reserved_bytes"]
+        print(f"estimate {est} B, budget {budget} B, held after the update {held} B", flush=True)
+        assert held > budget
+        with pytest.raises(hm.HMotifError) as ei:
+            srv.wait(srv.submit_update(gid, [0], np.zeros(1, np.int64), np.zeros(0, np.int32)))
+        assert ei.value.status == 2
+        c = srv.wait(srv.submit_count(gid))["counts"]
+        assert np.array_equal(c, g.count_static().astype(np.int64))  # the failed update changed nothing
+        srv.remove_graph(gid)
+        assert srv.stats()["reserved_bytes"] == 0
+
+
 def test_poll_until_done(gpu_ready):
     graphs = W.serving_graphs(3, lo=300, hi=600, seed=8)
     with hm.Server(1 << 30, 2) as srv:
