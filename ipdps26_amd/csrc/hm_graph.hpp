@@ -142,7 +142,7 @@ struct CountScratch {
 struct DynScratch {
   DevBuf deg, roff, rows, flag, ids, acc, tmp;
   DevBuf ins_off, ins_nodes, sizes, noff, nnodes, neoff, nsize, inc_node, inc_edge, srt_node,
-      srt_edge, vcnt, nvoff;
+      srt_edge, nvoff;
   DevBuf slen, soff, scur, spm;  // tiled delta: per (changed f, neighbour b) read position and mask
   // deg / roff / rows hold the full projection rows of the graph as it is now, built with tile
   // rows_tile (0 = one table). Cleared whenever the store changes.
@@ -153,7 +153,7 @@ struct DynScratch {
     rows_valid = false;
     for (DevBuf* b : {&deg, &roff, &rows, &flag, &ids, &acc, &tmp, &ins_off, &ins_nodes, &sizes,
                       &noff, &nnodes, &neoff, &nsize, &inc_node, &inc_edge, &srt_node, &srt_edge,
-                      &vcnt, &nvoff, &slen, &soff, &scur, &spm})
+                      &nvoff, &slen, &soff, &scur, &spm})
       b->release();
   }
 };
