@@ -20,7 +20,6 @@ constexpr int64_t kMaxPackedId = (int64_t)1 << (32 - kCntBits);
 
 // 64-bit masks use unsigned long long (the type of the 64-bit LDS atomics).
 using mask64_t = unsigned long long;
-__device__ inline int popc(uint8_t m) { return __popc((uint32_t)m); }
 __device__ inline int popc(uint32_t m) { return __popc(m); }
 __device__ inline int popc(mask64_t m) { return __popcll(m); }
 
@@ -91,14 +90,7 @@ __device__ inline void build_masks(MT* M, int32_t E, int32_t a, const uint32_t* 
     const uint32_t ks = voff[v], kt = voff[v + 1];
     for (uint32_t k = ks + threadIdx.x; k < kt; k += blockDim.x) {
       const int32_t x = vedges[k];
-      if (x == a) continue;
-      const int32_t y = rank ? rank[x] : x;
-      if constexpr (sizeof(MT) == 1) {  // 8-bit table: OR into the byte through its aligned word
-        uint32_t* word = reinterpret_cast<uint32_t*>(reinterpret_cast<uintptr_t>(M + y) & ~(uintptr_t)3);
-        atomicOr(word, (uint32_t)bit << (8 * (y & 3)));
-      } else {
-        atomicOr(&M[y], bit);
-      }
+      if (x != a) atomicOr(&M[rank ? rank[x] : x], bit);
     }
   }
   __syncthreads();

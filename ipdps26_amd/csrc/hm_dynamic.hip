@@ -331,13 +331,6 @@ __device__ inline void delta_body(
   if (threadIdx.x == 0 && s_scan) atomicAdd(&acc[kDynAccEntries], s_scan);
 }
 
-// Changed hyperedges of at most kDeltaByteNodes nodes can use an 8-bit mask table (2 B of LDS
-// per id with the size byte instead of 5): where a 32-bit table leaves one 16-wave workgroup per
-// CU, the 8-bit one leaves two. `sel` splits the changed list between two launches: 0 = every f,
-// 1 = f with at most kDeltaByteNodes nodes, 2 = the others (a workgroup of the other class leaves
-// at once).
-constexpr int kDeltaByteNodes = 8;
-
 template <typename MT, int NW, int kDeltaLoads>
 __global__ __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(kDeltaWavesPerEu(kDeltaLoads)))) void k_delta(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
@@ -345,15 +338,11 @@ __global__ __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(kDel
     const int32_t* __restrict__ vedges, const unsigned long long* __restrict__ roff,
     const unsigned long long* __restrict__ rlen, const uint32_t* __restrict__ rows,
     const int32_t* __restrict__ changed, const uint8_t* __restrict__ flag, int32_t parts,
-    long long sign, int32_t sel, unsigned long long* __restrict__ acc) {
+    long long sign, unsigned long long* __restrict__ acc) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int32_t f = changed[blockIdx.x / parts];
-  if (sel != 0) {
-    const bool small = (int32_t)(eoff[f + 1] - eoff[f]) <= kDeltaByteNodes;
-    if (small != (sel == 1)) return;
-  }
-  delta_body<MT, NW, kDeltaLoads>(reinterpret_cast<MT*>(smem), E, f, blockIdx.x % parts, parts, eoff,
-                                  enodes, esize, voff, vedges, roff, rlen, rows, flag, sign, acc);
+  delta_body<MT, NW, kDeltaLoads>(reinterpret_cast<MT*>(smem), E, changed[blockIdx.x / parts],
+                                  blockIdx.x % parts, parts, eoff, enodes, esize, voff, vedges,
+                                  roff, rlen, rows, flag, sign, acc);
 }
 
 // ---- tiled delta: hyperedge ids beyond one LDS table ------------------------------------------
@@ -945,8 +934,19 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
     HM_HIP(hipcub::DeviceScan::ExclusiveSum(d.tmp.p, tb, d.slen.as<unsigned long long>(),
                                             d.soff.as<unsigned long long>(), n + 1, st));
     unsigned long long U = 0;
+    std::vector<unsigned long long> lens((size_t)n);
     HM_HIP(hipMemcpyAsync(&U, d.soff.as<unsigned long long>() + n, 8, hipMemcpyDeviceToHost, st));
+    HM_HIP(hipMemcpyAsync(lens.data(), d.slen.p, sizeof(unsigned long long) * (size_t)n, hipMemcpyDeviceToHost, st));
     HM_HIP(hipStreamSynchronize(st));
+    // k_delta_tiled folds its counters into int32 once per tile: workgroup (f, part) takes at
+    // most ceil(d_f / parts) neighbours b of f, each with at most T row entries inside a tile
+    unsigned long long dmax = 0;
+    for (unsigned long long x : lens) dmax = std::max(dmax, x);
+    if ((dmax / (unsigned long long)parts + 1) * (unsigned long long)T >= (1ull << 31)) {
+      set_error("delta: a changed hyperedge with %llu neighbours over %d parts x tile of %d ids "
+                "exceeds the bound of the int32 per-tile counters (2^31)", dmax, parts, T);
+      return HM_ERR_UNSUPPORTED;
+    }
     HM_TRY(d.scur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(U, 1)));
     HM_TRY(d.spm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
     const size_t lds = ((size_t)T * sizeof(MT) + 15) & ~(size_t)15;
@@ -960,25 +960,13 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
   } else {
     const size_t lds = (((size_t)E * sizeof(MT) + (size_t)E) + 15) & ~(size_t)15;
     const bool one_group = lds + kDeltaStaticLds > kDeltaOneGroupLds;
-    const size_t lds8 = ((2 * (size_t)E) + 15) & ~(size_t)15;
-    const bool split = one_group && lds8 + kDeltaStaticLds <= kDeltaOneGroupLds;
-    if (split) {  // small f: 8-bit tables, two workgroups per CU
-      auto f8 = k_delta<uint8_t, kDeltaWaves, kDeltaLoadsNarrow>;
-      HM_TRY(allow_lds((const void*)f8, lds8));
-      f8<<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds8, st>>>(
-          E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
-          c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
-          d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ids.as<int32_t>(),
-          d.flag.as<uint8_t>(), parts, sign, 1, d.acc.as<unsigned long long>());
-      HM_HIP(hipGetLastError());
-    }
     auto fn = one_group ? k_delta<MT, kDeltaWaves, kDeltaLoadsWide> : k_delta<MT, kDeltaWaves, kDeltaLoadsNarrow>;
     HM_TRY(allow_lds((const void*)fn, lds));
     fn<<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
         E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
         c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
         d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ids.as<int32_t>(),
-        d.flag.as<uint8_t>(), parts, sign, split ? 2 : 0, d.acc.as<unsigned long long>());
+        d.flag.as<uint8_t>(), parts, sign, d.acc.as<unsigned long long>());
   }
   HM_HIP(hipGetLastError());
   if (evb >= 0) HM_HIP(hipEventRecord(g->ev[evb + 2], st));

@@ -240,3 +240,23 @@ def test_full_c3_store_updates(gpu_ready):
                       f"{ms_store[-1]:.3f} ms", flush=True)
     print(f"store update per batch: mean {np.mean(ms_store):.3f} ms, max {np.max(ms_store):.3f} ms",
           flush=True)
+
+
+@pytest.mark.parametrize("edges", [14_000, 24_000])
+def test_delta_one_group_per_cu_equals_recounts(gpu_ready, edges):
+    """Graphs whose delta table leaves one 16-wave workgroup per CU (about 11.5k to 28k ids: the
+    sixteen-load form): every batch's delta equals the difference of two recounts."""
+    p = W.scaled(W.DYNAMIC_C3, edges, int(edges * 0.4))
+    off, nodes = hm.generate_static(p)
+    live = np.arange(off.size - 1, dtype=np.int64)
+    with hm.Graph(off, nodes) as g:
+        before = g.count_static().astype(np.int64)
+        for b in range(1, 4):
+            dels, ioff, inodes = W.dynamic_batch(live, b, p, inserts=210, deletes=90)
+            ids, delta = g.update(dels, ioff, inodes)
+            after = g.count_static().astype(np.int64)
+            print(f"E={edges} batch {b}: touched {g.update_stats()['removed'] + g.update_stats()['added']}",
+                  flush=True)
+            assert np.array_equal(before + delta, after), f"batch {b}"
+            before = after
+            live = np.concatenate([np.setdiff1d(live, dels.astype(np.int64)), ids.astype(np.int64)])
