@@ -958,6 +958,22 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
         d.flag.as<uint8_t>(), parts, sign, d.soff.as<unsigned long long>(), d.scur.as<uint32_t>(),
         d.spm.as<MT>(), d.acc.as<unsigned long long>());
   } else {
+    // most work first: the changed hyperedges by projected degree, descending (a hub hyperedge's
+    // workgroups scan ~1000x the rows of a small one's; launched first, they do not form the tail)
+    HM_TRY(d.slen.ensure(sizeof(unsigned long long) * ((size_t)n + 1)));
+    HM_TRY(d.soff.ensure(sizeof(unsigned long long) * ((size_t)n + 1)));
+    HM_TRY(d.ord.ensure(sizeof(int32_t) * (size_t)n));
+    k_gather_len<<<grid_for((int64_t)n + 1), kBlock, 0, st>>>(d.ids.as<int32_t>(), n, d.deg.as<unsigned long long>(),
+                                                              d.slen.as<unsigned long long>());
+    size_t tb = 0;
+    HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, d.slen.as<unsigned long long>(),
+                                                        d.soff.as<unsigned long long>(), d.ids.as<int32_t>(),
+                                                        d.ord.as<int32_t>(), n, 0, 64, st));
+    HM_TRY(d.tmp.ensure(tb));
+    tb = d.tmp.bytes;
+    HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(d.tmp.p, tb, d.slen.as<unsigned long long>(),
+                                                        d.soff.as<unsigned long long>(), d.ids.as<int32_t>(),
+                                                        d.ord.as<int32_t>(), n, 0, 64, st));
     const size_t lds = (((size_t)E * sizeof(MT) + (size_t)E) + 15) & ~(size_t)15;
     const bool one_group = lds + kDeltaStaticLds > kDeltaOneGroupLds;
     auto fn = one_group ? k_delta<MT, kDeltaWaves, kDeltaLoadsWide> : k_delta<MT, kDeltaWaves, kDeltaLoadsNarrow>;
@@ -965,7 +981,7 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
     fn<<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
         E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
         c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
-        d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ids.as<int32_t>(),
+        d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ord.as<int32_t>(),
         d.flag.as<uint8_t>(), parts, sign, d.acc.as<unsigned long long>());
   }
   HM_HIP(hipGetLastError());

@@ -144,6 +144,7 @@ struct DynScratch {
   DevBuf ins_off, ins_nodes, sizes, noff, nnodes, neoff, nsize, inc_node, inc_edge, srt_node,
       srt_edge, nvoff;
   DevBuf slen, soff, scur, spm;  // tiled delta: per (changed f, neighbour b) read position and mask
+  DevBuf ord;                    // one-table delta: changed ids, most projected degree first
   // deg / roff / rows hold the full projection rows of the graph as it is now, built with tile
   // rows_tile (0 = one table). Cleared whenever the store changes.
   bool rows_valid = false;
@@ -153,7 +154,7 @@ struct DynScratch {
     rows_valid = false;
     for (DevBuf* b : {&deg, &roff, &rows, &flag, &ids, &acc, &tmp, &ins_off, &ins_nodes, &sizes,
                       &noff, &nnodes, &neoff, &nsize, &inc_node, &inc_edge, &srt_node, &srt_edge,
-                      &nvoff, &slen, &soff, &scur, &spm})
+                      &nvoff, &slen, &soff, &scur, &spm, &ord})
       b->release();
   }
 };

@@ -867,7 +867,7 @@ int64_t graph_device_bytes(const hm_graph* g) {
   const DynScratch& d = g->dyn;
   for (const DevBuf* b : {&d.deg, &d.roff, &d.rows, &d.flag, &d.ids, &d.acc, &d.tmp, &d.ins_off, &d.ins_nodes,
                           &d.sizes, &d.noff, &d.nnodes, &d.neoff, &d.nsize, &d.inc_node, &d.inc_edge,
-                          &d.srt_node, &d.srt_edge, &d.nvoff, &d.slen, &d.soff, &d.scur, &d.spm})
+                          &d.srt_node, &d.srt_edge, &d.nvoff, &d.slen, &d.soff, &d.scur, &d.spm, &d.ord})
     add(*b);
   add(g->uplan.in);
   return n;
