@@ -223,13 +223,13 @@ __device__ inline void delta_body(
       if (lane == 0) red[q][w] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
+    if (threadIdx.x < kWave) {  // wave 0: one atomic per bin, the touched total summed first
       unsigned long long t = 0;
-      for (int q = 0; q < NW; ++q) t += red[threadIdx.x][q];
-      if (t) {
-        atomicAdd(&acc[kOpenBin0 + threadIdx.x], (unsigned long long)((long long)t * sign));
-        atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], t);
-      }
+      if (threadIdx.x < 6)
+        for (int q = 0; q < NW; ++q) t += red[threadIdx.x][q];
+      if (t) atomicAdd(&acc[kOpenBin0 + threadIdx.x], (unsigned long long)((long long)t * sign));
+      const unsigned long long tot = wave_sum(t);
+      if (threadIdx.x == 0 && tot) atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], tot);
     }
   }
   // neighbours b of f (this part's share): closed pairs (re-booked) and triples open at b.
@@ -320,13 +320,14 @@ __device__ inline void delta_body(
     }
   }
   __syncthreads();
-  if (threadIdx.x < kStaticBins) {
+  if (threadIdx.x < kWave) {  // wave 0: one atomic per bin; the touched total (two's complement,
+                              // a bin's net count may be negative) summed across the wave first
     long long s = 0;
-    for (int q = 0; q < NW; ++q) s += h[q][threadIdx.x];
-    if (s) {
-      atomicAdd(&acc[threadIdx.x], (unsigned long long)(s * sign));
-      atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], (unsigned long long)s);
-    }
+    if (threadIdx.x < kStaticBins)
+      for (int q = 0; q < NW; ++q) s += h[q][threadIdx.x];
+    if (s) atomicAdd(&acc[threadIdx.x], (unsigned long long)(s * sign));
+    const unsigned long long tot = wave_sum((unsigned long long)s);
+    if (threadIdx.x == 0 && tot) atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], tot);
   }
   if (threadIdx.x == 0 && s_scan) atomicAdd(&acc[kDynAccEntries], s_scan);
 }
@@ -578,17 +579,18 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
       }
     }
     __syncthreads();
-    if (threadIdx.x < kStaticBins) {  // flush this tile's histogram (bounds the int32 counters)
+    if (threadIdx.x < kWave) {  // flush this tile's histogram (bounds the int32 counters); the
+                                // touched total summed across wave 0 first (one atomic, not 30)
       long long sum = 0;
-      for (int w2 = 0; w2 < NW; ++w2)
-        for (int c = 0; c < kDeltaCopies; ++c) {
-          sum += h[w2][threadIdx.x][c];
-          h[w2][threadIdx.x][c] = 0;
-        }
-      if (sum) {
-        atomicAdd(&acc[threadIdx.x], (unsigned long long)(sum * sign));
-        atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], (unsigned long long)sum);
-      }
+      if (threadIdx.x < kStaticBins)
+        for (int w2 = 0; w2 < NW; ++w2)
+          for (int c = 0; c < kDeltaCopies; ++c) {
+            sum += h[w2][threadIdx.x][c];
+            h[w2][threadIdx.x][c] = 0;
+          }
+      if (sum) atomicAdd(&acc[threadIdx.x], (unsigned long long)(sum * sign));
+      const unsigned long long tot = wave_sum((unsigned long long)sum);
+      if (threadIdx.x == 0 && tot) atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], tot);
     }
     for_tile_entries(tr, sf, total, vedges, [&](int, int32_t x) { M[x - t0] = (MT)0; });
   }
@@ -958,30 +960,36 @@ static hm_status local_count(hm_graph* g, DynScratch& d, const int32_t* ids, int
         d.flag.as<uint8_t>(), parts, sign, d.soff.as<unsigned long long>(), d.scur.as<uint32_t>(),
         d.spm.as<MT>(), d.acc.as<unsigned long long>());
   } else {
-    // most work first: the changed hyperedges by projected degree, descending (a hub hyperedge's
-    // workgroups scan ~1000x the rows of a small one's; launched first, they do not form the tail)
-    HM_TRY(d.slen.ensure(sizeof(unsigned long long) * ((size_t)n + 1)));
-    HM_TRY(d.soff.ensure(sizeof(unsigned long long) * ((size_t)n + 1)));
-    HM_TRY(d.ord.ensure(sizeof(int32_t) * (size_t)n));
-    k_gather_len<<<grid_for((int64_t)n + 1), kBlock, 0, st>>>(d.ids.as<int32_t>(), n, d.deg.as<unsigned long long>(),
-                                                              d.slen.as<unsigned long long>());
-    size_t tb = 0;
-    HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, d.slen.as<unsigned long long>(),
-                                                        d.soff.as<unsigned long long>(), d.ids.as<int32_t>(),
-                                                        d.ord.as<int32_t>(), n, 0, 64, st));
-    HM_TRY(d.tmp.ensure(tb));
-    tb = d.tmp.bytes;
-    HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(d.tmp.p, tb, d.slen.as<unsigned long long>(),
-                                                        d.soff.as<unsigned long long>(), d.ids.as<int32_t>(),
-                                                        d.ord.as<int32_t>(), n, 0, 64, st));
     const size_t lds = (((size_t)E * sizeof(MT) + (size_t)E) + 15) & ~(size_t)15;
     const bool one_group = lds + kDeltaStaticLds > kDeltaOneGroupLds;
+    // most work first on graphs whose table leaves one workgroup per CU: the changed hyperedges
+    // by projected degree, descending (a hub hyperedge's workgroups scan ~1000x the rows of a
+    // small one's; launched first, they do not form the tail). Small graphs keep id order: there
+    // the sort's launches cost more than the tail.
+    const int32_t* changed = d.ids.as<int32_t>();
+    if (one_group) {
+      HM_TRY(d.slen.ensure(sizeof(unsigned long long) * ((size_t)n + 1)));
+      HM_TRY(d.soff.ensure(sizeof(unsigned long long) * ((size_t)n + 1)));
+      HM_TRY(d.ord.ensure(sizeof(int32_t) * (size_t)n));
+      k_gather_len<<<grid_for((int64_t)n + 1), kBlock, 0, st>>>(d.ids.as<int32_t>(), n, d.deg.as<unsigned long long>(),
+                                                                d.slen.as<unsigned long long>());
+      size_t tb = 0;
+      HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, d.slen.as<unsigned long long>(),
+                                                          d.soff.as<unsigned long long>(), d.ids.as<int32_t>(),
+                                                          d.ord.as<int32_t>(), n, 0, 64, st));
+      HM_TRY(d.tmp.ensure(tb));
+      tb = d.tmp.bytes;
+      HM_HIP(hipcub::DeviceRadixSort::SortPairsDescending(d.tmp.p, tb, d.slen.as<unsigned long long>(),
+                                                          d.soff.as<unsigned long long>(), d.ids.as<int32_t>(),
+                                                          d.ord.as<int32_t>(), n, 0, 64, st));
+      changed = d.ord.as<int32_t>();
+    }
     auto fn = one_group ? k_delta<MT, kDeltaWaves, kDeltaLoadsWide> : k_delta<MT, kDeltaWaves, kDeltaLoadsNarrow>;
     HM_TRY(allow_lds((const void*)fn, lds));
     fn<<<(unsigned)n * (unsigned)parts, kDeltaWaves * kWave, lds, st>>>(
         E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.esize.as<uint8_t>(),
         c.voff.as<uint32_t>(), c.vedges.as<int32_t>(), d.roff.as<unsigned long long>(),
-        d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), d.ord.as<int32_t>(),
+        d.deg.as<unsigned long long>(), d.rows.as<uint32_t>(), changed,
         d.flag.as<uint8_t>(), parts, sign, d.acc.as<unsigned long long>());
   }
   HM_HIP(hipGetLastError());
