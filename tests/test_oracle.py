@@ -53,6 +53,26 @@ def test_golden_inputs_are_reproduced(tmp_path):
         assert hashlib.sha256(path.read_bytes()).hexdigest() == g["sha256_csv"], g["name"]
 
 
+def test_full_workload_goldens_match_generator(tmp_path):
+    """The two full-size oracle vectors (tests/golden/make_oracle_full.py) were computed on the
+    inputs the generator still emits: same bytes, same sizes."""
+    import hashlib
+    import json
+
+    st = json.load(open(os.path.join(GOLDEN, "static_c1_30bins.json")))
+    path = tmp_path / "c1.csv"
+    W.write_mochy_csv(str(path), *hm.generate_static(hm.GenParams(**st["params"])))
+    assert hashlib.sha256(path.read_bytes()).hexdigest() == st["sha256_csv"]
+    assert sum(st["counts30"]) > 0 and len(st["counts30"]) == 30
+    tp = json.load(open(os.path.join(GOLDEN, "temporal_c4_full.json")))
+    off, nodes, times = W.temporal_c4()
+    assert tp["params"]["num_edges"] == off.size - 1 and tp["delta"] == W.TEMPORAL_C4_DELTA_US
+    path = tmp_path / "c4.tsv"
+    W.write_thyme_tsv(str(path), off, nodes, times)
+    assert hashlib.sha256(path.read_bytes()).hexdigest() == tp["sha256_tsv"]
+    assert sum(tp["counts96"]) == tp["instances"] and len(tp["counts96"]) == 96
+
+
 # ----------------------------------------------------------------------------- static brute force
 @pytest.mark.parametrize("seed", range(12))
 def test_mochy_restatement_equals_brute_force(seed):

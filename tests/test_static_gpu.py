@@ -95,10 +95,7 @@ def test_c1_all_30_bins(gpu_ready):
     """All 30 bins of the 5,000-hyperedge static workload, including the four MoCHy does not print
     (0, 1, 4, 6), against the oracle's full-size vector (whose printed bins equal the reference
     MoCHy golden of the same input)."""
-    path = os.path.join(GOLDEN, "static_c1_30bins.json")
-    if not os.path.exists(path):
-        pytest.skip("tests/golden/static_c1_30bins.json not generated yet")
-    g = json.load(open(path))
+    g = json.load(open(os.path.join(GOLDEN, "static_c1_30bins.json")))
     off, nodes = hm.generate_static(hm.GenParams(**g["params"]))
     assert count(off, nodes).tolist() == g["counts30"]
 

@@ -135,10 +135,7 @@ def test_full_workload_equals_oracle(gpu_ready):
     """The full temporal workload (200,000 temporal hyperedges, delta = 3,600 s) bin for bin
     against the oracle's vector (THyMe+ restatement summed over anchor ranges in the build
     container; tests/golden/make_oracle_full.py)."""
-    path = os.path.join(GOLDEN, "temporal_c4_full.json")
-    if not os.path.exists(path):
-        pytest.skip("tests/golden/temporal_c4_full.json not generated yet")
-    g = json.load(open(path))
+    g = json.load(open(os.path.join(GOLDEN, "temporal_c4_full.json")))
     off, nodes, times = W.temporal_c4()
     assert g["delta"] == W.TEMPORAL_C4_DELTA_US and g["params"]["num_edges"] == off.size - 1
     got = tcount(off, nodes, times, g["delta"])
