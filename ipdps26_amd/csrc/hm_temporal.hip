@@ -17,11 +17,11 @@
 //   as (y << 7 | |x ∩ y|), ascending.
 //
 // Counting kernel (one workgroup per anchor i; every instance is counted at its newest edge i):
-//   M[y] = bitmask of i's nodes contained in y, for y in the window (LDS, dense over the window);
-//   NL = window positions with M != 0 (neighbours of i), ascending.
-//   open at i:  every ordered pair (x, y) of NL counted as if x ∩ y = ∅, from class histograms
+//   M[y] = bitmask of i's nodes contained in y, for y in the window (LDS, dense over the window),
+//   with W16[y] = (|y ∩ i|, |y \ i|); the neighbours of i are its backward row B(i).
+//   open at i:  every ordered pair (x, y) of B(i) counted as if x ∩ y = ∅, from class histograms
 //               (ordered_open_pairs: C with i, [node outside i], [C_x + C_y < |i|]);
-//   for every x in NL (one wave per x):
+//   for every x in B(i) (one wave per x):
 //     (b) y in F(x), y < i:   y a neighbour of i: the closed triangle (x, y, i) +1 in its bin,
 //                             -1 in the as-if-open bin counted above and -1 in its middle edge's
 //                             side (e) bin (below); else (x, y, i) is open at x.
@@ -194,13 +194,14 @@ __global__ __launch_bounds__(kBlock) void k_tproj_fill(
   compact_rows<MT>(M, base, base, x - 1, brows + boff[x], wcnt);
 }
 
-// Layout of the counting kernel's dynamic LDS for a window of W positions.
+// Layout of the counting kernel's dynamic LDS for a window of W positions: the anchor's masks
+// M[W] and, per position, W16 = C_yi | (s_y - C_yi) << 8 (overlap with the anchor, nodes outside
+// it).
 template <typename MT>
 struct TCountLds {
   static size_t bytes(int32_t W) {
     size_t b = ((size_t)W * sizeof(MT) + 15) & ~(size_t)15;  // M
-    b += ((size_t)W * sizeof(uint16_t) + 15) & ~(size_t)15;  // NL (relative positions)
-    b += ((size_t)W + 15) & ~(size_t)15;                     // sizes
+    b += ((size_t)W * sizeof(uint16_t) + 15) & ~(size_t)15;  // W16
     return b;
   }
 };
@@ -237,11 +238,16 @@ __device__ inline uint32_t row_load(__amdgpu_buffer_rsrc_t rs, uint32_t j) {
 // side (e) bin Y' | X' << 1 | I' << 2 | 8 | 16 by v and b2 = X' = [|y| > C_xy + C_yi]
 // (Y' = [x ⊄ y] is bit 0 or 5, I' = [i ⊄ y] bit 2 or 5). Without a triple region b is bit 2 and
 // b2 bit 1, and every pairwise-only region is non-empty. So one counter per (v, b, b2) holds all
-// three: slot = 2 ((low << 1 | b) + 16 mid) + b2 with x ∩ y ∩ i non-empty (0..255), 256 + low
-// without (256..263), low = bits 0-2 and mid = bits 3-5.
-constexpr int kTSlots = 264;
+// three: slot = b2 | b << 1 | low << 2 | mid << 5 with x ∩ y ∩ i non-empty (0..255), 264 + low
+// without (264..271), low = bits 0-2 and mid = bits 3-5 of v. Slots 256..263 count the triples
+// (x, y, i) open at x with y between x and i (y ∩ i = ∅) by X | Y << 1 | I << 2, X = [x has a
+// node in neither y nor i], Y = [y ⊄ x], I = [i ⊄ x]: Venn id X | Y << 1 | I << 2 | 8 | 32.
+constexpr int kTSlots = 272;
+constexpr int kTSlotOpenX = 256;
+constexpr int kTSlotNoTriple = 264;
 
-// Venn id, as-if-open id and side (e) id of the middle edge, of slot p.
+// Venn id, as-if-open id and side (e) id of the middle edge, of triangle slot p (p < 256 or
+// p >= kTSlotNoTriple).
 __device__ inline void tslot_bins(int p, int* v, int* as_open, int* side_e) {
   int vv, b, b2;
   if (p < 256) {
@@ -259,6 +265,54 @@ __device__ inline void tslot_bins(int p, int* v, int* as_open, int* side_e) {
   *v = vv;
   *as_open = 48 + fx + 2 * fy + 4 * b;  // as_open_id with [C_xi + C_yi < s_i] = b
   *side_e = ye | b2 << 1 | ie << 2 | 8 | 16;
+}
+
+// Adds the count s of slot p into the bin sums.
+__device__ inline void tslot_fold(int p, unsigned long long s, unsigned long long* bsum,
+                                  const int8_t* tbl) {
+  if (p >= kTSlotOpenX && p < kTSlotNoTriple) {
+    atomicAdd(&bsum[tbl[(p - kTSlotOpenX) | 8 | 32]], s);
+  } else {
+    int v, as_open, side_e;
+    tslot_bins(p, &v, &as_open, &side_e);
+    const unsigned long long neg = (unsigned long long)(-(long long)s);
+    atomicAdd(&bsum[tbl[v]], s);
+    atomicAdd(&bsum[tbl[as_open]], neg);
+    atomicAdd(&bsum[tbl[side_e]], neg);
+  }
+}
+
+// Slot of a forward entry (x, y) of anchor i from packed bytes. Each bit of the slot is [P > Q]
+// for small counts 0 <= P, Q <= 64, read as bit 7 of the byte 127 + P - Q (in 63..191, so four
+// such bytes add and subtract in one 32-bit word without carries between them):
+//   word A, bytes 0-3: x only   [xfree + g > C_xy]     word B: xy only [C_xy > g]
+//                      y only   [yfree + g > C_xy]             yi only [C_yi > g]
+//                      i only   [ifree + g > C_yi]             xi only [C_xi > g]
+//                      b        [ifree > C_yi]                 b2      [yfree > C_xy]
+// with xfree = s_x - C_xi, ifree = s_i - C_xi, yfree = s_y - C_yi and g = |x ∩ y ∩ i|. KA and
+// KB hold x's constants (bytes 127 + xfree, 127, ifree, ifree and 127, 127, 127 + C_xi, 127), y
+// comes in as W16. Two byte dot products with the slot bit weights gather the bits (times 128,
+// plus `base`); with g = 0 the slot is kTSlotNoTriple + low when y overlaps i and
+// kTSlotOpenX + (X | Y << 1 | I << 2) when it does not, which are the x-only, y-only and i-only
+// bits then (C_yi = 0), told apart by the yi-only bit. Returns 128 slot + base: the byte
+// offset of counter (slot, copy) of 32 copies per slot, copy fastest, for base = 4 copy.
+__device__ inline uint32_t tslot_offset(uint32_t KA, uint32_t KB, uint32_t w16, uint32_t cxy,
+                                        uint32_t g, uint32_t base) {
+  const uint32_t gm = g * 0x10101u;
+  const uint32_t ya = __builtin_amdgcn_perm(0u, w16, 0x0000010cu) ^ 0x7F7F0000u;  // 0, yfree, 127 - C_yi, 127 - C_yi
+  const uint32_t yb = __builtin_amdgcn_perm(0u, w16, 0x010c000cu);                // 0, C_yi, 0, yfree
+  const uint32_t c24 = __builtin_amdgcn_perm(0u, cxy, 0x000c0c0cu);               // C_xy << 24
+  const uint32_t A = (KA + ya + gm - cxy * 0x101u) & 0x80808080u;
+  const uint32_t B = (KB + yb + cxy - gm - c24) & 0x80808080u;
+  // slot bits: A -> 2, 3, 4, 1; B -> 5, 6, 7, 0 (times 128 by the byte value 128 of a set bit)
+  const uint32_t s1 = __builtin_amdgcn_udot4(
+      B, 32u | 64u << 8 | 128u << 16 | 1u << 24,
+      __builtin_amdgcn_udot4(A, 4u | 8u << 8 | 16u << 16 | 2u << 24, base, false), false);
+  // g = 0: low bits 0-2, the yi-only bit 3, from slot 256
+  const uint32_t s0 = __builtin_amdgcn_udot4(
+      B, 8u << 8, __builtin_amdgcn_udot4(A, 1u | 2u << 8 | 4u << 16, base + 128u * 256u, false),
+      false);
+  return g ? s1 : s0;
 }
 
 // Counts, for every ordered pair (x earlier, y later) of the nN neighbours of the anchor (in time
@@ -318,11 +372,13 @@ __device__ inline void ordered_open_pairs(uint32_t nN, int si, Load&& load, unsi
   for (int k = 0; k < 8; ++k) cnt[k] = wave_sum(cnt[k]);
 }
 
-// Histogram: per wave, four interleaved 32-bit copies (copy = lane & 3) updated with per-lane LDS
-// atomics; -1 contributions wrap, and a copy's value read as int32 is its exact net count (a wave
-// adds at most W^2 per anchor, W <= ~19k window entries from the LDS limit, so < 2^31). Measured on the 200k-edge temporal workload: 64-bit x 2 copies
-// 1.99 s, 32-bit x 2 1.94 s, x 4 1.58 s, x 8 2.71 s (also with 8 waves, or an odd stride);
-// copies by lane >> 4 1.93 s; ballot-aggregated bins slower still.
+// Histogram: 32 copies of the kTSlots 32-bit counters shared by the workgroup (copy = lane & 31,
+// copies of a slot adjacent), updated with per-lane LDS atomics; a counter holds counts of one
+// anchor (the workgroup adds at most W^2 per anchor, W <= ~19k window entries from the LDS
+// limit, so < 2^32). About half of the temporal workload's closed triangles share one Venn id,
+// so the copies decide the same-address conflicts: 2 copies per wave 613 ms, 16 shared copies
+// 415 ms, 32 shared copies 408 ms (34.8 KB, as for 2 per wave: two workgroups per CU).
+// The neighbours of the anchor are its backward row B(i) (ascending, with C_xi).
 template <typename MT, int NW>
 __global__ __launch_bounds__(NW * kWave) void k_tcount(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
@@ -332,169 +388,103 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount(
     const uint32_t* __restrict__ frows, const unsigned long long* __restrict__ boff,
     const unsigned long long* __restrict__ bdeg, const uint32_t* __restrict__ brows,
     const int32_t* __restrict__ order, int32_t Wmax, unsigned long long* __restrict__ acc) {
-  constexpr int kC = 2;
+  constexpr int kC = 32;  // counter copies per slot, shared by the workgroup (copy = lane & 31)
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ unsigned int hist[NW][kTSlots * kC];
+  __shared__ unsigned int hist[kTSlots * kC];
   __shared__ unsigned long long bsum[kTemporalBins];
   __shared__ int8_t tbl[128];
-  __shared__ uint32_t wcnt[NW];
-  __shared__ uint32_t s_nN, s_next;
+  __shared__ uint32_t s_next;
   const int32_t i = order[blockIdx.x];
   const int32_t base = lo[i];
   const int32_t W = i - base;
-  if (W < 2 || bdeg[i] == 0) return;
-  size_t off = ((size_t)Wmax * sizeof(MT) + 15) & ~(size_t)15;
+  const uint32_t nN = (uint32_t)bdeg[i];
+  if (W < 2 || nN == 0) return;
+  const size_t off = ((size_t)Wmax * sizeof(MT) + 15) & ~(size_t)15;
   MT* M = reinterpret_cast<MT*>(smem);
-  uint16_t* NL = reinterpret_cast<uint16_t*>(smem + off);
-  off += ((size_t)Wmax * sizeof(uint16_t) + 15) & ~(size_t)15;
-  uint8_t* sz = reinterpret_cast<uint8_t*>(smem + off);
+  uint16_t* W16 = reinterpret_cast<uint16_t*>(smem + off);
 
-  for (int k = threadIdx.x; k < NW * kTSlots * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < kTSlots * kC; k += blockDim.x) hist[k] = 0;
   for (int k = threadIdx.x; k < kTemporalBins; k += blockDim.x) bsum[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
   if (threadIdx.x == 0) s_next = 0;
-  for (int k = threadIdx.x; k < W; k += blockDim.x) sz[k] = esize[base + k];
   build_window_masks<MT>(M, i, base, i - 1, eoff, enodes, voff, vpos);
-
-  const int lane = lane_id(), w = wave_id();
-  // NL: neighbours of i inside the window, ascending (relative positions)
-  {
-    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t running = 0;
-    for (int32_t k0 = 0; k0 < W; k0 += blockDim.x) {
-      const int32_t k = k0 + (int32_t)threadIdx.x;
-      const bool hit = k < W && M[k] != (MT)0;
-      const unsigned long long bal = __ballot(hit);
-      if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
-      __syncthreads();
-      uint32_t woff = 0, tot = 0;
-      for (int q = 0; q < NW; ++q) {
-        woff += q < w ? wcnt[q] : 0;
-        tot += wcnt[q];
-      }
-      if (hit) NL[running + woff + (uint32_t)__popcll(bal & below)] = (uint16_t)k;
-      running += tot;
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) s_nN = running;
+  for (int k = threadIdx.x; k < W; k += blockDim.x) {
+    const uint32_t c = (uint32_t)popc(M[k]);
+    W16[k] = (uint16_t)(c | ((uint32_t)esize[base + k] - c) << 8);
   }
   __syncthreads();
-  const uint32_t nN = s_nN;
+
+  const int lane = lane_id(), w = wave_id();
   const int si = (int)esize[i];
-  unsigned int* hw = hist[w];
-  const int cp = lane & (kC - 1);
+  const uint32_t* NB = brows + boff[i];
+  unsigned char* hwb = reinterpret_cast<unsigned char*>(hist);
+  const uint32_t cbase = 4u * (uint32_t)(lane & (kC - 1));
   // the window tables addressed by absolute position (the window start folded into the base)
   const MT* Mabs = M - base;
-  const uint8_t* szabs = sz - base;
+  const uint16_t* Wabs = W16 - base;
 
   // open at i, counted for all ordered neighbour pairs as if disjoint (closed ones are taken
   // back below when their triangle is met)
   if (w == NW - 1) {
     unsigned long long cnt[8];
     ordered_open_pairs(nN, si, [&](uint32_t k, int& C, int& f) {
-      const int yr = NL[k];
-      C = popc(M[yr]);
-      f = (int)sz[yr] > C ? 1 : 0;
+      const uint32_t e = NB[k];
+      C = (int)(e & kCntMask);
+      f = (Wabs[e >> kCntBits] >> 8) != 0 ? 1 : 0;
     }, cnt);
     if (lane == 0)
       for (int k = 0; k < 8; ++k)
         if (cnt[k]) atomicAdd(&acc[kTAccHist + tbl[48 + k]], cnt[k]);
   }
 
-  // Triples open at a neighbour x (y overlaps x, not i) fall in one of 16 bins, fixed by the
-  // side of y (before x, or between x and i), I = [i ⊄ x], X = [x has a node in neither y nor i]
-  // and Y = [y ⊄ x]. Side (b), y between x and i: each lane counts them in registers, per x in
-  // a 64-bit word of four 16-bit fields (X + 2Y; a lane meets at most W / 64 + 1 < 2^16 entries
-  // of one row), folded into open_cnt[I][X + 2Y] after each x. Side (e), y before x: k_twedge
-  // counts every pair (y in B(x), i in F(x)) with y inside i's window by class, whether or not
-  // y overlaps i; the pairs that do are the triangles (y, x, i), met here as the forward entries
-  // of their earliest edge, and each takes its pair back through its slot (kTSlots).
-  uint32_t open_cnt[2][4] = {};  // [I][X + 2Y], side (b)
-  constexpr int kTLoads = 8;      // forward-row loads in flight per lane
+  // For every neighbour x (one wave each) the forward row of x inside (x, i): every entry y is a
+  // closed triangle (x, y, i) when y overlaps i, else a triple open at x with y between x and i;
+  // both kinds are one LDS atomic into the slot tslot_offset gives. The triples open at x with y
+  // before x (side (e)) are counted by k_twedge for every pair (y in B(x), i in F(x)) with y
+  // inside i's window, whether or not y overlaps i; the pairs that do are the triangles (y, x, i),
+  // met here as the forward entries of their earliest edge, and each takes its pair back through
+  // its slot.
+  constexpr int kTLoads = 8;  // forward-row loads in flight per lane
+  // entry e = y << kCntBits | C_xy (C_xy >= 1) is before the anchor iff e - 1 < lim; a read past
+  // the row returns 0, for which e - 1 wraps past every lim
+  const uint32_t lim = ((uint32_t)i << kCntBits) - 1u;
   for (;;) {
     uint32_t xi = 0;
     if (lane == 0) xi = atomicAdd(&s_next, 1u);
     xi = __shfl(xi, 0);
     if (xi >= nN) break;
-    const int32_t xr = NL[xi];
-    const int32_t x = base + xr;
-    const MT mx = M[xr];
-    const int sx = sz[xr];
-    const int cxi = popc(mx);
-    const int xfree = sx - cxi;  // X = [C_xy < xfree]
-    const int ifree = si - cxi;
-    unsigned long long fields_b = 0;
-    // (b) forward row of x inside (x, i): closed triangles and triples open at x. The row is
-    // ascending, so the scan stops at the first entry >= i. Rows are read through a buffer
-    // descriptor bounded by the row's length: a read past it returns 0, which no entry is (its
-    // overlap is >= 1), and (e - 1) >> kCntBits then lands past every anchor.
-    {
-      const uint32_t n = (uint32_t)fdeg[x];
-      const __amdgpu_buffer_rsrc_t rs = row_rsrc(frows + foff[x], n);
-      auto visit = [&](uint32_t e) {
-        const int32_t y = (int32_t)((e - 1u) >> kCntBits);
-        if (y >= i) return;
-        const int cxy = (int)(e & kCntMask);
-        const MT my = Mabs[y];
-        const int sy = szabs[y];
-        if (my) {
-          // Venn regions of (x, y, i) from the overlaps; every count below is >= 0
-          const int cyi = popc(my);
-          const int g = popc((MT)(mx & my));
-          const unsigned low = nz(xfree - cxy + g) | nz(sy - cxy - cyi + g) << 1 | nz(ifree - cyi + g) << 2;
-          const unsigned mid = nz(cxy - g) | nz(cyi - g) << 1 | nz(cxi - g) << 2;
-          const unsigned b = (unsigned)max(min(ifree - cyi, 1), 0);
-          const unsigned b2 = (unsigned)max(min(sy - cyi - cxy, 1), 0);
-          const unsigned slot = g ? (((low << 1 | b) + (mid << 4)) << 1 | b2) : (256u | low);
-          atomicAdd(&hw[slot * kC + cp], 1u);
-        } else {
-          // x ∩ y and x ∩ i are disjoint here, so C_xy <= xfree
-          fields_b += 1ull << (nz(xfree - cxy) << 4 | nz(sy - cxy) << 5);
-        }
-      };
-      for (uint32_t j0 = 0; j0 < n; j0 += kTLoads * kWave) {
-        uint32_t e[kTLoads];
+    const uint32_t ex = NB[xi];
+    const int32_t x = (int32_t)(ex >> kCntBits);
+    const uint32_t cxi = ex & kCntMask;
+    const MT mx = Mabs[x];
+    const uint32_t xfree = (uint32_t)Wabs[x] >> 8;
+    const uint32_t ifree = (uint32_t)si - cxi;
+    const uint32_t KA = (xfree + 127u) | 127u << 8 | ifree << 16 | ifree << 24;
+    const uint32_t KB = 127u | 127u << 8 | (127u + cxi) << 16 | 127u << 24;
+    // Rows are read through a buffer descriptor bounded by the row's length, and the scan stops
+    // at the first chunk that reaches the anchor (rows are ascending).
+    const uint32_t n = (uint32_t)fdeg[x];
+    const __amdgpu_buffer_rsrc_t rs = row_rsrc(frows + foff[x], n);
+    auto visit = [&](uint32_t e) {
+      if (e - 1u >= lim) return;
+      const uint32_t y = e >> kCntBits;
+      const uint32_t g = (uint32_t)popc((MT)(mx & Mabs[y]));
+      atomicAdd(reinterpret_cast<unsigned int*>(hwb + tslot_offset(KA, KB, Wabs[y], e & kCntMask, g, cbase)), 1u);
+    };
+    for (uint32_t j0 = 0; j0 < n; j0 += kTLoads * kWave) {
+      uint32_t e[kTLoads];
 #pragma unroll
-        for (int u = 0; u < kTLoads; ++u) e[u] = row_load(rs, j0 + lane + u * kWave);
+      for (int u = 0; u < kTLoads; ++u) e[u] = row_load(rs, j0 + lane + u * kWave);
 #pragma unroll
-        for (int u = 0; u < kTLoads; ++u) visit(e[u]);
-        // stop once these chunks reached past the anchor (or the row's end)
-        if (__ballot((int32_t)((e[kTLoads - 1] - 1u) >> kCntBits) >= i) != 0ull) break;
-      }
-    }
-    // (I is uniform in the wave; both branches index the counters with constants)
-    if (si > cxi) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) open_cnt[1][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) open_cnt[0][k] += (uint32_t)(fields_b >> (16 * k)) & 0xFFFFu;
+      for (int u = 0; u < kTLoads; ++u) visit(e[u]);
+      if (__ballot(e[kTLoads - 1] - 1u >= lim) != 0ull) break;
     }
   }
-  // open-at-x bins, side (b), time order (x, y, i): Venn id X | Y << 1 | I << 2 | 8 (xy) | 32 (ix)
-#pragma unroll
-  for (int I = 0; I < 2; ++I)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t c = wave_sum(open_cnt[I][k]);
-      const int X = k & 1, Y = k >> 1;
-      if (lane == 0 && c) atomicAdd(&bsum[tbl[X | Y << 1 | I << 2 | 8 | 32]], (unsigned long long)c);
-    }
   __syncthreads();
-  // each triangle slot: +count to its closed bin, -count to its as-if-open bin and to its middle
-  // edge's side (e) bin
   for (int p = threadIdx.x; p < kTSlots; p += blockDim.x) {
-    unsigned long long s = 0;
-    for (int q = 0; q < NW; ++q)
-      for (int c = 0; c < kC; ++c) s += hist[q][p * kC + c];
-    if (s) {
-      int v, as_open, side_e;
-      tslot_bins(p, &v, &as_open, &side_e);
-      const unsigned long long neg = (unsigned long long)(-(long long)s);
-      atomicAdd(&bsum[tbl[v]], s);
-      atomicAdd(&bsum[tbl[as_open]], neg);
-      atomicAdd(&bsum[tbl[side_e]], neg);
-    }
+    unsigned long long t = 0;
+    for (int c = 0; c < kC; ++c) t += hist[p * kC + c];
+    if (t) tslot_fold(p, t, bsum, tbl);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < kTemporalBins; b += blockDim.x)
@@ -857,7 +847,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
               const unsigned mid = nz(cxy - g) | nz(cyi - g) << 1 | nz(cxi - g) << 2;
               const unsigned b = (unsigned)max(min(ifree - cyi, 1), 0);
               const unsigned b2 = (unsigned)max(min(sy - cyi - cxy, 1), 0);
-              const unsigned slot = g ? (((low << 1 | b) + (mid << 4)) << 1 | b2) : (256u | low);
+              const unsigned slot = g ? (((low << 1 | b) + (mid << 4)) << 1 | b2) : ((unsigned)kTSlotNoTriple | low);
               atomicAdd(&hw[slot * kC + cp], 1u);
             } else {
               fields_b += 1ull << (nz(xfree - cxy) << 4 | nz(sy - cxy) << 5);
@@ -888,14 +878,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
           sum += hist[q][p * kC + c];
           hist[q][p * kC + c] = 0;
         }
-      if (sum) {
-        int v, as_open, side_e;
-        tslot_bins(p, &v, &as_open, &side_e);
-        const unsigned long long neg = (unsigned long long)(-(long long)sum);
-        atomicAdd(&bsum[tbl[v]], sum);
-        atomicAdd(&bsum[tbl[as_open]], neg);
-        atomicAdd(&bsum[tbl[side_e]], neg);
-      }
+      if (sum) tslot_fold(p, sum, bsum, tbl);
     }
     for_tile_entries(tr, si, total, vpos, [&](int, int32_t y) { M[y - t0] = (MT)0; });
   }
@@ -997,7 +980,7 @@ static size_t win_lds_bytes(int32_t R, bool wide) {
 // histograms 7.04 s, 8 waves aggregated 3.74 s, 8 waves atomics 2.46 s, 16 waves atomics 2.12 s.
 constexpr int kTCountWaves = 16;
 constexpr int kWedgeWaves = 4;  // k_twedge: one wave per middle edge
-// Static LDS of the tiled kernels (tile walk ranges, reductions; for counting also the 33.8 KB
+// Static LDS of the tiled kernels (tile walk ranges, reductions; for counting also the 34.8 KB
 // triangle-slot counters, the bin sums and the bin table), kept free next to the position table.
 constexpr size_t kTProjTiledStaticLds = 4 * 1024;
 constexpr size_t kTCountTiledStaticLds = 36 * 1024;
@@ -1055,8 +1038,7 @@ static hm_status count_temporal_impl(hm_tgraph* g, long long delta, unsigned lon
   const size_t static_cnt = sizeof(uint32_t) * 16 * kTSlots * 2 +
                             sizeof(unsigned long long) * kTemporalBins + 256;
   // one LDS table per window when it fits; position tiles otherwise (or when forced)
-  const bool tiled = g->tile > 0 || lds_proj > (size_t)lim || lds_cnt + static_cnt > (size_t)lim ||
-                     Wmax > 65535;
+  const bool tiled = g->tile > 0 || lds_proj > (size_t)lim || lds_cnt + static_cnt > (size_t)lim;
   int32_t Tp = 0, Tc = 0;
   if (tiled) {
     const int32_t most_p = (int32_t)((((size_t)lim - kTProjTiledStaticLds) / sizeof(MT)) & ~(size_t)63);
