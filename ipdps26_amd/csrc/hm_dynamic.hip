@@ -150,7 +150,7 @@ constexpr int kDeltaTiledLoads = 4;  // row chunks per step of the tiled delta
 // Slot = (low << 1 | A) + 16 mid + 16 [no triple region], low = bits 0-2, mid = bits 3-5:
 // 0..127 with a triple region, 128..143 without (mid = 7).
 constexpr int kDSlots = 144;
-constexpr int kDSlotCopies = 2;  // interleaved copies per wave (copy = lane & 1)
+constexpr int kDSlotCopies = 32;  // copies per slot shared by the workgroup (copy = lane & 31)
 
 // Venn id and as-if-open bin (at f) of slot p.
 __device__ inline void dslot_bins(int p, int* v, int* as_open) {
@@ -175,7 +175,7 @@ __device__ inline void delta_body(
   // size of x (at most 64) in bits 0..6, [x in S] in bit 7: one LDS read per row entry
   uint8_t* szf = reinterpret_cast<uint8_t*>(M + E);
   __shared__ int32_t h[NW][32];                       // open bins, then every bin
-  __shared__ uint32_t hs[NW][kDSlots * kDSlotCopies];  // closed pairs by slot
+  __shared__ uint32_t hs[kDSlots * kDSlotCopies];  // closed pairs by slot
   __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
   __shared__ unsigned long long red[6][NW];
   __shared__ int8_t tbl[128];
@@ -183,7 +183,7 @@ __device__ inline void delta_body(
   __shared__ unsigned long long s_scan;
   if (eoff[f + 1] == eoff[f]) return;
   for (int k = threadIdx.x; k < NW * 32; k += blockDim.x) (&h[0][0])[k] = 0;
-  for (int k = threadIdx.x; k < NW * kDSlots * kDSlotCopies; k += blockDim.x) (&hs[0][0])[k] = 0u;
+  for (int k = threadIdx.x; k < kDSlots * kDSlotCopies; k += blockDim.x) hs[k] = 0u;
   for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_delta_bin[threadIdx.x];
   if (threadIdx.x == 0) {
@@ -239,8 +239,7 @@ __device__ inline void delta_body(
   // folds them into open_cnt[F][X + 2Y] after each b.
   // A closed pair (b, c) is one LDS atomic into its slot (kDSlots), which fixes both its closed
   // bin (+1) and its as-if-open bin at f (-1).
-  uint32_t* hw = hs[w];
-  const int cp = lane & (kDSlotCopies - 1);
+  uint32_t* hw = hs + (lane & (kDSlotCopies - 1));
   const uint32_t* rowF = rows + roff[f];
   const uint32_t nF = (uint32_t)rlen[f];
   uint32_t open_cnt[2][4] = {};
@@ -276,7 +275,7 @@ __device__ inline void delta_body(
         const unsigned low = nz(ffree - cfc + g) | nz(bfree - cbc + g) << 1 | nz(sc - cfc - cbc + g) << 2;
         const unsigned mid = nz(cfb - g) | nz(cbc - g) << 1 | nz(cfc - g) << 2;
         const unsigned a = (unsigned)max(min(ffree - cfc, 1), 0);
-        atomicAdd(&hw[((low << 1 | a) + (mid << 4) + ((1u - nz(g)) << 4)) * kDSlotCopies + cp], 1u);
+        atomicAdd(&hw[((low << 1 | a) + (mid << 4) + ((1u - nz(g)) << 4)) * kDSlotCopies], 1u);
       }
       const unsigned long long one_o = 1ull << (16 * ((cbc < bfree ? 1 : 0) + (sc > cbc ? 2 : 0)));
       fields += mc ? 0ull : one_o;
@@ -310,8 +309,7 @@ __device__ inline void delta_body(
   __syncthreads();
   for (int p = threadIdx.x; p < kDSlots; p += blockDim.x) {  // slots into their two bins
     uint32_t n = 0;
-    for (int q = 0; q < NW; ++q)
-      for (int c = 0; c < kDSlotCopies; ++c) n += hs[q][p * kDSlotCopies + c];
+    for (int c = 0; c < kDSlotCopies; ++c) n += hs[p * kDSlotCopies + c];
     if (n) {
       int v, as_open;
       dslot_bins(p, &v, &as_open);
@@ -454,7 +452,7 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
   __shared__ int32_t h[NW][32][kDeltaCopies];
-  __shared__ uint32_t hs[NW][kDSlots * kDSlotCopies];  // closed pairs by slot, as in k_delta
+  __shared__ uint32_t hs[kDSlots * kDSlotCopies];  // closed pairs by slot, as in k_delta
   __shared__ uint32_t cls[2 * HM_MAX_EDGE_SIZE];
   __shared__ unsigned long long red[6][NW];
   __shared__ int8_t tbl[128];
@@ -468,7 +466,7 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
   const int sf = (int)(eoff[f + 1] - fs);
   if (sf == 0) return;
   for (int k = threadIdx.x; k < NW * 32 * kDeltaCopies; k += blockDim.x) (&h[0][0][0])[k] = 0;
-  for (int k = threadIdx.x; k < NW * kDSlots * kDSlotCopies; k += blockDim.x) (&hs[0][0])[k] = 0u;
+  for (int k = threadIdx.x; k < kDSlots * kDSlotCopies; k += blockDim.x) hs[k] = 0u;
   for (int k = threadIdx.x; k < 2 * HM_MAX_EDGE_SIZE; k += blockDim.x) cls[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_delta_bin[threadIdx.x];
   if ((int)threadIdx.x < sf) s_an[threadIdx.x] = enodes[fs + threadIdx.x];
@@ -476,8 +474,7 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
   for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
   const int lane = lane_id(), w = wave_id();
   int32_t* hc = &h[w][0][lane & (kDeltaCopies - 1)];
-  uint32_t* hsw = hs[w];
-  const int cps = lane & (kDSlotCopies - 1);
+  uint32_t* hsw = hs + (lane & (kDSlotCopies - 1));
   const uint32_t* rowF = rows + roff[f];
   const uint32_t nF = (uint32_t)rlen[f];
   const unsigned long long st = soff[q];
@@ -546,7 +543,7 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
                       nz(sf - cfb - cfc + g) | nz(sb - cfb - cbc + g) << 1 | nz(sc - cfc - cbc + g) << 2;
                   const unsigned mid = nz(cfb - g) | nz(cbc - g) << 1 | nz(cfc - g) << 2;
                   const unsigned a = (unsigned)max(min(sf - cfb - cfc, 1), 0);
-                  atomicAdd(&hsw[((low << 1 | a) + (mid << 4) + ((1u - nz(g)) << 4)) * kDSlotCopies + cps], 1u);
+                  atomicAdd(&hsw[((low << 1 | a) + (mid << 4) + ((1u - nz(g)) << 4)) * kDSlotCopies], 1u);
                 }
               } else {
                 atomicAdd(&hc[open_bin(sb, sf, sc, cfb, cbc) * kDeltaCopies], 1);
@@ -566,11 +563,10 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
     __syncthreads();
     for (int p = threadIdx.x; p < kDSlots; p += blockDim.x) {  // this tile's slots into their bins
       uint32_t n = 0;
-      for (int w2 = 0; w2 < NW; ++w2)
-        for (int c = 0; c < kDSlotCopies; ++c) {
-          n += hs[w2][p * kDSlotCopies + c];
-          hs[w2][p * kDSlotCopies + c] = 0u;
-        }
+      for (int c = 0; c < kDSlotCopies; ++c) {
+        n += hs[p * kDSlotCopies + c];
+        hs[p * kDSlotCopies + c] = 0u;
+      }
       if (n) {
         int v, as_open;
         dslot_bins(p, &v, &as_open);

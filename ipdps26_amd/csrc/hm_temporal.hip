@@ -766,10 +766,10 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
     const unsigned long long* __restrict__ bdeg, const uint32_t* __restrict__ brows,
     const MT* __restrict__ bmask, uint32_t* __restrict__ fcur,
     const int32_t* __restrict__ order, unsigned long long* __restrict__ acc) {
-  constexpr int kC = 2;
+  constexpr int kC = 32;  // counter copies per slot, shared by the workgroup, as in k_tcount
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
-  __shared__ unsigned int hist[NW][kTSlots * kC];
+  __shared__ unsigned int hist[kTSlots * kC];
   __shared__ unsigned long long bsum[kTemporalBins];
   __shared__ int8_t tbl[128];
   __shared__ TileRanges tr;
@@ -777,7 +777,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
   const int32_t base = lo[i];
   const uint32_t nN = (uint32_t)bdeg[i];
   if (i - base < 2 || nN == 0) return;
-  for (int k = threadIdx.x; k < NW * kTSlots * kC; k += blockDim.x) (&hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < kTSlots * kC; k += blockDim.x) hist[k] = 0;
   for (int k = threadIdx.x; k < kTemporalBins; k += blockDim.x) bsum[k] = 0;
   if (threadIdx.x < 128) tbl[threadIdx.x] = c_temporal_bin[threadIdx.x];
   for (int32_t k = threadIdx.x; k < T; k += blockDim.x) M[k] = (MT)0;
@@ -802,8 +802,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
         if (cnt[k]) atomicAdd(&acc[kTAccHist + tbl[48 + k]], cnt[k]);
   }
 
-  unsigned int* hw = hist[w];
-  const int cp = lane & (kC - 1);
+  unsigned int* hw = hist + (lane & (kC - 1));
   // triples open at x with the other edge later, per lane by (I, X + 2Y) as in k_tcount (those
   // with the other edge earlier: k_twedge, less the triangles taken back through their slots)
   uint32_t open_cnt[2][4] = {};
@@ -848,7 +847,7 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
               const unsigned b = (unsigned)max(min(ifree - cyi, 1), 0);
               const unsigned b2 = (unsigned)max(min(sy - cyi - cxy, 1), 0);
               const unsigned slot = g ? (((low << 1 | b) + (mid << 4)) << 1 | b2) : ((unsigned)kTSlotNoTriple | low);
-              atomicAdd(&hw[slot * kC + cp], 1u);
+              atomicAdd(&hw[slot * kC], 1u);
             } else {
               fields_b += 1ull << (nz(xfree - cxy) << 4 | nz(sy - cxy) << 5);
             }
@@ -873,11 +872,10 @@ __global__ __launch_bounds__(NW * kWave) void k_tcount_tiled(
     // slot counter stays far below 2^32)
     for (int p = threadIdx.x; p < kTSlots; p += blockDim.x) {
       unsigned long long sum = 0;
-      for (int q = 0; q < NW; ++q)
-        for (int c = 0; c < kC; ++c) {
-          sum += hist[q][p * kC + c];
-          hist[q][p * kC + c] = 0;
-        }
+      for (int c = 0; c < kC; ++c) {
+        sum += hist[p * kC + c];
+        hist[p * kC + c] = 0;
+      }
       if (sum) tslot_fold(p, sum, bsum, tbl);
     }
     for_tile_entries(tr, si, total, vpos, [&](int, int32_t y) { M[y - t0] = (MT)0; });
