@@ -514,6 +514,23 @@ def run_serving(slots: int = 4, n_requests: int = 400, batch_graphs: int = 16) -
     note(f"small graphs saturated: plain {s_plain['requests_per_s']:.0f} req/s, served "
          f"{s_served['requests_per_s']:.0f}, served with batched counts {s_batched['requests_per_s']:.0f}")
     same = same and same_as(s_plain, s_served) and same_as(s_plain, s_batched)
+    # the full configuration's own graphs up to 30k hyperedges, at their sizes
+    band = W.serving_c5_band()
+    breqs = W.serving_requests(band, 48, seed=5600)
+    b_plain = serving.run_plain(band, breqs, realtime=False)
+    b_served = serving.run_served(band, breqs, realtime=False, slots=slots)
+    b_kept = serving.run_served(band, breqs, realtime=False, slots=slots, maintain=True)
+    note(f"c5 graphs up to {W.SERVING_BAND_MAX} hyperedges saturated: plain {b_plain['requests_per_s']:.2f} "
+         f"req/s, served {b_served['requests_per_s']:.2f}, kept counts {b_kept['requests_per_s']:.2f}")
+    brate = 0.7 * b_plain["requests_per_s"]
+    btimed = W.serving_requests(band, 48, seed=5600, rate_per_s=brate)
+    b_plain_rt = serving.run_plain(band, btimed, realtime=True)
+    b_kept_rt = serving.run_served(band, btimed, realtime=True, slots=slots, maintain=True)
+    note(f"c5 graphs up to {W.SERVING_BAND_MAX} hyperedges at {brate:.2f} req/s: p50/p99 plain "
+         f"{b_plain_rt['p50_ms']:.0f}/{b_plain_rt['p99_ms']:.0f} ms, kept counts "
+         f"{b_kept_rt['p50_ms']:.0f}/{b_kept_rt['p99_ms']:.0f} ms")
+    same = (same and same_as(b_plain, b_served) and same_as(b_plain, b_kept)
+            and same_as(b_plain_rt, b_kept_rt))
     triples = serving.count_triples(reqs, served["results"])
     drop = ("results",)
 
@@ -543,7 +560,17 @@ def run_serving(slots: int = 4, n_requests: int = 400, batch_graphs: int = 16) -
                                     "seed 55; 2,000 requests (seed 5500), all queued at t=0",
                           "plain": strip(s_plain), "served": strip(s_served),
                           "served_batched_counts_64": strip(s_batched),
-                          "speedup_batched_counts": s_batched["requests_per_s"] / s_plain["requests_per_s"]}}}
+                          "speedup_batched_counts": s_batched["requests_per_s"] / s_plain["requests_per_s"]},
+                      "c5_graphs_at_size": {
+                          "graphs": f"the {len(band)} graphs of the full serving configuration (64 graphs, "
+                                    f"E log-uniform on [10k, 1M], seed 5) with at most {W.SERVING_BAND_MAX} "
+                                    "hyperedges, each at its own size; 48 requests (seed 5600)",
+                          "hyperedges": [int(off.size - 1) for off, _ in band],
+                          "saturated": {"plain": strip(b_plain), "served": strip(b_served),
+                                        "served_kept_counts": strip(b_kept),
+                                        "speedup_kept_counts": b_kept["requests_per_s"] / b_plain["requests_per_s"]},
+                          "poisson_at_70pct_plain_capacity": {"rate_per_s": brate, "plain": strip(b_plain_rt),
+                                                              "served_kept_counts": strip(b_kept_rt)}}}}
 
 
 def main() -> int:

@@ -82,16 +82,32 @@ def serving_c5_sizes(n_graphs: int = SERVING_C5_GRAPHS, lo: int = 10_000, hi: in
 SERVING_REDUCED_LO, SERVING_REDUCED_HI = 1_000, 6_000
 
 
+def _serving_graph(e: int, i: int, seed: int):
+    """Graph i of a serving set: the static_c2 family with E = e, V = E / 2, seed * 1000 + i."""
+    p = GenParams(num_nodes=max(e // 2, STATIC_C2.kmax), num_edges=e, alpha=STATIC_C2.alpha,
+                  kmin=STATIC_C2.kmin, kmax=STATIC_C2.kmax, zipf_s=STATIC_C2.zipf_s,
+                  seed=seed * 1000 + i)
+    return generate_static(p)
+
+
 def serving_graphs(n_graphs: int = SERVING_C5_GRAPHS, lo: int = SERVING_REDUCED_LO,
                    hi: int = SERVING_REDUCED_HI, seed: int = 5):
     """n_graphs hypergraphs of the static_c2 family, E_i log-uniform on [lo, hi], V_i = E_i / 2."""
-    out = []
-    for i, e in enumerate(serving_c5_sizes(n_graphs, lo, hi, seed)):
-        p = GenParams(num_nodes=max(e // 2, STATIC_C2.kmax), num_edges=e, alpha=STATIC_C2.alpha,
-                      kmin=STATIC_C2.kmin, kmax=STATIC_C2.kmax, zipf_s=STATIC_C2.zipf_s,
-                      seed=seed * 1000 + i)
-        out.append(generate_static(p))
-    return out
+    return [_serving_graph(e, i, seed) for i, e in enumerate(serving_c5_sizes(n_graphs, lo, hi, seed))]
+
+
+# Largest hyperedge count of the full serving configuration's graphs served at their own size.
+SERVING_BAND_MAX = 30_000
+
+
+def serving_c5_band(max_edges: int = SERVING_BAND_MAX, n_graphs: int = SERVING_C5_GRAPHS,
+                    seed: int = 5):
+    """The graphs of the full serving configuration (64 graphs, E log-uniform on [10k, 1M], seed 5)
+    that hold at most max_edges hyperedges, each at its own size and seed (16 graphs of 10,055 to
+    29,675 hyperedges for the defaults): the part of that configuration exact per-request counting
+    reaches (a 30k-hyperedge graph of the family holds ~1e12 triples)."""
+    sizes = serving_c5_sizes(n_graphs, 10_000, 1_000_000, seed)
+    return [_serving_graph(e, i, seed) for i, e in enumerate(sizes) if e <= max_edges]
 
 
 def serving_requests(graphs, n_requests: int, seed: int = 5000, count_frac: float = 0.5,
