@@ -57,6 +57,24 @@ __device__ inline void wave_hist_add(CT* h, int bin, bool valid, CT delta) {
   }
 }
 
+// Buffer descriptor over n packed row entries at p (raw, no stride; word 3 as for gfx9 raw
+// buffers): a read at entry index >= n returns 0. Indices passed to row_load must be
+// non-negative and small enough that 4 * index does not wrap.
+// p and n must be the same on every active lane (one row per wave); they are read from the
+// first lane so that the descriptor sits in scalar registers.
+__device__ inline __amdgpu_buffer_rsrc_t row_rsrc(const uint32_t* p, uint32_t n) {
+  const unsigned long long a = (unsigned long long)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  const uint32_t nn = __builtin_amdgcn_readfirstlane(n);
+  uint32_t* q = reinterpret_cast<uint32_t*>((unsigned long long)hi << 32 | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(q, (short)0, (int)(nn * 4u), 0x00020000);
+}
+
+__device__ inline uint32_t row_load(__amdgpu_buffer_rsrc_t rs, uint32_t j) {
+  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(j * 4u), 0, 0);
+}
+
 // Venn-region emptiness id of (x1, x2, x3) (MoCHy-master/motif_id.cpp:24-34):
 // bit0..2 = x1/x2/x3-only regions, bit3 = x1x2-only, bit4 = x2x3-only, bit5 = x3x1-only, bit6 = all.
 // [r > 0] of a region count r >= 0 (one unsigned min)

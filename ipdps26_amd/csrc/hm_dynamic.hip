@@ -280,16 +280,19 @@ __device__ inline void delta_body(
       const unsigned long long one_o = 1ull << (16 * ((cbc < bfree ? 1 : 0) + (sc > cbc ? 2 : 0)));
       fields += mc ? 0ull : one_o;
     };
-    // kDeltaLoads row loads in flight per lane before any is used (lane l reads l, l + 64, ...)
+    // kDeltaLoads row loads in flight per lane before any is used (lane l reads l, l + 64, ...),
+    // through a buffer descriptor: loads that wait only on the vector-memory counter (a generic
+    // load also holds the LDS counter, so the first LDS read would wait for all of them)
+    const __amdgpu_buffer_rsrc_t rs = row_rsrc(rowB, nB);
     uint32_t j = lane;
     for (; j + (kDeltaLoads - 1) * kWave < nB; j += kDeltaLoads * kWave) {
       uint32_t e[kDeltaLoads];
 #pragma unroll
-      for (int u = 0; u < kDeltaLoads; ++u) e[u] = rowB[j + u * kWave];
+      for (int u = 0; u < kDeltaLoads; ++u) e[u] = row_load(rs, j + u * kWave);
 #pragma unroll
       for (int u = 0; u < kDeltaLoads; ++u) visit(e[u]);
     }
-    for (; j < nB; j += kWave) visit(rowB[j]);
+    for (; j < nB; j += kWave) visit(row_load(rs, j));
     // (F is uniform in the wave; both branches index the counters with constants)
     if (sf > cfb) {
 #pragma unroll

@@ -212,24 +212,6 @@ __device__ inline int as_open_id(int fx, int fy, int cxi, int cyi, int si) {
   return 48 + fx + 2 * fy + 4 * (int)(cxi + cyi < si);
 }
 
-// Buffer descriptor over n packed row entries at p (raw, no stride; word 3 as for gfx9 raw
-// buffers): a read at entry index >= n returns 0. Indices passed to row_load must be
-// non-negative and small enough that 4 * index does not wrap.
-// p and n must be the same on every active lane (one row per wave); they are read from the
-// first lane so that the descriptor sits in scalar registers.
-__device__ inline __amdgpu_buffer_rsrc_t row_rsrc(const uint32_t* p, uint32_t n) {
-  const unsigned long long a = (unsigned long long)p;
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-  const uint32_t nn = __builtin_amdgcn_readfirstlane(n);
-  uint32_t* q = reinterpret_cast<uint32_t*>((unsigned long long)hi << 32 | lo);
-  return __builtin_amdgcn_make_buffer_rsrc(q, (short)0, (int)(nn * 4u), 0x00020000);
-}
-
-__device__ inline uint32_t row_load(__amdgpu_buffer_rsrc_t rs, uint32_t j) {
-  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(j * 4u), 0, 0);
-}
-
 // Slots of closed triangles (x, y, i) in time order. A triangle adds +1 to the bin of its Venn id
 // v (bits: 0-2 the private regions of x, y, i; 3-5 the pairwise-only regions xy, yi, xi; 6 the
 // triple region), -1 to its as-if-open bin (the pair x, y was counted as open at i) and -1 to
