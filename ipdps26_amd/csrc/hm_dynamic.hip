@@ -147,20 +147,45 @@ constexpr int kDeltaTiledLoads = 4;  // row chunks per step of the tiled delta
 // bits 0-2 private regions, 3-5 pairwise-only regions fb, bc, fc, 6 the triple region). Its
 // as-if-open bin at f, 20 + A + 2 ([b ⊄ f] + [c ⊄ f]), is fixed by v ([b ⊄ f] is bit 1 or 4,
 // [c ⊄ f] bit 2 or 4) and A = [|f| > C_fb + C_fc], which is bit 0 without a triple region.
-// Slot = (low << 1 | A) + 16 mid + 16 [no triple region], low = bits 0-2, mid = bits 3-5:
-// 0..127 with a triple region, 128..143 without (mid = 7).
-constexpr int kDSlots = 144;
+// Slot = (low << 1 | A) + 16 mid, low = bits 0-2, mid = bits 3-5: 0..127 with a triple region,
+// 144 + (low << 1 | A) without (mid = 7). Slots 128..143 count the triples (f, b, c) open at b
+// (c ∩ f = ∅) by F | F << 1 | X << 2 | Y << 3, bin 20 + X + 2 (F + Y): F = [f ⊄ b],
+// X = [b has a node in neither f nor c], Y = [c ⊄ b].
+constexpr int kDSlots = 160;
+constexpr int kDSlotOpenB = 128;
+constexpr int kDSlotNoTriple = 144;
 constexpr int kDSlotCopies = 32;  // copies per slot shared by the workgroup (copy = lane & 31)
 
-// Venn id and as-if-open bin (at f) of slot p.
+// Venn id and as-if-open bin (at f) of closed-pair slot p (p < 128 or p >= kDSlotNoTriple).
 __device__ inline void dslot_bins(int p, int* v, int* as_open) {
-  const int q = p < 128 ? p : p - 128;
+  const int q = p < 128 ? p : p - kDSlotNoTriple;
   const int a = q & 1, low = (q >> 1) & 7;
   const int vv = p < 128 ? (low | (p >> 4) << 3 | 64) : (low | 56);
   const int bf = ((vv >> 1) & 1) | ((vv >> 4) & 1), cf = ((vv >> 2) & 1) | ((vv >> 4) & 1);
   *v = vv;
   *as_open = kOpenBin0 + a + 2 * (bf + cf);
 }
+// Byte offset of the counter (slot, copy) of a row entry (b, c) of neighbour b of f, for 32
+// copies per slot (copy fastest) and base = 4 copy. Each slot bit is [P > Q] over counts 0..64,
+// bit 7 of the byte 127 + P - Q; word A holds f only [ffree + g > C_fc], b only
+// [bfree + g > C_bc], c only [s_c + g > C_fc + C_bc] and A [ffree > C_fc], word B fb only
+// [C_fb > g], bc only [C_bc > g], fc only [C_fc > g] and a constant 128 (ffree = |f| - C_fb,
+// bfree = |b| - C_fb, g = |f ∩ b ∩ c|). KA holds b's constants (127 + ffree, 127 + bfree, 127,
+// 127 + ffree) and KB (127 + C_fb, 127, 127, 128). Byte dot products with the slot bit weights
+// gather the bits (times 128): with g = 0 the slot is kDSlotNoTriple + (low << 1 | A) for c
+// overlapping f and kDSlotOpenB + (F | F << 1 | X << 2 | Y << 3) for c not (its f-only and A bits
+// are then F, its b-only bit X, its c-only bit Y), told apart by the fc-only bit.
+__device__ inline uint32_t dslot_offset(uint32_t KA, uint32_t KB, uint32_t cfc, uint32_t sc,
+                                        uint32_t cbc, uint32_t g, uint32_t base) {
+  const uint32_t gm = g * 0x10101u;
+  const uint32_t c24 = __builtin_amdgcn_perm(0u, cfc, 0x000c0c0cu);  // C_fc << 24
+  const uint32_t A = (KA + (sc << 16) + gm - cfc * 0x10001u - c24 - cbc * 0x10100u) & 0x80808080u;
+  const uint32_t B = (KB + (cbc << 8) + (cfc << 16) - gm) & 0x80808080u;
+  const uint32_t sA = __builtin_amdgcn_udot4(A, 2u | 4u << 8 | 8u << 16 | 1u << 24, base, false);
+  return g ? __builtin_amdgcn_udot4(B, 16u | 32u << 8 | 64u << 16, sA, false)
+           : __builtin_amdgcn_udot4(B, 16u << 16 | 128u << 24, sA, false);
+}
+
 constexpr size_t kDeltaTiledStaticLds = 40 * 1024;  // k_delta_tiled: histograms, classes, tables
 constexpr size_t kDeltaStaticLds = 24 * 1024;       // k_delta: slot counters, bins, classes, tables
 
@@ -172,7 +197,8 @@ __device__ inline void delta_body(
     const unsigned long long* __restrict__ roff, const unsigned long long* __restrict__ rlen,
     const uint32_t* __restrict__ rows, const uint8_t* __restrict__ flag, long long sign,
     unsigned long long* __restrict__ acc) {
-  // size of x (at most 64) in bits 0..6, [x in S] in bit 7: one LDS read per row entry
+  // size of x (at most 64) in bits 0..6, bit 7 set for x = f and for an x of S before f (their
+  // triples with f belong to an earlier changed hyperedge): one LDS read per row entry
   uint8_t* szf = reinterpret_cast<uint8_t*>(M + E);
   __shared__ int32_t h[NW][32];                       // open bins, then every bin
   __shared__ uint32_t hs[kDSlots * kDSlotCopies];  // closed pairs by slot
@@ -190,7 +216,8 @@ __device__ inline void delta_body(
     s_next = 0;
     s_scan = 0;
   }
-  for (int32_t x = threadIdx.x; x < E; x += blockDim.x) szf[x] = (uint8_t)(esize[x] | (flag[x] ? 0x80 : 0));
+  for (int32_t x = threadIdx.x; x < E; x += blockDim.x)
+    szf[x] = (uint8_t)(esize[x] | ((flag[x] && x < f) || x == f ? 0x80 : 0));
   build_masks<MT>(M, E, f, eoff, enodes, voff, vedges, nullptr);
   const int sf = (int)(eoff[f + 1] - eoff[f]);
   const int lane = lane_id(), w = wave_id();
@@ -200,7 +227,7 @@ __device__ inline void delta_body(
     for (int32_t x = threadIdx.x; x < E; x += blockDim.x) {
       const MT m = M[x];
       const int zx = szf[x];
-      if (m && (!(zx & 0x80) || x > f)) {
+      if (m && !(zx & 0x80)) {
         const int C = popc(m);
         atomicAdd(&cls[2 * (C - 1) + ((zx & 0x7F) > C ? 1 : 0)], 1u);
       }
@@ -232,17 +259,13 @@ __device__ inline void delta_body(
       if (threadIdx.x == 0 && tot) atomicAdd(&acc[sign < 0 ? kDynAccRemoved : kDynAccAdded], tot);
     }
   }
-  // neighbours b of f (this part's share): closed pairs (re-booked) and triples open at b.
-  // A triple open at b (c overlaps b, not f) has bin 20 + X + 2 (F + Y) with F = [f ⊄ b] (fixed
-  // per b), X = [b has a node in neither f nor c] and Y = [c ⊄ b]; each lane counts them per b in
-  // four 16-bit fields of one word (a lane meets at most d_b / 64 + 1 < 2^16 entries of a row) and
-  // folds them into open_cnt[F][X + 2Y] after each b.
-  // A closed pair (b, c) is one LDS atomic into its slot (kDSlots), which fixes both its closed
-  // bin (+1) and its as-if-open bin at f (-1).
-  uint32_t* hw = hs + (lane & (kDSlotCopies - 1));
+  // neighbours b of f (this part's share): closed pairs (b, c), b < c, re-booked, and triples
+  // open at b (c overlaps b, not f); each is one LDS atomic into its slot (dslot_offset), which
+  // fixes the closed bin (+1) and the as-if-open bin at f (-1) of a closed pair, or the open bin.
+  unsigned char* hwb = reinterpret_cast<unsigned char*>(hs);
+  const uint32_t cbase = 4u * (uint32_t)(lane & (kDSlotCopies - 1));
   const uint32_t* rowF = rows + roff[f];
   const uint32_t nF = (uint32_t)rlen[f];
-  uint32_t open_cnt[2][4] = {};
   for (;;) {
     uint32_t k = 0;
     if (lane == 0) k = atomicAdd(&s_next, 1u);
@@ -250,35 +273,24 @@ __device__ inline void delta_body(
     const uint32_t ib = part + k * (uint32_t)parts;
     if (ib >= nF) break;
     const int32_t b = (int32_t)(rowF[ib] >> kCntBits);
-    const int zb = szf[b];
-    if ((zb & 0x80) && b < f) continue;
+    const uint32_t zb = szf[b];
+    if (zb & 0x80u) continue;
     const MT mb = M[b];
-    const int cfb = popc(mb);
-    const int sb = zb & 0x7F;
-    const int bfree = sb - cfb;  // X = [C_bc < bfree]
-    const int ffree = sf - cfb;  // A = [C_fc < ffree]
+    const uint32_t cfb = (uint32_t)popc(mb);
+    const uint32_t ffree = (uint32_t)sf - cfb, bfree = zb - cfb;
+    const uint32_t KA = (127u + ffree) | (127u + bfree) << 8 | 127u << 16 | (127u + ffree) << 24;
+    const uint32_t KB = (127u + cfb) | 127u << 8 | 127u << 16 | 128u << 24;
     const uint32_t* rowB = rows + roff[b];
     const uint32_t nB = (uint32_t)rlen[b];
     if (lane == 0) atomicAdd(&s_scan, (unsigned long long)nB);
-    unsigned long long fields = 0;
     auto visit = [&](uint32_t e) {
-      const int32_t c = (int32_t)(e >> kCntBits);
-      const int cbc = (int)(e & kCntMask);
-      const int zc = szf[c];
-      if (c == f || ((zc & 0x80) && c < f)) return;
+      const uint32_t c = e >> kCntBits;
+      const uint32_t zc = szf[c];
       const MT mc = M[c];
-      const int sc = zc & 0x7F;
-      const int cfc = popc(mc);
-      if (mc && c > b) {
-        // Venn regions of (f, b, c) from the overlaps; every count below is >= 0
-        const int g = popc((MT)(mb & mc));
-        const unsigned low = nz(ffree - cfc + g) | nz(bfree - cbc + g) << 1 | nz(sc - cfc - cbc + g) << 2;
-        const unsigned mid = nz(cfb - g) | nz(cbc - g) << 1 | nz(cfc - g) << 2;
-        const unsigned a = (unsigned)max(min(ffree - cfc, 1), 0);
-        atomicAdd(&hw[((low << 1 | a) + (mid << 4) + ((1u - nz(g)) << 4)) * kDSlotCopies], 1u);
-      }
-      const unsigned long long one_o = 1ull << (16 * ((cbc < bfree ? 1 : 0) + (sc > cbc ? 2 : 0)));
-      fields += mc ? 0ull : one_o;
+      if ((zc & 0x80u) || (mc && c < (uint32_t)b)) return;
+      atomicAdd(reinterpret_cast<uint32_t*>(hwb + dslot_offset(KA, KB, (uint32_t)popc(mc), zc, e & kCntMask,
+                                                               (uint32_t)popc((MT)(mb & mc)), cbase)),
+                1u);
     };
     // kDeltaLoads row loads in flight per lane before any is used (lane l reads l, l + 64, ...),
     // through a buffer descriptor: loads that wait only on the vector-memory counter (a generic
@@ -293,27 +305,16 @@ __device__ inline void delta_body(
       for (int u = 0; u < kDeltaLoads; ++u) visit(e[u]);
     }
     for (; j < nB; j += kWave) visit(row_load(rs, j));
-    // (F is uniform in the wave; both branches index the counters with constants)
-    if (sf > cfb) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) open_cnt[1][q] += (uint32_t)(fields >> (16 * q)) & 0xFFFFu;
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) open_cnt[0][q] += (uint32_t)(fields >> (16 * q)) & 0xFFFFu;
-    }
   }
-#pragma unroll
-  for (int F = 0; F < 2; ++F)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t c = wave_sum(open_cnt[F][q]);
-      if (lane == 0 && c) atomicAdd(&h[w][kOpenBin0 + (q & 1) + 2 * (F + (q >> 1))], (int32_t)c);
-    }
   __syncthreads();
-  for (int p = threadIdx.x; p < kDSlots; p += blockDim.x) {  // slots into their two bins
+  for (int p = threadIdx.x; p < kDSlots; p += blockDim.x) {  // slots into their bins
     uint32_t n = 0;
     for (int c = 0; c < kDSlotCopies; ++c) n += hs[p * kDSlotCopies + c];
-    if (n) {
+    if (!n) continue;
+    if (p >= kDSlotOpenB && p < kDSlotNoTriple) {
+      const int q = p - kDSlotOpenB;  // F | F << 1 | X << 2 | Y << 3
+      atomicAdd(&h[0][kOpenBin0 + ((q >> 2) & 1) + 2 * ((q & 1) + ((q >> 3) & 1))], (int32_t)n);
+    } else {
       int v, as_open;
       dslot_bins(p, &v, &as_open);
       atomicAdd(&h[0][tbl[v]], (int32_t)n);
@@ -546,7 +547,7 @@ __global__ __launch_bounds__(NW * kWave) void k_delta_tiled(
                       nz(sf - cfb - cfc + g) | nz(sb - cfb - cbc + g) << 1 | nz(sc - cfc - cbc + g) << 2;
                   const unsigned mid = nz(cfb - g) | nz(cbc - g) << 1 | nz(cfc - g) << 2;
                   const unsigned a = (unsigned)max(min(sf - cfb - cfc, 1), 0);
-                  atomicAdd(&hsw[((low << 1 | a) + (mid << 4) + ((1u - nz(g)) << 4)) * kDSlotCopies], 1u);
+                  atomicAdd(&hsw[((low << 1 | a) + (mid << 4) + ((1u - nz(g)) << 5)) * kDSlotCopies], 1u);
                 }
               } else {
                 atomicAdd(&hc[open_bin(sb, sf, sc, cfb, cbc) * kDeltaCopies], 1);
