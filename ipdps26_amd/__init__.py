@@ -70,6 +70,20 @@ class _GraphStats(ctypes.Structure):
     ]
 
 
+class _HubStats(ctypes.Structure):
+    _fields_ = [
+        ("hubs", ctypes.c_int32),
+        ("classes", ctypes.c_int64),
+        ("light_pairs", ctypes.c_int64),
+        ("pair_types", ctypes.c_int64),
+        ("light_wedges", ctypes.c_int64),
+        ("ms_classes", ctypes.c_double),
+        ("ms_wedges", ctypes.c_double),
+        ("ms_types", ctypes.c_double),
+        ("ms_triangles", ctypes.c_double),
+    ]
+
+
 _P = ctypes.POINTER
 _i32p = _P(ctypes.c_int32)
 _i64p = _P(ctypes.c_int64)
@@ -97,6 +111,8 @@ _SIGNATURES = [
     ("hm_graph_get_stats", ctypes.c_int, [ctypes.c_void_p, _P(_GraphStats)]),
     ("hm_graph_phase_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _P(ctypes.c_double)]),
     ("hm_graph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("hm_graph_set_hubs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("hm_graph_get_hub_stats", ctypes.c_int, [ctypes.c_void_p, _P(_HubStats)]),
     ("hm_graph_set_replay", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_set_tri_form", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("hm_graph_get_tri_form", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
@@ -315,6 +331,16 @@ class Graph:
     def set_tile(self, tile_ids: int) -> None:
         """Key tile of count_static: 0 = automatic, > 0 = forced tiles of that many ids."""
         _check(lib().hm_graph_set_tile(self._h, int(tile_ids)))
+
+    def set_hubs(self, hubs: int) -> None:
+        """Hub-aware count_static: 0 = off (triangles of the full projection), 1..31 = that many
+        hubs, -1 = hub count from the cost model (hm_graph_set_hubs)."""
+        _check(lib().hm_graph_set_hubs(self._h, int(hubs)))
+
+    def hub_stats(self) -> dict:
+        st = _HubStats()
+        _check(lib().hm_graph_get_hub_stats(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in _HubStats._fields_}
 
     def update(self, deleted, ins_offsets=None, ins_nodes=None, delta: bool = True):
         """Deletes the ids in `deleted`, inserts the CSR (ins_offsets, ins_nodes); returns

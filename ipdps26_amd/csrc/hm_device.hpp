@@ -93,17 +93,20 @@ __device__ inline int open_bin(int sx, int sy, int sz, int cxy, int cxz) {
 }
 
 // Mask table: M[key(x)] |= bit(i) for every node a_i of hyperedge a and every x in V2H(a_i), x != a.
+// With `hub` given, nodes v with hub[v] >= 0 are skipped (the table then holds a's light nodes only).
 template <typename MT>
 __device__ inline void build_masks(MT* M, int32_t E, int32_t a, const uint32_t* __restrict__ eoff,
                                    const int32_t* __restrict__ enodes,
                                    const uint32_t* __restrict__ voff,
                                    const int32_t* __restrict__ vedges,
-                                   const int32_t* __restrict__ rank) {
+                                   const int32_t* __restrict__ rank,
+                                   const int8_t* __restrict__ hub = nullptr) {
   for (int x = threadIdx.x; x < E; x += blockDim.x) M[x] = (MT)0;
   __syncthreads();
   const uint32_t s = eoff[a], t = eoff[a + 1];
   for (uint32_t i = s; i < t; ++i) {
     const int32_t v = enodes[i];
+    if (hub && hub[v] >= 0) continue;  // uniform across the workgroup
     const MT bit = (MT)1 << (i - s);
     const uint32_t ks = voff[v], kt = voff[v + 1];
     for (uint32_t k = ks + threadIdx.x; k < kt; k += blockDim.x) {

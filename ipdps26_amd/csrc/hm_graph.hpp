@@ -138,6 +138,29 @@ struct CountScratch {
   }
 };
 
+// Scratch of the hub-aware static count (hm_hub.hip), reused across calls.
+struct HubScratch {
+  DevBuf hubidx;   // int8[V]     hub index of a node (-1 = light)
+  DevBuf hmask;    // uint32[E]   hub pattern of a hyperedge
+  DevBuf lsz;      // uint8[E]    light-part size
+  DevBuf ckey;     // uint64[E]   class keys (hub pattern << 1 | [light part non-empty])
+  DevBuf ckey2;    // uint64[E]   sorted class keys
+  DevBuf ukey;     // uint64[K]   distinct class keys
+  DevBuf ucnt;     // uint32[K]   hyperedges per class
+  DevBuf nruns;    // uint64[4]   class count, light-pair count, pair-type count
+  DevBuf hmr;      // uint32[E]   hub pattern by light rank
+  DevBuf lszr;     // uint8[E]    light size by light rank
+  DevBuf types;    // uint64[LP]  pair-type key per light pair
+  DevBuf types2;   // uint64[LP]  sorted pair-type keys
+  DevBuf tkey;     // uint64[T]   distinct pair types
+  DevBuf tcnt;     // uint32[T]   light pairs per type
+  void release() {
+    for (DevBuf* b : {&hubidx, &hmask, &lsz, &ckey, &ckey2, &ukey, &ucnt, &nruns, &hmr, &lszr, &types,
+                      &types2, &tkey, &tcnt})
+      b->release();
+  }
+};
+
 // Scratch of the dynamic path (full projection rows, batch staging), reused across updates.
 struct DynScratch {
   DevBuf deg, roff, rows, flag, ids, acc, tmp;
@@ -228,4 +251,21 @@ struct hm_graph {
   int32_t last_tri_form = -1;  // form of the last one-table count (-1: none or tiled)
   bool last_tri_tuned = false;
   hm::UpdPlan uplan;  // captured update with a delta
+  // hub-aware static count (hm_graph_set_hubs): 0 = off, -1 = automatic, h > 0 = h hubs
+  int32_t hubs = 0;
+  hm::HubScratch hub;
+  hm_hub_stats hub_stats{};
 };
+
+namespace hm {
+// Most hubs the hub-aware count takes (a hub pattern is a 32-bit mask; pair-type keys hold two
+// (pattern << 1 | flag) words).
+constexpr int32_t kHubMax = 31;
+// Hub-aware static count of g with h hubs (1..kHubMax) into the accumulator layout of the
+// triangle path (30 bins; the projection statistics are not computed).
+hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc);
+// Whether g's hub-aware count fits one LDS table on `device`.
+bool hub_fits(const hm_graph* g, int device);
+// Hub count minimising the hub-aware cost model (class triples + light wedges + pair types).
+hm_status hub_choose(hm_graph* g, int32_t* h_out, double* cost_out);
+}  // namespace hm

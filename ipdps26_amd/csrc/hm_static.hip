@@ -870,6 +870,10 @@ int64_t graph_device_bytes(const hm_graph* g) {
                           &d.srt_node, &d.srt_edge, &d.nvoff, &d.slen, &d.soff, &d.scur, &d.spm, &d.ord})
     add(*b);
   add(g->uplan.in);
+  const HubScratch& h = g->hub;
+  for (const DevBuf* b : {&h.hubidx, &h.hmask, &h.lsz, &h.ckey, &h.ckey2, &h.ukey, &h.ucnt, &h.nruns, &h.hmr,
+                          &h.lszr, &h.types, &h.types2, &h.tkey, &h.tcnt})
+    add(*b);
   return n;
 }
 
@@ -1025,6 +1029,20 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
       return fail(HM_ERR_UNSUPPORTED, "hyperedge larger than 64 nodes");
     if ((int64_t)E >= kMaxPackedId) return fail(HM_ERR_UNSUPPORTED, "more than 2^25 hyperedges");
     HM_HIP(hipSetDevice(g->device));
+    g->hub_stats.hubs = 0;
+    if (g->hubs != 0) {
+      const bool fits = hub_fits(g, g->device);
+      if (!fits && g->hubs > 0)
+        return fail(HM_ERR_UNSUPPORTED, "hub-aware count: the graph does not fit one LDS table");
+      if (fits) {
+        int32_t h = g->hubs;
+        if (h < 0) HM_TRY(hub_choose(g, &h, nullptr));
+        HM_TRY(count_static_hub(g, h, acc));
+        for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = acc[kAccHist + k];
+        fill_count_stats(g, out, acc);
+        return HM_OK;
+      }
+    }
     const int lim = lds_limit_bytes(g->device);
     const bool wide = g->csr.max_size > 32;
     CountImpl impl;
@@ -1102,6 +1120,7 @@ hm_status hm_graph_destroy(hm_graph* g) {
   g->uplan.release();
   g->scratch.release();
   g->dyn.release();
+  g->hub.release();
   g->csr.release();
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1128,6 +1147,21 @@ hm_status hm_graph_set_tile(hm_graph* g, int32_t tile_ids) {
   if (tile_ids < 0) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_set_tile: negative tile");
   if (g->tile != tile_ids) ++g->version;
   g->tile = tile_ids;
+  return HM_OK;
+}
+
+hm_status hm_graph_set_hubs(hm_graph* g, int32_t hubs) {
+  hm::clear_error();
+  if (!g) return hm::fail(HM_ERR_BAD_INPUT, "null graph");
+  if (hubs < -1 || hubs > hm::kHubMax) return hm::fail(HM_ERR_BAD_INPUT, "hubs must be -1, 0 or 1..31");
+  g->hubs = hubs;
+  return HM_OK;
+}
+
+hm_status hm_graph_get_hub_stats(hm_graph* g, hm_hub_stats* out) {
+  hm::clear_error();
+  if (!g || !out) return hm::fail(HM_ERR_BAD_INPUT, "null argument");
+  *out = g->hub_stats;
   return HM_OK;
 }
 

@@ -116,7 +116,7 @@ __device__ inline void proj_count_item(MT* M, int32_t E, int32_t a, const uint32
 
 // Hyperedge a (kBlock lanes): its upper row — ranks r > rank[a] with M[r] != 0, ascending, as
 // packed (r << 7 | C_ar) words at rows[roff[rank[a]]...] — and its length rlen[rank[a]]. M spans
-// E ranks.
+// E ranks. With `hub` given, only a's nodes v with hub[v] < 0 take part (the light projection).
 template <typename MT>
 __device__ inline void proj_fill_item(MT* M, int32_t E, int32_t a, const uint32_t* __restrict__ eoff,
                                       const int32_t* __restrict__ enodes,
@@ -124,10 +124,11 @@ __device__ inline void proj_fill_item(MT* M, int32_t E, int32_t a, const uint32_
                                       const int32_t* __restrict__ vedges,
                                       const int32_t* __restrict__ rank,
                                       const unsigned long long* __restrict__ roff,
-                                      uint32_t* __restrict__ rlen, uint32_t* __restrict__ rows) {
+                                      uint32_t* __restrict__ rlen, uint32_t* __restrict__ rows,
+                                      const int8_t* __restrict__ hub = nullptr) {
   __shared__ uint32_t wcnt[kWavesPerBlock];
   const int32_t ra = rank[a];
-  build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, rank);
+  build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, rank, hub);
 
   const unsigned long long base = roff[ra];
   const int lane = lane_id(), w = wave_id();

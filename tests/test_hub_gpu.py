@@ -1,0 +1,131 @@
+"""GPU parity of the hub-aware static count (hm_graph_set_hubs + hm_count_static, through the C
+ABI) against the oracle, brute force, the reference's golden vectors and the triangle path.
+Integer counts: bit-exact."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import GOLDEN, csr, load_golden, random_hypergraph
+
+import ipdps26_amd as hm
+from ipdps26_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def fits_one_table(off, nodes):
+    """Mirror of the library's rule: mask table + 8-byte neighbour list (k_hub_centre) and mask
+    table + light sizes (k_hub_tri) within 160 KB of LDS, 12 KB kept for static counters."""
+    E = off.size - 1
+    wide = int(np.diff(off).max()) > 32
+    mask = 8 if wide else 4
+    return max(E * (mask + 8), E * (mask + 1)) + 12 * 1024 <= 160 * 1024
+
+
+def count(off, nodes, hubs):
+    with hm.Graph(off, nodes) as g:
+        g.set_hubs(hubs)
+        c = g.count_static().astype(np.int64)
+        return c, g.hub_stats()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_matches_oracle(gpu_ready, seed):
+    rng = np.random.default_rng(5000 + seed)
+    kmax = [4, 8, 20, 40][seed % 4]  # 40: 64-bit light masks
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(3, 400)),
+                                   num_nodes=int(rng.integers(kmax, 120)), kmax=kmax, dup_frac=0.2)
+    want = O.static_mochy(off, nodes)
+    for h in (1, 2, 3, 5, 8, 16, 31):
+        print(f"seed={seed} E={off.size - 1} hubs={h}", flush=True)
+        c, st = count(off, nodes, h)
+        assert np.array_equal(c, want), h
+        assert st["hubs"] == min(h, int(nodes.max()) + 1)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_tiny_matches_brute_force(gpu_ready, seed):
+    rng = np.random.default_rng(6000 + seed)
+    off, nodes = random_hypergraph(rng, num_edges=60, num_nodes=15, kmax=6, dup_frac=0.3)
+    want = O.static_brute(off, nodes)
+    for h in (1, 4, 15):
+        assert np.array_equal(count(off, nodes, h)[0], want), h
+
+
+@pytest.mark.parametrize("name", ["tiny_dense_e60", "c1_family_e800", "c2_family_e3000",
+                                  "c3_family_e6000", "c2_family_e8000", "c5_family_e10000", "dblp"])
+def test_reference_goldens(gpu_ready, name):
+    g = [x for x in load_golden("static") if x["name"] == name][0]
+    if g["kind"] == "fixture":
+        off, nodes = W.read_mochy_csv(os.path.join(GOLDEN, g["file"]))
+    else:
+        off, nodes = hm.generate_static(hm.GenParams(**g["params"]))
+    if not fits_one_table(off, nodes):
+        pytest.skip("beyond one LDS table of the hub-aware count")
+    for h in (4, 12, -1):
+        print(f"golden={name} hubs={h}", flush=True)
+        c, _ = count(off, nodes, h)
+        assert hm.printed_motifs(c) == g["printed_motifs"], h
+
+
+def test_static_c1_all_bins(gpu_ready):
+    g = json.load(open(os.path.join(GOLDEN, "static_c1_30bins.json")))
+    off, nodes = hm.generate_static(W.STATIC_C1)
+    for h in (2, 8, 16, -1):
+        c, st = count(off, nodes, h)
+        print(f"static_c1 hubs={h} stats={st}", flush=True)
+        assert c.tolist() == g["counts30"], h
+        assert st["hubs"] > 0
+
+
+def test_equals_triangle_path_c2_family(gpu_ready):
+    off, nodes = hm.generate_static(W.scaled(W.STATIC_C2, 9_000, 4_500, seed=21))
+    assert fits_one_table(off, nodes)
+    plain, _ = count(off, nodes, 0)
+    for h in (8, 24, -1):
+        print(f"c2 family 10k hubs={h}", flush=True)
+        assert np.array_equal(count(off, nodes, h)[0], plain), h
+
+
+def test_after_deletes_and_inserts(gpu_ready):
+    rng = np.random.default_rng(77)
+    off, nodes = random_hypergraph(rng, num_edges=300, num_nodes=60, kmax=10)
+    with hm.Graph(off, nodes) as g:
+        g.update(np.arange(0, 300, 7, dtype=np.int32), *csr([[1, 2, 3], [2, 3, 4, 50], [0, 1]]))
+        plain = g.count_static().astype(np.int64)
+        g.set_hubs(6)
+        assert np.array_equal(g.count_static().astype(np.int64), plain)
+        st = g.stats()
+        assert st["proj_pairs"] == -1 and st["triangles"] >= 0
+
+
+def test_closed_forms(gpu_ready):
+    # star sharing one node: C(k, 3) in bin 3; the hub is node 0
+    k = 500
+    c, _ = count(*csr([[0, 10_000 + i] for i in range(k)]), 1)
+    assert c[3] == k * (k - 1) * (k - 2) // 6 and c.sum() == c[3]
+    c, _ = count(*csr([[4, 5, 6]] * 3), 2)
+    assert c[0] == 1 and c.sum() == 1
+    c, _ = count(*csr([[1, 3, 10], [1, 2, 11], [2, 3, 12]]), 1)
+    assert c[29] == 1 and c.sum() == 1
+    assert count(*csr([[0, 1], [1, 2]]), 3)[0].sum() == 0
+
+
+def test_forced_hubs_beyond_one_table_is_an_error_auto_falls_back(gpu_ready):
+    off, nodes = hm.generate_static(W.scaled(W.STATIC_C2, 20_000, 10_000, seed=22))
+    with hm.Graph(off, nodes) as g:
+        g.set_hubs(8)
+        with pytest.raises(hm.HMotifError):
+            g.count_static()
+        g.set_hubs(-1)
+        auto = g.count_static().astype(np.int64)
+        assert g.hub_stats()["hubs"] == 0  # counted by triangles
+        g.set_hubs(0)
+        assert np.array_equal(g.count_static().astype(np.int64), auto)
+    with hm.Graph(*csr([[0, 1], [1, 2], [2, 0]])) as g:
+        for bad in (-2, 32):
+            with pytest.raises(hm.HMotifError):
+                g.set_hubs(bad)
