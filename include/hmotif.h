@@ -163,27 +163,28 @@ typedef struct hm_graph_stats {
 } hm_graph_stats;
 hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out);
 
-/* Hub-aware static count (no counterpart in the reference, whose counters enumerate
- * every triple: MoCHy-master/main_exact_par.cpp:122-165, dynamicHyperGraph/src/HMotifCount.cu:16-95).
- * hubs = 0 (default): hm_count_static enumerates triangles of the full projection; hubs in 1..31:
- * the hubs most frequent nodes are hubs, the count combines hyperedge classes (hub pattern, light
- * part non-empty), pair types of the hyperedge pairs that share a light node, light wedges and
- * light triangles, and never enumerates the cliques hubs put into the projection; hubs = -1: the
- * hub count is chosen by a cost model. The 30 bins are identical either way. Graphs whose
- * hub-aware count does not fit one LDS table (about 12k hyperedges) count by triangles when
- * hubs = -1 and fail with HM_ERR_UNSUPPORTED when hubs > 0. After a hub-aware count the
- * projection fields of hm_graph_stats (proj_pairs, max_proj_degree, oriented_row_sum, wedges) are
- * -1. */
+/* Hub-aware static count (no counterpart in the reference, whose counters enumerate every triple:
+ * MoCHy-master/main_exact_par.cpp:122-165, dynamicHyperGraph/src/HMotifCount.cu:16-95).
+ * hubs = 0 (default): hm_count_static enumerates the triangles of the full projection; hubs in
+ * 1..23: the `hubs` most frequent nodes are hubs, and the count combines hyperedge classes (hub
+ * pattern, light part non-empty), the types of the hyperedge pairs that share a light node, the
+ * pairs of light neighbours of every hyperedge grouped by key, and the light triangles, without
+ * enumerating the cliques that hubs put into the projection; hubs = -1: the hub count is chosen by
+ * a cost model (and lowered by a quarter while some hyperedge has more distinct neighbour keys
+ * than its LDS histogram holds; if even one hub overflows, the triangle path counts). The 30 bins are identical
+ * either way. A forced hub count whose histograms overflow fails with HM_ERR_UNSUPPORTED. After a
+ * hub-aware count hm_graph_stats holds the projection statistics computed from the projected
+ * degrees alone (no rows: upper-row totals are not kept). */
 hm_status hm_graph_set_hubs(hm_graph* g, int32_t hubs);
 typedef struct hm_hub_stats {
   int32_t hubs;           /* hubs of the last hub-aware count on g (0: it counted by triangles) */
   int64_t classes;        /* distinct (hub pattern, [light part non-empty]) classes */
-  int64_t light_pairs;    /* hyperedge pairs sharing a light node */
+  int64_t light_pairs;    /* hyperedge pairs sharing a light node (pairs of the light projection) */
   int64_t pair_types;     /* distinct pair types among the light pairs */
-  int64_t light_wedges;   /* reserved (-1) */
-  double ms_classes;      /* class sort and class triples */
+  int64_t light_triangles; /* hyperedge triples whose three pairs share light nodes */
+  double ms_classes;      /* class keys and their sort */
   double ms_wedges;       /* light neighbours, pair-type keys and light wedges */
-  double ms_types;        /* pair-type sort and pair types x classes */
+  double ms_types;        /* class triples, pair-type sort and pair types x classes */
   double ms_triangles;    /* light projection rows and light triangles */
 } hm_hub_stats;
 hm_status hm_graph_get_hub_stats(hm_graph* g, hm_hub_stats* out);

@@ -76,7 +76,7 @@ class _HubStats(ctypes.Structure):
         ("classes", ctypes.c_int64),
         ("light_pairs", ctypes.c_int64),
         ("pair_types", ctypes.c_int64),
-        ("light_wedges", ctypes.c_int64),
+        ("light_triangles", ctypes.c_int64),
         ("ms_classes", ctypes.c_double),
         ("ms_wedges", ctypes.c_double),
         ("ms_types", ctypes.c_double),

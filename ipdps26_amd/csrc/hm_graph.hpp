@@ -150,13 +150,15 @@ struct HubScratch {
   DevBuf nruns;    // uint64[4]   class count, light-pair count, pair-type count
   DevBuf hmr;      // uint32[E]   hub pattern by light rank
   DevBuf lszr;     // uint8[E]    light size by light rank
-  DevBuf types;    // uint64[LP]  pair-type key per light pair
-  DevBuf types2;   // uint64[LP]  sorted pair-type keys
+  DevBuf types;    // uint64[N]   pair-type key per (centre, neighbour key) entry
+  DevBuf tcraw;    // uint64[N]   light pairs of the entry
+  DevBuf types2;   // uint64[N]   sorted pair-type keys
+  DevBuf tcsrt;    // uint64[N]   their counts
   DevBuf tkey;     // uint64[T]   distinct pair types
-  DevBuf tcnt;     // uint32[T]   light pairs per type
+  DevBuf tcnt;     // uint64[T]   light pairs per type
   void release() {
     for (DevBuf* b : {&hubidx, &hmask, &lsz, &ckey, &ckey2, &ukey, &ucnt, &nruns, &hmr, &lszr, &types,
-                      &types2, &tkey, &tcnt})
+                      &tcraw, &types2, &tcsrt, &tkey, &tcnt})
       b->release();
   }
 };
@@ -258,14 +260,17 @@ struct hm_graph {
 };
 
 namespace hm {
-// Most hubs the hub-aware count takes (a hub pattern is a 32-bit mask; pair-type keys hold two
-// (pattern << 1 | flag) words).
-constexpr int32_t kHubMax = 31;
+// Most hubs the hub-aware count takes (a neighbour key holds the hub pattern in 23 bits beside the
+// overlap and a flag).
+constexpr int32_t kHubMax = 23;
 // Hub-aware static count of g with h hubs (1..kHubMax) into the accumulator layout of the
-// triangle path (30 bins; the projection statistics are not computed).
-hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc);
-// Whether g's hub-aware count fits one LDS table on `device`.
-bool hub_fits(const hm_graph* g, int device);
-// Hub count minimising the hub-aware cost model (class triples + light wedges + pair types).
+// triangle path (30 bins; the projection statistics are not computed). *key_overflow is set when
+// some hyperedge's light neighbours hold more distinct keys than its LDS histogram (fewer hubs
+// then fit).
+hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow);
+// Hub count minimising the hub-aware cost model.
 hm_status hub_choose(hm_graph* g, int32_t* h_out, double* cost_out);
+// Projection statistics (sum and max of the projected degrees, wedges, oriented row sum, largest
+// node degree) into their accumulator slots, without rows or triangles.
+hm_status projection_stats(hm_graph* g, unsigned long long* h_acc);
 }  // namespace hm

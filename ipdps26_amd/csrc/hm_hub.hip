@@ -40,6 +40,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -162,83 +164,195 @@ __device__ inline unsigned long long type_key(uint32_t sa, uint32_t sx) {
   return (unsigned long long)hi << 32 | lo;
 }
 
-// Per hyperedge a: light mask table M[x] (a's light nodes contained in x), its light neighbours
-// compacted into LDS as (hub mask, C_ax | [|L(x)| > C_ax] << 7), the light degree dL[a], one
-// pair-type key per light neighbour x > a appended to types[] (for (B); the slot count keeps
-// counting past cap so the host can grow the buffer), and (C) over all pairs of light neighbours.
+// Key histogram of a centre's light neighbours (LDS, open addressing): key = hub mask << 8 |
+// C_ax << 1 | [|L(x)| > C_ax] (hub masks of at most 23 bits), the neighbours with that key and
+// those of them with a larger id. A centre with more than kHubKeysMax distinct keys reports an
+// overflow (the count is then redone with fewer hubs).
+constexpr int kHubHashBits = 13;
+constexpr int kHubHash = 1 << kHubHashBits;
+constexpr uint32_t kHubKeysMax = kHubHash - kHubHash / 8;
+constexpr uint32_t kHubEmpty = 0xFFFFFFFFu;
+constexpr int32_t kHubKeyHubs = 23;
+
+__device__ inline bool hub_key_add(uint32_t* hk, uint32_t* hn, uint32_t* hu, uint32_t key, bool up,
+                                   uint32_t* used) {
+  uint32_t slot = (key * 2654435761u) >> (32 - kHubHashBits);
+  for (int p = 0; p < kHubHash; ++p) {
+    const uint32_t prev = atomicCAS(&hk[slot], kHubEmpty, key);
+    if (prev == kHubEmpty && atomicAdd(used, 1u) >= kHubKeysMax) return false;
+    if (prev == kHubEmpty || prev == key) {
+      atomicAdd(&hn[slot], 1u);
+      if (up) atomicAdd(&hu[slot], 1u);
+      return true;
+    }
+    slot = (slot + 1) & (kHubHash - 1);
+  }
+  return false;
+}
+
+// a's light nodes, ascending, into s_ln (wave 0); returns nothing, callers sync before use.
+__device__ inline void light_nodes(int32_t* s_ln, int32_t a, const uint32_t* __restrict__ eoff,
+                                   const int32_t* __restrict__ enodes, const int8_t* __restrict__ hub) {
+  if (threadIdx.x < kWave) {
+    const uint32_t s = eoff[a];
+    const int sa = (int)(eoff[a + 1] - s), i = (int)threadIdx.x;
+    int32_t v = 0;
+    bool light = false;
+    if (i < sa) {
+      v = enodes[s + i];
+      light = hub[v] < 0;
+    }
+    const unsigned long long bal = __ballot(light);
+    const unsigned long long below = (i == 0) ? 0ull : (~0ull >> (64 - i));
+    if (light) s_ln[__popcll(bal & below)] = v;
+  }
+}
+
+// Per hyperedge a: its light neighbours x over id tiles of T (the tile's mask table built from
+// the parts of the V2H rows of a's light nodes that fall in it, each entry visited once by the
+// lane of a's lowest light node in x), their keys in the LDS histogram, the light degree dL[a];
+// then (C) over all pairs of keys (n_i n_j pairs of neighbours, C(n_i, 2) within one key) into
+// accC, and one pair type per key with neighbours above a, with their number, appended to
+// (tkeys, tcnts) for (B) (the slot count keeps counting past cap so the host can grow them).
+// Dynamic LDS: T masks, then the histogram (keys, counts, counts above a: kHubHash each).
 template <typename MT>
 __global__ __launch_bounds__(kBlock) void k_hub_centre(
-    int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges, const int8_t* __restrict__ hub,
     const uint32_t* __restrict__ hmask, const uint8_t* __restrict__ lsz, uint32_t* __restrict__ dL,
-    unsigned long long* __restrict__ types, unsigned long long* __restrict__ ntypes,
-    unsigned long long cap, unsigned long long* __restrict__ acc) {
+    unsigned long long* __restrict__ tkeys, unsigned long long* __restrict__ tcnts,
+    unsigned long long* __restrict__ ntent, unsigned long long cap, uint32_t* __restrict__ overflow,
+    unsigned long long* __restrict__ accC) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int32_t hs[kHubIds * kHubCopies];
-  __shared__ uint32_t wcnt[kWavesPerBlock];
+  __shared__ unsigned long long hs[kHubIds * kHubCopies];
+  __shared__ int32_t s_ln[HM_MAX_EDGE_SIZE];
+  __shared__ TileRanges tr;
+  __shared__ uint32_t s_used, s_g;
+  __shared__ uint32_t s_d[kWavesPerBlock];
+  __shared__ int s_over;
   const int32_t a = blockIdx.x;
   const int la = lsz[a];
   if (la == 0) {  // no light node: no light neighbour
     if (threadIdx.x == 0) dL[a] = 0;
     return;
   }
+  const size_t region = (size_t)T * sizeof(MT);
   MT* M = reinterpret_cast<MT*>(smem);
-  uint2* list = reinterpret_cast<uint2*>(smem + (((size_t)E * sizeof(MT) + 7) & ~(size_t)7));
-  for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0;
-  build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, nullptr, hub);
-
+  uint32_t* hk = reinterpret_cast<uint32_t*>(smem + region);
+  uint32_t* hn = hk + kHubHash;
+  uint32_t* hu = hn + kHubHash;
+  light_nodes(s_ln, a, eoff, enodes, hub);
+  for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0ull;
+  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+  for (int k = threadIdx.x; k < kHubHash; k += blockDim.x) {
+    hk[k] = kHubEmpty;
+    hn[k] = 0u;
+    hu[k] = 0u;
+  }
+  if (threadIdx.x == 0) {
+    s_used = 0u;
+    s_g = 0u;
+    s_over = 0;
+  }
   const uint32_t Aa = hmask[a];
+  uint32_t d = 0;
+  bool ok = true;
+  for (int32_t t0 = 0; t0 < E; t0 += T) {
+    const int32_t t1 = min(E, t0 + T);
+    const uint32_t total = tile_ranges(tr, la, s_ln, voff, vedges, t0, t1);
+    if (total == 0) continue;
+    for_tile_entries(tr, la, total, vedges, [&](int i, int32_t x) {
+      if (x != a) atomicOr(&M[x - t0], (MT)1 << i);
+    });
+    __syncthreads();
+    for_tile_entries(tr, la, total, vedges, [&](int i, int32_t x) {
+      if (x == a) return;
+      const MT m = M[x - t0];
+      if (m && low_bit(m) == i) {
+        const int C = popc(m);
+        const uint32_t key = hmask[x] << 8 | (uint32_t)C << 1 | (lsz[x] > C ? 1u : 0u);
+        ok = hub_key_add(hk, hn, hu, key, x > a, &s_used) && ok;
+        ++d;
+        M[x - t0] = (MT)0;
+      }
+    });
+  }
+  if (!ok) s_over = 1;
+  d = wave_sum(d);
+  if (lane_id() == 0) s_d[wave_id()] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t dd = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) dd += s_d[w];
+    dL[a] = dd;
+    if (s_over) atomicOr(overflow, 1u);
+  }
+  __syncthreads();     // s_d is reused by the compaction below
+  if (s_over) return;  // uniform: read after the barriers above
+
+  // compact the histogram in place (stable: slot k moves to the number of keys before it, never
+  // past k, and a chunk is read whole before it is written); pair types of the neighbours above a
+  uint32_t* lk = hk;
+  uint32_t* ln = hn;
   const int lane = lane_id(), w = wave_id();
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t running = 0;
-  for (int32_t x0 = 0; x0 < E; x0 += blockDim.x) {
-    const int32_t x = x0 + (int32_t)threadIdx.x;
-    const MT m = (x < E) ? M[x] : (MT)0;
-    const unsigned long long bal = __ballot(m != (MT)0);
-    if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+  uint32_t placed = 0;
+  for (int k0 = 0; k0 < kHubHash; k0 += blockDim.x) {
+    const int k = k0 + (int)threadIdx.x;
+    const uint32_t key = hk[k];
+    const bool have = key != kHubEmpty;
+    const uint32_t n = have ? hn[k] : 0u, up = have ? hu[k] : 0u;
+    const unsigned long long bal = __ballot(have);
+    if (lane == 0) s_d[w] = (uint32_t)__popcll(bal);
     __syncthreads();
     uint32_t woff = 0, tot = 0;
     for (int q = 0; q < kWavesPerBlock; ++q) {
-      woff += q < w ? wcnt[q] : 0;
-      tot += wcnt[q];
+      woff += q < w ? s_d[q] : 0;
+      tot += s_d[q];
     }
-    uint32_t sx = 0, Ax = 0;
-    int C = 0;
-    if (m) {
-      C = popc(m);
-      Ax = hmask[x];
-      const uint32_t f = lsz[x] > C ? 1u : 0u;
-      list[running + woff + (uint32_t)__popcll(bal & below)] = make_uint2(Ax, (uint32_t)C | f << 7);
-      sx = Ax << 1 | f;
+    if (have) {
+      const uint32_t at = placed + woff + (uint32_t)__popcll(bal & below);
+      lk[at] = key;
+      ln[at] = n;
     }
-    const bool emit = m && x > a;
+    placed += tot;
+    const bool emit = up > 0;
     const unsigned long long eb = __ballot(emit);
     if (eb) {
       const int leader = __ffsll((long long)eb) - 1;
       unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(ntypes, (unsigned long long)__popcll(eb));
+      if (lane == leader) base = atomicAdd(ntent, (unsigned long long)__popcll(eb));
       base = __shfl(base, leader);
       if (emit) {
         const unsigned long long pos = base + (unsigned long long)__popcll(eb & below);
-        if (pos < cap) types[pos] = type_key(Aa << 1 | (la > C ? 1u : 0u), sx);
+        if (pos < cap) {
+          const int C = (int)((key >> 1) & 127u);
+          const uint32_t sa = Aa << 1 | (la > C ? 1u : 0u), sx = (key >> 8) << 1 | (key & 1u);
+          tkeys[pos] = type_key(sa, sx);
+          tcnts[pos] = up;
+        }
       }
     }
-    running += tot;
     __syncthreads();
   }
-  const uint32_t n = running;
-  int32_t* hc = hs + (lane & (kHubCopies - 1));
-  for (uint32_t i = w; i < n; i += kWavesPerBlock) {
-    const uint2 bi = list[i];
-    const uint32_t Ab = bi.x;
-    const int Cb = (int)(bi.y & 127u);
-    const unsigned fb = bi.y >> 7;
+  if (threadIdx.x == 0) s_g = placed;
+  __syncthreads();
+  const uint32_t G = s_g;
+  unsigned long long* hc = hs + (lane & (kHubCopies - 1));
+  for (uint32_t i = w; i < G; i += kWavesPerBlock) {
+    const uint32_t bi = lk[i];
+    const unsigned long long ni = ln[i];
+    const uint32_t Ab = bi >> 8;
+    const int Cb = (int)((bi >> 1) & 127u);
+    const unsigned fb = bi & 1u;
     const uint32_t Aab = Aa & Ab, Aa_b = Aa & ~Ab, Ab_a = Ab & ~Aa;
-    for (uint32_t j = i + 1 + lane; j < n; j += kWave) {
-      const uint2 cj = list[j];
-      const uint32_t Ac = cj.x;
-      const int Cc = (int)(cj.y & 127u);
-      const unsigned fc = cj.y >> 7;
+    for (uint32_t j = i + lane; j < G; j += kWave) {
+      const uint32_t cj = lk[j];
+      const unsigned long long w = (j == i) ? binom2(ni) : ni * (unsigned long long)ln[j];
+      if (!w) continue;
+      const uint32_t Ac = cj >> 8;
+      const int Cc = (int)((cj >> 1) & 127u);
+      const unsigned fc = cj & 1u;
       const unsigned h = nzm(Aa_b & ~Ac) | nzm(Ab_a & ~Ac) << 1 | nzm(Ac & ~(Aa | Ab)) << 2 |
                          nzm(Aab & ~Ac) << 3 | nzm(Ab_a & Ac) << 4 | nzm(Aa_b & Ac) << 5 |
                          nzm(Aab & Ac) << 6;
@@ -246,15 +360,14 @@ __global__ __launch_bounds__(kBlock) void k_hub_centre(
       const unsigned p1ab = h | 8u | (la > Cb ? 1u : 0u) | fb << 1 | 4u;
       const unsigned p1ac = h | 32u | (la > Cc ? 1u : 0u) | fc << 2 | 2u;
       const unsigned p0 = h | 7u;
-      atomicAdd(&hc[ta * kHubCopies], 1);
-      atomicAdd(&hc[p0 * kHubCopies], 1);
-      atomicAdd(&hc[p1ab * kHubCopies], -1);
-      atomicAdd(&hc[p1ac * kHubCopies], -1);
+      atomicAdd(&hc[ta * kHubCopies], w);
+      atomicAdd(&hc[p0 * kHubCopies], w);
+      atomicAdd(&hc[p1ab * kHubCopies], 0ull - w);
+      atomicAdd(&hc[p1ac * kHubCopies], 0ull - w);
     }
   }
   __syncthreads();
-  fold_ids(hs, 1ull, acc);
-  if (threadIdx.x == 0) dL[a] = n;
+  fold_ids(hs, 1ull, accC);
 }
 
 // (B): one workgroup per distinct pair type t = (Aa, fa, Ab, fb) with nt light pairs; every class
@@ -265,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_types(int32_t K, int32_t k0,
                                                       const unsigned long long* __restrict__ ukey,
                                                       const uint32_t* __restrict__ ucnt,
                                                       const unsigned long long* __restrict__ tkey,
-                                                      const uint32_t* __restrict__ tcnt,
+                                                      const unsigned long long* __restrict__ tcnt,
                                                       unsigned long long* __restrict__ acc) {
   __shared__ long long hs[kHubIds * kHubCopies];
   for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0;
@@ -311,92 +424,376 @@ __global__ void k_hub_rank(int32_t E, const int32_t* __restrict__ perm, const ui
   lszr[r] = lsz[a];
 }
 
+// Upper rows of the light projection (one LDS table by rank): as proj_fill_item, from a's light
+// nodes, never writing past the row's capacity roff[ra + 1] - roff[ra] (the light degree the
+// centre pass counted); a row that would is reported in *bad.
 template <typename MT>
 __global__ __launch_bounds__(kBlock) void k_hub_fill(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges, const int8_t* __restrict__ hub,
     const uint8_t* __restrict__ lsz, const int32_t* __restrict__ rank,
-    const unsigned long long* __restrict__ roff, uint32_t* __restrict__ rlen, uint32_t* __restrict__ rows) {
+    const unsigned long long* __restrict__ roff, uint32_t* __restrict__ rlen, uint32_t* __restrict__ rows,
+    uint32_t* __restrict__ bad) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t wcnt[kWavesPerBlock];
   const int32_t a = blockIdx.x;
+  const int32_t ra = rank[a];
   if (lsz[a] == 0) {
-    if (threadIdx.x == 0) rlen[rank[a]] = 0;
+    if (threadIdx.x == 0) rlen[ra] = 0;
     return;
   }
-  proj_fill_item<MT>(reinterpret_cast<MT*>(smem), E, a, eoff, enodes, voff, vedges, rank, roff, rlen,
-                     rows, hub);
+  MT* M = reinterpret_cast<MT*>(smem);
+  build_masks<MT>(M, E, a, eoff, enodes, voff, vedges, rank, hub);
+  const unsigned long long base = roff[ra];
+  const uint32_t cap = (uint32_t)(roff[ra + 1] - base);
+  const int lane = lane_id(), w = wave_id();
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t running = 0;
+  for (int32_t r0 = ra + 1; r0 < E; r0 += blockDim.x) {
+    const int32_t r = r0 + (int32_t)threadIdx.x;
+    const MT m = (r < E) ? M[r] : (MT)0;
+    const unsigned long long bal = __ballot(m != (MT)0);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int q = 0; q < kWavesPerBlock; ++q) {
+      woff += q < w ? wcnt[q] : 0;
+      tot += wcnt[q];
+    }
+    if (m) {
+      const uint32_t pos = running + woff + (uint32_t)__popcll(bal & below);
+      if (pos < cap) rows[base + pos] = ((uint32_t)r << kCntBits) | (uint32_t)popc(m);
+    }
+    running += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    rlen[ra] = min(running, cap);
+    if (running > cap) atomicOr(bad, 1u);
+  }
 }
+
+// The 8 id contributions of a light triangle (roles a, b, c in ascending rank) into the int32
+// slot counters hc (copy already selected); Aab, Aa_b, Ab_a are a's and b's hub-mask parts.
+__device__ inline void light_triangle(int32_t* hc, uint32_t Aab, uint32_t Aa_b, uint32_t Ab_a, uint32_t Ac,
+                                      int sa, int sb, int sc, int cab, int cbc, int cac, int g) {
+  const unsigned h = nzm(Aa_b & ~Ac) | nzm(Ab_a & ~Ac) << 1 | nzm(Ac & ~(Aa_b | Ab_a | Aab)) << 2 |
+                     nzm(Aab & ~Ac) << 3 | nzm(Ab_a & Ac) << 4 | nzm(Aa_b & Ac) << 5 | nzm(Aab & Ac) << 6;
+  const unsigned tv = h | (unsigned)venn_id(sa, sb, sc, cab, cbc, cac, g);
+  const unsigned p0 = h | 7u;
+  const unsigned p1ab = h | 8u | (sa > cab ? 1u : 0u) | (sb > cab ? 2u : 0u) | 4u;
+  const unsigned p1bc = h | 16u | (sb > cbc ? 2u : 0u) | (sc > cbc ? 4u : 0u) | 1u;
+  const unsigned p1ca = h | 32u | (sc > cac ? 4u : 0u) | (sa > cac ? 1u : 0u) | 2u;
+  const unsigned ta = h | 8u | 32u | (sa > cab + cac ? 1u : 0u) | (sb > cab ? 2u : 0u) | (sc > cac ? 4u : 0u);
+  const unsigned tb = h | 8u | 16u | (sb > cab + cbc ? 2u : 0u) | (sa > cab ? 1u : 0u) | (sc > cbc ? 4u : 0u);
+  const unsigned tc = h | 16u | 32u | (sc > cac + cbc ? 4u : 0u) | (sa > cac ? 1u : 0u) | (sb > cbc ? 2u : 0u);
+  atomicAdd(&hc[tv * kHubCopies], 1);
+  atomicAdd(&hc[p0 * kHubCopies], -1);
+  atomicAdd(&hc[p1ab * kHubCopies], 1);
+  atomicAdd(&hc[p1bc * kHubCopies], 1);
+  atomicAdd(&hc[p1ca * kHubCopies], 1);
+  atomicAdd(&hc[ta * kHubCopies], -1);
+  atomicAdd(&hc[tb * kHubCopies], -1);
+  atomicAdd(&hc[tc * kHubCopies], -1);
+}
+
+constexpr int kHubLoads = 4;  // row loads in flight per lane in the triangle kernels
 
 // (D): centre rank r; every light triangle r < rb < rc (rb in r's upper light row, rc in rb's)
 // adds F(true) - F(P0) + F(P1ab) + F(P1bc) + F(P1ca) - F(Ta) - F(Tb) - F(Tc) (roles a, b, c =
-// r, rb, rc; every member has a light node). M (light masks by rank) spans E ranks; the E light
-// sizes by rank follow it.
+// r, rb, rc; every member has a light node). Dynamic LDS by rank: light masks M (E), hub masks
+// (E words) and light sizes (E bytes). acc[kAccLightTri] counts the light triangles.
 template <typename MT>
 __global__ __launch_bounds__(kBlock) void k_hub_tri(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges, const int8_t* __restrict__ hub,
     const int32_t* __restrict__ rank, const int32_t* __restrict__ perm, const uint32_t* __restrict__ hmr,
     const uint8_t* __restrict__ lszr, const unsigned long long* __restrict__ roff,
-    const uint32_t* __restrict__ rlen, const uint32_t* __restrict__ rows, unsigned long long* __restrict__ acc) {
+    const uint32_t* __restrict__ rlen, const uint32_t* __restrict__ rows, unsigned long long* __restrict__ acc,
+    int32_t r0) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t hs[kHubIds * kHubCopies];
-  const int32_t r = blockIdx.x;
+  const int32_t r = r0 + (int32_t)blockIdx.x;
   const uint32_t nA = rlen[r];
   if (nA < 2) return;
   MT* M = reinterpret_cast<MT*>(smem);
-  uint8_t* sz = reinterpret_cast<uint8_t*>(M + E);
+  uint32_t* hm = reinterpret_cast<uint32_t*>(M + E);
+  uint8_t* sz = reinterpret_cast<uint8_t*>(hm + E);
   for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0;
-  for (int x = threadIdx.x; x < E; x += blockDim.x) sz[x] = lszr[x];
+  for (int x = threadIdx.x; x < E; x += blockDim.x) {
+    sz[x] = lszr[x];
+    hm[x] = hmr[x];
+  }
   build_masks<MT>(M, E, perm[r], eoff, enodes, voff, vedges, rank, hub);
 
   const int lane = lane_id(), w = wave_id();
   int32_t* hc = hs + (lane & (kHubCopies - 1));
   const int sa = sz[r];
-  const uint32_t Aa = hmr[r];
+  const uint32_t Aa = hm[r];
   const uint32_t* rowA = rows + roff[r];
+  unsigned long long ntri = 0;
   for (uint32_t ib = w; ib < nA; ib += kWavesPerBlock) {
     const uint32_t eb = rowA[ib];
     const int32_t rb = (int32_t)(eb >> kCntBits);
     const int cab = (int)(eb & kCntMask);
     const MT mab = M[rb];
     const int sb = sz[rb];
-    const uint32_t Ab = hmr[rb];
+    const uint32_t Ab = hm[rb];
     const uint32_t Aab = Aa & Ab, Aa_b = Aa & ~Ab, Ab_a = Ab & ~Aa;
     const uint32_t* rowB = rows + roff[rb];
     const uint32_t nB = rlen[rb];
-    for (uint32_t j = lane; j < nB; j += kWave) {
-      const uint32_t ec = rowB[j];
+    auto visit = [&](uint32_t ec) {
       const int32_t rc = (int32_t)(ec >> kCntBits);
       const MT mc = M[rc];
-      if (!mc) continue;
-      const int cbc = (int)(ec & kCntMask);
-      const int cac = popc(mc);
-      const int g = popc((MT)(mab & mc));
-      const int sc = sz[rc];
-      const uint32_t Ac = hmr[rc];
-      const unsigned h = nzm(Aa_b & ~Ac) | nzm(Ab_a & ~Ac) << 1 | nzm(Ac & ~(Aa | Ab)) << 2 |
-                         nzm(Aab & ~Ac) << 3 | nzm(Ab_a & Ac) << 4 | nzm(Aa_b & Ac) << 5 |
-                         nzm(Aab & Ac) << 6;
-      const unsigned tv = h | (unsigned)venn_id(sa, sb, sc, cab, cbc, cac, g);
-      const unsigned p0 = h | 7u;
-      const unsigned p1ab = h | 8u | (sa > cab ? 1u : 0u) | (sb > cab ? 2u : 0u) | 4u;
-      const unsigned p1bc = h | 16u | (sb > cbc ? 2u : 0u) | (sc > cbc ? 4u : 0u) | 1u;
-      const unsigned p1ca = h | 32u | (sc > cac ? 4u : 0u) | (sa > cac ? 1u : 0u) | 2u;
-      const unsigned ta = h | 8u | 32u | (sa > cab + cac ? 1u : 0u) | (sb > cab ? 2u : 0u) | (sc > cac ? 4u : 0u);
-      const unsigned tb = h | 8u | 16u | (sb > cab + cbc ? 2u : 0u) | (sa > cab ? 1u : 0u) | (sc > cbc ? 4u : 0u);
-      const unsigned tc = h | 16u | 32u | (sc > cac + cbc ? 4u : 0u) | (sa > cac ? 1u : 0u) | (sb > cbc ? 2u : 0u);
-      atomicAdd(&hc[tv * kHubCopies], 1);
-      atomicAdd(&hc[p0 * kHubCopies], -1);
-      atomicAdd(&hc[p1ab * kHubCopies], 1);
-      atomicAdd(&hc[p1bc * kHubCopies], 1);
-      atomicAdd(&hc[p1ca * kHubCopies], 1);
-      atomicAdd(&hc[ta * kHubCopies], -1);
-      atomicAdd(&hc[tb * kHubCopies], -1);
-      atomicAdd(&hc[tc * kHubCopies], -1);
+      if (mc) {
+        light_triangle(hc, Aab, Aa_b, Ab_a, hm[rc], sa, sb, sz[rc], cab, (int)(ec & kCntMask), popc(mc),
+                       popc((MT)(mab & mc)));
+        ++ntri;
+      }
+    };
+    uint32_t j = lane;
+    for (; j + (kHubLoads - 1) * kWave < nB; j += kHubLoads * kWave) {
+      uint32_t e[kHubLoads];
+#pragma unroll
+      for (int u = 0; u < kHubLoads; ++u) e[u] = rowB[j + u * kWave];
+#pragma unroll
+      for (int u = 0; u < kHubLoads; ++u) visit(e[u]);
     }
+    for (; j < nB; j += kWave) visit(rowB[j]);
   }
+  ntri = wave_sum(ntri);
+  if (lane == 0 && ntri) atomicAdd(&acc[kAccLightTri], ntri);
   __syncthreads();
   fold_ids(hs, 1ull, acc);
+}
+
+// Upper rows of the light projection over rank tiles of T (vrank = V2H relabelled to light ranks,
+// ascending per node): entries (rank << 7 | C) grouped by tile in ascending tile order, as
+// k_proj_fill_tiled writes them, from a's light nodes only.
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_hub_fill_tiled(
+    int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vrank, const int8_t* __restrict__ hub,
+    const uint8_t* __restrict__ lsz, const int32_t* __restrict__ rank,
+    const unsigned long long* __restrict__ roff, uint32_t* __restrict__ rlen, uint32_t* __restrict__ rows,
+    uint32_t* __restrict__ bad) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  __shared__ int32_t s_ln[HM_MAX_EDGE_SIZE];
+  __shared__ TileRanges tr;
+  __shared__ uint32_t s_cnt;
+  const int32_t a = blockIdx.x;
+  const int32_t ra = rank[a];
+  const int la = lsz[a];
+  if (la == 0) {
+    if (threadIdx.x == 0) rlen[ra] = 0;
+    return;
+  }
+  light_nodes(s_ln, a, eoff, enodes, hub);
+  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+  if (threadIdx.x == 0) s_cnt = 0;
+  uint32_t* out = rows + roff[ra];
+  const uint32_t cap = (uint32_t)(roff[ra + 1] - roff[ra]);
+  for (int32_t t0 = ((ra + 1) / T) * T; t0 < E; t0 += T) {
+    const int32_t t1 = min(E, t0 + T);
+    const uint32_t total = tile_ranges(tr, la, s_ln, voff, vrank, max(t0, ra + 1), t1);
+    if (total == 0) continue;
+    for_tile_entries(tr, la, total, vrank, [&](int i, int32_t x) { atomicOr(&M[x - t0], (MT)1 << i); });
+    __syncthreads();
+    for_tile_entries(tr, la, total, vrank, [&](int i, int32_t x) {
+      const MT m = M[x - t0];
+      if (m && low_bit(m) == i) {
+        const uint32_t pos = atomicAdd(&s_cnt, 1u);
+        if (pos < cap) out[pos] = ((uint32_t)x << kCntBits) | (uint32_t)popc(m);
+        M[x - t0] = (MT)0;
+      }
+    });
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    rlen[ra] = min(s_cnt, cap);
+    if (s_cnt > cap) atomicOr(bad, 1u);
+  }
+}
+
+// (D) over rank tiles of c, as k_tri_tiled walks the full projection: per b in r's upper light
+// row, state at uoff[r] + ib keeps a's light mask in b and the read position in b's row (grouped
+// by tile); tiles without work are skipped. The int32 slot counters are folded and cleared per
+// tile (the host bounds their count per tile).
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_hub_tri_tiled(
+    int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
+    const uint32_t* __restrict__ voff, const int32_t* __restrict__ vrank, const int8_t* __restrict__ hub,
+    const int32_t* __restrict__ perm, const uint32_t* __restrict__ hmr, const uint8_t* __restrict__ lszr,
+    const unsigned long long* __restrict__ roff, const uint32_t* __restrict__ rlen,
+    const uint32_t* __restrict__ rows, const unsigned long long* __restrict__ uoff,
+    uint32_t* __restrict__ cur, MT* __restrict__ pm, unsigned long long* __restrict__ acc, int32_t r0) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MT* M = reinterpret_cast<MT*>(smem);
+  uint32_t* hmt = reinterpret_cast<uint32_t*>(M + T);
+  uint8_t* szt = reinterpret_cast<uint8_t*>(hmt + T);
+  __shared__ int32_t hs[kHubIds * kHubCopies];
+  __shared__ unsigned long long hb[32];
+  __shared__ int32_t s_ln[HM_MAX_EDGE_SIZE];
+  __shared__ TileRanges tr;
+  __shared__ uint32_t s_end;
+  __shared__ int32_t s_next;
+  const int32_t r = r0 + (int32_t)blockIdx.x;
+  const uint32_t nA = rlen[r];
+  if (nA < 2) return;
+  const int32_t a = perm[r];
+  const int la = lszr[r];
+  light_nodes(s_ln, a, eoff, enodes, hub);
+  for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0;
+  if (threadIdx.x < 32) hb[threadIdx.x] = 0ull;
+  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+
+  const int lane = lane_id(), w = wave_id();
+  int32_t* hc = hs + (lane & (kHubCopies - 1));
+  const uint32_t Aa = hmr[r];
+  const uint32_t* rowA = rows + roff[r];
+  const unsigned long long st = uoff[r];
+  uint32_t prev_end = 0;
+  unsigned long long ntri = 0;
+  __syncthreads();
+  int32_t k = (int32_t)(rowA[0] >> kCntBits) / T;
+  for (;;) {
+    const int32_t t0 = k * T, t1 = min(E, t0 + T);
+    if (threadIdx.x == 0) {  // end of the b entries whose rank is below t1 (row grouped by tile)
+      uint32_t lo = prev_end, hi = nA;
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if ((int32_t)(rowA[mid] >> kCntBits) < t1) lo = mid + 1;
+        else hi = mid;
+      }
+      s_end = lo;
+    }
+    const uint32_t total = tile_ranges(tr, la, s_ln, voff, vrank, max(t0, r + 1), t1);
+    if (threadIdx.x == 0) s_next = INT32_MAX;
+    for_tile_entries(tr, la, total, vrank, [&](int i, int32_t x) {
+      atomicOr(&M[x - t0], (MT)1 << i);
+      szt[x - t0] = lszr[x];
+      hmt[x - t0] = hmr[x];
+    });
+    __syncthreads();
+    const uint32_t end = s_end;
+    for (uint32_t base = (uint32_t)w * kWave; base < end; base += kWavesPerBlock * kWave) {
+      const uint32_t ib = base + lane;
+      const bool have = ib < end;
+      int32_t rb = 0;
+      int cab = 0;
+      uint32_t cb = 0, nB = 0;
+      unsigned long long ob = 0;
+      MT mab = (MT)0;
+      bool work = false;
+      if (have) {
+        const uint32_t eb = rowA[ib];
+        rb = (int32_t)(eb >> kCntBits);
+        cab = (int)(eb & kCntMask);
+        ob = roff[rb];
+        nB = rlen[rb];
+        if (ib >= prev_end) {
+          mab = pair_mask<MT>(s_ln, la, eoff, enodes, perm[rb]);
+          pm[st + ib] = mab;
+        } else {
+          mab = pm[st + ib];
+          cb = cur[st + ib];
+        }
+        work = cb < nB && (int32_t)(rows[ob + cb] >> kCntBits) < t1;
+      }
+      unsigned long long wl = __ballot(work);
+      while (wl) {
+        const int leader = __ffsll((long long)wl) - 1;
+        wl &= wl - 1;
+        const int32_t rbL = __shfl(rb, leader);
+        const int cabL = __shfl(cab, leader);
+        const uint32_t nBL = __shfl(nB, leader);
+        const unsigned long long obL = __shfl(ob, leader);
+        const MT mabL = __shfl(mab, leader);
+        uint32_t pos = __shfl(cb, leader);
+        const int sbL = lszr[rbL];
+        const uint32_t AbL = hmr[rbL];
+        const uint32_t Aab = Aa & AbL, Aa_b = Aa & ~AbL, Ab_a = AbL & ~Aa;
+        const uint32_t* rowB = rows + obL;
+        // kHubLoads chunks of b's row per step; the row is grouped by tile, so the entries inside
+        // the tile are a prefix from pos and each chunk's valid entries follow the previous chunk's
+        for (;;) {
+          uint32_t e[kHubLoads];
+          bool v[kHubLoads];
+#pragma unroll
+          for (int u = 0; u < kHubLoads; ++u) {
+            const uint32_t j = pos + (uint32_t)(u * kWave) + lane;
+            e[u] = j < nBL ? rowB[j] : 0xFFFFFFFFu;
+            v[u] = j < nBL && (int32_t)(e[u] >> kCntBits) < t1;
+          }
+          uint32_t nv = 0;
+#pragma unroll
+          for (int u = 0; u < kHubLoads; ++u) {
+            if (v[u]) {
+              const int32_t rc = (int32_t)(e[u] >> kCntBits);
+              const MT mc = M[rc - t0];
+              if (mc) {
+                light_triangle(hc, Aab, Aa_b, Ab_a, hmt[rc - t0], la, sbL, szt[rc - t0], cabL,
+                               (int)(e[u] & kCntMask), popc(mc), popc((MT)(mabL & mc)));
+                ++ntri;
+              }
+            }
+            nv += (uint32_t)__popcll(__ballot(v[u]));
+          }
+          pos += nv;
+          if (nv < (uint32_t)(kHubLoads * kWave)) break;
+        }
+        if (lane == leader) cb = pos;
+      }
+      if (have) {
+        cur[st + ib] = cb;
+        if (cb < nB) atomicMin(&s_next, (int32_t)(rows[ob + cb] >> kCntBits) / T);
+      }
+    }
+    if (threadIdx.x == 0 && end < nA) atomicMin(&s_next, (int32_t)(rowA[end] >> kCntBits) / T);
+    __syncthreads();
+    for (int id = threadIdx.x; id < kHubIds; id += blockDim.x) {
+      long long n = 0;
+      for (int c = 0; c < kHubCopies; ++c) {
+        n += hs[id * kHubCopies + c];
+        hs[id * kHubCopies + c] = 0;
+      }
+      const int bin = c_hub_bin[id];
+      if (n && bin >= 0) atomicAdd(&hb[bin], (unsigned long long)n);
+    }
+    __syncthreads();
+    if (threadIdx.x < kStaticBins && hb[threadIdx.x]) {
+      atomicAdd(&acc[kAccHist + threadIdx.x], hb[threadIdx.x]);
+      hb[threadIdx.x] = 0ull;
+    }
+    for_tile_entries(tr, la, total, vrank, [&](int, int32_t x) { M[x - t0] = (MT)0; });
+    const int32_t kn = s_next;
+    prev_end = end;
+    if (kn == INT32_MAX) break;
+    k = kn;
+  }
+  ntri = wave_sum(ntri);
+  if (lane == 0 && ntri) atomicAdd(&acc[kAccLightTri], ntri);
+}
+
+// V2H entries as (node << 32 | light rank of the hyperedge), sorted to relabel the rows.
+__global__ void k_hub_node_keys(int32_t V, int64_t I, const uint32_t* __restrict__ voff,
+                                const int32_t* __restrict__ vedges, const int32_t* __restrict__ rank,
+                                unsigned long long* __restrict__ keys) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= I) return;
+  int32_t lo = 0, hi = V;  // largest v with voff[v] <= k
+  while (hi - lo > 1) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if ((int64_t)voff[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  keys[k] = ((unsigned long long)lo << 32) | (unsigned long long)(uint32_t)rank[vedges[k]];
+}
+__global__ void k_hub_low_words(const unsigned long long* __restrict__ keys, int32_t* __restrict__ out,
+                                int64_t I) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < I) out[k] = (int32_t)(uint32_t)keys[k];
 }
 
 __global__ void k_hub_zero(unsigned long long* p, int n) {
@@ -416,13 +813,10 @@ __global__ void k_hub_widen(const uint32_t* __restrict__ in, unsigned long long*
 // ---------------------------------------------------------------------------------------------
 // Host side
 
-// Dynamic LDS of k_hub_centre (mask table + 8-byte neighbour list) and of k_hub_tri (mask table
-// + light sizes by rank); the static slot counters (8 KB) and fold bins sit beside them.
-template <typename MT>
-static size_t hub_centre_lds(int32_t E) { return (((size_t)E * sizeof(MT) + 7) & ~(size_t)7) + (size_t)E * 8; }
-template <typename MT>
-static size_t hub_tri_lds(int32_t E) { return (size_t)E * (sizeof(MT) + 1); }
-constexpr size_t kHubStaticLds = 12 * 1024;
+// Static LDS beside the dynamic tables: k_hub_centre's u64 slot counters (16 KB) and small
+// arrays; k_hub_tri(_tiled)'s int32 counters (8 KB).
+constexpr size_t kHubCentreStatic = 17 * 1024 + 1024;
+constexpr size_t kHubTriStatic = 12 * 1024;
 
 static int hub_lds_limit(int device) {
   int v = 0;
@@ -431,19 +825,16 @@ static int hub_lds_limit(int device) {
   return v;
 }
 
-bool hub_fits(const hm_graph* g, int device) {
-  const int lim = hub_lds_limit(device);
-  const int32_t E = g->csr.E;
-  if (g->csr.max_size > 32)
-    return hub_centre_lds<mask64_t>(E) + kHubStaticLds <= (size_t)lim &&
-           hub_tri_lds<mask64_t>(E) + kHubStaticLds <= (size_t)lim;
-  return hub_centre_lds<uint32_t>(E) + kHubStaticLds <= (size_t)lim &&
-         hub_tri_lds<uint32_t>(E) + kHubStaticLds <= (size_t)lim;
-}
+// Centre tile: ids per mask table of k_hub_centre; the region also holds the compacted histogram.
+template <typename MT>
+static int32_t hub_centre_tile(int32_t E) { return std::min<int32_t>(E, sizeof(MT) == 4 ? 8192 : 4096); }
+template <typename MT>
+static size_t hub_centre_lds(int32_t T) { return (size_t)T * sizeof(MT) + (size_t)kHubHash * 12; }
 
-// Hub count chosen by the cost model: the h in {1, 2, 4, ..., 31} minimising class triples
-// K^3 / 6 + light wedges (bound: C(sum over e's light nodes of (deg - 1), 2) per hyperedge) +
-// pair types (bound: min(light pairs, 4 K^2)) x K, from the store read back to the host.
+// Hub count chosen by the cost model over h in {2, 4, ..., 23}: class triples K^3 / 6, plus light
+// node triples sum_{v light} C(deg v, 3) (a proxy of the light triangles, 8 slot updates each) and
+// light node pairs sum_{v light} C(deg v, 2) (pair types and key histograms), from the store read
+// back to the host.
 hm_status hub_choose(hm_graph* g, int32_t* h_out, double* cost_out) {
   DeviceCsr& c = g->csr;
   const int32_t E = c.E, V = c.V;
@@ -456,7 +847,7 @@ hm_status hub_choose(hm_graph* g, int32_t* h_out, double* cost_out) {
   HM_HIP(hipStreamSynchronize(g->stream));
   std::vector<int32_t> order((size_t)V);
   std::iota(order.begin(), order.end(), 0);
-  const int32_t top = std::min<int32_t>(kHubMax, V);
+  const int32_t top = std::min<int32_t>(kHubKeyHubs, V);
   auto deg = [&](int32_t v) { return (double)(voff[v + 1] - voff[v]); };
   std::partial_sort(order.begin(), order.begin() + top, order.end(), [&](int32_t x, int32_t y) {
     const uint32_t dx = voff[x + 1] - voff[x], dy = voff[y + 1] - voff[y];
@@ -464,33 +855,36 @@ hm_status hub_choose(hm_graph* g, int32_t* h_out, double* cost_out) {
   });
   std::vector<int8_t> idx((size_t)std::max(V, 1), (int8_t)-1);
   for (int32_t i = 0; i < top; ++i) idx[order[i]] = (int8_t)i;
-  double lp_all = 0;
-  for (int32_t v = 0; v < V; ++v) lp_all += deg(v) * (deg(v) - 1) / 2;
+  double lp_all = 0, lt_all = 0;
+  for (int32_t v = 0; v < V; ++v) {
+    const double d = deg(v);
+    lp_all += d * (d - 1) / 2;
+    lt_all += d * (d - 1) * (d - 2) / 6;
+  }
   int32_t best_h = 1;
   double best = -1;
   std::vector<unsigned long long> keys((size_t)E);
-  for (int32_t h : {1, 2, 4, 6, 8, 10, 12, 16, 20, 24, 31}) {
+  for (int32_t h : {2, 4, 6, 8, 10, 12, 14, 16, 20, 23}) {
     if (h > top) break;
-    double lw = 0, lp = lp_all;
-    for (int32_t i = 0; i < h; ++i) lp -= deg(order[i]) * (deg(order[i]) - 1) / 2;
+    double lp = lp_all, lt = lt_all;
+    for (int32_t i = 0; i < h; ++i) {
+      const double d = deg(order[i]);
+      lp -= d * (d - 1) / 2;
+      lt -= d * (d - 1) * (d - 2) / 6;
+    }
     for (int32_t e = 0; e < E; ++e) {
       uint32_t m = 0;
       bool light = false;
-      double sum = 0;
       for (uint32_t i = eoff[e]; i < eoff[e + 1]; ++i) {
         const int32_t v = enodes[i];
         if (idx[v] >= 0 && idx[v] < h) m |= 1u << idx[v];
-        else {
-          light = true;
-          sum += deg(v) - 1;
-        }
+        else light = true;
       }
       keys[e] = (unsigned long long)m << 1 | (light ? 1ull : 0ull);
-      lw += sum * (sum - 1) / 2;
     }
     std::sort(keys.begin(), keys.end());
     const double K = (double)(std::unique(keys.begin(), keys.end()) - keys.begin());
-    const double cost = K * K * K / 6 + lw + std::min(lp, 4 * K * K) * K;
+    const double cost = K * K * K / 6 + 8 * lt + 4 * lp;
     if (best < 0 || cost < best) {
       best = cost;
       best_h = h;
@@ -501,18 +895,50 @@ hm_status hub_choose(hm_graph* g, int32_t* h_out, double* cost_out) {
   return HM_OK;
 }
 
+// The fill's capacity flag: a light row longer than the light degree the centre pass counted would
+// mean the two passes disagree; the count stops rather than use the truncated rows.
+static hm_status check_fill(const unsigned long long* flag, hipStream_t st) {
+  unsigned long long f = 0;
+  HM_HIP(hipMemcpyAsync(&f, flag, sizeof(f), hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  if (f) return fail(HM_ERR_DEVICE, "hub-aware count: a light row outgrew its light degree");
+  return HM_OK;
+}
+
 template <typename MT>
-static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long long* h_acc) {
+static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow) {
   DeviceCsr& c = g->csr;
   CountScratch& s = g->scratch;
   HubScratch& hb = g->hub;
   const int32_t E = c.E, V = c.V;
+  const int64_t I = c.I;
   hipStream_t st = g->stream;
-  const size_t lds_centre = hub_centre_lds<MT>(E), lds_tri = hub_tri_lds<MT>(E);
-  const size_t lds_fill = (size_t)E * sizeof(MT);
+  *key_overflow = false;
+  const bool verbose = std::getenv("HM_VERBOSE") != nullptr;
+  const int lim = hub_lds_limit(g->device);
+  // tiles: automatic, or forced by hm_graph_set_tile (tests and tuning; results do not change)
+  const int32_t Tc = g->tile > 0 ? std::min<int32_t>(E, std::min<int32_t>(g->tile, hub_centre_tile<MT>(E)))
+                                 : hub_centre_tile<MT>(E);
+  const size_t lds_centre = hub_centre_lds<MT>(Tc);
+  if (lds_centre + kHubCentreStatic > (size_t)lim)
+    return fail(HM_ERR_UNSUPPORTED, "hub-aware count: the centre tables exceed the LDS limit");
+  constexpr size_t kTriKey = sizeof(MT) + 5;  // light mask, hub mask, light size per key
+  const bool dense = g->tile <= 0 && (size_t)E * kTriKey + kHubTriStatic <= (size_t)lim;
+  const int32_t Tt = dense ? E
+                           : g->tile > 0 ? g->tile
+                                         : (int32_t)(((size_t)lim - kHubTriStatic - 1024) / kTriKey) & ~255;
+  if (!dense && (size_t)Tt * kTriKey + kHubTriStatic > (size_t)lim)
+    return fail(HM_ERR_BAD_INPUT, "hub-aware count: the forced tile exceeds the LDS limit");
   HM_TRY(allow_lds((const void*)k_hub_centre<MT>, lds_centre));
-  HM_TRY(allow_lds((const void*)k_hub_fill<MT>, lds_fill));
-  HM_TRY(allow_lds((const void*)k_hub_tri<MT>, lds_tri));
+  if (dense) {
+    HM_TRY(allow_lds((const void*)k_hub_fill<MT>, (size_t)E * sizeof(MT)));
+    HM_TRY(allow_lds((const void*)k_hub_tri<MT>, (size_t)E * kTriKey));
+  } else {
+    HM_TRY(allow_lds((const void*)k_hub_fill_tiled<MT>, (size_t)Tt * sizeof(MT)));
+    HM_TRY(allow_lds((const void*)k_hub_tri_tiled<MT>, (size_t)Tt * kTriKey));
+  }
+  if (dense ? false : I >= ((int64_t)1 << 31))
+    return fail(HM_ERR_UNSUPPORTED, "hub-aware count: tiled path supports fewer than 2^31 incidences");
 
   // hubs: the h nodes of largest V2H degree (ties by smaller id); light pairs are bounded by the
   // sum over light nodes of C(deg, 2)
@@ -528,12 +954,12 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   });
   std::vector<int8_t> hubidx((size_t)std::max(V, 1), (int8_t)-1);
   for (int32_t i = 0; i < hh; ++i) hubidx[order[i]] = (int8_t)i;
-  unsigned long long lp_bound = 0;
-  for (int32_t v = 0; v < V; ++v)
-    if (hubidx[v] < 0) {
-      const unsigned long long d = voff[v + 1] - voff[v];
-      lp_bound += d * (d - (d > 0)) / 2;
-    }
+  unsigned long long lp_bound = 0, dmax = 0;
+  for (int32_t v = 0; v < V; ++v) {
+    const unsigned long long d = voff[v + 1] - voff[v];
+    dmax = std::max(dmax, d);
+    if (hubidx[v] < 0) lp_bound += d * (d - (d > 0)) / 2;
+  }
   lp_bound = std::min<unsigned long long>(lp_bound, (unsigned long long)E * (unsigned long long)(E - 1) / 2);
   g->hub_stats = hm_hub_stats{};
   g->hub_stats.hubs = hh;
@@ -545,13 +971,9 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   HM_TRY(hb.ckey2.ensure(sizeof(unsigned long long) * E));
   HM_TRY(hb.ukey.ensure(sizeof(unsigned long long) * E));
   HM_TRY(hb.ucnt.ensure(sizeof(uint32_t) * E));
-  HM_TRY(hb.nruns.ensure(sizeof(unsigned long long) * 4));
+  HM_TRY(hb.nruns.ensure(sizeof(unsigned long long) * 8));
   HM_TRY(hb.hmr.ensure(sizeof(uint32_t) * E));
   HM_TRY(hb.lszr.ensure((size_t)E));
-  HM_TRY(hb.types.ensure(sizeof(unsigned long long) * std::max<unsigned long long>(lp_bound, 1)));
-  HM_TRY(hb.types2.ensure(sizeof(unsigned long long) * std::max<unsigned long long>(lp_bound, 1)));
-  HM_TRY(hb.tcnt.ensure(sizeof(uint32_t) * std::max<unsigned long long>(lp_bound, 1)));
-  HM_TRY(hb.tkey.ensure(sizeof(unsigned long long) * std::max<unsigned long long>(lp_bound, 1)));
   HM_TRY(s.deg.ensure(sizeof(uint32_t) * E));
   HM_TRY(s.ids.ensure(sizeof(int32_t) * E));
   HM_TRY(s.perm.ensure(sizeof(int32_t) * E));
@@ -561,23 +983,25 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   HM_TRY(s.rlen.ensure(sizeof(uint32_t) * E));
   HM_TRY(s.work2.ensure(sizeof(unsigned long long) * E));
   HM_TRY(s.acc.ensure(sizeof(unsigned long long) * kAccCount));
+  HM_TRY(s.acc2.ensure(sizeof(unsigned long long) * kAccCount));
   unsigned long long* acc = s.acc.as<unsigned long long>();
-  unsigned long long* runs = hb.nruns.as<unsigned long long>();  // [0] classes, [1] light pairs, [2] types
+  unsigned long long* accC = s.acc2.as<unsigned long long>();
+  // runs: [0] classes, [1] pair-type entries, [2] distinct pair types, [3] key-overflow flag
+  unsigned long long* runs = hb.nruns.as<unsigned long long>();
   const unsigned eb = (unsigned)((E + kBlock - 1) / kBlock);
   const int tbits = 32 + hh + 1;
-  const unsigned long long tcap = hb.types.bytes / sizeof(unsigned long long);
 
   HM_HIP(hipMemcpyAsync(hb.hubidx.p, hubidx.data(), (size_t)std::max(V, 1), hipMemcpyHostToDevice, st));
   HM_HIP(hipEventRecord(g->ev[0], st));
   k_hub_zero<<<1, 64, 0, st>>>(acc, kAccCount);
-  k_hub_zero<<<1, 64, 0, st>>>(runs, 4);
+  k_hub_zero<<<1, 64, 0, st>>>(runs, 8);  // [4]: a light row outgrew the light degree
   k_hub_class<<<eb, kBlock, 0, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), hb.hubidx.as<int8_t>(),
                                      hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(),
                                      hb.ckey.as<unsigned long long>());
   HM_HIP(hipGetLastError());
 
   // (A) classes: sorted unique keys with counts
-  size_t tb1 = 0, tb2 = 0, tb3 = 0, tb4 = 0, tb5 = 0;
+  size_t tb1 = 0, tb2 = 0, tb3 = 0, tb4 = 0;
   HM_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, hb.ckey.as<unsigned long long>(),
                                            hb.ckey2.as<unsigned long long>(), E, 0, hh + 1, st));
   HM_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, hb.ckey2.as<unsigned long long>(),
@@ -587,11 +1011,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
                                             s.ids.as<int32_t>(), s.perm.as<int32_t>(), E, 0, 32, st));
   HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb4, s.degr.as<unsigned long long>(),
                                           s.roff.as<unsigned long long>(), E + 1, st));
-  HM_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb5, hb.types.as<unsigned long long>(),
-                                           hb.types2.as<unsigned long long>(), (int)std::min<unsigned long long>(tcap, INT32_MAX),
-                                           0, tbits, st));
-  size_t tbm = std::max(std::max(tb1, tb2), std::max(std::max(tb3, tb4), tb5));
-  HM_TRY(s.tmp.ensure(tbm));
+  HM_TRY(s.tmp.ensure(std::max(std::max(tb1, tb2), std::max(tb3, tb4))));
   size_t tb = s.tmp.bytes;
   HM_HIP(hipcub::DeviceRadixSort::SortKeys(s.tmp.p, tb, hb.ckey.as<unsigned long long>(),
                                            hb.ckey2.as<unsigned long long>(), E, 0, hh + 1, st));
@@ -607,38 +1027,72 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   int32_t k0 = 0;  // classes with no hub: keys 0 and 1
   while (k0 < K && k0 < 2 && (first_keys[k0] >> 1) == 0) ++k0;
   g->hub_stats.classes = K;
+  if (verbose) std::fprintf(stderr, "[hm hub] %d hubs, %d classes\n", hh, K);
+  HM_HIP(hipEventRecord(g->ev[1], st));
+
+  // light neighbours: (C) into accC, light degrees, pair-type entries for (B). A pass whose entries
+  // outgrow the buffer runs again with a buffer of the reported size (accC starts from zero).
+  unsigned long long cap = std::max<unsigned long long>(1, std::min<unsigned long long>(
+                                                              lp_bound, std::max<unsigned long long>((unsigned long long)E * 64, 1u << 20)));
+  unsigned long long nt = 0;
+  for (int attempt = 0;; ++attempt) {
+    HM_TRY(hb.types.ensure(sizeof(unsigned long long) * cap));
+    HM_TRY(hb.tcraw.ensure(sizeof(unsigned long long) * cap));
+    cap = std::min(hb.types.bytes, hb.tcraw.bytes) / sizeof(unsigned long long);
+    k_hub_zero<<<1, 64, 0, st>>>(accC, kAccCount);
+    k_hub_zero<<<1, 64, 0, st>>>(runs + 1, 3);
+    k_hub_centre<MT><<<E, kBlock, lds_centre, st>>>(
+        E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+        hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
+        hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), runs + 1, cap,
+        reinterpret_cast<uint32_t*>(runs + 3), accC);
+    HM_HIP(hipGetLastError());
+    unsigned long long r2[3] = {0, 0, 0};
+    HM_HIP(hipMemcpyAsync(r2, runs + 1, sizeof(r2), hipMemcpyDeviceToHost, st));
+    HM_HIP(hipStreamSynchronize(st));
+    if (r2[2]) {
+      *key_overflow = true;
+      set_error("hub-aware count: a hyperedge's light neighbours hold more than %u distinct (hub pattern, "
+                "overlap) keys with %d hubs; use fewer hubs", kHubKeysMax, hh);
+      return HM_ERR_UNSUPPORTED;
+    }
+    nt = r2[0];
+    if (nt <= cap) break;
+    if (attempt > 0) return fail(HM_ERR_DEVICE, "hub-aware count: pair-type entries grew between passes");
+    cap = nt;
+  }
+  g->hub_stats.light_pairs = -1;
+  HM_HIP(hipEventRecord(g->ev[2], st));
+  // (A) once the histograms are known to hold (class triples are the costly part at many hubs)
   if (K > k0) {
     k_hub_classes<<<(unsigned)(K - k0), kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(),
                                                          hb.ucnt.as<uint32_t>(), acc);
     HM_HIP(hipGetLastError());
   }
-  HM_HIP(hipEventRecord(g->ev[1], st));
+  if (verbose) std::fprintf(stderr, "[hm hub] classes and light neighbours done, %llu pair-type entries\n", nt);
 
-  // light neighbours: (C), light degrees, pair types of the light pairs for (B)
-  k_hub_centre<MT><<<E, kBlock, lds_centre, st>>>(
-      E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
-      hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
-      hb.types.as<unsigned long long>(), runs + 1, tcap, acc);
-  HM_HIP(hipGetLastError());
-  unsigned long long nt = 0;
-  HM_HIP(hipMemcpyAsync(&nt, runs + 1, sizeof(nt), hipMemcpyDeviceToHost, st));
-  HM_HIP(hipStreamSynchronize(st));
-  if (nt > tcap) return fail(HM_ERR_DEVICE, "hub-aware count: more light pairs than their bound");
-  g->hub_stats.light_pairs = (int64_t)nt;
-  HM_HIP(hipEventRecord(g->ev[2], st));
+  // (B) pair types: sum the entries' counts per type, then every type against every class
   if (nt > 0) {
+    size_t need1 = 0, need2 = 0;
+    HM_TRY(hb.types2.ensure(sizeof(unsigned long long) * nt));
+    HM_TRY(hb.tcsrt.ensure(sizeof(unsigned long long) * nt));
+    HM_TRY(hb.tkey.ensure(sizeof(unsigned long long) * nt));
+    HM_TRY(hb.tcnt.ensure(sizeof(unsigned long long) * nt));
+    HM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need1, hb.types.as<unsigned long long>(),
+                                              hb.types2.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(),
+                                              hb.tcsrt.as<unsigned long long>(), (int)nt, 0, tbits, st));
+    HM_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, need2, hb.types2.as<unsigned long long>(),
+                                             hb.tkey.as<unsigned long long>(), hb.tcsrt.as<unsigned long long>(),
+                                             hb.tcnt.as<unsigned long long>(), runs + 2, hipcub::Sum(), (int)nt, st));
+    HM_TRY(s.tmp.ensure(std::max(need1, need2)));
     tb = s.tmp.bytes;
-    HM_HIP(hipcub::DeviceRadixSort::SortKeys(s.tmp.p, tb, hb.types.as<unsigned long long>(),
-                                             hb.types2.as<unsigned long long>(), (int)nt, 0, tbits, st));
-    size_t need = 0;
-    HM_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, need, hb.types2.as<unsigned long long>(),
-                                                 hb.tkey.as<unsigned long long>(), hb.tcnt.as<uint32_t>(),
-                                                 runs + 2, (int)nt, st));
-    HM_TRY(s.tmp.ensure(need));
+    HM_HIP(hipcub::DeviceRadixSort::SortPairs(s.tmp.p, tb, hb.types.as<unsigned long long>(),
+                                              hb.types2.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(),
+                                              hb.tcsrt.as<unsigned long long>(), (int)nt, 0, tbits, st));
     tb = s.tmp.bytes;
-    HM_HIP(hipcub::DeviceRunLengthEncode::Encode(s.tmp.p, tb, hb.types2.as<unsigned long long>(),
-                                                 hb.tkey.as<unsigned long long>(), hb.tcnt.as<uint32_t>(),
-                                                 runs + 2, (int)nt, st));
+    HM_HIP(hipcub::DeviceReduce::ReduceByKey(s.tmp.p, tb, hb.types2.as<unsigned long long>(),
+                                             hb.tkey.as<unsigned long long>(), hb.tcsrt.as<unsigned long long>(),
+                                             hb.tcnt.as<unsigned long long>(), runs + 2, hipcub::Sum(), (int)nt, st));
     unsigned long long T = 0;
     HM_HIP(hipMemcpyAsync(&T, runs + 2, sizeof(T), hipMemcpyDeviceToHost, st));
     HM_HIP(hipStreamSynchronize(st));
@@ -646,7 +1100,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
     if (T > 0 && K > k0) {
       k_hub_types<<<(unsigned)T, kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(),
                                                    hb.ucnt.as<uint32_t>(), hb.tkey.as<unsigned long long>(),
-                                                   hb.tcnt.as<uint32_t>(), acc);
+                                                   hb.tcnt.as<unsigned long long>(), acc);
       HM_HIP(hipGetLastError());
     }
   }
@@ -667,26 +1121,94 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   unsigned long long total = 0;
   HM_HIP(hipMemcpyAsync(&total, s.roff.as<unsigned long long>() + E, sizeof(total), hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
-  g->hub_stats.light_wedges = -1;
+  g->hub_stats.light_pairs = (int64_t)(total / 2);
+  if (verbose) std::fprintf(stderr, "[hm hub] pair types done; %llu light pairs\n", total / 2);
   HM_TRY(s.rows.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(total, 1)));
-  k_hub_fill<MT><<<E, kBlock, lds_fill, st>>>(
-      E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
-      hb.hubidx.as<int8_t>(), hb.lsz.as<uint8_t>(), s.rank.as<int32_t>(), s.roff.as<unsigned long long>(),
-      s.rlen.as<uint32_t>(), s.rows.as<uint32_t>());
-  HM_HIP(hipGetLastError());
-  k_hub_tri<MT><<<E, kBlock, lds_tri, st>>>(
-      E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
-      hb.hubidx.as<int8_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(),
-      hb.lszr.as<uint8_t>(), s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), acc);
-  HM_HIP(hipGetLastError());
+  if (dense) {
+    k_hub_fill<MT><<<E, kBlock, (size_t)E * sizeof(MT), st>>>(
+        E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+        hb.hubidx.as<int8_t>(), hb.lsz.as<uint8_t>(), s.rank.as<int32_t>(), s.roff.as<unsigned long long>(),
+        s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), reinterpret_cast<uint32_t*>(runs + 4));
+    HM_HIP(hipGetLastError());
+    HM_TRY(check_fill(runs + 4, st));
+    k_hub_tri<MT><<<E, kBlock, (size_t)E * kTriKey, st>>>(
+        E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+        hb.hubidx.as<int8_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(),
+        hb.lszr.as<uint8_t>(), s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), acc, 0);
+    HM_HIP(hipGetLastError());
+  } else {
+    // V2H relabelled to light ranks, ascending within every node's row
+    HM_TRY(s.vkeys.ensure(sizeof(unsigned long long) * std::max<int64_t>(I, 1)));
+    HM_TRY(s.vkeys2.ensure(sizeof(unsigned long long) * std::max<int64_t>(I, 1)));
+    HM_TRY(s.vrank.ensure(sizeof(int32_t) * std::max<int64_t>(I, 1)));
+    HM_TRY(s.uoff.ensure(sizeof(unsigned long long) * (E + 1)));
+    int vbits = 1;
+    while (vbits < 31 && ((int64_t)1 << vbits) <= V) ++vbits;
+    size_t need = 0;
+    HM_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, need, s.vkeys.as<unsigned long long>(),
+                                             s.vkeys2.as<unsigned long long>(), (int)I, 0, 32 + vbits, st));
+    HM_TRY(s.tmp.ensure(need));
+    if (I > 0) {
+      const unsigned ib = (unsigned)((I + kBlock - 1) / kBlock);
+      k_hub_node_keys<<<ib, kBlock, 0, st>>>(V, I, c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+                                             s.rank.as<int32_t>(), s.vkeys.as<unsigned long long>());
+      tb = s.tmp.bytes;
+      HM_HIP(hipcub::DeviceRadixSort::SortKeys(s.tmp.p, tb, s.vkeys.as<unsigned long long>(),
+                                               s.vkeys2.as<unsigned long long>(), (int)I, 0, 32 + vbits, st));
+      k_hub_low_words<<<ib, kBlock, 0, st>>>(s.vkeys2.as<unsigned long long>(), s.vrank.as<int32_t>(), I);
+    }
+    k_hub_fill_tiled<MT><<<E, kBlock, (size_t)Tt * sizeof(MT), st>>>(
+        E, Tt, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), s.vrank.as<int32_t>(),
+        hb.hubidx.as<int8_t>(), hb.lsz.as<uint8_t>(), s.rank.as<int32_t>(), s.roff.as<unsigned long long>(),
+        s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), reinterpret_cast<uint32_t*>(runs + 4));
+    HM_HIP(hipGetLastError());
+    HM_TRY(check_fill(runs + 4, st));
+    // per (a, b) state at the scan of the upper-row lengths; the int32 slot counters of a tile
+    // take at most d+_max (T / 16 + 2) updates per slot and copy
+    std::vector<uint32_t> rl((size_t)E);
+    HM_HIP(hipMemcpyAsync(rl.data(), s.rlen.p, sizeof(uint32_t) * E, hipMemcpyDeviceToHost, st));
+    HM_HIP(hipStreamSynchronize(st));
+    unsigned long long U = 0, umax = 0;
+    for (uint32_t x : rl) {
+      U += x;
+      umax = std::max<unsigned long long>(umax, x);
+    }
+    if (umax * ((unsigned long long)Tt / kHubCopies + 4 * kHubLoads + 1) >= (1ull << 31))
+      return fail(HM_ERR_UNSUPPORTED, "hub-aware count: an upper light row is too long for the tile counters");
+    k_hub_widen<<<(unsigned)((E + 1 + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+        s.rlen.as<uint32_t>(), s.degr.as<unsigned long long>(), E);
+    tb = s.tmp.bytes;
+    HM_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp.p, tb, s.degr.as<unsigned long long>(),
+                                            s.uoff.as<unsigned long long>(), E + 1, st));
+    HM_TRY(s.cur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(U, 1)));
+    HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
+    // centres in chunks: with HM_VERBOSE set, a line per chunk reports the progress of long counts
+    const int32_t chunk = verbose ? std::max<int32_t>(4096, E / 16) : E;
+    for (int32_t r0 = 0; r0 < E; r0 += chunk) {
+      const int32_t n = std::min(chunk, E - r0);
+      k_hub_tri_tiled<MT><<<(unsigned)n, kBlock, (size_t)Tt * kTriKey, st>>>(
+          E, Tt, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), s.vrank.as<int32_t>(),
+          hb.hubidx.as<int8_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(), hb.lszr.as<uint8_t>(),
+          s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
+          s.uoff.as<unsigned long long>(), s.cur.as<uint32_t>(), s.pm.as<MT>(), acc, r0);
+      HM_HIP(hipGetLastError());
+      if (verbose) {
+        HM_HIP(hipStreamSynchronize(st));
+        std::fprintf(stderr, "[hm hub] light triangles: centres %d..%d of %d done\n", r0, r0 + n, E);
+      }
+    }
+  }
   HM_HIP(hipEventRecord(g->ev[4], st));
+  unsigned long long hC[kAccCount];
   HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipMemcpyAsync(hC, accC, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
-  // the projection statistics are not computed on this path
-  h_acc[kAccP] = h_acc[kAccMaxDeg] = h_acc[kAccOrient] = h_acc[kAccWedges] = h_acc[kAccUpper] = ~0ull;
-  unsigned long long dmax = 0;
-  for (int32_t v = 0; v < V; ++v) dmax = std::max<unsigned long long>(dmax, voff[v + 1] - voff[v]);
-  h_acc[kAccMaxNodeDeg] = dmax;
+  for (int k = 0; k < kStaticBins; ++k) h_acc[kAccHist + k] += hC[kAccHist + k];
+  // projection statistics from the projected degrees alone (the wedge identity
+  // sum(open) + 3 sum(closed) = sum_a C(d_a, 2) checks the bins against them)
+  HM_TRY(projection_stats(g, h_acc));
+  (void)dmax;
+  g->hub_stats.light_triangles = (int64_t)h_acc[kAccLightTri];
   float m01 = 0, m12 = 0, m23 = 0, m34 = 0;
   HM_HIP(hipEventElapsedTime(&m01, g->ev[0], g->ev[1]));
   HM_HIP(hipEventElapsedTime(&m12, g->ev[1], g->ev[2]));
@@ -701,10 +1223,10 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   return HM_OK;
 }
 
-hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc) {
-  if (h < 1 || h > kHubMax) return fail(HM_ERR_BAD_INPUT, "hub count outside 1..31");
-  return g->csr.max_size > 32 ? count_static_hub_impl<mask64_t>(g, h, h_acc)
-                              : count_static_hub_impl<uint32_t>(g, h, h_acc);
+hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow) {
+  if (h < 1 || h > kHubKeyHubs) return fail(HM_ERR_BAD_INPUT, "hub count outside 1..23");
+  return g->csr.max_size > 32 ? count_static_hub_impl<mask64_t>(g, h, h_acc, key_overflow)
+                              : count_static_hub_impl<uint32_t>(g, h, h_acc, key_overflow);
 }
 
 }  // namespace hm

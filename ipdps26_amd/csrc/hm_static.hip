@@ -877,6 +877,55 @@ int64_t graph_device_bytes(const hm_graph* g) {
   return n;
 }
 
+// Projection statistics of g (the kAccP .. kAccMaxNodeDeg slots of h_acc) from the projected and
+// lower degrees alone: k_proj_count(_tiled) and k_stats, no rows and no triangles (upper-row
+// total 0). The hub-aware count reports them beside its bins.
+hm_status projection_stats(hm_graph* g, unsigned long long* h_acc) {
+  DeviceCsr& c = g->csr;
+  CountScratch& s = g->scratch;
+  const int32_t E = c.E;
+  hipStream_t st = g->stream;
+  const int lim = lds_limit_bytes(g->device);
+  const bool wide = c.max_size > 32;
+  HM_TRY(s.deg.ensure(sizeof(uint32_t) * E));
+  HM_TRY(s.dlow.ensure(sizeof(uint32_t) * E));
+  HM_TRY(s.rlen.ensure(sizeof(uint32_t) * E));
+  HM_TRY(s.acc2.ensure(sizeof(unsigned long long) * kAccCount));
+  unsigned long long* acc = s.acc2.as<unsigned long long>();
+  k_zero_u64<<<1, 64, 0, st>>>(acc, kAccCount);
+  HM_HIP(hipMemsetAsync(s.rlen.p, 0, sizeof(uint32_t) * E, st));
+  auto run = [&](auto mt) -> hm_status {
+    using MT = decltype(mt);
+    const size_t dense = dense_lds_bytes<MT>(E, false);
+    if (dense + kTiledStaticLds <= (size_t)lim) {
+      HM_TRY(set_lds_attr<MT>((const void*)k_proj_count<MT>, dense));
+      k_proj_count<MT><<<E, kBlock, dense, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                                  c.esize.as<uint8_t>(), c.voff.as<uint32_t>(),
+                                                  c.vedges.as<int32_t>(), s.deg.as<uint32_t>(),
+                                                  s.dlow.as<uint32_t>(), acc);
+    } else {
+      const int32_t T = auto_tile<MT>(lim);
+      const size_t lds = dense_lds_bytes<MT>(T, false);
+      HM_TRY(set_lds_attr<MT>((const void*)k_proj_count_tiled<MT>, lds));
+      k_proj_count_tiled<MT><<<E, kBlock, lds, st>>>(E, T, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(),
+                                                      c.esize.as<uint8_t>(), c.voff.as<uint32_t>(),
+                                                      c.vedges.as<int32_t>(), s.deg.as<uint32_t>(),
+                                                      s.dlow.as<uint32_t>(), acc);
+    }
+    HM_HIP(hipGetLastError());
+    return HM_OK;
+  };
+  HM_TRY(wide ? run(mask64_t{}) : run(uint32_t{}));
+  k_stats<<<64, kBlock, 0, st>>>(E, c.V, c.voff.as<uint32_t>(), s.deg.as<uint32_t>(), s.dlow.as<uint32_t>(),
+                                 s.rlen.as<uint32_t>(), acc);
+  HM_HIP(hipGetLastError());
+  unsigned long long tmp[kAccCount];
+  HM_HIP(hipMemcpyAsync(tmp, acc, sizeof(tmp), hipMemcpyDeviceToHost, st));
+  HM_HIP(hipStreamSynchronize(st));
+  for (int k : {kAccP, kAccMaxDeg, kAccOrient, kAccWedges, kAccUpper, kAccMaxNodeDeg}) h_acc[k] = tmp[k];
+  return HM_OK;
+}
+
 // Graph statistics of the count just made (hm_graph_get_stats).
 static void fill_count_stats(hm_graph* g, const uint64_t out[HM_STATIC_BINS], const unsigned long long* acc) {
   const int32_t E = g->csr.E;
@@ -1031,17 +1080,26 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
     HM_HIP(hipSetDevice(g->device));
     g->hub_stats.hubs = 0;
     if (g->hubs != 0) {
-      const bool fits = hub_fits(g, g->device);
-      if (!fits && g->hubs > 0)
-        return fail(HM_ERR_UNSUPPORTED, "hub-aware count: the graph does not fit one LDS table");
-      if (fits) {
-        int32_t h = g->hubs;
-        if (h < 0) HM_TRY(hub_choose(g, &h, nullptr));
-        HM_TRY(count_static_hub(g, h, acc));
-        for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = acc[kAccHist + k];
-        fill_count_stats(g, out, acc);
-        return HM_OK;
+      // forced hub count: its result or its error; automatic: the cost model's choice, halved
+      // while some hyperedge's neighbour keys overflow their histogram, else the triangle path
+      int32_t h = g->hubs;
+      if (h < 0) HM_TRY(hub_choose(g, &h, nullptr));
+      for (;;) {
+        bool overflow = false;
+        const hm_status st = count_static_hub(g, h, acc, &overflow);
+        if (st == HM_OK) {
+          for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = acc[kAccHist + k];
+          fill_count_stats(g, out, acc);
+          return HM_OK;
+        }
+        if (g->hubs > 0 || !overflow) return st;
+        if (h == 1) break;
+        h = std::max<int32_t>(1, h * 3 / 4);
+        clear_error();
       }
+      g->hub_stats.hubs = 0;
+      for (int k = 0; k < kAccCount; ++k) acc[k] = 0;
+      clear_error();
     }
     const int lim = lds_limit_bytes(g->device);
     const bool wide = g->csr.max_size > 32;
@@ -1153,7 +1211,7 @@ hm_status hm_graph_set_tile(hm_graph* g, int32_t tile_ids) {
 hm_status hm_graph_set_hubs(hm_graph* g, int32_t hubs) {
   hm::clear_error();
   if (!g) return hm::fail(HM_ERR_BAD_INPUT, "null graph");
-  if (hubs < -1 || hubs > hm::kHubMax) return hm::fail(HM_ERR_BAD_INPUT, "hubs must be -1, 0 or 1..31");
+  if (hubs < -1 || hubs > hm::kHubMax) return hm::fail(HM_ERR_BAD_INPUT, "hubs must be -1, 0 or 1..23");
   g->hubs = hubs;
   return HM_OK;
 }

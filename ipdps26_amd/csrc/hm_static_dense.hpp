@@ -30,6 +30,7 @@ enum AccSlot : int {
   kAccWedges = 35,     // sum_a C(d_a, 2)
   kAccUpper = 36,      // sum_a d+_a (upper-row entries)
   kAccMaxNodeDeg = 37, // max_v |V2H(v)|
+  kAccLightTri = 38,   // light triangles (hub-aware count)
   kAccCount = 40
 };
 
