@@ -215,8 +215,8 @@ __device__ inline void light_nodes(int32_t* s_ln, int32_t a, const uint32_t* __r
 // accC, and one pair type per key with neighbours above a, with their number, appended to
 // (tkeys, tcnts) for (B) (the slot count keeps counting past cap so the host can grow them).
 // Dynamic LDS: T masks, then the histogram (keys, counts, counts above a: kHubHash each).
-template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_hub_centre(
+template <typename MT, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_hub_centre(
     int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges, const int8_t* __restrict__ hub,
     const uint32_t* __restrict__ hmask, const uint8_t* __restrict__ lsz, uint32_t* __restrict__ dL,
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_centre(
   __shared__ int32_t s_ln[HM_MAX_EDGE_SIZE];
   __shared__ TileRanges tr;
   __shared__ uint32_t s_used, s_g;
-  __shared__ uint32_t s_d[kWavesPerBlock];
+  __shared__ uint32_t s_d[NW];
   __shared__ int s_over;
   const int32_t a = blockIdx.x;
   const int la = lsz[a];
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_centre(
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t dd = 0;
-    for (int w = 0; w < kWavesPerBlock; ++w) dd += s_d[w];
+    for (int w = 0; w < NW; ++w) dd += s_d[w];
     dL[a] = dd;
     if (s_over) atomicOr(overflow, 1u);
   }
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_centre(
     if (lane == 0) s_d[w] = (uint32_t)__popcll(bal);
     __syncthreads();
     uint32_t woff = 0, tot = 0;
-    for (int q = 0; q < kWavesPerBlock; ++q) {
+    for (int q = 0; q < NW; ++q) {
       woff += q < w ? s_d[q] : 0;
       tot += s_d[q];
     }
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_centre(
   __syncthreads();
   const uint32_t G = s_g;
   unsigned long long* hc = hs + (lane & (kHubCopies - 1));
-  for (uint32_t i = w; i < G; i += kWavesPerBlock) {
+  for (uint32_t i = w; i < G; i += NW) {
     const uint32_t bi = lk[i];
     const unsigned long long ni = ln[i];
     const uint32_t Ab = bi >> 8;
@@ -503,8 +503,8 @@ constexpr int kHubLoads = 4;  // row loads in flight per lane in the triangle ke
 // adds F(true) - F(P0) + F(P1ab) + F(P1bc) + F(P1ca) - F(Ta) - F(Tb) - F(Tc) (roles a, b, c =
 // r, rb, rc; every member has a light node). Dynamic LDS by rank: light masks M (E), hub masks
 // (E words) and light sizes (E bytes). acc[kAccLightTri] counts the light triangles.
-template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_hub_tri(
+template <typename MT, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_hub_tri(
     int32_t E, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges, const int8_t* __restrict__ hub,
     const int32_t* __restrict__ rank, const int32_t* __restrict__ perm, const uint32_t* __restrict__ hmr,
@@ -532,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_tri(
   const uint32_t Aa = hm[r];
   const uint32_t* rowA = rows + roff[r];
   unsigned long long ntri = 0;
-  for (uint32_t ib = w; ib < nA; ib += kWavesPerBlock) {
+  for (uint32_t ib = w; ib < nA; ib += NW) {
     const uint32_t eb = rowA[ib];
     const int32_t rb = (int32_t)(eb >> kCntBits);
     const int cab = (int)(eb & kCntMask);
@@ -620,8 +620,8 @@ __global__ __launch_bounds__(kBlock) void k_hub_fill_tiled(
 // row, state at uoff[r] + ib keeps a's light mask in b and the read position in b's row (grouped
 // by tile); tiles without work are skipped. The int32 slot counters are folded and cleared per
 // tile (the host bounds their count per tile).
-template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_hub_tri_tiled(
+template <typename MT, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_hub_tri_tiled(
     int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vrank, const int8_t* __restrict__ hub,
     const int32_t* __restrict__ perm, const uint32_t* __restrict__ hmr, const uint8_t* __restrict__ lszr,
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_tri_tiled(
     });
     __syncthreads();
     const uint32_t end = s_end;
-    for (uint32_t base = (uint32_t)w * kWave; base < end; base += kWavesPerBlock * kWave) {
+    for (uint32_t base = (uint32_t)w * kWave; base < end; base += NW * kWave) {
       const uint32_t ib = base + lane;
       const bool have = ib < end;
       int32_t rb = 0;
@@ -817,6 +817,7 @@ __global__ void k_hub_widen(const uint32_t* __restrict__ in, unsigned long long*
 // arrays; k_hub_tri(_tiled)'s int32 counters (8 KB).
 constexpr size_t kHubCentreStatic = 17 * 1024 + 1024;
 constexpr size_t kHubTriStatic = 12 * 1024;
+constexpr int kHubWaves = 16;  // waves per workgroup of the LDS-heavy kernels (one workgroup per CU)
 
 static int hub_lds_limit(int device) {
   int v = 0;
@@ -929,13 +930,13 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
                                          : (int32_t)(((size_t)lim - kHubTriStatic - 1024) / kTriKey) & ~255;
   if (!dense && (size_t)Tt * kTriKey + kHubTriStatic > (size_t)lim)
     return fail(HM_ERR_BAD_INPUT, "hub-aware count: the forced tile exceeds the LDS limit");
-  HM_TRY(allow_lds((const void*)k_hub_centre<MT>, lds_centre));
+  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves>, lds_centre));
   if (dense) {
     HM_TRY(allow_lds((const void*)k_hub_fill<MT>, (size_t)E * sizeof(MT)));
-    HM_TRY(allow_lds((const void*)k_hub_tri<MT>, (size_t)E * kTriKey));
+    HM_TRY(allow_lds((const void*)k_hub_tri<MT, kHubWaves>, (size_t)E * kTriKey));
   } else {
     HM_TRY(allow_lds((const void*)k_hub_fill_tiled<MT>, (size_t)Tt * sizeof(MT)));
-    HM_TRY(allow_lds((const void*)k_hub_tri_tiled<MT>, (size_t)Tt * kTriKey));
+    HM_TRY(allow_lds((const void*)k_hub_tri_tiled<MT, kHubWaves>, (size_t)Tt * kTriKey));
   }
   if (dense ? false : I >= ((int64_t)1 << 31))
     return fail(HM_ERR_UNSUPPORTED, "hub-aware count: tiled path supports fewer than 2^31 incidences");
@@ -1041,7 +1042,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
     cap = std::min(hb.types.bytes, hb.tcraw.bytes) / sizeof(unsigned long long);
     k_hub_zero<<<1, 64, 0, st>>>(accC, kAccCount);
     k_hub_zero<<<1, 64, 0, st>>>(runs + 1, 3);
-    k_hub_centre<MT><<<E, kBlock, lds_centre, st>>>(
+    k_hub_centre<MT, kHubWaves><<<E, kHubWaves * kWave, lds_centre, st>>>(
         E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
         hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
         hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), runs + 1, cap,
@@ -1131,7 +1132,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
         s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), reinterpret_cast<uint32_t*>(runs + 4));
     HM_HIP(hipGetLastError());
     HM_TRY(check_fill(runs + 4, st));
-    k_hub_tri<MT><<<E, kBlock, (size_t)E * kTriKey, st>>>(
+    k_hub_tri<MT, kHubWaves><<<E, kHubWaves * kWave, (size_t)E * kTriKey, st>>>(
         E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
         hb.hubidx.as<int8_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(),
         hb.lszr.as<uint8_t>(), s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), acc, 0);
@@ -1186,7 +1187,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
     const int32_t chunk = verbose ? std::max<int32_t>(4096, E / 16) : E;
     for (int32_t r0 = 0; r0 < E; r0 += chunk) {
       const int32_t n = std::min(chunk, E - r0);
-      k_hub_tri_tiled<MT><<<(unsigned)n, kBlock, (size_t)Tt * kTriKey, st>>>(
+      k_hub_tri_tiled<MT, kHubWaves><<<(unsigned)n, kHubWaves * kWave, (size_t)Tt * kTriKey, st>>>(
           E, Tt, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), s.vrank.as<int32_t>(),
           hb.hubidx.as<int8_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(), hb.lszr.as<uint8_t>(),
           s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
