@@ -14,9 +14,15 @@ Workloads (inputs are built and copied to HBM before the timed region):
                          is one batch: hm_graph_update with its exact delta (store update + delta).
                          value = triples touched by the batches (removed + added) per second.
   serving_c5             plain versus served request streams over 64 reduced graphs (p50/p99).
-The full static 1M-node / 2M-hyperedge graph is out of reach of exact per-triple counting (its
-largest node sits in ~780k hyperedges, i.e. >7e16 triples); the temporal workload is the
-largest configuration the engine counts at full size on one GPU.
+  static_c3_base         MoCHy's 30 static bins of the dynamic workload's full 500,000-hyperedge
+                         base graph (3.16e15 triples, its largest node in 198,573 hyperedges) by
+                         the hub-aware count (hm_graph_set_hubs, hub count from the cost model); a
+                         step takes ~42 s, so this workload defaults to one step, no warmup.
+The full static 1M-node / 2M-hyperedge graph stays out of reach: even the hub-aware count leaves
+~4.3e14 light triangles with 12 hubs (~47 min at the measured rate) and more light-row state than
+HBM holds. The temporal workload is the largest full-size configuration whose step is short
+enough for repeated timing; the default line adds the static counts of c1 and of a 60,000-
+hyperedge graph of the 2M-hyperedge graph's family by the hub-aware count (config.static_hub).
 
 value = h-motif instances classified per second = (sum of the bins) / step time.
 roofline = the dominant kernel (k_tcount, k_tri or k_delta), bound "hbm": achieved = the HBM bytes
@@ -67,14 +73,19 @@ DESC = {
     "serving_c5": "serving_c5 (reduced): 64 graphs of the static_c2 family, E log-uniform on "
                   "[1k, 6k], V = E/2, seed 5; 400 requests, half count queries, half update "
                   "batches of 0.5-5% of E (70/30 insert/delete), seed 5000",
+    "static_c3_base": "static_c3_base: the dynamic workload's base graph at full size, 500,000 "
+                      "hyperedges over 200,000 nodes, sizes ~k^-2.2 on [2,30], Zipf s=1.1, seed 3; "
+                      "hub-aware count (hubs from the cost model)",
 }
-WHY_DEFAULT = ("the largest configuration the engine counts exactly at full size on one GPU: the "
-               "2M-hyperedge static graph (one node in 780,282 hyperedges, > 7.9e16 triples), the "
-               "500k-hyperedge dynamic base and the [10k, 1M]-hyperedge serving graphs are beyond "
-               "exact per-triple counting, and no exact hub-aware method is built")
+WHY_DEFAULT = ("the largest full-size configuration with a step short enough for repeated timing: "
+               "the 2M-hyperedge static graph (one node in 780,282 hyperedges, > 7.9e16 triples) is "
+               "beyond per-triple enumeration and, with the hub-aware count, still leaves ~4.3e14 "
+               "light triangles at 12 hubs (~47 min) and more light-row state than HBM; the "
+               "500k-hyperedge dynamic base counts exactly in ~42 s per step (--workload "
+               "static_c3_base)")
 # Bounded CPU samples: a prefix of the same input sized for ~10-30 s of reference-program time.
 CPU_SAMPLE_EDGES = {"temporal_c4": 700, "static_c1": 4000, "dynamic_c3": 4600}
-DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20, "dynamic_c3": 10, "serving_c5": 1}
+DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20, "dynamic_c3": 10, "serving_c5": 1, "static_c3_base": 1}
 DYN_BASE_EDGES, DYN_BASE_NODES = 20_000, 8_000
 
 
@@ -240,7 +251,7 @@ def roofline(workload: str, r: dict) -> dict:
            "traffic_source": f"profiles/traffic_{workload}.txt: rocprofv3 --pmc FETCH_SIZE and "
                              "WRITE_SIZE of one launch, scaled x2 and x1 by a 1 GiB device copy",
            "model_bytes": r["kernel_bytes"],
-           "model_gbs": r["kernel_bytes"] / (kms * 1e-3) / 1e9,
+           "model_gbs": r["kernel_bytes"] / (kms * 1e-3) / 1e9 if r["kernel_bytes"] else None,
            "model_note": "cost model of the counting path (projection pairs, neighbour-row entries, "
                          "bins at 8 B each); mostly served by LDS, L2 and the Infinity Cache, so not "
                          "an HBM lower bound and not the roofline numerator"}
@@ -304,6 +315,74 @@ def run_static(steps: int, warmup: int, with_cpu: bool) -> dict:
             "extra": {"hyperedges": int(off.size - 1), "triples": triples,
                       "bytes_model_total": hm.algorithmic_bytes_static(st)},
             "cpu": cpu_baseline_static(off, nodes) if with_cpu else None}
+
+
+def wedge_identity(counts: np.ndarray, st: dict) -> bool:
+    """sum(open) + 3 sum(closed) = sum_a C(d_a, 2), the projected degrees from the projection
+    kernels alone (bins 20..25 are the open triples)."""
+    c = counts.astype(np.int64)
+    closed = int(c.sum() - c[20:26].sum())
+    return int(c[20:26].sum()) + 3 * closed == st["wedges"]
+
+
+def run_static_c3_base(steps: int, warmup: int, with_cpu: bool) -> dict:
+    """The full 500k-hyperedge dynamic base by the hub-aware count. The dominant kernel is the
+    light-triangle kernel (k_hub_tri_tiled), timed by the count's own HIP events."""
+    off, nodes = hm.generate_static(W.DYNAMIC_C3)
+    g = hm.Graph(off, nodes)
+    g.set_hubs(-1)
+    for _ in range(warmup):
+        counts = g.count_static()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        counts = g.count_static()
+        kms.append(g.hub_stats()["ms_tri_kernel"])
+    t1 = time.perf_counter()
+    st, hs = g.stats(), g.hub_stats()
+    g.close()
+    triples = int(counts.astype(np.int64).sum())
+    return {"ms": (t1 - t0) * 1e3 / steps, "total": triples, "unit": "triples/s",
+            "kernel": "k_hub_tri_tiled", "kernel_ms": float(np.mean(kms)), "kernel_bytes": None,
+            "extra": {"hyperedges": int(off.size - 1), "triples": triples, "hub_stats": hs,
+                      "wedge_identity": wedge_identity(counts, st),
+                      "largest_node_degree": st["max_node_degree"]},
+            "cpu": None}
+
+
+def static_hub_summary() -> dict:
+    """Static counts by the hub-aware path beside the triangle path (identical bins): static c1
+    (5,000 hyperedges) and a 60,000-hyperedge graph of the 2M-hyperedge graph's family (V = E/2,
+    seed 22), each the best of a few calls after one untimed call."""
+    out = {}
+    for name, p, plain in (("static_c1", W.STATIC_C1, True),
+                           ("static_c2_family_60k", W.scaled(W.STATIC_C2, 60_000, 30_000, seed=22), False)):
+        off, nodes = hm.generate_static(p)
+        rec = {"hyperedges": int(off.size - 1)}
+        with hm.Graph(off, nodes) as g:
+            if plain:
+                base = g.count_static()
+                ts = []
+                for _ in range(5):
+                    t = time.perf_counter()
+                    base = g.count_static()
+                    ts.append(time.perf_counter() - t)
+                rec["triangle_path_ms"] = min(ts) * 1e3
+            g.set_hubs(-1)
+            c = g.count_static()
+            ts = []
+            for _ in range(3):
+                t = time.perf_counter()
+                c = g.count_static()
+                ts.append(time.perf_counter() - t)
+            st = g.stats()
+            rec.update({"hub_path_ms": min(ts) * 1e3, "triples": int(c.astype(np.int64).sum()),
+                        "triples_per_s": float(c.astype(np.int64).sum()) / min(ts),
+                        "hubs": g.hub_stats()["hubs"], "wedge_identity": wedge_identity(c, st)})
+            if plain:
+                rec["identical_bins"] = bool(np.array_equal(c, base))
+        out[name] = rec
+    return out
 
 
 def run_dynamic(steps: int, warmup: int, with_cpu: bool) -> dict:
@@ -613,13 +692,16 @@ def main() -> int:
                 "roofline": None, "cpu_baseline": None}
         print(json.dumps(line), flush=True)
         return 0
-    run = {"temporal_c4": run_temporal, "static_c1": run_static, "dynamic_c3": run_dynamic}[args.workload]
-    r = run(steps, args.warmup, not args.no_cpu)
+    run = {"temporal_c4": run_temporal, "static_c1": run_static, "dynamic_c3": run_dynamic,
+           "static_c3_base": run_static_c3_base}[args.workload]
+    warmup = 0 if args.workload == "static_c3_base" and args.steps is None else args.warmup
+    r = run(steps, warmup, not args.no_cpu)
     config = {"workload": DESC[args.workload], **r["extra"]}
     if args.workload == "temporal_c4":
         config["why_this_workload"] = WHY_DEFAULT
         if not args.no_serving:
             config["serving"] = serving_summary()
+            config["static_hub"] = static_hub_summary()
     if r["cpu"] is not None:
         r["cpu"]["host"] = host_cpu()
     line = {
@@ -628,7 +710,7 @@ def main() -> int:
         "unit": r["unit"],
         "n_gpus": 1,
         "steps": steps,
-        "warmup": args.warmup,
+        "warmup": warmup,
         "ms_per_step": r["ms"],
         "higher_is_better": True,
         "scaling": "weak",

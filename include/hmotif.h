@@ -186,6 +186,7 @@ typedef struct hm_hub_stats {
   double ms_wedges;       /* light neighbours, pair-type keys and light wedges */
   double ms_types;        /* class triples, pair-type sort and pair types x classes */
   double ms_triangles;    /* light projection rows and light triangles */
+  double ms_tri_kernel;   /* the light-triangle kernel alone (k_hub_tri or k_hub_tri_tiled) */
 } hm_hub_stats;
 hm_status hm_graph_get_hub_stats(hm_graph* g, hm_hub_stats* out);
 

@@ -81,6 +81,7 @@ class _HubStats(ctypes.Structure):
         ("ms_wedges", ctypes.c_double),
         ("ms_types", ctypes.c_double),
         ("ms_triangles", ctypes.c_double),
+        ("ms_tri_kernel", ctypes.c_double),
     ]
 
 

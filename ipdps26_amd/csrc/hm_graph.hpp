@@ -255,6 +255,8 @@ struct hm_graph {
   hm::UpdPlan uplan;  // captured update with a delta
   // hub-aware static count (hm_graph_set_hubs): 0 = off, -1 = automatic, h > 0 = h hubs
   int32_t hubs = 0;
+  int32_t hub_auto_h = 0;          // hub count the automatic mode settled on (0: none yet)
+  uint64_t hub_auto_version = 0;   // store version it was chosen for
   hm::HubScratch hub;
   hm_hub_stats hub_stats{};
 };

@@ -1132,6 +1132,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
         s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), reinterpret_cast<uint32_t*>(runs + 4));
     HM_HIP(hipGetLastError());
     HM_TRY(check_fill(runs + 4, st));
+    HM_HIP(hipEventRecord(g->ev[5], st));
     k_hub_tri<MT, kHubWaves><<<E, kHubWaves * kWave, (size_t)E * kTriKey, st>>>(
         E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
         hb.hubidx.as<int8_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(),
@@ -1183,6 +1184,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
                                             s.uoff.as<unsigned long long>(), E + 1, st));
     HM_TRY(s.cur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(U, 1)));
     HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
+    HM_HIP(hipEventRecord(g->ev[5], st));
     // centres in chunks: with HM_VERBOSE set, a line per chunk reports the progress of long counts
     const int32_t chunk = verbose ? std::max<int32_t>(4096, E / 16) : E;
     for (int32_t r0 = 0; r0 < E; r0 += chunk) {
@@ -1199,7 +1201,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
       }
     }
   }
-  HM_HIP(hipEventRecord(g->ev[4], st));
+  HM_HIP(hipEventRecord(g->ev[4], st));  // the triangle kernel ends the phase
   unsigned long long hC[kAccCount];
   HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipMemcpyAsync(hC, accC, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
@@ -1210,7 +1212,9 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   HM_TRY(projection_stats(g, h_acc));
   (void)dmax;
   g->hub_stats.light_triangles = (int64_t)h_acc[kAccLightTri];
-  float m01 = 0, m12 = 0, m23 = 0, m34 = 0;
+  float m01 = 0, m12 = 0, m23 = 0, m34 = 0, m54 = 0;
+  HM_HIP(hipEventElapsedTime(&m54, g->ev[5], g->ev[4]));
+  g->hub_stats.ms_tri_kernel = m54;
   HM_HIP(hipEventElapsedTime(&m01, g->ev[0], g->ev[1]));
   HM_HIP(hipEventElapsedTime(&m12, g->ev[1], g->ev[2]));
   HM_HIP(hipEventElapsedTime(&m23, g->ev[2], g->ev[3]));

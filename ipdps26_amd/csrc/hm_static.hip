@@ -1083,7 +1083,14 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
       // forced hub count: its result or its error; automatic: the cost model's choice, halved
       // while some hyperedge's neighbour keys overflow their histogram, else the triangle path
       int32_t h = g->hubs;
-      if (h < 0) HM_TRY(hub_choose(g, &h, nullptr));
+      if (h < 0) {  // the cost model's choice is kept while the store is unchanged
+        if (g->hub_auto_version != g->version || g->hub_auto_h <= 0) {
+          HM_TRY(hub_choose(g, &h, nullptr));
+          g->hub_auto_h = h;
+          g->hub_auto_version = g->version;
+        }
+        h = g->hub_auto_h;
+      }
       for (;;) {
         bool overflow = false;
         const hm_status st = count_static_hub(g, h, acc, &overflow);
@@ -1095,9 +1102,11 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
         if (g->hubs > 0 || !overflow) return st;
         if (h == 1) break;
         h = std::max<int32_t>(1, h * 3 / 4);
+        g->hub_auto_h = h;
         clear_error();
       }
       g->hub_stats.hubs = 0;
+      g->hub_auto_h = 0;
       for (int k = 0; k < kAccCount; ++k) acc[k] = 0;
       clear_error();
     }

@@ -153,3 +153,20 @@ def test_beyond_one_table_equals_triangle_path(gpu_ready):
         for bad in (-2, 24):
             with pytest.raises(hm.HMotifError):
                 g.set_hubs(bad)
+
+
+def test_dynamic_base_full_size(gpu_ready):
+    """The 500,000-hyperedge dynamic base (3.16e15 triples, its largest node in 198,573
+    hyperedges): the automatic hub count against the vector two hub counts agreed on, and the wedge
+    identity against the projected degrees."""
+    g = json.load(open(os.path.join(GOLDEN, "static_c3_base_hub.json")))
+    off, nodes = hm.generate_static(W.DYNAMIC_C3)
+    assert g["params"]["num_edges"] == off.size - 1
+    with hm.Graph(off, nodes) as gr:
+        gr.set_hubs(-1)
+        c = gr.count_static().astype(np.int64)
+        st, hs = gr.stats(), gr.hub_stats()
+    print(f"dynamic base: {int(c.sum())} triples, hubs {hs}", flush=True)
+    assert hs["hubs"] > 0
+    assert c.tolist() == g["counts30"]
+    assert st["wedges"] == g["wedges"] and wedge_identity(c, st)
