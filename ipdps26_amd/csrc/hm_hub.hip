@@ -236,6 +236,12 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
     if (threadIdx.x == 0) dL[a] = 0;
     return;
   }
+  // once some centre's histogram has overflowed the pass is redone with fewer hubs: stop early
+  // (one read, broadcast through LDS, so the whole workgroup leaves or stays together)
+  if (threadIdx.x == 0) s_over = (int)*(volatile uint32_t*)overflow;
+  __syncthreads();
+  if (s_over) return;
+  __syncthreads();  // s_over is reset below
   const size_t region = (size_t)T * sizeof(MT);
   MT* M = reinterpret_cast<MT*>(smem);
   uint32_t* hk = reinterpret_cast<uint32_t*>(smem + region);
@@ -1034,7 +1040,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   // light neighbours: (C) into accC, light degrees, pair-type entries for (B). A pass whose entries
   // outgrow the buffer runs again with a buffer of the reported size (accC starts from zero).
   unsigned long long cap = std::max<unsigned long long>(1, std::min<unsigned long long>(
-                                                              lp_bound, std::max<unsigned long long>((unsigned long long)E * 64, 1u << 20)));
+                                                              lp_bound, std::max<unsigned long long>((unsigned long long)E * 1024, 1u << 20)));
   unsigned long long nt = 0;
   for (int attempt = 0;; ++attempt) {
     HM_TRY(hb.types.ensure(sizeof(unsigned long long) * cap));
