@@ -13,6 +13,19 @@ constexpr int kBlock = 256;                 // 4 waves per workgroup
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kStaticBins = 30;
 constexpr int kOpenBin0 = 20;               // bins 20..25: open triples (MoCHy printed motifs 17-22)
+// Accumulator slots of one graph's count (uint64).
+enum AccSlot : int {
+  kAccHist = 0,        // 30 bins (two's-complement wrap: intermediate -1 contributions are exact)
+  kAccP = 32,          // sum_a d_a
+  kAccMaxDeg = 33,     // max_a d_a
+  kAccOrient = 34,     // sum_a sum_{b in N(a), b > a} d_b
+  kAccWedges = 35,     // sum_a C(d_a, 2)
+  kAccUpper = 36,      // sum_a d+_a (upper-row entries)
+  kAccMaxNodeDeg = 37, // max_v |V2H(v)|
+  kAccLightTri = 38,   // light triangles (hub-aware count)
+  kAccCount = 40
+};
+
 // Packed projection-row entry: (neighbour << kCntBits) | overlap.
 constexpr int kCntBits = 7;
 constexpr uint32_t kCntMask = (1u << kCntBits) - 1u;

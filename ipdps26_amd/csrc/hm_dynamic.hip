@@ -1448,6 +1448,81 @@ using namespace hm;
 
 extern "C" {
 
+// Update with an exact delta by the hub-aware parts (graphs whose frequent nodes put the full
+// projection rows k_delta needs out of reach): (A) + (B) + (C) of the graph before and after the
+// batch, and (D) over the light triangles that hold a deleted hyperedge (before) or an inserted
+// one (after) — the changed ids are ranked first in the light order, so (D) over those centres
+// alone enumerates exactly them, and the light triangles without a changed member are common to
+// both graphs and cancel. Both graphs use one hub set: the one chosen on the graph before.
+static hm_status update_hub(hm_graph* g, DynScratch& d, const int32_t* del_ids, int32_t n_del,
+                            const std::vector<uint8_t>& delmask, const std::vector<uint32_t>& ins_off32,
+                            const std::vector<int32_t>& inodes, int32_t n_ins, int32_t newV, int32_t maxsz,
+                            int32_t* assigned_ids, int64_t delta_out[HM_STATIC_BINS]) {
+  const int32_t E = g->csr.E;
+  std::vector<int32_t> dels(del_ids, del_ids + n_del);
+  std::sort(dels.begin(), dels.end());
+  int32_t h = g->hubs;
+  if (h < 0) {  // the automatic hub count is chosen once and kept across updates (a batch changes
+                // the graph by a few percent; the cost model reads the whole store back)
+    if (g->hub_auto_h <= 0) {
+      HM_TRY(hub_choose(g, &h, nullptr));
+      g->hub_auto_h = h;
+    }
+    h = g->hub_auto_h;
+  }
+  std::vector<int32_t> hubs;
+  unsigned long long abc0[kAccCount], d0[kAccCount], abc1[kAccCount], d1[kAccCount];
+  for (;;) {  // graph before the batch; an overflowing hub count is lowered (automatic mode)
+    HubRun r0;
+    r0.hubs_out = &hubs;
+    r0.first = dels.data();
+    r0.n_first = n_del;
+    r0.only_first = true;
+    r0.stats = false;
+    bool overflow = false;
+    const hm_status st = hub_parts(g, h, r0, abc0, d0, &overflow);
+    if (st == HM_OK) break;
+    if (g->hubs > 0 || !overflow || h == 1) return st;
+    h = std::max<int32_t>(1, h * 3 / 4);
+    g->hub_auto_h = h;
+    clear_error();
+  }
+  ++g->version;
+  g->hub_auto_version = g->version;  // the kept hub count stays valid for the graph after the batch
+  d.rows_valid = false;  // the held rows (k_delta's) describe no graph after this update
+  HM_HIP(hipEventRecord(g->ev[6], g->stream));  // the hub parts use events 0..5
+  HM_TRY(apply_batch(g, d, delmask, ins_off32, inodes, n_ins, newV));
+  HM_HIP(hipEventRecord(g->ev[7], g->stream));
+  g->csr.max_size = maxsz;
+  for (int32_t k = 0; k < n_del; ++k) g->live[(size_t)del_ids[k]] = 0;
+  g->live.resize((size_t)E + n_ins, 1);
+  for (int32_t k = 0; k < n_ins; ++k) assigned_ids[k] = E + k;
+  std::vector<int32_t> ins((size_t)n_ins);
+  for (int32_t k = 0; k < n_ins; ++k) ins[(size_t)k] = E + k;
+  HubRun r1;
+  r1.hub_nodes = &hubs;
+  r1.first = ins.data();
+  r1.n_first = n_ins;
+  r1.only_first = true;
+  r1.stats = false;
+  bool overflow = false;
+  const hm_status st = hub_parts(g, (int32_t)hubs.size(), r1, abc1, d1, &overflow);
+  if (st != HM_OK) {
+    set_error("hm_graph_update: the batch is applied, but its hub-aware delta failed on the graph after it (%s); "
+              "recount the graph", overflow ? "a neighbour-key histogram overflowed" : "see the earlier error");
+    return st;
+  }
+  for (int k = 0; k < HM_STATIC_BINS; ++k)
+    delta_out[k] = (int64_t)((abc1[kAccHist + k] - abc0[kAccHist + k]) + (d1[kAccHist + k] - d0[kAccHist + k]));
+  g->upd_removed = (int64_t)d0[kAccLightTri];  // light triangles enumerated before / after the batch
+  g->upd_added = (int64_t)d1[kAccLightTri];
+  g->upd_entries = -1;
+  HM_HIP(hipEventSynchronize(g->ev[7]));
+  HM_HIP(hipEventElapsedTime(&g->ms_upd_store, g->ev[6], g->ev[7]));
+  g->have_stats = false;
+  return HM_OK;
+}
+
 hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
                           const int64_t* ins_offsets, const int32_t* ins_nodes, int32_t n_ins,
                           int32_t* assigned_ids, int64_t delta_out[HM_STATIC_BINS]) {
@@ -1482,6 +1557,13 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
     return fail(HM_ERR_UNSUPPORTED, "hm_graph_update: a delta needs hyperedges of at most 64 nodes");
   HM_HIP(hipSetDevice(g->device));
   DynScratch& d = dyn_scratch(g);
+  if (want && g->hubs != 0) {
+    g->ms_upd_rows = g->ms_upd_delta = 0.f;
+    g->upd_removed = g->upd_added = g->upd_entries = -1;
+    g->uplan.last_captured = g->uplan.last_overflow = false;
+    return update_hub(g, d, del_ids, n_del, delmask, ins_off32, inodes, n_ins, newV, maxsz, assigned_ids,
+                      delta_out);
+  }
   ++g->version;  // a captured count of the old graph is never replayed
   g->ms_upd_rows = g->ms_upd_delta = 0.f;
   g->upd_removed = g->upd_added = g->upd_entries = -1;

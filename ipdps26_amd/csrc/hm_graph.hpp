@@ -156,9 +156,13 @@ struct HubScratch {
   DevBuf tcsrt;    // uint64[N]   their counts
   DevBuf tkey;     // uint64[T]   distinct pair types
   DevBuf tcnt;     // uint64[T]   light pairs per type
+  DevBuf rkey;     // uint32[E]   light-order sort keys
+  DevBuf first;    // uint8[E]    hyperedges ranked first (an update's changed ids)
+  DevBuf firstids; // int32[n]    their ids
+  DevBuf accd;     // uint64[40]  (D)'s bins
   void release() {
     for (DevBuf* b : {&hubidx, &hmask, &lsz, &ckey, &ckey2, &ukey, &ucnt, &nruns, &hmr, &lszr, &types,
-                      &tcraw, &types2, &tcsrt, &tkey, &tcnt})
+                      &tcraw, &types2, &tcsrt, &tkey, &tcnt, &rkey, &first, &firstids, &accd})
       b->release();
   }
 };
@@ -259,12 +263,28 @@ struct hm_graph {
   uint64_t hub_auto_version = 0;   // store version it was chosen for
   hm::HubScratch hub;
   hm_hub_stats hub_stats{};
+  std::vector<int32_t> hub_nodes;  // hub set of the last hub-aware count (most frequent first)
 };
 
 namespace hm {
 // Most hubs the hub-aware count takes (a neighbour key holds the hub pattern in 23 bits beside the
 // overlap and a flag).
 constexpr int32_t kHubMax = 23;
+// One run of the hub-aware pipeline: an optional pinned hub set, ids ranked first in the light
+// order (with only_first, (D) runs over those centres alone: the light triangles holding one of
+// them), and whether the projection statistics are computed.
+struct HubRun {
+  const std::vector<int32_t>* hub_nodes = nullptr;
+  std::vector<int32_t>* hubs_out = nullptr;  // receives the hub set used
+  const int32_t* first = nullptr;
+  int32_t n_first = 0;
+  bool only_first = false;
+  bool stats = true;
+};
+// Parts (A) + (B) + (C) into h_abc's bins (plus the projection statistics when run.stats) and
+// (D) into h_d's bins (kAccLightTri: the light triangles enumerated).
+hm_status hub_parts(hm_graph* g, int32_t h, const HubRun& run, unsigned long long* h_abc, unsigned long long* h_d,
+                    bool* key_overflow);
 // Hub-aware static count of g with h hubs (1..kHubMax) into the accumulator layout of the
 // triangle path (30 bins; the projection statistics are not computed). *key_overflow is set when
 // some hyperedge's light neighbours hold more distinct keys than its LDS histogram (fewer hubs

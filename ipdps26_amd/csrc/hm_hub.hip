@@ -417,17 +417,35 @@ __global__ __launch_bounds__(kBlock) void k_hub_types(int32_t K, int32_t k0,
   fold_ids(hs, (unsigned long long)tcnt[blockIdx.x], acc);
 }
 
-// Light-projection order: rank by ascending light degree (sorted pairs), hub mask and light size
-// by rank.
+// Sort keys of the light-projection order: ascending light degree, with the hyperedges flagged in
+// `first` (if given) ahead of all others. Any total order counts every light triangle once at its
+// lowest-ranked member; ranking changed hyperedges first makes that member a changed one for every
+// light triangle that holds one, so (D) over the first centres alone enumerates exactly those.
+__global__ void k_hub_rank_key(int32_t E, const uint32_t* __restrict__ dL, const uint8_t* __restrict__ first,
+                               uint32_t* __restrict__ key) {
+  const int32_t a = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (a >= E) return;
+  key[a] = (first && first[a]) ? 0u : min(dL[a], 0xFFFFFFFEu) + 1u;
+}
+__global__ void k_hub_flag(int32_t n, const int32_t* __restrict__ ids, uint8_t* __restrict__ flag) {
+  const int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k < n) flag[ids[k]] = 1;
+}
+
+// Light-projection order from the sorted keys: rank of each hyperedge, hub mask, light size and
+// light degree (the row capacity; degr[E] = 0 for the scan) by rank.
 __global__ void k_hub_rank(int32_t E, const int32_t* __restrict__ perm, const uint32_t* __restrict__ hmask,
-                           const uint8_t* __restrict__ lsz, int32_t* __restrict__ rank,
-                           uint32_t* __restrict__ hmr, uint8_t* __restrict__ lszr) {
+                           const uint8_t* __restrict__ lsz, const uint32_t* __restrict__ dL,
+                           int32_t* __restrict__ rank, uint32_t* __restrict__ hmr, uint8_t* __restrict__ lszr,
+                           unsigned long long* __restrict__ degr) {
   const int32_t r = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r == E) degr[E] = 0ull;
   if (r >= E) return;
   const int32_t a = perm[r];
   rank[a] = r;
   hmr[r] = hmask[a];
   lszr[r] = lsz[a];
+  degr[r] = dL[a];
 }
 
 // Upper rows of the light projection (one LDS table by rank): as proj_fill_item, from a's light
@@ -913,7 +931,8 @@ static hm_status check_fill(const unsigned long long* flag, hipStream_t st) {
 }
 
 template <typename MT>
-static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow) {
+static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run, unsigned long long* h_acc,
+                                       unsigned long long* h_d, bool* key_overflow) {
   DeviceCsr& c = g->csr;
   CountScratch& s = g->scratch;
   HubScratch& hb = g->hub;
@@ -960,7 +979,23 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
     return dx != dy ? dx > dy : x < y;
   });
   std::vector<int8_t> hubidx((size_t)std::max(V, 1), (int8_t)-1);
-  for (int32_t i = 0; i < hh; ++i) hubidx[order[i]] = (int8_t)i;
+  if (run.hub_nodes) {  // a pinned hub set (an update's two graphs share it)
+    if ((int32_t)run.hub_nodes->size() != hh) return fail(HM_ERR_BAD_INPUT, "hub-aware count: hub set size");
+    for (int32_t i = 0; i < hh; ++i) {
+      const int32_t v = (*run.hub_nodes)[(size_t)i];
+      if (v < 0 || v >= V) return fail(HM_ERR_BAD_INPUT, "hub-aware count: hub node out of range");
+      hubidx[(size_t)v] = (int8_t)i;
+    }
+  } else {
+    for (int32_t i = 0; i < hh; ++i) hubidx[order[i]] = (int8_t)i;
+  }
+  if (run.hubs_out) {
+    run.hubs_out->clear();
+    for (int32_t v = 0; v < V; ++v)
+      if (hubidx[(size_t)v] >= 0) run.hubs_out->push_back(v);
+    std::sort(run.hubs_out->begin(), run.hubs_out->end(),
+              [&](int32_t x, int32_t y) { return hubidx[(size_t)x] < hubidx[(size_t)y]; });
+  }
   unsigned long long lp_bound = 0, dmax = 0;
   for (int32_t v = 0; v < V; ++v) {
     const unsigned long long d = voff[v + 1] - voff[v];
@@ -993,6 +1028,8 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   HM_TRY(s.acc2.ensure(sizeof(unsigned long long) * kAccCount));
   unsigned long long* acc = s.acc.as<unsigned long long>();
   unsigned long long* accC = s.acc2.as<unsigned long long>();
+  HM_TRY(hb.accd.ensure(sizeof(unsigned long long) * kAccCount));
+  unsigned long long* accD = hb.accd.as<unsigned long long>();
   // runs: [0] classes, [1] pair-type entries, [2] distinct pair types, [3] key-overflow flag
   unsigned long long* runs = hb.nruns.as<unsigned long long>();
   const unsigned eb = (unsigned)((E + kBlock - 1) / kBlock);
@@ -1001,6 +1038,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   HM_HIP(hipMemcpyAsync(hb.hubidx.p, hubidx.data(), (size_t)std::max(V, 1), hipMemcpyHostToDevice, st));
   HM_HIP(hipEventRecord(g->ev[0], st));
   k_hub_zero<<<1, 64, 0, st>>>(acc, kAccCount);
+  k_hub_zero<<<1, 64, 0, st>>>(accD, kAccCount);
   k_hub_zero<<<1, 64, 0, st>>>(runs, 8);  // [4]: a light row outgrew the light degree
   k_hub_class<<<eb, kBlock, 0, st>>>(E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), hb.hubidx.as<int8_t>(),
                                      hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(),
@@ -1113,15 +1151,28 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   }
   HM_HIP(hipEventRecord(g->ev[3], st));
 
-  // (D) light triangles over the light projection, ranked by ascending light degree
+  // (D) light triangles over the light projection, ranked by ascending light degree (the ids in
+  // run.first ahead of all others)
+  const uint8_t* firstflag = nullptr;
+  if (run.n_first > 0) {
+    HM_TRY(hb.first.ensure((size_t)E));
+    HM_TRY(hb.firstids.ensure(sizeof(int32_t) * (size_t)run.n_first));
+    HM_HIP(hipMemsetAsync(hb.first.p, 0, (size_t)E, st));
+    HM_HIP(hipMemcpyAsync(hb.firstids.p, run.first, sizeof(int32_t) * (size_t)run.n_first, hipMemcpyHostToDevice, st));
+    k_hub_flag<<<(unsigned)((run.n_first + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+        run.n_first, hb.firstids.as<int32_t>(), hb.first.as<uint8_t>());
+    firstflag = hb.first.as<uint8_t>();
+  }
+  HM_TRY(hb.rkey.ensure(sizeof(uint32_t) * (size_t)E));
+  k_hub_rank_key<<<eb, kBlock, 0, st>>>(E, s.deg.as<uint32_t>(), firstflag, hb.rkey.as<uint32_t>());
   k_hub_iota<<<eb, kBlock, 0, st>>>(s.ids.as<int32_t>(), E);
   tb = s.tmp.bytes;
-  HM_HIP(hipcub::DeviceRadixSort::SortPairs(s.tmp.p, tb, s.deg.as<uint32_t>(), s.work2.as<uint32_t>(),
+  HM_HIP(hipcub::DeviceRadixSort::SortPairs(s.tmp.p, tb, hb.rkey.as<uint32_t>(), s.work2.as<uint32_t>(),
                                             s.ids.as<int32_t>(), s.perm.as<int32_t>(), E, 0, 32, st));
-  k_hub_rank<<<eb, kBlock, 0, st>>>(E, s.perm.as<int32_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(),
-                                    s.rank.as<int32_t>(), hb.hmr.as<uint32_t>(), hb.lszr.as<uint8_t>());
-  k_hub_widen<<<(unsigned)((E + 1 + kBlock - 1) / kBlock), kBlock, 0, st>>>(
-      s.work2.as<uint32_t>(), s.degr.as<unsigned long long>(), E);
+  k_hub_rank<<<(unsigned)((E + 1 + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+      E, s.perm.as<int32_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
+      s.rank.as<int32_t>(), hb.hmr.as<uint32_t>(), hb.lszr.as<uint8_t>(), s.degr.as<unsigned long long>());
+  const int32_t nD = run.only_first ? run.n_first : E;  // centres of (D)
   tb = s.tmp.bytes;
   HM_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp.p, tb, s.degr.as<unsigned long long>(),
                                           s.roff.as<unsigned long long>(), E + 1, st));
@@ -1139,11 +1190,14 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
     HM_HIP(hipGetLastError());
     HM_TRY(check_fill(runs + 4, st));
     HM_HIP(hipEventRecord(g->ev[5], st));
-    k_hub_tri<MT, kHubWaves><<<E, kHubWaves * kWave, (size_t)E * kTriKey, st>>>(
-        E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
-        hb.hubidx.as<int8_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(),
-        hb.lszr.as<uint8_t>(), s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), acc, 0);
-    HM_HIP(hipGetLastError());
+    if (nD > 0) {
+      k_hub_tri<MT, kHubWaves><<<(unsigned)nD, kHubWaves * kWave, (size_t)E * kTriKey, st>>>(
+          E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+          hb.hubidx.as<int8_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(),
+          hb.lszr.as<uint8_t>(), s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
+          accD, 0);
+      HM_HIP(hipGetLastError());
+    }
   } else {
     // V2H relabelled to light ranks, ascending within every node's row
     HM_TRY(s.vkeys.ensure(sizeof(unsigned long long) * std::max<int64_t>(I, 1)));
@@ -1192,18 +1246,18 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
     HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
     HM_HIP(hipEventRecord(g->ev[5], st));
     // centres in chunks: with HM_VERBOSE set, a line per chunk reports the progress of long counts
-    const int32_t chunk = verbose ? std::max<int32_t>(4096, E / 16) : E;
-    for (int32_t r0 = 0; r0 < E; r0 += chunk) {
-      const int32_t n = std::min(chunk, E - r0);
+    const int32_t chunk = verbose ? std::max<int32_t>(4096, nD / 16) : std::max<int32_t>(nD, 1);
+    for (int32_t r0 = 0; r0 < nD; r0 += chunk) {
+      const int32_t n = std::min(chunk, nD - r0);
       k_hub_tri_tiled<MT, kHubWaves><<<(unsigned)n, kHubWaves * kWave, (size_t)Tt * kTriKey, st>>>(
           E, Tt, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), s.vrank.as<int32_t>(),
           hb.hubidx.as<int8_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(), hb.lszr.as<uint8_t>(),
           s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
-          s.uoff.as<unsigned long long>(), s.cur.as<uint32_t>(), s.pm.as<MT>(), acc, r0);
+          s.uoff.as<unsigned long long>(), s.cur.as<uint32_t>(), s.pm.as<MT>(), accD, r0);
       HM_HIP(hipGetLastError());
       if (verbose) {
         HM_HIP(hipStreamSynchronize(st));
-        std::fprintf(stderr, "[hm hub] light triangles: centres %d..%d of %d done\n", r0, r0 + n, E);
+        std::fprintf(stderr, "[hm hub] light triangles: centres %d..%d of %d done\n", r0, r0 + n, nD);
       }
     }
   }
@@ -1211,13 +1265,14 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   unsigned long long hC[kAccCount];
   HM_HIP(hipMemcpyAsync(h_acc, acc, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipMemcpyAsync(hC, accC, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
+  HM_HIP(hipMemcpyAsync(h_d, accD, sizeof(unsigned long long) * kAccCount, hipMemcpyDeviceToHost, st));
   HM_HIP(hipStreamSynchronize(st));
   for (int k = 0; k < kStaticBins; ++k) h_acc[kAccHist + k] += hC[kAccHist + k];
   // projection statistics from the projected degrees alone (the wedge identity
   // sum(open) + 3 sum(closed) = sum_a C(d_a, 2) checks the bins against them)
-  HM_TRY(projection_stats(g, h_acc));
+  if (run.stats) HM_TRY(projection_stats(g, h_acc));
   (void)dmax;
-  g->hub_stats.light_triangles = (int64_t)h_acc[kAccLightTri];
+  g->hub_stats.light_triangles = (int64_t)h_d[kAccLightTri];
   float m01 = 0, m12 = 0, m23 = 0, m34 = 0, m54 = 0;
   HM_HIP(hipEventElapsedTime(&m54, g->ev[5], g->ev[4]));
   g->hub_stats.ms_tri_kernel = m54;
@@ -1234,10 +1289,20 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, unsigned long lon
   return HM_OK;
 }
 
-hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow) {
+hm_status hub_parts(hm_graph* g, int32_t h, const HubRun& run, unsigned long long* h_abc, unsigned long long* h_d,
+                    bool* key_overflow) {
   if (h < 1 || h > kHubKeyHubs) return fail(HM_ERR_BAD_INPUT, "hub count outside 1..23");
-  return g->csr.max_size > 32 ? count_static_hub_impl<mask64_t>(g, h, h_acc, key_overflow)
-                              : count_static_hub_impl<uint32_t>(g, h, h_acc, key_overflow);
+  return g->csr.max_size > 32 ? count_static_hub_impl<mask64_t>(g, h, run, h_abc, h_d, key_overflow)
+                              : count_static_hub_impl<uint32_t>(g, h, run, h_abc, h_d, key_overflow);
+}
+
+hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow) {
+  HubRun run;
+  run.hubs_out = &g->hub_nodes;
+  unsigned long long d[kAccCount];
+  HM_TRY(hub_parts(g, h, run, h_acc, d, key_overflow));
+  for (int k = 0; k < kStaticBins; ++k) h_acc[kAccHist + k] += d[kAccHist + k];
+  return HM_OK;
 }
 
 }  // namespace hm

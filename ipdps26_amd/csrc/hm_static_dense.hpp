@@ -21,18 +21,6 @@ namespace hm {
 
 static __constant__ int8_t c_static_bin[128] = HM_STATIC_BIN_TABLE;
 
-// Accumulator slots of one graph's count (uint64).
-enum AccSlot : int {
-  kAccHist = 0,        // 30 bins (two's-complement wrap: intermediate -1 contributions are exact)
-  kAccP = 32,          // sum_a d_a
-  kAccMaxDeg = 33,     // max_a d_a
-  kAccOrient = 34,     // sum_a sum_{b in N(a), b > a} d_b
-  kAccWedges = 35,     // sum_a C(d_a, 2)
-  kAccUpper = 36,      // sum_a d+_a (upper-row entries)
-  kAccMaxNodeDeg = 37, // max_v |V2H(v)|
-  kAccLightTri = 38,   // light triangles (hub-aware count)
-  kAccCount = 40
-};
 
 // OpenAll(a): every unordered pair of neighbours of a, classified as an open triple centred at a
 // from the neighbour classes cls[2 (C_ab - 1) + [b ⊄ a]] (sa = |a|). Adds to acc bins 20..25.

@@ -170,3 +170,84 @@ def test_dynamic_base_full_size(gpu_ready):
     assert hs["hubs"] > 0
     assert c.tolist() == g["counts30"]
     assert st["wedges"] == g["wedges"] and wedge_identity(c, st)
+
+
+# ---- hub-aware deltas (hm_graph_update on a graph with hubs set) ---------------------------------
+def _rows(off, nodes):
+    return [list(nodes[off[e]:off[e + 1]]) for e in range(off.size - 1)]
+
+
+def _batch(rng, rows, live, n_del, n_ins, num_nodes, kmax):
+    dels = sorted(rng.choice(sorted(live), size=min(n_del, len(live)), replace=False).tolist())
+    new_rows = []
+    for _ in range(n_ins):
+        if rng.random() < 0.2 and live:  # duplicate of a live hyperedge
+            new_rows.append(list(rows[int(rng.choice(sorted(live)))]))
+        else:
+            k = int(rng.integers(1, kmax + 1))
+            new_rows.append(list(rng.choice(num_nodes, size=min(k, num_nodes), replace=False)))
+    return dels, new_rows
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_hub_deltas_match_oracle_recount(gpu_ready, seed):
+    rng = np.random.default_rng(8000 + seed)
+    nv, kmax = int(rng.integers(10, 80)), [4, 8, 40][seed % 3]
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(20, 250)), num_nodes=nv, kmax=kmax)
+    rows = _rows(off, nodes)
+    live = set(range(len(rows)))
+    hubs = [1, 3, 6, -1][seed % 4]
+    with hm.Graph(off, nodes) as g:
+        g.set_hubs(hubs)
+        if seed % 2:
+            g.set_tile(64)  # tiled light triangles
+        before = O.static_mochy(*csr(rows))
+        for b in range(4):
+            dels, new_rows = _batch(rng, rows, live, int(rng.integers(0, 15)), int(rng.integers(0, 15)), nv, kmax)
+            ids, delta = g.update(dels, *csr(new_rows))
+            for x in dels:
+                rows[x] = []
+                live.discard(x)
+            rows.extend(new_rows)
+            live.update(ids.tolist())
+            after = O.static_mochy(*csr(rows))
+            print(f"seed={seed} hubs={hubs} batch={b} E={len(rows)}", flush=True)
+            assert np.array_equal(delta, after - before)
+            assert np.array_equal(g.count_static().astype(np.int64), after)
+            before = after
+
+
+def test_hub_deltas_equal_triangle_path_deltas(gpu_ready):
+    """The dynamic workload's generator at 20k hyperedges: the same batches as hub-aware deltas
+    and as k_delta deltas, equal bin for bin, and the hub-aware counts after them equal."""
+    p = W.scaled(W.DYNAMIC_C3, 20_000, 8_000)
+    off, nodes = hm.generate_static(p)
+    with hm.Graph(off, nodes) as ga, hm.Graph(off, nodes) as gb:
+        ga.set_hubs(-1)
+        live = np.arange(off.size - 1)
+        for b in range(1, 4):
+            dels, ioff, inodes = W.dynamic_batch(live, b, params=p, inserts=280, deletes=120)
+            ids_a, da = ga.update(dels, ioff, inodes)
+            ids_b, db = gb.update(dels, ioff, inodes)
+            print(f"batch {b}: {int(np.abs(da).sum())} changed triples, hubs {ga.hub_stats()}", flush=True)
+            assert np.array_equal(ids_a, ids_b) and np.array_equal(da, db)
+            live = np.setdiff1d(live, dels)
+            live = np.concatenate([live, ids_a])
+        assert np.array_equal(ga.count_static(), gb.count_static())
+
+
+def test_dynamic_workload_full_size_first_batches(gpu_ready):
+    """The full dynamic workload's first two batches on the 500,000-hyperedge base: hub-aware
+    deltas equal to those of the full 100-batch run, whose recounts after batches 33, 66 and 100
+    matched the running sums (tests/golden/dynamic_c3_full_hub.json)."""
+    g = json.load(open(os.path.join(GOLDEN, "dynamic_c3_full_hub.json")))
+    off, nodes = hm.generate_static(W.DYNAMIC_C3)
+    with hm.Graph(off, nodes) as gr:
+        gr.set_hubs(-1)
+        live = np.arange(off.size - 1)
+        for b in (1, 2):
+            dels, ioff, inodes = W.dynamic_batch(live, b)
+            ids, delta = gr.update(dels, ioff, inodes)
+            print(f"batch {b}: sum|delta| {int(np.abs(delta).sum())}, {gr.update_stats()}", flush=True)
+            assert delta.tolist() == g["deltas30"][b - 1]
+            live = np.concatenate([np.setdiff1d(live, dels), ids])
