@@ -174,7 +174,13 @@ hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out);
  * than its LDS histogram holds; if even one hub overflows, the triangle path counts). The 30 bins are identical
  * either way. A forced hub count whose histograms overflow fails with HM_ERR_UNSUPPORTED. After a
  * hub-aware count hm_graph_stats holds the projection statistics computed from the projected
- * degrees alone (no rows: upper-row totals are not kept). */
+ * degrees alone (no rows: upper-row totals are not kept).
+ * With hubs != 0, hm_graph_update's delta is hub-aware too: the class, pair-type and light-wedge
+ * parts of the graph before and after the batch, and the light triangles that hold a changed
+ * hyperedge (deleted ones before the batch, inserted ones after it), with one hub set for both
+ * graphs (chosen on the graph before; the automatic hub count is kept across updates). If the
+ * graph after the batch overflows a neighbour-key histogram with that hub set, the batch stays
+ * applied and the update fails (recount the graph). */
 hm_status hm_graph_set_hubs(hm_graph* g, int32_t hubs);
 typedef struct hm_hub_stats {
   int32_t hubs;           /* hubs of the last hub-aware count on g (0: it counted by triangles) */
@@ -225,12 +231,15 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
  * HM_ERR_BAD_INPUT if the last update had no delta. No counterpart in the reference, whose anchor
  * rule does not separate removed and added triples (dynamicHyperGraph/src/HMotifCountUpdate.cu:151-226). */
 typedef struct hm_update_stats {
-  int64_t removed;      /* triples that contained a deleted hyperedge before the batch */
-  int64_t added;        /* triples that contain an inserted hyperedge after it (delta sums to added - removed) */
-  int64_t row_entries;  /* projection-row entries scanned by the delta kernel, both phases */
+  int64_t removed;      /* triples that contained a deleted hyperedge before the batch (hub-aware
+                           delta: the light triangles among them) */
+  int64_t added;        /* triples that contain an inserted hyperedge after it (delta sums to added - removed;
+                           hub-aware delta: the light triangles among them) */
+  int64_t row_entries;  /* projection-row entries scanned by the delta kernel, both phases (-1: hub-aware) */
   double ms_rows;       /* projection rows of the graph after the batch, and before it unless the
                            rows built by the previous update are still held (HIP events) */
-  double ms_delta;      /* delta kernel, both phases (HIP events) */
+  double ms_delta;      /* delta kernel, both phases (HIP events); hub-aware delta: its parts on
+                           both graphs (host time, ms_rows 0) */
   double ms_store;      /* store update: H2V / V2H rebuild, stream time including its host wait */
   int32_t captured;     /* 1: the update ran as one captured graph (hm_graph_set_update_capture);
                            ms_delta then holds the whole launch, ms_rows and ms_store 0 */

@@ -27,6 +27,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -1472,6 +1473,9 @@ static hm_status update_hub(hm_graph* g, DynScratch& d, const int32_t* del_ids, 
   }
   std::vector<int32_t> hubs;
   unsigned long long abc0[kAccCount], d0[kAccCount], abc1[kAccCount], d1[kAccCount];
+  // the hub parts read their results back before returning: host time is their time
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
   for (;;) {  // graph before the batch; an overflowing hub count is lowered (automatic mode)
     HubRun r0;
     r0.hubs_out = &hubs;
@@ -1487,6 +1491,7 @@ static hm_status update_hub(hm_graph* g, DynScratch& d, const int32_t* del_ids, 
     g->hub_auto_h = h;
     clear_error();
   }
+  const double ms_before = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
   ++g->version;
   g->hub_auto_version = g->version;  // the kept hub count stays valid for the graph after the batch
   d.rows_valid = false;  // the held rows (k_delta's) describe no graph after this update
@@ -1499,6 +1504,7 @@ static hm_status update_hub(hm_graph* g, DynScratch& d, const int32_t* del_ids, 
   for (int32_t k = 0; k < n_ins; ++k) assigned_ids[k] = E + k;
   std::vector<int32_t> ins((size_t)n_ins);
   for (int32_t k = 0; k < n_ins; ++k) ins[(size_t)k] = E + k;
+  const auto t1 = clk::now();
   HubRun r1;
   r1.hub_nodes = &hubs;
   r1.first = ins.data();
@@ -1517,6 +1523,7 @@ static hm_status update_hub(hm_graph* g, DynScratch& d, const int32_t* del_ids, 
   g->upd_removed = (int64_t)d0[kAccLightTri];  // light triangles enumerated before / after the batch
   g->upd_added = (int64_t)d1[kAccLightTri];
   g->upd_entries = -1;
+  g->ms_upd_delta = (float)(ms_before + std::chrono::duration<double, std::milli>(clk::now() - t1).count());
   HM_HIP(hipEventSynchronize(g->ev[7]));
   HM_HIP(hipEventElapsedTime(&g->ms_upd_store, g->ev[6], g->ev[7]));
   g->have_stats = false;
