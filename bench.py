@@ -595,16 +595,21 @@ def run_serving(slots: int = 4, n_requests: int = 400, batch_graphs: int = 16) -
     same = same and same_as(s_plain, s_served) and same_as(s_plain, s_batched)
     # the full configuration's own graphs up to 30k hyperedges, at their sizes
     band = W.serving_c5_band()
+    hub_min, budget = W.SERVING_HUB_MIN_EDGES, 160 << 30
     breqs = W.serving_requests(band, 48, seed=5600)
-    b_plain = serving.run_plain(band, breqs, realtime=False)
-    b_served = serving.run_served(band, breqs, realtime=False, slots=slots)
-    b_kept = serving.run_served(band, breqs, realtime=False, slots=slots, maintain=True)
+    b_plain = serving.run_plain(band, breqs, realtime=False, hub_min_edges=hub_min)
+    note(f"c5 graphs up to {W.SERVING_BAND_MAX} hyperedges, plain saturated: {b_plain['requests_per_s']:.2f} req/s")
+    b_served = serving.run_served(band, breqs, realtime=False, slots=slots, budget_bytes=budget,
+                                  hub_min_edges=hub_min)
+    b_kept = serving.run_served(band, breqs, realtime=False, slots=slots, maintain=True, budget_bytes=budget,
+                                hub_min_edges=hub_min)
     note(f"c5 graphs up to {W.SERVING_BAND_MAX} hyperedges saturated: plain {b_plain['requests_per_s']:.2f} "
          f"req/s, served {b_served['requests_per_s']:.2f}, kept counts {b_kept['requests_per_s']:.2f}")
     brate = 0.7 * b_plain["requests_per_s"]
     btimed = W.serving_requests(band, 48, seed=5600, rate_per_s=brate)
-    b_plain_rt = serving.run_plain(band, btimed, realtime=True)
-    b_kept_rt = serving.run_served(band, btimed, realtime=True, slots=slots, maintain=True)
+    b_plain_rt = serving.run_plain(band, btimed, realtime=True, hub_min_edges=hub_min)
+    b_kept_rt = serving.run_served(band, btimed, realtime=True, slots=slots, maintain=True, budget_bytes=budget,
+                                   hub_min_edges=hub_min)
     note(f"c5 graphs up to {W.SERVING_BAND_MAX} hyperedges at {brate:.2f} req/s: p50/p99 plain "
          f"{b_plain_rt['p50_ms']:.0f}/{b_plain_rt['p99_ms']:.0f} ms, kept counts "
          f"{b_kept_rt['p50_ms']:.0f}/{b_kept_rt['p99_ms']:.0f} ms")
@@ -643,7 +648,8 @@ def run_serving(slots: int = 4, n_requests: int = 400, batch_graphs: int = 16) -
                       "c5_graphs_at_size": {
                           "graphs": f"the {len(band)} graphs of the full serving configuration (64 graphs, "
                                     f"E log-uniform on [10k, 1M], seed 5) with at most {W.SERVING_BAND_MAX} "
-                                    "hyperedges, each at its own size; 48 requests (seed 5600)",
+                                    "hyperedges, each at its own size; 48 requests (seed 5600); graphs of "
+                                    f"{hub_min} hyperedges or more counted and updated hub-aware in every form",
                           "hyperedges": [int(off.size - 1) for off, _ in band],
                           "saturated": {"plain": strip(b_plain), "served": strip(b_served),
                                         "served_kept_counts": strip(b_kept),

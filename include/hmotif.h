@@ -408,6 +408,11 @@ typedef struct hm_server_config {
   /* 1: every graph's updates run as one captured hipGraph each (hm_graph_set_update_capture):
    * one launch and one wait per update. Deltas identical either way. */
   int32_t capture_updates;
+  /* n > 0: a graph added with at least n hyperedges is counted and updated hub-aware
+   * (hm_graph_set_hubs(g, -1): its counts and deltas by hyperedge classes, pair types, light
+   * wedges and light triangles; never in a batched count; the captured form is not used for its
+   * updates); 0: every graph by triangles. Counts and deltas identical either way. */
+  int32_t hub_min_edges;
 } hm_server_config;
 
 typedef struct hm_result {

@@ -568,7 +568,7 @@ class Batch:
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("slots", ctypes.c_int32),
                 ("maintain_counts", ctypes.c_int32), ("batch_graphs", ctypes.c_int32),
-                ("capture_updates", ctypes.c_int32)]
+                ("capture_updates", ctypes.c_int32), ("hub_min_edges", ctypes.c_int32)]
 
 
 class _Result(ctypes.Structure):
@@ -595,14 +595,15 @@ class Server:
     maintain_counts=True keeps each graph's counts current from the exact update deltas after
     its first count and answers count queries from them without a launch; batch_graphs=n > 1
     lets a worker count up to n ready graphs in one batched launch sequence; capture_updates=True
-    runs every graph's updates as one captured hipGraph each."""
+    runs every graph's updates as one captured hipGraph each; hub_min_edges=n > 0 counts and
+    updates every graph of at least n hyperedges hub-aware (Graph.set_hubs(-1))."""
 
     COUNT, UPDATE = 0, 1
 
     def __init__(self, hbm_budget_bytes: int, slots: int = 4, maintain_counts: bool = False,
-                 batch_graphs: int = 0, capture_updates: bool = False):
+                 batch_graphs: int = 0, capture_updates: bool = False, hub_min_edges: int = 0):
         cfg = _ServerConfig(int(hbm_budget_bytes), int(slots), int(bool(maintain_counts)),
-                            int(batch_graphs), int(bool(capture_updates)))
+                            int(batch_graphs), int(bool(capture_updates)), int(hub_min_edges))
         self._h = ctypes.c_void_p()
         _check(lib().hm_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
 

@@ -27,6 +27,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -62,20 +63,31 @@ struct GraphSlot {
   bool queued = false;   // the graph id sits in the ready queue
   bool removed = false;
   double count_ms = -1.0;    // duration of the graph's last count (its share of a batched one)
+  bool hub = false;          // counted and updated hub-aware (never batched)
   bool have_counts = false;  // maintain_counts: `counts` equal the graph's counts after its last
                              // finished request
   uint64_t counts[HM_STATIC_BINS] = {0};
 };
 
-int64_t estimate_bytes(const int64_t* off, const int32_t* nodes, int32_t E) {
+// Device footprint of a graph: CSR, projection rows (4 B per directed pair, at most E^2) and
+// per-hyperedge scratch. A hub-aware graph holds no full rows: its light upper rows (at most 12 B
+// per entry, pairs through nodes other than the 8 most frequent) and pair-type entries (16 B, at
+// most 1,024 per hyperedge before they are counted exactly) instead.
+int64_t estimate_bytes(const int64_t* off, const int32_t* nodes, int32_t E, bool hub) {
   const int64_t I = off[E];
   std::unordered_map<int32_t, int64_t> deg;
   deg.reserve((size_t)std::min<int64_t>(I, 1 << 22));
   for (int64_t k = 0; k < I; ++k) deg[nodes[k]]++;
+  std::vector<int64_t> d;
+  d.reserve(deg.size());
+  for (auto& kv : deg) d.push_back(kv.second);
+  const size_t top = hub ? std::min<size_t>(8, d.size()) : 0;
+  std::partial_sort(d.begin(), d.begin() + top, d.end(), std::greater<int64_t>());
   int64_t pairs = 0;
-  for (auto& kv : deg) pairs += kv.second * (kv.second - 1);
+  for (size_t k = top; k < d.size(); ++k) pairs += d[k] * (d[k] - 1);
   const int64_t csr = 4 * (2 * I + 2 * (int64_t)E + 2 + 2 * (int64_t)deg.size());
-  const int64_t rows = 4 * std::min<int64_t>(pairs, (int64_t)E * (int64_t)E);
+  const int64_t rows = hub ? 6 * pairs + 16 * std::min<int64_t>(pairs / 2 + (1 << 20), 1024 * (int64_t)E)
+                           : 4 * std::min<int64_t>(pairs, (int64_t)E * (int64_t)E);
   return csr + rows + 64 * (int64_t)E + (1 << 20);
 }
 
@@ -125,7 +137,9 @@ struct hm_server {
   // make every request of the batch wait for the whole batch.
   static constexpr double kBatchSmallMs = 1.0;
   static constexpr double kBatchBudgetMs = 4.0;  // expected duration of one batch at most
-  static bool small_count(const GraphSlot& g) { return g.count_ms >= 0.0 && g.count_ms < kBatchSmallMs; }
+  static bool small_count(const GraphSlot& g) {
+    return !g.hub && g.count_ms >= 0.0 && g.count_ms < kBatchSmallMs;
+  }
 
   // Takes the head request of graph gid, plus every directly following count when the head is a
   // count (s->mu held; the slot becomes busy).
@@ -279,10 +293,10 @@ hm_status hm_server_create(const hm_server_config* cfg, hm_server** out) {
   if (!cfg || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_server_create: null argument");
   if (cfg->slots < 1 || cfg->slots > 64 || cfg->hbm_budget_bytes <= 0 ||
       (cfg->maintain_counts != 0 && cfg->maintain_counts != 1) || cfg->batch_graphs < 0 ||
-      (cfg->capture_updates != 0 && cfg->capture_updates != 1))
+      (cfg->capture_updates != 0 && cfg->capture_updates != 1) || cfg->hub_min_edges < 0)
     return hm::fail(HM_ERR_BAD_INPUT,
                     "hm_server_create: need 1 <= slots <= 64, a positive budget, maintain_counts 0 or 1, "
-                    "batch_graphs >= 0 and capture_updates 0 or 1");
+                    "batch_graphs >= 0, capture_updates 0 or 1 and hub_min_edges >= 0");
   *out = nullptr;
   size_t free_b = 0, total_b = 0;
   if (hipSetDevice(hm::current_device()) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
@@ -317,7 +331,8 @@ hm_status hm_server_add_graph(hm_server* s, const int64_t* offsets, const int32_
   hm::clear_error();
   if (!s || !graph_id || !offsets || num_edges < 0)
     return hm::fail(HM_ERR_BAD_INPUT, "hm_server_add_graph: bad argument");
-  const int64_t need = estimate_bytes(offsets, nodes, num_edges);
+  const bool hub = s->cfg.hub_min_edges > 0 && num_edges >= s->cfg.hub_min_edges;
+  const int64_t need = estimate_bytes(offsets, nodes, num_edges, hub);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->used_bytes + need > s->cfg.hbm_budget_bytes) {
@@ -330,6 +345,11 @@ hm_status hm_server_add_graph(hm_server* s, const int64_t* offsets, const int32_
   hm_graph* g = nullptr;
   hm_status st = hm_graph_create(offsets, nodes, num_edges, 0, &g);
   if (st == HM_OK && s->cfg.capture_updates) st = hm_graph_set_update_capture(g, 1);
+  if (st == HM_OK && hub) st = hm_graph_set_hubs(g, -1);
+  if (st != HM_OK && g) {
+    hm_graph_destroy(g);
+    g = nullptr;
+  }
   std::lock_guard<std::mutex> lk(s->mu);
   if (st != HM_OK) {
     s->used_bytes -= need;
@@ -338,6 +358,7 @@ hm_status hm_server_add_graph(hm_server* s, const int64_t* offsets, const int32_
   GraphSlot slot;
   slot.g = g;
   slot.bytes = need;
+  slot.hub = hub;
   s->graphs.push_back(std::move(slot));
   *graph_id = (int32_t)s->graphs.size() - 1;
   return HM_OK;

@@ -8,6 +8,8 @@ served  the same requests submitted to an hm_server (slots run concurrently, one
         with maintain=True the server also keeps each graph's counts current from the exact
         update deltas, so count queries after a graph's first count need no launch; with
         batch_graphs=n a worker counts up to n ready graphs in one batched launch sequence.
+hub_min_edges=n > 0 (both forms) counts and updates every graph of at least n hyperedges
+hub-aware (hm_graph_set_hubs(g, -1)).
 Both start from identical graphs and apply identical batches, so every result must match.
 """
 from __future__ import annotations
@@ -25,8 +27,11 @@ def _latency_stats(lat_ms: list[float]) -> dict:
             "mean_ms": float(a.mean())}
 
 
-def run_plain(graphs, reqs, realtime: bool) -> dict:
+def run_plain(graphs, reqs, realtime: bool, hub_min_edges: int = 0) -> dict:
     handles = [Graph(off, nodes) for off, nodes in graphs]
+    for h, (off, _) in zip(handles, graphs):
+        if 0 < hub_min_edges <= off.size - 1:
+            h.set_hubs(-1)
     for h in handles:  # allocate every graph's count scratch outside the measured stream
         h.count_static()
     results, lat = [], []
@@ -52,9 +57,10 @@ def run_plain(graphs, reqs, realtime: bool) -> dict:
 
 
 def run_served(graphs, reqs, realtime: bool, slots: int = 4, budget_bytes: int = 64 << 30,
-               maintain: bool = False, batch_graphs: int = 0, capture: bool = False) -> dict:
+               maintain: bool = False, batch_graphs: int = 0, capture: bool = False,
+               hub_min_edges: int = 0) -> dict:
     srv = Server(budget_bytes, slots, maintain_counts=maintain, batch_graphs=batch_graphs,
-                 capture_updates=capture)
+                 capture_updates=capture, hub_min_edges=hub_min_edges)
     gids = [srv.add_graph(off, nodes) for off, nodes in graphs]
     for t in [srv.submit_count(g) for g in gids]:  # same warm-up as the plain path
         srv.wait(t)

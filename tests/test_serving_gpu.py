@@ -242,6 +242,26 @@ def test_server_config_rejects_bad_capture_flag(gpu_ready):
     from ipdps26_amd import _ServerConfig, lib
 
     h = ctypes.c_void_p()
-    cfg = _ServerConfig(1 << 30, 2, 0, 0, 2)
+    cfg = _ServerConfig(1 << 30, 2, 0, 0, 2, 0)
     assert lib().hm_server_create(ctypes.byref(cfg), ctypes.byref(h)) != 0
     assert "capture_updates" in lib().hm_last_error().decode()
+    cfg = _ServerConfig(1 << 30, 2, 0, 0, 0, -1)
+    assert lib().hm_server_create(ctypes.byref(cfg), ctypes.byref(h)) != 0
+    assert "hub_min_edges" in lib().hm_last_error().decode()
+
+
+# ---- hub-aware graphs (hub_min_edges) -----------------------------------------------------------
+@pytest.mark.parametrize("maintain", [False, True])
+def test_hub_graphs_served_equal_plain_by_triangles(gpu_ready, maintain):
+    """Graphs of 1,500 hyperedges or more counted and updated hub-aware by the server (and by the
+    plain path in the second run) while smaller graphs go by triangles: every result equals the
+    plain path's by triangles, request by request."""
+    graphs = W.serving_graphs(8, lo=400, hi=4000, seed=41)
+    reqs = W.serving_requests(graphs, 160, seed=410)
+    assert any(off.size - 1 >= 1500 for off, _ in graphs) and any(off.size - 1 < 1500 for off, _ in graphs)
+    plain = serving.run_plain(graphs, reqs, realtime=False)
+    served = serving.run_served(graphs, reqs, realtime=False, slots=4, maintain=maintain, batch_graphs=4,
+                                hub_min_edges=1500)
+    plain_hub = serving.run_plain(graphs, reqs, realtime=False, hub_min_edges=1500)
+    for k, (a, b, c) in enumerate(zip(plain["results"], served["results"], plain_hub["results"])):
+        assert np.array_equal(a, b) and np.array_equal(a, c), f"request {k} ({reqs[k]['kind']})"
