@@ -196,6 +196,21 @@ typedef struct hm_hub_stats {
 } hm_hub_stats;
 hm_status hm_graph_get_hub_stats(hm_graph* g, hm_hub_stats* out);
 
+/* One part of a hub-aware count with `hubs` hubs (1..23), so that a long count can be split over
+ * calls, processes or devices (no counterpart in the reference, whose counters run one process per
+ * graph: MoCHy-master/main_exact_par.cpp:122-165). The light triangles are enumerated once each, at
+ * their centre of lowest light rank (ascending light degree, ties by id: the same order in every
+ * process holding the same graph); this call enumerates those of the centres with ranks in
+ * [centre_begin, centre_end) (centre_end = -1: up to the number of hyperedges) and, with
+ * other_parts = 1, adds the class, pair-type and light-wedge parts, which involve every hyperedge.
+ * The bins of calls whose ranges partition [0, E), other_parts set in exactly one, sum (mod 2^64)
+ * to hm_count_static's with hm_graph_set_hubs(g, hubs). *light_triangles (may be NULL) receives the
+ * light triangles of the range. With other_parts = 1, hm_graph_get_stats then holds the
+ * projection statistics (wedges = sum_a C(d_a, 2)) with triangles = -1. A neighbour-key histogram
+ * overflow (other_parts = 1) returns HM_ERR_UNSUPPORTED. */
+hm_status hm_graph_count_hub_part(hm_graph* g, int32_t hubs, int64_t centre_begin, int64_t centre_end,
+                                  int32_t other_parts, uint64_t out[HM_STATIC_BINS], int64_t* light_triangles);
+
 /* Kernel timing of the last hm_count_static on g, measured with HIP events on the handle's
  * stream: names are "project" and "triangles"; "store" is the store-update part of the last eager
  * hm_graph_update (0 after a captured one); ms receives the duration of that phase. */

@@ -114,6 +114,8 @@ _SIGNATURES = [
     ("hm_graph_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_set_hubs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_get_hub_stats", ctypes.c_int, [ctypes.c_void_p, _P(_HubStats)]),
+    ("hm_graph_count_hub_part", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                               ctypes.c_int32, _P(ctypes.c_uint64), _P(ctypes.c_int64)]),
     ("hm_graph_set_replay", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_set_tri_form", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("hm_graph_get_tri_form", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
@@ -334,7 +336,7 @@ class Graph:
         _check(lib().hm_graph_set_tile(self._h, int(tile_ids)))
 
     def set_hubs(self, hubs: int) -> None:
-        """Hub-aware count_static: 0 = off (triangles of the full projection), 1..31 = that many
+        """Hub-aware count_static: 0 = off (triangles of the full projection), 1..23 = that many
         hubs, -1 = hub count from the cost model (hm_graph_set_hubs)."""
         _check(lib().hm_graph_set_hubs(self._h, int(hubs)))
 
@@ -342,6 +344,18 @@ class Graph:
         st = _HubStats()
         _check(lib().hm_graph_get_hub_stats(self._h, ctypes.byref(st)))
         return {name: getattr(st, name) for name, _ in _HubStats._fields_}
+
+    def count_hub_part(self, hubs: int, centre_begin: int = 0, centre_end: int = -1,
+                       other_parts: bool = True):
+        """One part of a hub-aware count (hm_graph_count_hub_part): the light triangles centred at
+        light ranks [centre_begin, centre_end), plus the other parts if other_parts. Returns
+        (uint64[30] bins, light triangles of the range)."""
+        out = np.zeros(STATIC_BINS, dtype=np.uint64)
+        lt = ctypes.c_int64(0)
+        _check(lib().hm_graph_count_hub_part(self._h, int(hubs), int(centre_begin), int(centre_end),
+                                             int(bool(other_parts)),
+                                             out.ctypes.data_as(_P(ctypes.c_uint64)), ctypes.byref(lt)))
+        return out, int(lt.value)
 
     def update(self, deleted, ins_offsets=None, ins_nodes=None, delta: bool = True):
         """Deletes the ids in `deleted`, inserts the CSR (ins_offsets, ins_nodes); returns

@@ -160,10 +160,15 @@ struct HubScratch {
   DevBuf first;    // uint8[E]    hyperedges ranked first (an update's changed ids)
   DevBuf firstids; // int32[n]    their ids
   DevBuf accd;     // uint64[40]  (D)'s bins
+  DevBuf accr;     // uint64[40]  (C) of one range of centres
   void release() {
     for (DevBuf* b : {&hubidx, &hmask, &lsz, &ckey, &ckey2, &ukey, &ucnt, &nruns, &hmr, &lszr, &types,
-                      &tcraw, &types2, &tcsrt, &tkey, &tcnt, &rkey, &first, &firstids, &accd})
+                      &tcraw, &types2, &tcsrt, &tkey, &tcnt, &rkey, &first, &firstids, &accd, &accr})
       b->release();
+  }
+  // the pair-type buffers, once (B) has run (they are the largest of a big graph's count)
+  void release_types() {
+    for (DevBuf* b : {&types, &tcraw, &types2, &tcsrt}) b->release();
   }
 };
 
@@ -280,6 +285,9 @@ struct HubRun {
   int32_t n_first = 0;
   bool only_first = false;
   bool stats = true;
+  bool abc = true;        // false: (A), (B), (C) skipped (their bins stay 0; no key histograms)
+  int64_t d_begin = 0;    // (D) over the centres of light ranks [d_begin, d_end) (d_end < 0: all)
+  int64_t d_end = -1;
 };
 // Parts (A) + (B) + (C) into h_abc's bins (plus the projection statistics when run.stats) and
 // (D) into h_d's bins (kAccLightTri: the light triangles enumerated).

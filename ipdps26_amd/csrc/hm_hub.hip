@@ -214,6 +214,7 @@ __device__ inline void light_nodes(int32_t* s_ln, int32_t a, const uint32_t* __r
 // then (C) over all pairs of keys (n_i n_j pairs of neighbours, C(n_i, 2) within one key) into
 // accC, and one pair type per key with neighbours above a, with their number, appended to
 // (tkeys, tcnts) for (B) (the slot count keeps counting past cap so the host can grow them).
+// Centres a0 + blockIdx.x. keys = 0: the light degree alone (no histogram, no (C), no pair types).
 // Dynamic LDS: T masks, then the histogram (keys, counts, counts above a: kHubHash each).
 template <typename MT, int NW>
 __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
     const uint32_t* __restrict__ hmask, const uint8_t* __restrict__ lsz, uint32_t* __restrict__ dL,
     unsigned long long* __restrict__ tkeys, unsigned long long* __restrict__ tcnts,
     unsigned long long* __restrict__ ntent, unsigned long long cap, uint32_t* __restrict__ overflow,
-    unsigned long long* __restrict__ accC) {
+    unsigned long long* __restrict__ accC, int32_t a0, int keys) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ unsigned long long hs[kHubIds * kHubCopies];
   __shared__ int32_t s_ln[HM_MAX_EDGE_SIZE];
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
   __shared__ uint32_t s_used, s_g;
   __shared__ uint32_t s_d[NW];
   __shared__ int s_over;
-  const int32_t a = blockIdx.x;
+  const int32_t a = a0 + (int32_t)blockIdx.x;
   const int la = lsz[a];
   if (la == 0) {  // no light node: no light neighbour
     if (threadIdx.x == 0) dL[a] = 0;
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
   }
   // once some centre's histogram has overflowed the pass is redone with fewer hubs: stop early
   // (one read, broadcast through LDS, so the whole workgroup leaves or stays together)
-  if (threadIdx.x == 0) s_over = (int)*(volatile uint32_t*)overflow;
+  if (threadIdx.x == 0) s_over = keys ? (int)*(volatile uint32_t*)overflow : 0;
   __syncthreads();
   if (s_over) return;
   __syncthreads();  // s_over is reset below
@@ -276,8 +277,10 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
       const MT m = M[x - t0];
       if (m && low_bit(m) == i) {
         const int C = popc(m);
-        const uint32_t key = hmask[x] << 8 | (uint32_t)C << 1 | (lsz[x] > C ? 1u : 0u);
-        ok = hub_key_add(hk, hn, hu, key, x > a, &s_used) && ok;
+        if (keys) {
+          const uint32_t key = hmask[x] << 8 | (uint32_t)C << 1 | (lsz[x] > C ? 1u : 0u);
+          ok = hub_key_add(hk, hn, hu, key, x > a, &s_used) && ok;
+        }
         ++d;
         M[x - t0] = (MT)0;
       }
@@ -293,8 +296,8 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
     dL[a] = dd;
     if (s_over) atomicOr(overflow, 1u);
   }
-  __syncthreads();     // s_d is reused by the compaction below
-  if (s_over) return;  // uniform: read after the barriers above
+  __syncthreads();               // s_d is reused by the compaction below
+  if (s_over || !keys) return;  // uniform: read after the barriers above
 
   // compact the histogram in place (stable: slot k moves to the number of keys before it, never
   // past k, and a chunk is read whole before it is written); pair types of the neighbours above a
@@ -600,7 +603,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_fill_tiled(
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vrank, const int8_t* __restrict__ hub,
     const uint8_t* __restrict__ lsz, const int32_t* __restrict__ rank,
     const unsigned long long* __restrict__ roff, uint32_t* __restrict__ rlen, uint32_t* __restrict__ rows,
-    uint32_t* __restrict__ bad) {
+    uint32_t* __restrict__ bad, int count_only) {
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
   __shared__ int32_t s_ln[HM_MAX_EDGE_SIZE];
@@ -616,8 +619,8 @@ __global__ __launch_bounds__(kBlock) void k_hub_fill_tiled(
   light_nodes(s_ln, a, eoff, enodes, hub);
   for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
   if (threadIdx.x == 0) s_cnt = 0;
-  uint32_t* out = rows + roff[ra];
-  const uint32_t cap = (uint32_t)(roff[ra + 1] - roff[ra]);
+  uint32_t* out = count_only ? nullptr : rows + roff[ra];
+  const uint32_t cap = count_only ? 0u : (uint32_t)(roff[ra + 1] - roff[ra]);
   for (int32_t t0 = ((ra + 1) / T) * T; t0 < E; t0 += T) {
     const int32_t t1 = min(E, t0 + T);
     const uint32_t total = tile_ranges(tr, la, s_ln, voff, vrank, max(t0, ra + 1), t1);
@@ -635,13 +638,17 @@ __global__ __launch_bounds__(kBlock) void k_hub_fill_tiled(
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    rlen[ra] = min(s_cnt, cap);
-    if (s_cnt > cap) atomicOr(bad, 1u);
+    if (count_only) {  // the upper light degree alone: sizes the rows before they are written
+      rlen[ra] = s_cnt;
+    } else {
+      rlen[ra] = min(s_cnt, cap);
+      if (s_cnt > cap) atomicOr(bad, 1u);
+    }
   }
 }
 
 // (D) over rank tiles of c, as k_tri_tiled walks the full projection: per b in r's upper light
-// row, state at uoff[r] + ib keeps a's light mask in b and the read position in b's row (grouped
+// row, state at uoff[r] - ubase + ib keeps a's light mask in b and the read position in b's row (grouped
 // by tile); tiles without work are skipped. The int32 slot counters are folded and cleared per
 // tile (the host bounds their count per tile).
 template <typename MT, int NW>
@@ -651,7 +658,8 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_tri_tiled(
     const int32_t* __restrict__ perm, const uint32_t* __restrict__ hmr, const uint8_t* __restrict__ lszr,
     const unsigned long long* __restrict__ roff, const uint32_t* __restrict__ rlen,
     const uint32_t* __restrict__ rows, const unsigned long long* __restrict__ uoff,
-    uint32_t* __restrict__ cur, MT* __restrict__ pm, unsigned long long* __restrict__ acc, int32_t r0) {
+    uint32_t* __restrict__ cur, MT* __restrict__ pm, unsigned long long* __restrict__ acc, int32_t r0,
+    unsigned long long ubase) {
   extern __shared__ __align__(16) unsigned char smem[];
   MT* M = reinterpret_cast<MT*>(smem);
   uint32_t* hmt = reinterpret_cast<uint32_t*>(M + T);
@@ -676,7 +684,7 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_tri_tiled(
   int32_t* hc = hs + (lane & (kHubCopies - 1));
   const uint32_t Aa = hmr[r];
   const uint32_t* rowA = rows + roff[r];
-  const unsigned long long st = uoff[r];
+  const unsigned long long st = uoff[r] - ubase;  // state of this launch's centres starts at uoff[r0]
   uint32_t prev_end = 0;
   unsigned long long ntri = 0;
   __syncthreads();
@@ -824,6 +832,10 @@ __global__ void k_hub_zero(unsigned long long* p, int n) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i < n) p[i] = 0ull;
 }
+__global__ void k_hub_add(unsigned long long* to, const unsigned long long* from, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) to[i] += from[i];
+}
 __global__ void k_hub_iota(int32_t* p, int32_t n) {
   const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i < n) p[i] = i;
@@ -917,6 +929,62 @@ hm_status hub_choose(hm_graph* g, int32_t* h_out, double* cost_out) {
   }
   *h_out = best_h;
   if (cost_out) *cost_out = best;
+  return HM_OK;
+}
+
+// Pair-type entries held at once (the centre pass runs in ranges of centres beyond it), and light
+// rows beyond which the tiled path sizes them by the upper light degrees (one more fill pass)
+// instead of the light degrees.
+constexpr unsigned long long kHubTypeCap = 1ull << 28;
+constexpr unsigned long long kHubRowsByDegree = 1ull << 32;
+// Per-launch (a, b) state of the tiled light-triangle kernel: centres are launched in ranges whose
+// upper rows hold at most this many entries in all (one centre's row may exceed it alone).
+constexpr unsigned long long kHubStateCap = 1ull << 30;
+// The three limits above, lowered by the environment variables HM_HUB_TYPE_CAP, HM_HUB_ROWS_BY_DEGREE
+// and HM_HUB_STATE_CAP (entries; tests use them to run the split paths on small graphs).
+static unsigned long long hub_limit(const char* name, unsigned long long dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  char* end = nullptr;
+  const unsigned long long x = std::strtoull(v, &end, 10);
+  return (end && *end == '\0') ? std::min(x, dflt) : dflt;
+}
+
+// (keys, counts)[0, n) sorted by key into (k2, c2), then reduced by key into (kout, cout); the
+// number of distinct keys goes to *nout (device). kout / cout may alias keys / counts.
+static hm_status sort_reduce_types(CountScratch& s, unsigned long long* keys, unsigned long long* counts,
+                                   unsigned long long* k2, unsigned long long* c2, unsigned long long* kout,
+                                   unsigned long long* cout, unsigned long long n, int bits,
+                                   unsigned long long* nout, hipStream_t st) {
+  size_t need1 = 0, need2 = 0;
+  HM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need1, keys, k2, counts, c2, (int)n, 0, bits, st));
+  HM_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, need2, k2, kout, c2, cout, nout, hipcub::Sum(), (int)n, st));
+  HM_TRY(s.tmp.ensure(std::max(need1, need2)));
+  size_t tb = s.tmp.bytes;
+  HM_HIP(hipcub::DeviceRadixSort::SortPairs(s.tmp.p, tb, keys, k2, counts, c2, (int)n, 0, bits, st));
+  tb = s.tmp.bytes;
+  HM_HIP(hipcub::DeviceReduce::ReduceByKey(s.tmp.p, tb, k2, kout, c2, cout, nout, hipcub::Sum(), (int)n, st));
+  return HM_OK;
+}
+
+// b grown to at least need bytes with its first keep bytes kept.
+static hm_status grow_keep(DevBuf& b, size_t need, size_t keep, hipStream_t st) {
+  if (need <= b.bytes && b.p) return HM_OK;
+  if (keep == 0) return b.ensure(need);
+  DevBuf n;
+  HM_TRY(n.ensure(std::max(need, b.bytes + b.bytes / 2)));
+  hipError_t e = hipMemcpyAsync(n.p, b.p, keep, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    n.release();
+    set_error("hub-aware count: copy of the pair types failed: %s", hipGetErrorString(e));
+    return HM_ERR_DEVICE;
+  }
+  b.release();
+  b.p = n.p;
+  b.bytes = n.bytes;
+  n.p = nullptr;
+  n.bytes = 0;
   return HM_OK;
 }
 
@@ -1075,79 +1143,120 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
   if (verbose) std::fprintf(stderr, "[hm hub] %d hubs, %d classes\n", hh, K);
   HM_HIP(hipEventRecord(g->ev[1], st));
 
-  // light neighbours: (C) into accC, light degrees, pair-type entries for (B). A pass whose entries
-  // outgrow the buffer runs again with a buffer of the reported size (accC starts from zero).
-  unsigned long long cap = std::max<unsigned long long>(1, std::min<unsigned long long>(
-                                                              lp_bound, std::max<unsigned long long>((unsigned long long)E * 1024, 1u << 20)));
-  unsigned long long nt = 0;
-  for (int attempt = 0;; ++attempt) {
-    HM_TRY(hb.types.ensure(sizeof(unsigned long long) * cap));
-    HM_TRY(hb.tcraw.ensure(sizeof(unsigned long long) * cap));
-    cap = std::min(hb.types.bytes, hb.tcraw.bytes) / sizeof(unsigned long long);
-    k_hub_zero<<<1, 64, 0, st>>>(accC, kAccCount);
+  // light neighbours in ranges of centres: (C), light degrees, and the range's pair-type entries for
+  // (B), sorted, reduced by type and merged into (tkey, tcnt) (nT distinct types). A range whose
+  // entries outgrow the buffer runs again, with a buffer of the reported size while that stays
+  // within kHubTypeCap entries, else as a shorter range; a range's (C) goes to accR first and into
+  // accC once the range has held. Without run.abc the pass computes the light degrees alone.
+  k_hub_zero<<<1, 64, 0, st>>>(accC, kAccCount);
+  unsigned long long nt_all = 0, nT = 0;
+  if (!run.abc) {
     k_hub_zero<<<1, 64, 0, st>>>(runs + 1, 3);
     k_hub_centre<MT, kHubWaves><<<E, kHubWaves * kWave, lds_centre, st>>>(
         E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
-        hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
-        hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), runs + 1, cap,
-        reinterpret_cast<uint32_t*>(runs + 3), accC);
+        hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(), nullptr,
+        nullptr, runs + 1, 0ull, reinterpret_cast<uint32_t*>(runs + 3), accC, 0, 0);
     HM_HIP(hipGetLastError());
-    unsigned long long r2[3] = {0, 0, 0};
-    HM_HIP(hipMemcpyAsync(r2, runs + 1, sizeof(r2), hipMemcpyDeviceToHost, st));
-    HM_HIP(hipStreamSynchronize(st));
-    if (r2[2]) {
-      *key_overflow = true;
-      set_error("hub-aware count: a hyperedge's light neighbours hold more than %u distinct (hub pattern, "
-                "overlap) keys with %d hubs; use fewer hubs", kHubKeysMax, hh);
-      return HM_ERR_UNSUPPORTED;
+  } else {
+    HM_TRY(hb.accr.ensure(sizeof(unsigned long long) * kAccCount));
+    unsigned long long* accR = hb.accr.as<unsigned long long>();
+    const unsigned long long type_cap = std::max<unsigned long long>(1, hub_limit("HM_HUB_TYPE_CAP", kHubTypeCap));
+    unsigned long long cap = std::max<unsigned long long>(
+        1, std::min<unsigned long long>({lp_bound, std::max<unsigned long long>((unsigned long long)E * 1024, 1u << 20),
+                                         type_cap}));
+    int32_t a0 = 0, span = E;
+    while (a0 < E) {
+      const int32_t n = std::min(span, E - a0);
+      HM_TRY(hb.types.ensure(sizeof(unsigned long long) * cap));
+      HM_TRY(hb.tcraw.ensure(sizeof(unsigned long long) * cap));
+      cap = std::min<unsigned long long>(std::min(hb.types.bytes, hb.tcraw.bytes) / sizeof(unsigned long long),
+                                         type_cap);
+      k_hub_zero<<<1, 64, 0, st>>>(accR, kAccCount);
+      k_hub_zero<<<1, 64, 0, st>>>(runs + 1, 3);
+      k_hub_centre<MT, kHubWaves><<<(unsigned)n, kHubWaves * kWave, lds_centre, st>>>(
+          E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+          hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
+          hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), runs + 1, cap,
+          reinterpret_cast<uint32_t*>(runs + 3), accR, a0, 1);
+      HM_HIP(hipGetLastError());
+      unsigned long long r2[3] = {0, 0, 0};
+      HM_HIP(hipMemcpyAsync(r2, runs + 1, sizeof(r2), hipMemcpyDeviceToHost, st));
+      HM_HIP(hipStreamSynchronize(st));
+      if (r2[2]) {
+        *key_overflow = true;
+        set_error("hub-aware count: a hyperedge's light neighbours hold more than %u distinct (hub pattern, "
+                  "overlap) keys with %d hubs; use fewer hubs", kHubKeysMax, hh);
+        return HM_ERR_UNSUPPORTED;
+      }
+      const unsigned long long nt = r2[0];
+      if (nt > cap) {  // the range again: a larger buffer, or fewer centres
+        if (nt <= type_cap) {
+          cap = nt;
+        } else {
+          if (n == 1) {
+            set_error("hub-aware count: one hyperedge has %llu pair-type entries", nt);
+            return HM_ERR_UNSUPPORTED;
+          }
+          span = std::max<int32_t>(1, (int32_t)((double)n * (double)cap / (double)nt * 0.9));
+        }
+        continue;
+      }
+      k_hub_add<<<1, 64, 0, st>>>(accC, accR, kAccCount);
+      nt_all += nt;
+      if (nt > 0) {  // sort and reduce the range's entries, append them, merge with the earlier ranges
+        HM_TRY(hb.types2.ensure(sizeof(unsigned long long) * std::max(nt, nT)));
+        HM_TRY(hb.tcsrt.ensure(sizeof(unsigned long long) * std::max(nt, nT)));
+        HM_TRY(sort_reduce_types(s, hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(),
+                                 hb.types2.as<unsigned long long>(), hb.tcsrt.as<unsigned long long>(),
+                                 hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), nt, tbits,
+                                 runs + 2, st));
+        unsigned long long nR = 0;
+        HM_HIP(hipMemcpyAsync(&nR, runs + 2, sizeof(nR), hipMemcpyDeviceToHost, st));
+        HM_HIP(hipStreamSynchronize(st));
+        HM_TRY(grow_keep(hb.tkey, sizeof(unsigned long long) * (nT + nR), sizeof(unsigned long long) * nT, st));
+        HM_TRY(grow_keep(hb.tcnt, sizeof(unsigned long long) * (nT + nR), sizeof(unsigned long long) * nT, st));
+        HM_HIP(hipMemcpyAsync(hb.tkey.as<unsigned long long>() + nT, hb.types.p, sizeof(unsigned long long) * nR,
+                              hipMemcpyDeviceToDevice, st));
+        HM_HIP(hipMemcpyAsync(hb.tcnt.as<unsigned long long>() + nT, hb.tcraw.p, sizeof(unsigned long long) * nR,
+                              hipMemcpyDeviceToDevice, st));
+        if (nT > 0) {
+          const unsigned long long nm = nT + nR;
+          HM_TRY(hb.types2.ensure(sizeof(unsigned long long) * nm));
+          HM_TRY(hb.tcsrt.ensure(sizeof(unsigned long long) * nm));
+          HM_TRY(sort_reduce_types(s, hb.tkey.as<unsigned long long>(), hb.tcnt.as<unsigned long long>(),
+                                   hb.types2.as<unsigned long long>(), hb.tcsrt.as<unsigned long long>(),
+                                   hb.tkey.as<unsigned long long>(), hb.tcnt.as<unsigned long long>(), nm, tbits,
+                                   runs + 2, st));
+          HM_HIP(hipMemcpyAsync(&nR, runs + 2, sizeof(nR), hipMemcpyDeviceToHost, st));
+          HM_HIP(hipStreamSynchronize(st));
+          nT = nR;
+        } else {
+          nT = nR;
+        }
+      }
+      if (verbose && n < E)
+        std::fprintf(stderr, "[hm hub] light neighbours: centres %d..%d of %d done, %llu pair types\n", a0, a0 + n,
+                     E, nT);
+      a0 += n;
     }
-    nt = r2[0];
-    if (nt <= cap) break;
-    if (attempt > 0) return fail(HM_ERR_DEVICE, "hub-aware count: pair-type entries grew between passes");
-    cap = nt;
   }
   g->hub_stats.light_pairs = -1;
   HM_HIP(hipEventRecord(g->ev[2], st));
   // (A) once the histograms are known to hold (class triples are the costly part at many hubs)
-  if (K > k0) {
+  if (run.abc && K > k0) {
     k_hub_classes<<<(unsigned)(K - k0), kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(),
                                                          hb.ucnt.as<uint32_t>(), acc);
     HM_HIP(hipGetLastError());
   }
-  if (verbose) std::fprintf(stderr, "[hm hub] classes and light neighbours done, %llu pair-type entries\n", nt);
+  if (verbose) std::fprintf(stderr, "[hm hub] classes and light neighbours done, %llu pair-type entries\n", nt_all);
 
-  // (B) pair types: sum the entries' counts per type, then every type against every class
-  if (nt > 0) {
-    size_t need1 = 0, need2 = 0;
-    HM_TRY(hb.types2.ensure(sizeof(unsigned long long) * nt));
-    HM_TRY(hb.tcsrt.ensure(sizeof(unsigned long long) * nt));
-    HM_TRY(hb.tkey.ensure(sizeof(unsigned long long) * nt));
-    HM_TRY(hb.tcnt.ensure(sizeof(unsigned long long) * nt));
-    HM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need1, hb.types.as<unsigned long long>(),
-                                              hb.types2.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(),
-                                              hb.tcsrt.as<unsigned long long>(), (int)nt, 0, tbits, st));
-    HM_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, need2, hb.types2.as<unsigned long long>(),
-                                             hb.tkey.as<unsigned long long>(), hb.tcsrt.as<unsigned long long>(),
-                                             hb.tcnt.as<unsigned long long>(), runs + 2, hipcub::Sum(), (int)nt, st));
-    HM_TRY(s.tmp.ensure(std::max(need1, need2)));
-    tb = s.tmp.bytes;
-    HM_HIP(hipcub::DeviceRadixSort::SortPairs(s.tmp.p, tb, hb.types.as<unsigned long long>(),
-                                              hb.types2.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(),
-                                              hb.tcsrt.as<unsigned long long>(), (int)nt, 0, tbits, st));
-    tb = s.tmp.bytes;
-    HM_HIP(hipcub::DeviceReduce::ReduceByKey(s.tmp.p, tb, hb.types2.as<unsigned long long>(),
-                                             hb.tkey.as<unsigned long long>(), hb.tcsrt.as<unsigned long long>(),
-                                             hb.tcnt.as<unsigned long long>(), runs + 2, hipcub::Sum(), (int)nt, st));
-    unsigned long long T = 0;
-    HM_HIP(hipMemcpyAsync(&T, runs + 2, sizeof(T), hipMemcpyDeviceToHost, st));
-    HM_HIP(hipStreamSynchronize(st));
-    g->hub_stats.pair_types = (int64_t)T;
-    if (T > 0 && K > k0) {
-      k_hub_types<<<(unsigned)T, kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(),
-                                                   hb.ucnt.as<uint32_t>(), hb.tkey.as<unsigned long long>(),
-                                                   hb.tcnt.as<unsigned long long>(), acc);
-      HM_HIP(hipGetLastError());
-    }
+  // (B) every distinct pair type against every class
+  g->hub_stats.pair_types = (int64_t)nT;
+  if (run.abc && nT > 0 && K > k0) {
+    k_hub_types<<<(unsigned)nT, kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(), hb.ucnt.as<uint32_t>(),
+                                                 hb.tkey.as<unsigned long long>(), hb.tcnt.as<unsigned long long>(),
+                                                 acc);
+    HM_HIP(hipGetLastError());
   }
   HM_HIP(hipEventRecord(g->ev[3], st));
 
@@ -1173,6 +1282,8 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
       E, s.perm.as<int32_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
       s.rank.as<int32_t>(), hb.hmr.as<uint32_t>(), hb.lszr.as<uint8_t>(), s.degr.as<unsigned long long>());
   const int32_t nD = run.only_first ? run.n_first : E;  // centres of (D)
+  const int32_t d0 = std::min<int64_t>(std::max<int64_t>(run.d_begin, 0), nD);
+  const int32_t d1 = run.d_end < 0 ? nD : (int32_t)std::min<int64_t>(std::max<int64_t>(run.d_end, d0), nD);
   tb = s.tmp.bytes;
   HM_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp.p, tb, s.degr.as<unsigned long long>(),
                                           s.roff.as<unsigned long long>(), E + 1, st));
@@ -1181,8 +1292,10 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
   HM_HIP(hipStreamSynchronize(st));
   g->hub_stats.light_pairs = (int64_t)(total / 2);
   if (verbose) std::fprintf(stderr, "[hm hub] pair types done; %llu light pairs\n", total / 2);
-  HM_TRY(s.rows.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(total, 1)));
+  const unsigned long long rows_by_degree = hub_limit("HM_HUB_ROWS_BY_DEGREE", kHubRowsByDegree);
+  if (total > rows_by_degree) hb.release_types();  // (B) is done; the rows need the memory
   if (dense) {
+    HM_TRY(s.rows.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(total, 1)));
     k_hub_fill<MT><<<E, kBlock, (size_t)E * sizeof(MT), st>>>(
         E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
         hb.hubidx.as<int8_t>(), hb.lsz.as<uint8_t>(), s.rank.as<int32_t>(), s.roff.as<unsigned long long>(),
@@ -1190,12 +1303,12 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
     HM_HIP(hipGetLastError());
     HM_TRY(check_fill(runs + 4, st));
     HM_HIP(hipEventRecord(g->ev[5], st));
-    if (nD > 0) {
-      k_hub_tri<MT, kHubWaves><<<(unsigned)nD, kHubWaves * kWave, (size_t)E * kTriKey, st>>>(
+    if (d1 > d0) {
+      k_hub_tri<MT, kHubWaves><<<(unsigned)(d1 - d0), kHubWaves * kWave, (size_t)E * kTriKey, st>>>(
           E, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
           hb.hubidx.as<int8_t>(), s.rank.as<int32_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(),
           hb.lszr.as<uint8_t>(), s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
-          accD, 0);
+          accD, d0);
       HM_HIP(hipGetLastError());
     }
   } else {
@@ -1219,10 +1332,26 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
                                                s.vkeys2.as<unsigned long long>(), (int)I, 0, 32 + vbits, st));
       k_hub_low_words<<<ib, kBlock, 0, st>>>(s.vkeys2.as<unsigned long long>(), s.vrank.as<int32_t>(), I);
     }
+    if (total > rows_by_degree) {  // rows by the upper light degree: a counting pass first
+      k_hub_fill_tiled<MT><<<E, kBlock, (size_t)Tt * sizeof(MT), st>>>(
+          E, Tt, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), s.vrank.as<int32_t>(),
+          hb.hubidx.as<int8_t>(), hb.lsz.as<uint8_t>(), s.rank.as<int32_t>(), s.roff.as<unsigned long long>(),
+          s.rlen.as<uint32_t>(), nullptr, reinterpret_cast<uint32_t*>(runs + 4), 1);
+      HM_HIP(hipGetLastError());
+      k_hub_widen<<<(unsigned)((E + 1 + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+          s.rlen.as<uint32_t>(), s.degr.as<unsigned long long>(), E);
+      tb = s.tmp.bytes;
+      HM_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp.p, tb, s.degr.as<unsigned long long>(),
+                                              s.roff.as<unsigned long long>(), E + 1, st));
+      HM_HIP(hipMemcpyAsync(&total, s.roff.as<unsigned long long>() + E, sizeof(total), hipMemcpyDeviceToHost, st));
+      HM_HIP(hipStreamSynchronize(st));
+      if (verbose) std::fprintf(stderr, "[hm hub] upper light rows: %llu entries\n", total);
+    }
+    HM_TRY(s.rows.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(total, 1)));
     k_hub_fill_tiled<MT><<<E, kBlock, (size_t)Tt * sizeof(MT), st>>>(
         E, Tt, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), s.vrank.as<int32_t>(),
         hb.hubidx.as<int8_t>(), hb.lsz.as<uint8_t>(), s.rank.as<int32_t>(), s.roff.as<unsigned long long>(),
-        s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), reinterpret_cast<uint32_t*>(runs + 4));
+        s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(), reinterpret_cast<uint32_t*>(runs + 4), 0);
     HM_HIP(hipGetLastError());
     HM_TRY(check_fill(runs + 4, st));
     // per (a, b) state at the scan of the upper-row lengths; the int32 slot counters of a tile
@@ -1230,10 +1359,11 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
     std::vector<uint32_t> rl((size_t)E);
     HM_HIP(hipMemcpyAsync(rl.data(), s.rlen.p, sizeof(uint32_t) * E, hipMemcpyDeviceToHost, st));
     HM_HIP(hipStreamSynchronize(st));
-    unsigned long long U = 0, umax = 0;
-    for (uint32_t x : rl) {
-      U += x;
-      umax = std::max<unsigned long long>(umax, x);
+    std::vector<unsigned long long> pre((size_t)E + 1, 0ull);
+    unsigned long long umax = 0;
+    for (int32_t r = 0; r < E; ++r) {
+      pre[(size_t)r + 1] = pre[(size_t)r] + rl[(size_t)r];
+      umax = std::max<unsigned long long>(umax, rl[(size_t)r]);
     }
     if (umax * ((unsigned long long)Tt / kHubCopies + 4 * kHubLoads + 1) >= (1ull << 31))
       return fail(HM_ERR_UNSUPPORTED, "hub-aware count: an upper light row is too long for the tile counters");
@@ -1242,18 +1372,30 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
     tb = s.tmp.bytes;
     HM_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp.p, tb, s.degr.as<unsigned long long>(),
                                             s.uoff.as<unsigned long long>(), E + 1, st));
-    HM_TRY(s.cur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(U, 1)));
-    HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(U, 1)));
+    // centres in launches whose (a, b) state stays within kHubStateCap entries (a lone centre may
+    // exceed it); with HM_VERBOSE set, also at most a sixteenth of the centres per launch, each
+    // reported as it ends
+    const int32_t most = verbose ? std::max<int32_t>(4096, nD / 16) : std::max<int32_t>(nD, 1);
+    const unsigned long long state_cap = hub_limit("HM_HUB_STATE_CAP", kHubStateCap);
+    std::vector<int32_t> cut{d0};
+    unsigned long long smax = 0;
+    while (cut.back() < d1) {
+      const int32_t r0 = cut.back();
+      int32_t r1 = r0 + 1;
+      while (r1 < d1 && r1 - r0 < most && pre[(size_t)r1 + 1] - pre[(size_t)r0] <= state_cap) ++r1;
+      smax = std::max(smax, pre[(size_t)r1] - pre[(size_t)r0]);
+      cut.push_back(r1);
+    }
+    HM_TRY(s.cur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(smax, 1)));
+    HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(smax, 1)));
     HM_HIP(hipEventRecord(g->ev[5], st));
-    // centres in chunks: with HM_VERBOSE set, a line per chunk reports the progress of long counts
-    const int32_t chunk = verbose ? std::max<int32_t>(4096, nD / 16) : std::max<int32_t>(nD, 1);
-    for (int32_t r0 = 0; r0 < nD; r0 += chunk) {
-      const int32_t n = std::min(chunk, nD - r0);
+    for (size_t q = 0; q + 1 < cut.size(); ++q) {
+      const int32_t r0 = cut[q], n = cut[q + 1] - cut[q];
       k_hub_tri_tiled<MT, kHubWaves><<<(unsigned)n, kHubWaves * kWave, (size_t)Tt * kTriKey, st>>>(
           E, Tt, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), s.vrank.as<int32_t>(),
           hb.hubidx.as<int8_t>(), s.perm.as<int32_t>(), hb.hmr.as<uint32_t>(), hb.lszr.as<uint8_t>(),
           s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
-          s.uoff.as<unsigned long long>(), s.cur.as<uint32_t>(), s.pm.as<MT>(), accD, r0);
+          s.uoff.as<unsigned long long>(), s.cur.as<uint32_t>(), s.pm.as<MT>(), accD, r0, pre[(size_t)r0]);
       HM_HIP(hipGetLastError());
       if (verbose) {
         HM_HIP(hipStreamSynchronize(st));

@@ -140,6 +140,77 @@ def test_forced_tiles_match_oracle(gpu_ready, seed):
                 assert np.array_equal(g.count_static().astype(np.int64), want), (tile, h)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_parts_over_centre_ranges_sum_to_the_count(gpu_ready, seed):
+    """hm_graph_count_hub_part over a partition of the light ranks (the other parts once) sums to
+    the oracle's bins; the light triangles of the ranges sum to the full count's; the projection
+    statistics of the call with the other parts satisfy the wedge identity."""
+    rng = np.random.default_rng(9100 + seed)
+    kmax = [4, 12, 40][seed % 3]
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(150, 400)),
+                                   num_nodes=int(rng.integers(kmax + 10, 90)), kmax=kmax, dup_frac=0.1)
+    E = off.size - 1
+    hubs = [2, 5, 9][seed % 3]
+    want = O.static_mochy(off, nodes)
+    cuts = sorted(set([0, E] + rng.integers(0, E, 3).tolist()))
+    with hm.Graph(off, nodes) as g:
+        if seed % 2:
+            g.set_tile(64)
+        g.set_hubs(hubs)
+        full = g.count_static().astype(np.int64)
+        full_lt = g.hub_stats()["light_triangles"]
+        total, lt = np.zeros(hm.STATIC_BINS, dtype=np.uint64), 0
+        for i, (a, b) in enumerate(zip(cuts[:-1], cuts[1:])):
+            end = -1 if b == E and i % 2 else b
+            print(f"seed={seed} E={E} hubs={hubs} range [{a}, {end}) other={i == 0}", flush=True)
+            bins, n = g.count_hub_part(hubs, a, end, other_parts=(i == 0))
+            if i == 0:
+                st = g.stats()
+                assert st["triangles"] == -1
+                wedges = st["wedges"]
+            total += bins
+            lt += n
+    assert np.array_equal(total.astype(np.int64), want) and np.array_equal(full, want)
+    assert lt == full_lt
+    closed = int(want.sum() - want[OPEN].sum())
+    assert int(want[OPEN].sum()) + 3 * closed == wedges
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_split_paths_match_oracle(gpu_ready, monkeypatch, seed):
+    """The paths of graphs too big for one pass, forced on small ones: the centre pass in ranges
+    whose pair types are merged (HM_HUB_TYPE_CAP just above one hyperedge's entries), rows sized by
+    the upper light degrees (HM_HUB_ROWS_BY_DEGREE=0) and light-triangle launches of a few dozen
+    row entries (HM_HUB_STATE_CAP=40)."""
+    rng = np.random.default_rng(9300 + seed)
+    kmax = [6, 40][seed % 2]
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(150, 500)),
+                                   num_nodes=int(rng.integers(kmax + 10, 120)), kmax=kmax, dup_frac=0.2)
+    E = off.size - 1
+    want = O.static_mochy(off, nodes)
+    monkeypatch.setenv("HM_HUB_TYPE_CAP", str(E + 1))
+    monkeypatch.setenv("HM_HUB_ROWS_BY_DEGREE", "0")
+    monkeypatch.setenv("HM_HUB_STATE_CAP", "40")
+    for tile in (0, 64):
+        for h in (1, 4, 12):
+            print(f"seed={seed} E={E} tile={tile} hubs={h}", flush=True)
+            with hm.Graph(off, nodes) as g:
+                g.set_tile(tile)
+                g.set_hubs(h)
+                assert np.array_equal(g.count_static().astype(np.int64), want), (tile, h)
+                bins, _ = g.count_hub_part(h, 0, E // 2, other_parts=True)
+                rest, _ = g.count_hub_part(h, E // 2, -1, other_parts=False)
+                assert np.array_equal((bins + rest).astype(np.int64), want), (tile, h)
+
+
+def test_part_count_rejects_bad_arguments(gpu_ready):
+    off, nodes = csr([[0, 1], [1, 2], [2, 0], [0, 3]])
+    with hm.Graph(off, nodes) as g:
+        for args in ((0, 0, -1), (24, 0, -1), (3, -1, -1), (3, 3, 2)):
+            with pytest.raises(hm.HMotifError):
+                g.count_hub_part(*args)
+
+
 def test_beyond_one_table_equals_triangle_path(gpu_ready):
     off, nodes = hm.generate_static(W.scaled(W.STATIC_C2, 20_000, 10_000, seed=22))
     with hm.Graph(off, nodes) as g:
