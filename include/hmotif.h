@@ -193,6 +193,8 @@ typedef struct hm_hub_stats {
   double ms_types;        /* class triples, pair-type sort and pair types x classes */
   double ms_triangles;    /* light projection rows and light triangles */
   double ms_tri_kernel;   /* the light-triangle kernel alone (k_hub_tri or k_hub_tri_tiled) */
+  int64_t big_centres;    /* hyperedges whose neighbour keys outgrew the LDS histogram (7,168 keys),
+                             counted with a global-memory one (57,344 keys) */
 } hm_hub_stats;
 hm_status hm_graph_get_hub_stats(hm_graph* g, hm_hub_stats* out);
 
@@ -207,9 +209,13 @@ hm_status hm_graph_get_hub_stats(hm_graph* g, hm_hub_stats* out);
  * to hm_count_static's with hm_graph_set_hubs(g, hubs). *light_triangles (may be NULL) receives the
  * light triangles of the range. With other_parts = 1, hm_graph_get_stats then holds the
  * projection statistics (wedges = sum_a C(d_a, 2)) with triangles = -1. A neighbour-key histogram
- * overflow (other_parts = 1) returns HM_ERR_UNSUPPORTED. */
+ * overflow (other_parts = 1) returns HM_ERR_UNSUPPORTED. seconds > 0 bounds the call: the range's
+ * light triangles are enumerated in launches of at most 1/256 of the centres, and none starts once
+ * `seconds` have passed since the call began; *centre_done (may be NULL) receives the end of the
+ * ranks enumerated (the range's end unless the time ran out), where the next part begins. */
 hm_status hm_graph_count_hub_part(hm_graph* g, int32_t hubs, int64_t centre_begin, int64_t centre_end,
-                                  int32_t other_parts, uint64_t out[HM_STATIC_BINS], int64_t* light_triangles);
+                                  int32_t other_parts, double seconds, uint64_t out[HM_STATIC_BINS],
+                                  int64_t* light_triangles, int64_t* centre_done);
 
 /* Kernel timing of the last hm_count_static on g, measured with HIP events on the handle's
  * stream: names are "project" and "triangles"; "store" is the store-update part of the last eager

@@ -82,6 +82,7 @@ class _HubStats(ctypes.Structure):
         ("ms_types", ctypes.c_double),
         ("ms_triangles", ctypes.c_double),
         ("ms_tri_kernel", ctypes.c_double),
+        ("big_centres", ctypes.c_int64),
     ]
 
 
@@ -115,7 +116,8 @@ _SIGNATURES = [
     ("hm_graph_set_hubs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_get_hub_stats", ctypes.c_int, [ctypes.c_void_p, _P(_HubStats)]),
     ("hm_graph_count_hub_part", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
-                                               ctypes.c_int32, _P(ctypes.c_uint64), _P(ctypes.c_int64)]),
+                                               ctypes.c_int32, ctypes.c_double, _P(ctypes.c_uint64),
+                                               _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     ("hm_graph_set_replay", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("hm_graph_set_tri_form", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("hm_graph_get_tri_form", ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
@@ -346,16 +348,18 @@ class Graph:
         return {name: getattr(st, name) for name, _ in _HubStats._fields_}
 
     def count_hub_part(self, hubs: int, centre_begin: int = 0, centre_end: int = -1,
-                       other_parts: bool = True):
+                       other_parts: bool = True, seconds: float = 0.0, with_done: bool = False):
         """One part of a hub-aware count (hm_graph_count_hub_part): the light triangles centred at
-        light ranks [centre_begin, centre_end), plus the other parts if other_parts. Returns
-        (uint64[30] bins, light triangles of the range)."""
+        light ranks [centre_begin, centre_end), plus the other parts if other_parts; seconds > 0
+        stops starting launches after that long. Returns (uint64[30] bins, light triangles of the
+        range), plus the end of the ranks enumerated if with_done."""
         out = np.zeros(STATIC_BINS, dtype=np.uint64)
-        lt = ctypes.c_int64(0)
+        lt, done = ctypes.c_int64(0), ctypes.c_int64(0)
         _check(lib().hm_graph_count_hub_part(self._h, int(hubs), int(centre_begin), int(centre_end),
-                                             int(bool(other_parts)),
-                                             out.ctypes.data_as(_P(ctypes.c_uint64)), ctypes.byref(lt)))
-        return out, int(lt.value)
+                                             int(bool(other_parts)), float(seconds),
+                                             out.ctypes.data_as(_P(ctypes.c_uint64)), ctypes.byref(lt),
+                                             ctypes.byref(done)))
+        return (out, int(lt.value), int(done.value)) if with_done else (out, int(lt.value))
 
     def update(self, deleted, ins_offsets=None, ins_nodes=None, delta: bool = True):
         """Deletes the ids in `deleted`, inserts the CSR (ins_offsets, ins_nodes); returns

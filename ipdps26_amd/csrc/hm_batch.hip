@@ -596,8 +596,8 @@ hm_status hm_batch_count_static(hm_batch* b, hm_graph* const* graphs, int32_t n,
       return hm::fail(HM_ERR_BAD_INPUT, "hm_batch_count_static: graph on another device than the batch");
     if (graphs[k]->csr.E >= 3 && graphs[k]->csr.max_size > HM_MAX_EDGE_SIZE)
       return hm::fail(HM_ERR_UNSUPPORTED, "hm_batch_count_static: hyperedge larger than 64 nodes");
-    if ((int64_t)graphs[k]->csr.E >= hm::kMaxPackedId)
-      return hm::fail(HM_ERR_UNSUPPORTED, "hm_batch_count_static: more than 2^25 hyperedges");
+    if ((int64_t)graphs[k]->csr.E >= hm::kMaxIds)
+      return hm::fail(HM_ERR_UNSUPPORTED, "hm_batch_count_static: more than 2^22 - 1 hyperedges");
   }
   HM_HIP(hipSetDevice(b->device));
   b->last = hm_batch_stats{};

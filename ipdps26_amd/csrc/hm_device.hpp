@@ -30,6 +30,10 @@ enum AccSlot : int {
 constexpr int kCntBits = 7;
 constexpr uint32_t kCntMask = (1u << kCntBits) - 1u;
 constexpr int64_t kMaxPackedId = (int64_t)1 << (32 - kCntBits);
+// Ids (hyperedges, temporal hyperedges) a graph may hold: the kernels launch one workgroup of up to
+// 1,024 work-items per id, and a launch's work-item count (grid x block) is a 32-bit quantity.
+constexpr int64_t kMaxIds = (int64_t)1 << 22;
+static_assert(kMaxIds <= kMaxPackedId, "ids must fit the packed row entries");
 
 // 64-bit masks use unsigned long long (the type of the 64-bit LDS atomics).
 using mask64_t = unsigned long long;

@@ -1280,7 +1280,7 @@ static hm_status make_plan(hm_graph* g, int32_t E2, int32_t n_del, int32_t n_ins
   p.icap = std::max<int32_t>(64, 2 * n_ins);
   p.iicap = std::max<int64_t>(512, 2 * Iins);
   const int64_t ewant = (int64_t)E2 + std::max<int64_t>({(int64_t)4 * p.icap, (int64_t)E2 / 8, 256});
-  p.Ecap = (int32_t)std::min<int64_t>({ewant, (int64_t)dense_max_ids(g, p.wide), kMaxPackedId - 1});
+  p.Ecap = (int32_t)std::min<int64_t>({ewant, (int64_t)dense_max_ids(g, p.wide), kMaxIds - 1});
   if (p.Ecap < E2) return fail(HM_ERR_UNSUPPORTED, "captured update: ids beyond one delta table");
   p.Vcap = (int32_t)std::min<int64_t>((int64_t)V2 + std::max<int64_t>((int64_t)V2 / 8, p.iicap), (1ll << 30));
   if (p.Vcap < V2) return fail(HM_ERR_UNSUPPORTED, "captured update: node ids too large");
@@ -1539,7 +1539,7 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
   if ((n_del > 0 && !del_ids) || (n_ins > 0 && (!ins_offsets || !assigned_ids)))
     return fail(HM_ERR_BAD_INPUT, "hm_graph_update: null batch array");
   const int32_t E = g->csr.E;
-  if ((int64_t)E + n_ins >= kMaxPackedId) return fail(HM_ERR_OVER_BUDGET, "hm_graph_update: id space exhausted");
+  if ((int64_t)E + n_ins >= kMaxIds) return fail(HM_ERR_OVER_BUDGET, "hm_graph_update: id space exhausted");
   std::vector<uint8_t> delmask((size_t)std::max(E, 1), 0);
   for (int32_t k = 0; k < n_del; ++k) {
     const int32_t x = del_ids[k];

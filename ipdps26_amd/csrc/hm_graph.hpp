@@ -161,14 +161,17 @@ struct HubScratch {
   DevBuf firstids; // int32[n]    their ids
   DevBuf accd;     // uint64[40]  (D)'s bins
   DevBuf accr;     // uint64[40]  (C) of one range of centres
+  DevBuf big;      // int32[E]    centres whose keys outgrew the LDS histogram
+  DevBuf bighash;  // uint32[]    their histograms in global memory (3 * 2^16 words per workgroup)
   void release() {
     for (DevBuf* b : {&hubidx, &hmask, &lsz, &ckey, &ckey2, &ukey, &ucnt, &nruns, &hmr, &lszr, &types,
-                      &tcraw, &types2, &tcsrt, &tkey, &tcnt, &rkey, &first, &firstids, &accd, &accr})
+                      &tcraw, &types2, &tcsrt, &tkey, &tcnt, &rkey, &first, &firstids, &accd, &accr, &big,
+                      &bighash})
       b->release();
   }
   // the pair-type buffers, once (B) has run (they are the largest of a big graph's count)
   void release_types() {
-    for (DevBuf* b : {&types, &tcraw, &types2, &tcsrt}) b->release();
+    for (DevBuf* b : {&types, &tcraw, &types2, &tcsrt, &bighash}) b->release();
   }
 };
 
@@ -288,6 +291,8 @@ struct HubRun {
   bool abc = true;        // false: (A), (B), (C) skipped (their bins stay 0; no key histograms)
   int64_t d_begin = 0;    // (D) over the centres of light ranks [d_begin, d_end) (d_end < 0: all)
   int64_t d_end = -1;
+  double seconds = 0;     // > 0: (D)'s launches stop once this long has passed since the count began
+  int64_t* d_done = nullptr;  // receives the end of the light ranks (D) enumerated
 };
 // Parts (A) + (B) + (C) into h_abc's bins (plus the projection statistics when run.stats) and
 // (D) into h_d's bins (kAccLightTri: the light triangles enumerated).

@@ -40,6 +40,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -164,28 +165,35 @@ __device__ inline unsigned long long type_key(uint32_t sa, uint32_t sx) {
   return (unsigned long long)hi << 32 | lo;
 }
 
-// Key histogram of a centre's light neighbours (LDS, open addressing): key = hub mask << 8 |
-// C_ax << 1 | [|L(x)| > C_ax] (hub masks of at most 23 bits), the neighbours with that key and
-// those of them with a larger id. A centre with more than kHubKeysMax distinct keys reports an
-// overflow (the count is then redone with fewer hubs).
+// Key histogram of a centre's light neighbours (open addressing over 2^HB slots: in LDS with HB =
+// kHubHashBits, in global memory with HB = kHubBigBits for the centres whose keys the LDS table
+// cannot hold): key = hub mask << 8 | C_ax << 1 | [|L(x)| > C_ax] (hub masks of at most 23 bits),
+// the neighbours with that key and those of them with a larger id. At most 7/8 of the slots fill.
 constexpr int kHubHashBits = 13;
 constexpr int kHubHash = 1 << kHubHashBits;
-constexpr uint32_t kHubKeysMax = kHubHash - kHubHash / 8;
+constexpr uint32_t kHubKeysMax = kHubHash - kHubHash / 8;  // lowered by HM_HUB_LDS_KEYS (tests)
+constexpr int kHubBigBits = 16;
+constexpr uint32_t kHubBigKeysMax = (1u << kHubBigBits) - (1u << kHubBigBits) / 8;
+constexpr int32_t kHubBigSlices = 1024;  // workgroups, each with its global histogram, of the second pass
 constexpr uint32_t kHubEmpty = 0xFFFFFFFFu;
 constexpr int32_t kHubKeyHubs = 23;
 
+// Adds a neighbour with this key; false once more than `most` (at most 7/8 of the slots) keys
+// would be held.
+template <int HB>
 __device__ inline bool hub_key_add(uint32_t* hk, uint32_t* hn, uint32_t* hu, uint32_t key, bool up,
-                                   uint32_t* used) {
-  uint32_t slot = (key * 2654435761u) >> (32 - kHubHashBits);
-  for (int p = 0; p < kHubHash; ++p) {
+                                   uint32_t* used, uint32_t most) {
+  constexpr uint32_t size = 1u << HB;
+  uint32_t slot = (key * 2654435761u) >> (32 - HB);
+  for (uint32_t p = 0; p < size; ++p) {
     const uint32_t prev = atomicCAS(&hk[slot], kHubEmpty, key);
-    if (prev == kHubEmpty && atomicAdd(used, 1u) >= kHubKeysMax) return false;
+    if (prev == kHubEmpty && atomicAdd(used, 1u) >= most) return false;
     if (prev == kHubEmpty || prev == key) {
       atomicAdd(&hn[slot], 1u);
       if (up) atomicAdd(&hu[slot], 1u);
       return true;
     }
-    slot = (slot + 1) & (kHubHash - 1);
+    slot = (slot + 1) & (size - 1);
   }
   return false;
 }
@@ -210,174 +218,202 @@ __device__ inline void light_nodes(int32_t* s_ln, int32_t a, const uint32_t* __r
 
 // Per hyperedge a: its light neighbours x over id tiles of T (the tile's mask table built from
 // the parts of the V2H rows of a's light nodes that fall in it, each entry visited once by the
-// lane of a's lowest light node in x), their keys in the LDS histogram, the light degree dL[a];
+// lane of a's lowest light node in x), their keys in the key histogram, the light degree dL[a];
 // then (C) over all pairs of keys (n_i n_j pairs of neighbours, C(n_i, 2) within one key) into
 // accC, and one pair type per key with neighbours above a, with their number, appended to
 // (tkeys, tcnts) for (B) (the slot count keeps counting past cap so the host can grow them).
-// Centres a0 + blockIdx.x. keys = 0: the light degree alone (no histogram, no (C), no pair types).
-// Dynamic LDS: T masks, then the histogram (keys, counts, counts above a: kHubHash each).
-template <typename MT, int NW>
+// keys = 0: the light degree alone (no histogram, no (C), no pair types).
+// BIG = false: centres a0 + blockIdx.x, histogram in LDS (dynamic LDS: T masks, then keys, counts
+// and counts above a, kHubHash each); a centre with more than `most` keys adds its light degree
+// alone and is appended to big[*nbig]. BIG = true: the *nbig centres in big, spread over the
+// workgroups, each workgroup's histogram of 2^kHubBigBits slots at gh (3 words per slot; dynamic
+// LDS: T masks); a centre with more than `most` keys flags *overflow. Once any centre has flagged it,
+// the later ones stop early (the count is redone with fewer hubs).
+template <typename MT, int NW, bool BIG>
 __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
     int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges, const int8_t* __restrict__ hub,
     const uint32_t* __restrict__ hmask, const uint8_t* __restrict__ lsz, uint32_t* __restrict__ dL,
     unsigned long long* __restrict__ tkeys, unsigned long long* __restrict__ tcnts,
     unsigned long long* __restrict__ ntent, unsigned long long cap, uint32_t* __restrict__ overflow,
-    unsigned long long* __restrict__ accC, int32_t a0, int keys) {
+    unsigned long long* __restrict__ accC, int32_t a0, int keys, int32_t* __restrict__ big,
+    uint32_t* __restrict__ nbig, uint32_t* __restrict__ gh, uint32_t most) {
+  constexpr int HB = BIG ? kHubBigBits : kHubHashBits;
+  constexpr int HS = 1 << HB;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ unsigned long long hs[kHubIds * kHubCopies];
   __shared__ int32_t s_ln[HM_MAX_EDGE_SIZE];
   __shared__ TileRanges tr;
   __shared__ uint32_t s_used, s_g;
   __shared__ uint32_t s_d[NW];
-  __shared__ int s_over;
-  const int32_t a = a0 + (int32_t)blockIdx.x;
-  const int la = lsz[a];
-  if (la == 0) {  // no light node: no light neighbour
-    if (threadIdx.x == 0) dL[a] = 0;
-    return;
-  }
-  // once some centre's histogram has overflowed the pass is redone with fewer hubs: stop early
-  // (one read, broadcast through LDS, so the whole workgroup leaves or stays together)
-  if (threadIdx.x == 0) s_over = keys ? (int)*(volatile uint32_t*)overflow : 0;
-  __syncthreads();
-  if (s_over) return;
-  __syncthreads();  // s_over is reset below
+  __shared__ int s_over, s_full;
   const size_t region = (size_t)T * sizeof(MT);
   MT* M = reinterpret_cast<MT*>(smem);
-  uint32_t* hk = reinterpret_cast<uint32_t*>(smem + region);
-  uint32_t* hn = hk + kHubHash;
-  uint32_t* hu = hn + kHubHash;
-  light_nodes(s_ln, a, eoff, enodes, hub);
-  for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0ull;
-  for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
-  for (int k = threadIdx.x; k < kHubHash; k += blockDim.x) {
-    hk[k] = kHubEmpty;
-    hn[k] = 0u;
-    hu[k] = 0u;
-  }
-  if (threadIdx.x == 0) {
-    s_used = 0u;
-    s_g = 0u;
-    s_over = 0;
-  }
-  const uint32_t Aa = hmask[a];
-  uint32_t d = 0;
-  bool ok = true;
-  for (int32_t t0 = 0; t0 < E; t0 += T) {
-    const int32_t t1 = min(E, t0 + T);
-    const uint32_t total = tile_ranges(tr, la, s_ln, voff, vedges, t0, t1);
-    if (total == 0) continue;
-    for_tile_entries(tr, la, total, vedges, [&](int i, int32_t x) {
-      if (x != a) atomicOr(&M[x - t0], (MT)1 << i);
-    });
+  uint32_t* hk = BIG ? gh + (size_t)blockIdx.x * 3 * HS : reinterpret_cast<uint32_t*>(smem + region);
+  uint32_t* hn = hk + HS;
+  uint32_t* hu = hn + HS;
+  const int32_t nwork = BIG ? (int32_t)*nbig : (int32_t)blockIdx.x + 1;
+  for (int32_t it = (int32_t)blockIdx.x; it < nwork; it += BIG ? (int32_t)gridDim.x : 1) {
+    __syncthreads();  // the previous centre's reads of the shared state are done
+    const int32_t a = BIG ? big[it] : a0 + it;
+    const int la = lsz[a];
+    if (la == 0) {  // no light node: no light neighbour
+      if (threadIdx.x == 0) dL[a] = 0;
+      continue;
+    }
+    // a count that will be redone stops early (one read, broadcast through LDS, so the whole
+    // workgroup leaves or stays together)
+    if (threadIdx.x == 0) s_over = keys ? (int)*(volatile uint32_t*)overflow : 0;
     __syncthreads();
-    for_tile_entries(tr, la, total, vedges, [&](int i, int32_t x) {
-      if (x == a) return;
-      const MT m = M[x - t0];
-      if (m && low_bit(m) == i) {
-        const int C = popc(m);
-        if (keys) {
-          const uint32_t key = hmask[x] << 8 | (uint32_t)C << 1 | (lsz[x] > C ? 1u : 0u);
-          ok = hub_key_add(hk, hn, hu, key, x > a, &s_used) && ok;
-        }
-        ++d;
-        M[x - t0] = (MT)0;
+    if (s_over) continue;
+    __syncthreads();  // s_over is reset below
+    light_nodes(s_ln, a, eoff, enodes, hub);
+    for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0ull;
+    for (int32_t x = threadIdx.x; x < T; x += blockDim.x) M[x] = (MT)0;
+    if (keys) {
+      for (int k = threadIdx.x; k < HS; k += blockDim.x) {
+        hk[k] = kHubEmpty;
+        hn[k] = 0u;
+        hu[k] = 0u;
       }
-    });
-  }
-  if (!ok) s_over = 1;
-  d = wave_sum(d);
-  if (lane_id() == 0) s_d[wave_id()] = d;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t dd = 0;
-    for (int w = 0; w < NW; ++w) dd += s_d[w];
-    dL[a] = dd;
-    if (s_over) atomicOr(overflow, 1u);
-  }
-  __syncthreads();               // s_d is reused by the compaction below
-  if (s_over || !keys) return;  // uniform: read after the barriers above
+    }
+    if (threadIdx.x == 0) {
+      s_used = 0u;
+      s_g = 0u;
+      s_over = 0;
+      s_full = 0;
+    }
+    // A global histogram is cleared with plain stores, filled with atomics (performed in L2) and
+    // read back with plain loads: a workgroup barrier orders none of these against L2 and the CU's
+    // L1, so each phase ends with a device-scope fence (stores reach L2, L1 is invalidated) before
+    // the barrier that starts the next one.
+    if (BIG) __threadfence();
+    const uint32_t Aa = hmask[a];
+    uint32_t d = 0;
+    for (int32_t t0 = 0; t0 < E; t0 += T) {
+      const int32_t t1 = min(E, t0 + T);
+      const uint32_t total = tile_ranges(tr, la, s_ln, voff, vedges, t0, t1);
+      if (total == 0) continue;
+      for_tile_entries(tr, la, total, vedges, [&](int i, int32_t x) {
+        if (x != a) atomicOr(&M[x - t0], (MT)1 << i);
+      });
+      __syncthreads();
+      for_tile_entries(tr, la, total, vedges, [&](int i, int32_t x) {
+        if (x == a) return;
+        const MT m = M[x - t0];
+        if (m && low_bit(m) == i) {
+          // once the histogram is full the centre only counts (it is redone, or the count fails)
+          if (keys && !*(volatile int*)&s_full) {
+            const int C = popc(m);
+            const uint32_t key = hmask[x] << 8 | (uint32_t)C << 1 | (lsz[x] > C ? 1u : 0u);
+            if (!hub_key_add<HB>(hk, hn, hu, key, x > a, &s_used, most)) s_full = 1;
+          }
+          ++d;
+          M[x - t0] = (MT)0;
+        }
+      });
+    }
+    d = wave_sum(d);
+    if (lane_id() == 0) s_d[wave_id()] = d;
+    if (BIG) __threadfence();  // the histogram's atomics are done: compaction reads come from L2
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t dd = 0;
+      for (int w = 0; w < NW; ++w) dd += s_d[w];
+      dL[a] = dd;
+      if (s_full) {
+        if (BIG) atomicOr(overflow, 1u);
+        else big[atomicAdd(nbig, 1u)] = a;
+      }
+    }
+    __syncthreads();                         // s_d is reused by the compaction below
+    if (s_full || !keys) continue;  // uniform: read after the barriers above
 
-  // compact the histogram in place (stable: slot k moves to the number of keys before it, never
-  // past k, and a chunk is read whole before it is written); pair types of the neighbours above a
-  uint32_t* lk = hk;
-  uint32_t* ln = hn;
-  const int lane = lane_id(), w = wave_id();
-  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t placed = 0;
-  for (int k0 = 0; k0 < kHubHash; k0 += blockDim.x) {
-    const int k = k0 + (int)threadIdx.x;
-    const uint32_t key = hk[k];
-    const bool have = key != kHubEmpty;
-    const uint32_t n = have ? hn[k] : 0u, up = have ? hu[k] : 0u;
-    const unsigned long long bal = __ballot(have);
-    if (lane == 0) s_d[w] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t woff = 0, tot = 0;
-    for (int q = 0; q < NW; ++q) {
-      woff += q < w ? s_d[q] : 0;
-      tot += s_d[q];
-    }
-    if (have) {
-      const uint32_t at = placed + woff + (uint32_t)__popcll(bal & below);
-      lk[at] = key;
-      ln[at] = n;
-    }
-    placed += tot;
-    const bool emit = up > 0;
-    const unsigned long long eb = __ballot(emit);
-    if (eb) {
-      const int leader = __ffsll((long long)eb) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(ntent, (unsigned long long)__popcll(eb));
-      base = __shfl(base, leader);
-      if (emit) {
-        const unsigned long long pos = base + (unsigned long long)__popcll(eb & below);
-        if (pos < cap) {
-          const int C = (int)((key >> 1) & 127u);
-          const uint32_t sa = Aa << 1 | (la > C ? 1u : 0u), sx = (key >> 8) << 1 | (key & 1u);
-          tkeys[pos] = type_key(sa, sx);
-          tcnts[pos] = up;
+    // compact the histogram in place (stable: slot k moves to the number of keys before it, never
+    // past k, and a chunk is read whole before it is written); pair types of the neighbours above a
+    uint32_t* lk = hk;
+    uint32_t* ln = hn;
+    const int lane = lane_id(), w = wave_id();
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t placed = 0;
+    for (int k0 = 0; k0 < HS; k0 += blockDim.x) {
+      const int k = k0 + (int)threadIdx.x;
+      const uint32_t key = hk[k];
+      const bool have = key != kHubEmpty;
+      const uint32_t n = have ? hn[k] : 0u, up = have ? hu[k] : 0u;
+      const unsigned long long bal = __ballot(have);
+      if (lane == 0) s_d[w] = (uint32_t)__popcll(bal);
+      __syncthreads();
+      uint32_t woff = 0, tot = 0;
+      for (int q = 0; q < NW; ++q) {
+        woff += q < w ? s_d[q] : 0;
+        tot += s_d[q];
+      }
+      if (have) {
+        const uint32_t at = placed + woff + (uint32_t)__popcll(bal & below);
+        lk[at] = key;
+        ln[at] = n;
+      }
+      placed += tot;
+      const bool emit = up > 0;
+      const unsigned long long eb = __ballot(emit);
+      if (eb) {
+        const int leader = __ffsll((long long)eb) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(ntent, (unsigned long long)__popcll(eb));
+        base = __shfl(base, leader);
+        if (emit) {
+          const unsigned long long pos = base + (unsigned long long)__popcll(eb & below);
+          if (pos < cap) {
+            const int C = (int)((key >> 1) & 127u);
+            const uint32_t sa = Aa << 1 | (la > C ? 1u : 0u), sx = (key >> 8) << 1 | (key & 1u);
+            tkeys[pos] = type_key(sa, sx);
+            tcnts[pos] = up;
+          }
         }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) s_g = placed;
+    if (BIG) __threadfence();  // the compacted keys are in L2 before the other waves read them
+    __syncthreads();
+    const uint32_t G = s_g;
+    unsigned long long* hc = hs + (lane & (kHubCopies - 1));
+    for (uint32_t i = w; i < G; i += NW) {
+      const uint32_t bi = lk[i];
+      const unsigned long long ni = ln[i];
+      const uint32_t Ab = bi >> 8;
+      const int Cb = (int)((bi >> 1) & 127u);
+      const unsigned fb = bi & 1u;
+      const uint32_t Aab = Aa & Ab, Aa_b = Aa & ~Ab, Ab_a = Ab & ~Aa;
+      for (uint32_t j = i + lane; j < G; j += kWave) {
+        const uint32_t cj = lk[j];
+        const unsigned long long wt = (j == i) ? binom2(ni) : ni * (unsigned long long)ln[j];
+        if (!wt) continue;
+        const uint32_t Ac = cj >> 8;
+        const int Cc = (int)((cj >> 1) & 127u);
+        const unsigned fc = cj & 1u;
+        const unsigned h = nzm(Aa_b & ~Ac) | nzm(Ab_a & ~Ac) << 1 | nzm(Ac & ~(Aa | Ab)) << 2 |
+                           nzm(Aab & ~Ac) << 3 | nzm(Ab_a & Ac) << 4 | nzm(Aa_b & Ac) << 5 |
+                           nzm(Aab & Ac) << 6;
+        const unsigned ta = h | 8u | 32u | (la > Cb + Cc ? 1u : 0u) | fb << 1 | fc << 2;
+        const unsigned p1ab = h | 8u | (la > Cb ? 1u : 0u) | fb << 1 | 4u;
+        const unsigned p1ac = h | 32u | (la > Cc ? 1u : 0u) | fc << 2 | 2u;
+        const unsigned p0 = h | 7u;
+        atomicAdd(&hc[ta * kHubCopies], wt);
+        atomicAdd(&hc[p0 * kHubCopies], wt);
+        atomicAdd(&hc[p1ab * kHubCopies], 0ull - wt);
+        atomicAdd(&hc[p1ac * kHubCopies], 0ull - wt);
       }
     }
     __syncthreads();
+    fold_ids(hs, 1ull, accC);
   }
-  if (threadIdx.x == 0) s_g = placed;
-  __syncthreads();
-  const uint32_t G = s_g;
-  unsigned long long* hc = hs + (lane & (kHubCopies - 1));
-  for (uint32_t i = w; i < G; i += NW) {
-    const uint32_t bi = lk[i];
-    const unsigned long long ni = ln[i];
-    const uint32_t Ab = bi >> 8;
-    const int Cb = (int)((bi >> 1) & 127u);
-    const unsigned fb = bi & 1u;
-    const uint32_t Aab = Aa & Ab, Aa_b = Aa & ~Ab, Ab_a = Ab & ~Aa;
-    for (uint32_t j = i + lane; j < G; j += kWave) {
-      const uint32_t cj = lk[j];
-      const unsigned long long w = (j == i) ? binom2(ni) : ni * (unsigned long long)ln[j];
-      if (!w) continue;
-      const uint32_t Ac = cj >> 8;
-      const int Cc = (int)((cj >> 1) & 127u);
-      const unsigned fc = cj & 1u;
-      const unsigned h = nzm(Aa_b & ~Ac) | nzm(Ab_a & ~Ac) << 1 | nzm(Ac & ~(Aa | Ab)) << 2 |
-                         nzm(Aab & ~Ac) << 3 | nzm(Ab_a & Ac) << 4 | nzm(Aa_b & Ac) << 5 |
-                         nzm(Aab & Ac) << 6;
-      const unsigned ta = h | 8u | 32u | (la > Cb + Cc ? 1u : 0u) | fb << 1 | fc << 2;
-      const unsigned p1ab = h | 8u | (la > Cb ? 1u : 0u) | fb << 1 | 4u;
-      const unsigned p1ac = h | 32u | (la > Cc ? 1u : 0u) | fc << 2 | 2u;
-      const unsigned p0 = h | 7u;
-      atomicAdd(&hc[ta * kHubCopies], w);
-      atomicAdd(&hc[p0 * kHubCopies], w);
-      atomicAdd(&hc[p1ab * kHubCopies], 0ull - w);
-      atomicAdd(&hc[p1ac * kHubCopies], 0ull - w);
-    }
-  }
-  __syncthreads();
-  fold_ids(hs, 1ull, accC);
 }
+
+// Workgroups of one launch over a data-sized list (types, classes): a launch's work-item count
+// (grid x block) is a 32-bit quantity.
+constexpr unsigned long long kHubLaunch = 1ull << 20;
 
 // (B): one workgroup per distinct pair type t = (Aa, fa, Ab, fb) with nt light pairs; every class
 // (Ac, lc) with nc hyperedges contributes nc (F(P1ab) - F(P0)); the pair's own two hyperedges
@@ -388,11 +424,12 @@ __global__ __launch_bounds__(kBlock) void k_hub_types(int32_t K, int32_t k0,
                                                       const uint32_t* __restrict__ ucnt,
                                                       const unsigned long long* __restrict__ tkey,
                                                       const unsigned long long* __restrict__ tcnt,
-                                                      unsigned long long* __restrict__ acc) {
+                                                      unsigned long long* __restrict__ acc, unsigned long long t0) {
   __shared__ long long hs[kHubIds * kHubCopies];
   for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0;
   __syncthreads();
-  const unsigned long long t = tkey[blockIdx.x];
+  const unsigned long long ti = t0 + blockIdx.x;  // type t0 + blockIdx.x (launches of kHubLaunch types)
+  const unsigned long long t = tkey[ti];
   const uint32_t s1 = (uint32_t)(t >> 32), s2 = (uint32_t)t;
   const uint32_t Aa = s1 >> 1, Ab = s2 >> 1;
   const unsigned fa = s1 & 1u, fb = s2 & 1u;
@@ -417,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_types(int32_t K, int32_t k0,
     add_class(Ab, 1u, -1);
   }
   __syncthreads();
-  fold_ids(hs, (unsigned long long)tcnt[blockIdx.x], acc);
+  fold_ids(hs, (unsigned long long)tcnt[ti], acc);
 }
 
 // Sort keys of the light-projection order: ascending light degree, with the hyperedges flagged in
@@ -1009,6 +1046,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
   hipStream_t st = g->stream;
   *key_overflow = false;
   const bool verbose = std::getenv("HM_VERBOSE") != nullptr;
+  const auto t_start = std::chrono::steady_clock::now();
   const int lim = hub_lds_limit(g->device);
   // tiles: automatic, or forced by hm_graph_set_tile (tests and tuning; results do not change)
   const int32_t Tc = g->tile > 0 ? std::min<int32_t>(E, std::min<int32_t>(g->tile, hub_centre_tile<MT>(E)))
@@ -1023,7 +1061,8 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
                                          : (int32_t)(((size_t)lim - kHubTriStatic - 1024) / kTriKey) & ~255;
   if (!dense && (size_t)Tt * kTriKey + kHubTriStatic > (size_t)lim)
     return fail(HM_ERR_BAD_INPUT, "hub-aware count: the forced tile exceeds the LDS limit");
-  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves>, lds_centre));
+  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves, false>, lds_centre));
+  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves, true>, (size_t)Tc * sizeof(MT)));
   if (dense) {
     HM_TRY(allow_lds((const void*)k_hub_fill<MT>, (size_t)E * sizeof(MT)));
     HM_TRY(allow_lds((const void*)k_hub_tri<MT, kHubWaves>, (size_t)E * kTriKey));
@@ -1152,13 +1191,15 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
   unsigned long long nt_all = 0, nT = 0;
   if (!run.abc) {
     k_hub_zero<<<1, 64, 0, st>>>(runs + 1, 3);
-    k_hub_centre<MT, kHubWaves><<<E, kHubWaves * kWave, lds_centre, st>>>(
+    k_hub_centre<MT, kHubWaves, false><<<E, kHubWaves * kWave, lds_centre, st>>>(
         E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
         hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(), nullptr,
-        nullptr, runs + 1, 0ull, reinterpret_cast<uint32_t*>(runs + 3), accC, 0, 0);
+        nullptr, runs + 1, 0ull, reinterpret_cast<uint32_t*>(runs + 3), accC, 0, 0, nullptr, nullptr, nullptr, 0u);
     HM_HIP(hipGetLastError());
   } else {
     HM_TRY(hb.accr.ensure(sizeof(unsigned long long) * kAccCount));
+    HM_TRY(hb.big.ensure(sizeof(int32_t) * (size_t)E));
+    const uint32_t lds_keys = (uint32_t)std::max<unsigned long long>(1, hub_limit("HM_HUB_LDS_KEYS", kHubKeysMax));
     unsigned long long* accR = hb.accr.as<unsigned long long>();
     const unsigned long long type_cap = std::max<unsigned long long>(1, hub_limit("HM_HUB_TYPE_CAP", kHubTypeCap));
     unsigned long long cap = std::max<unsigned long long>(
@@ -1173,19 +1214,38 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
                                          type_cap);
       k_hub_zero<<<1, 64, 0, st>>>(accR, kAccCount);
       k_hub_zero<<<1, 64, 0, st>>>(runs + 1, 3);
-      k_hub_centre<MT, kHubWaves><<<(unsigned)n, kHubWaves * kWave, lds_centre, st>>>(
+      k_hub_zero<<<1, 64, 0, st>>>(runs + 5, 1);
+      k_hub_centre<MT, kHubWaves, false><<<(unsigned)n, kHubWaves * kWave, lds_centre, st>>>(
           E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
           hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
           hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), runs + 1, cap,
-          reinterpret_cast<uint32_t*>(runs + 3), accR, a0, 1);
+          reinterpret_cast<uint32_t*>(runs + 3), accR, a0, 1, hb.big.as<int32_t>(),
+          reinterpret_cast<uint32_t*>(runs + 5), nullptr, lds_keys);
       HM_HIP(hipGetLastError());
-      unsigned long long r2[3] = {0, 0, 0};
+      unsigned long long r2[5] = {0, 0, 0, 0, 0};
       HM_HIP(hipMemcpyAsync(r2, runs + 1, sizeof(r2), hipMemcpyDeviceToHost, st));
       HM_HIP(hipStreamSynchronize(st));
+      const uint32_t nbig = (uint32_t)r2[4];
+      if (nbig > 0) {  // the centres the LDS histogram could not hold, with histograms in global memory
+        const int32_t slices = std::min<int32_t>((int32_t)nbig, kHubBigSlices);
+        HM_TRY(hb.bighash.ensure(sizeof(uint32_t) * 3 * ((size_t)1 << kHubBigBits) * (size_t)slices));
+        k_hub_centre<MT, kHubWaves, true><<<(unsigned)slices, kHubWaves * kWave, (size_t)Tc * sizeof(MT), st>>>(
+            E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+            hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
+            hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), runs + 1, cap,
+            reinterpret_cast<uint32_t*>(runs + 3), accR, a0, 1, hb.big.as<int32_t>(),
+            reinterpret_cast<uint32_t*>(runs + 5), hb.bighash.as<uint32_t>(), kHubBigKeysMax);
+        HM_HIP(hipGetLastError());
+        HM_HIP(hipMemcpyAsync(r2, runs + 1, sizeof(r2), hipMemcpyDeviceToHost, st));
+        HM_HIP(hipStreamSynchronize(st));
+        if (verbose)
+          std::fprintf(stderr, "[hm hub] centres %d..%d: %u with more keys than LDS holds, in global memory\n",
+                       a0, a0 + n, nbig);
+      }
       if (r2[2]) {
         *key_overflow = true;
         set_error("hub-aware count: a hyperedge's light neighbours hold more than %u distinct (hub pattern, "
-                  "overlap) keys with %d hubs; use fewer hubs", kHubKeysMax, hh);
+                  "overlap) keys with %d hubs; use fewer hubs", kHubBigKeysMax, hh);
         return HM_ERR_UNSUPPORTED;
       }
       const unsigned long long nt = r2[0];
@@ -1202,6 +1262,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
         continue;
       }
       k_hub_add<<<1, 64, 0, st>>>(accC, accR, kAccCount);
+      g->hub_stats.big_centres += nbig;
       nt_all += nt;
       if (nt > 0) {  // sort and reduce the range's entries, append them, merge with the earlier ranges
         HM_TRY(hb.types2.ensure(sizeof(unsigned long long) * std::max(nt, nT)));
@@ -1243,20 +1304,26 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
   g->hub_stats.light_pairs = -1;
   HM_HIP(hipEventRecord(g->ev[2], st));
   // (A) once the histograms are known to hold (class triples are the costly part at many hubs)
-  if (run.abc && K > k0) {
-    k_hub_classes<<<(unsigned)(K - k0), kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(),
-                                                         hb.ucnt.as<uint32_t>(), acc);
-    HM_HIP(hipGetLastError());
+  if (run.abc) {
+    for (int32_t i0 = k0; i0 < K; i0 += (int32_t)kHubLaunch) {  // class i0 + blockIdx.x
+      const int32_t n = (int32_t)std::min<int64_t>(kHubLaunch, K - i0);
+      k_hub_classes<<<(unsigned)n, kBlock, 0, st>>>(K, i0, hb.ukey.as<unsigned long long>(),
+                                                    hb.ucnt.as<uint32_t>(), acc);
+      HM_HIP(hipGetLastError());
+    }
   }
   if (verbose) std::fprintf(stderr, "[hm hub] classes and light neighbours done, %llu pair-type entries\n", nt_all);
 
   // (B) every distinct pair type against every class
   g->hub_stats.pair_types = (int64_t)nT;
-  if (run.abc && nT > 0 && K > k0) {
-    k_hub_types<<<(unsigned)nT, kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(), hb.ucnt.as<uint32_t>(),
-                                                 hb.tkey.as<unsigned long long>(), hb.tcnt.as<unsigned long long>(),
-                                                 acc);
-    HM_HIP(hipGetLastError());
+  if (run.abc && K > k0) {
+    for (unsigned long long t0 = 0; t0 < nT; t0 += kHubLaunch) {
+      const unsigned long long n = std::min(kHubLaunch, nT - t0);
+      k_hub_types<<<(unsigned)n, kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(), hb.ucnt.as<uint32_t>(),
+                                                  hb.tkey.as<unsigned long long>(), hb.tcnt.as<unsigned long long>(),
+                                                  acc, t0);
+      HM_HIP(hipGetLastError());
+    }
   }
   HM_HIP(hipEventRecord(g->ev[3], st));
 
@@ -1311,6 +1378,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
           accD, d0);
       HM_HIP(hipGetLastError());
     }
+    if (run.d_done) *run.d_done = d1;
   } else {
     // V2H relabelled to light ranks, ascending within every node's row
     HM_TRY(s.vkeys.ensure(sizeof(unsigned long long) * std::max<int64_t>(I, 1)));
@@ -1374,8 +1442,10 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
                                             s.uoff.as<unsigned long long>(), E + 1, st));
     // centres in launches whose (a, b) state stays within kHubStateCap entries (a lone centre may
     // exceed it); with HM_VERBOSE set, also at most a sixteenth of the centres per launch, each
-    // reported as it ends
-    const int32_t most = verbose ? std::max<int32_t>(4096, nD / 16) : std::max<int32_t>(nD, 1);
+    // reported as it ends; with a time limit, at most 1/256 of them, the limit checked after each
+    const int32_t most = run.seconds > 0 ? std::max<int32_t>(1024, nD / 256)
+                         : verbose       ? std::max<int32_t>(4096, nD / 16)
+                                         : std::max<int32_t>(nD, 1);
     const unsigned long long state_cap = hub_limit("HM_HUB_STATE_CAP", kHubStateCap);
     std::vector<int32_t> cut{d0};
     unsigned long long smax = 0;
@@ -1389,6 +1459,7 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
     HM_TRY(s.cur.ensure(sizeof(uint32_t) * (size_t)std::max<unsigned long long>(smax, 1)));
     HM_TRY(s.pm.ensure(sizeof(MT) * (size_t)std::max<unsigned long long>(smax, 1)));
     HM_HIP(hipEventRecord(g->ev[5], st));
+    if (run.d_done) *run.d_done = d1;
     for (size_t q = 0; q + 1 < cut.size(); ++q) {
       const int32_t r0 = cut[q], n = cut[q + 1] - cut[q];
       k_hub_tri_tiled<MT, kHubWaves><<<(unsigned)n, kHubWaves * kWave, (size_t)Tt * kTriKey, st>>>(
@@ -1397,9 +1468,12 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
           s.roff.as<unsigned long long>(), s.rlen.as<uint32_t>(), s.rows.as<uint32_t>(),
           s.uoff.as<unsigned long long>(), s.cur.as<uint32_t>(), s.pm.as<MT>(), accD, r0, pre[(size_t)r0]);
       HM_HIP(hipGetLastError());
-      if (verbose) {
-        HM_HIP(hipStreamSynchronize(st));
-        std::fprintf(stderr, "[hm hub] light triangles: centres %d..%d of %d done\n", r0, r0 + n, nD);
+      if (verbose || run.seconds > 0) HM_HIP(hipStreamSynchronize(st));
+      if (verbose) std::fprintf(stderr, "[hm hub] light triangles: centres %d..%d of %d done\n", r0, r0 + n, nD);
+      if (run.seconds > 0 && q + 2 < cut.size() &&
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() >= run.seconds) {
+        if (run.d_done) *run.d_done = cut[q + 1];
+        break;
       }
     }
   }

@@ -1076,7 +1076,7 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
   if (E >= 3) {
     if (g->csr.max_size > HM_MAX_EDGE_SIZE)
       return fail(HM_ERR_UNSUPPORTED, "hyperedge larger than 64 nodes");
-    if ((int64_t)E >= kMaxPackedId) return fail(HM_ERR_UNSUPPORTED, "more than 2^25 hyperedges");
+    if ((int64_t)E >= kMaxIds) return fail(HM_ERR_UNSUPPORTED, "more than 2^22 - 1 hyperedges");
     HM_HIP(hipSetDevice(g->device));
     g->hub_stats.hubs = 0;
     if (g->hubs != 0) {
@@ -1233,21 +1233,23 @@ hm_status hm_graph_get_hub_stats(hm_graph* g, hm_hub_stats* out) {
 }
 
 hm_status hm_graph_count_hub_part(hm_graph* g, int32_t hubs, int64_t centre_begin, int64_t centre_end,
-                                  int32_t other_parts, uint64_t out[HM_STATIC_BINS], int64_t* light_triangles) {
+                                  int32_t other_parts, double seconds, uint64_t out[HM_STATIC_BINS],
+                                  int64_t* light_triangles, int64_t* centre_done) {
   hm::clear_error();
   if (!g || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_count_hub_part: null argument");
   if (hubs < 1 || hubs > hm::kHubMax || (other_parts != 0 && other_parts != 1) || centre_begin < 0 ||
-      (centre_end >= 0 && centre_end < centre_begin))
+      (centre_end >= 0 && centre_end < centre_begin) || !(seconds >= 0))
     return hm::fail(HM_ERR_BAD_INPUT,
-                    "hm_graph_count_hub_part: need hubs in 1..23, other_parts 0 or 1 and 0 <= centre_begin <= "
-                    "centre_end (or centre_end = -1)");
+                    "hm_graph_count_hub_part: need hubs in 1..23, other_parts 0 or 1, 0 <= centre_begin <= "
+                    "centre_end (or centre_end = -1) and seconds >= 0");
   for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = 0;
   if (light_triangles) *light_triangles = 0;
   const int32_t E = g->csr.E;
+  if (centre_done) *centre_done = centre_end < 0 ? E : std::min<int64_t>(centre_end, E);
   g->hub_stats = hm_hub_stats{};
   if (E < 3) return HM_OK;
   if (g->csr.max_size > HM_MAX_EDGE_SIZE) return hm::fail(HM_ERR_UNSUPPORTED, "hyperedge larger than 64 nodes");
-  if ((int64_t)E >= hm::kMaxPackedId) return hm::fail(HM_ERR_UNSUPPORTED, "more than 2^25 hyperedges");
+  if ((int64_t)E >= hm::kMaxIds) return hm::fail(HM_ERR_UNSUPPORTED, "more than 2^22 - 1 hyperedges");
   HM_HIP(hipSetDevice(g->device));
   hm::HubRun run;
   run.hubs_out = &g->hub_nodes;
@@ -1255,11 +1257,15 @@ hm_status hm_graph_count_hub_part(hm_graph* g, int32_t hubs, int64_t centre_begi
   run.stats = other_parts != 0;
   run.d_begin = centre_begin;
   run.d_end = centre_end;
+  run.seconds = seconds;
+  int64_t done = 0;
+  run.d_done = &done;
   unsigned long long abc[hm::kAccCount] = {0}, d[hm::kAccCount] = {0};
   bool overflow = false;
   HM_TRY(hm::hub_parts(g, hubs, run, abc, d, &overflow));
   for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = abc[hm::kAccHist + k] + d[hm::kAccHist + k];
   if (light_triangles) *light_triangles = (int64_t)d[hm::kAccLightTri];
+  if (centre_done) *centre_done = done;
   if (other_parts) {  // the projection statistics; the closed-triple total is not known from one part
     hm::fill_count_stats(g, out, abc);
     g->stats.triangles = -1;

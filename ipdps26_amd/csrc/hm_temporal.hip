@@ -1148,8 +1148,8 @@ hm_status hm_tgraph_create(const int64_t* offsets, const int32_t* nodes, const i
   if (!out) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_create: null out");
   *out = nullptr;
   if (num_edges > 0 && !times) return hm::fail(HM_ERR_BAD_INPUT, "hm_tgraph_create: null times");
-  if ((int64_t)num_edges >= hm::kMaxPackedId)
-    return hm::fail(HM_ERR_UNSUPPORTED, "hm_tgraph_create: more than 2^25 temporal hyperedges");
+  if ((int64_t)num_edges >= hm::kMaxIds)
+    return hm::fail(HM_ERR_UNSUPPORTED, "hm_tgraph_create: more than 2^22 - 1 temporal hyperedges");
   std::vector<int64_t> off;
   std::vector<int32_t> flat;
   int32_t vmax = -1;

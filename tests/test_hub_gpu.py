@@ -203,12 +203,56 @@ def test_split_paths_match_oracle(gpu_ready, monkeypatch, seed):
                 assert np.array_equal((bins + rest).astype(np.int64), want), (tile, h)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_global_memory_histograms_match_oracle(gpu_ready, monkeypatch, seed):
+    """Centres whose neighbour keys outgrow the LDS histogram are counted again with a histogram in
+    global memory: with the LDS table cut to 3 keys (HM_HUB_LDS_KEYS) nearly every centre takes that
+    path, in one table and in tiles, and the bins still equal the oracle's."""
+    rng = np.random.default_rng(9500 + seed)
+    kmax = [6, 40][seed % 2]
+    off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(150, 500)),
+                                   num_nodes=int(rng.integers(kmax + 10, 120)), kmax=kmax, dup_frac=0.2)
+    want = O.static_mochy(off, nodes)
+    monkeypatch.setenv("HM_HUB_LDS_KEYS", "3")
+    for tile in (0, 64):
+        for h in (1, 4, 12):
+            print(f"seed={seed} E={off.size - 1} tile={tile} hubs={h}", flush=True)
+            with hm.Graph(off, nodes) as g:
+                g.set_tile(tile)
+                g.set_hubs(h)
+                assert np.array_equal(g.count_static().astype(np.int64), want), (tile, h)
+                assert g.hub_stats()["big_centres"] > 0
+
+
+def test_time_bounded_parts_resume_where_they_stopped(gpu_ready):
+    """A part whose time has run out stops after a launch and reports the rank it reached; parts
+    resumed from there, until every rank is done, sum to the count (tiled path, 1,024-centre
+    launches)."""
+    off, nodes = hm.generate_static(W.scaled(W.STATIC_C2, 4_000, 2_000, seed=23))
+    E = off.size - 1
+    with hm.Graph(off, nodes) as g:
+        g.set_tile(512)
+        g.set_hubs(6)
+        want = g.count_static().astype(np.int64)
+        total, _, done = g.count_hub_part(6, 0, -1, other_parts=True, seconds=1e-6, with_done=True)
+        parts = 1
+        assert 0 < done < E
+        while done < E:
+            bins, _, nxt = g.count_hub_part(6, done, -1, other_parts=False, seconds=1e-6, with_done=True)
+            assert nxt > done
+            total, done, parts = total + bins, nxt, parts + 1
+        print(f"E={E}: {parts} parts", flush=True)
+        assert parts >= 3 and np.array_equal(total.astype(np.int64), want)
+
+
 def test_part_count_rejects_bad_arguments(gpu_ready):
     off, nodes = csr([[0, 1], [1, 2], [2, 0], [0, 3]])
     with hm.Graph(off, nodes) as g:
         for args in ((0, 0, -1), (24, 0, -1), (3, -1, -1), (3, 3, 2)):
             with pytest.raises(hm.HMotifError):
                 g.count_hub_part(*args)
+        with pytest.raises(hm.HMotifError):
+            g.count_hub_part(3, 0, -1, seconds=-1.0)
 
 
 def test_beyond_one_table_equals_triangle_path(gpu_ready):
