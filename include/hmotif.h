@@ -194,7 +194,7 @@ typedef struct hm_hub_stats {
   double ms_triangles;    /* light projection rows and light triangles */
   double ms_tri_kernel;   /* the light-triangle kernel alone (k_hub_tri or k_hub_tri_tiled) */
   int64_t big_centres;    /* hyperedges whose neighbour keys outgrew the LDS histogram (7,168 keys),
-                             counted with a global-memory one (57,344 keys) */
+                             counted with global-memory ones (57,344, then 229,376, then 917,504 keys) */
 } hm_hub_stats;
 hm_status hm_graph_get_hub_stats(hm_graph* g, hm_hub_stats* out);
 

@@ -161,12 +161,13 @@ struct HubScratch {
   DevBuf firstids; // int32[n]    their ids
   DevBuf accd;     // uint64[40]  (D)'s bins
   DevBuf accr;     // uint64[40]  (C) of one range of centres
-  DevBuf big;      // int32[E]    centres whose keys outgrew the LDS histogram
-  DevBuf bighash;  // uint32[]    their histograms in global memory (3 * 2^16 words per workgroup)
+  DevBuf big;      // int32[E]    centres whose keys outgrew the LDS histogram (and the next levels')
+  DevBuf big2;     // int32[E]    the other list of the level-by-level passes
+  DevBuf bighash;  // uint32[]    their histograms in global memory (3 * 2^HB words per workgroup)
   void release() {
     for (DevBuf* b : {&hubidx, &hmask, &lsz, &ckey, &ckey2, &ukey, &ucnt, &nruns, &hmr, &lszr, &types,
                       &tcraw, &types2, &tcsrt, &tkey, &tcnt, &rkey, &first, &firstids, &accd, &accr, &big,
-                      &bighash})
+                      &big2, &bighash})
       b->release();
   }
   // the pair-type buffers, once (B) has run (they are the largest of a big graph's count)

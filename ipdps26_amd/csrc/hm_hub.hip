@@ -41,6 +41,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <type_traits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -172,9 +173,12 @@ __device__ inline unsigned long long type_key(uint32_t sa, uint32_t sx) {
 constexpr int kHubHashBits = 13;
 constexpr int kHubHash = 1 << kHubHashBits;
 constexpr uint32_t kHubKeysMax = kHubHash - kHubHash / 8;  // lowered by HM_HUB_LDS_KEYS (tests)
-constexpr int kHubBigBits = 16;
-constexpr uint32_t kHubBigKeysMax = (1u << kHubBigBits) - (1u << kHubBigBits) / 8;
-constexpr int32_t kHubBigSlices = 1024;  // workgroups, each with its global histogram, of the second pass
+// global histograms: 2^16, 2^18 and 2^20 slots, each level for the centres the one before could not
+// hold, over 1024, 256 and 64 workgroups (768 MB of histograms per level)
+constexpr int kHubBigLevels = 3;
+constexpr int kHubBigBits[kHubBigLevels] = {16, 18, 20};
+constexpr int32_t kHubBigSlices[kHubBigLevels] = {1024, 256, 64};
+constexpr uint32_t kHubBigKeysMax = (1u << 20) - (1u << 20) / 8;
 constexpr uint32_t kHubEmpty = 0xFFFFFFFFu;
 constexpr int32_t kHubKeyHubs = 23;
 
@@ -223,22 +227,23 @@ __device__ inline void light_nodes(int32_t* s_ln, int32_t a, const uint32_t* __r
 // accC, and one pair type per key with neighbours above a, with their number, appended to
 // (tkeys, tcnts) for (B) (the slot count keeps counting past cap so the host can grow them).
 // keys = 0: the light degree alone (no histogram, no (C), no pair types).
-// BIG = false: centres a0 + blockIdx.x, histogram in LDS (dynamic LDS: T masks, then keys, counts
-// and counts above a, kHubHash each); a centre with more than `most` keys adds its light degree
-// alone and is appended to big[*nbig]. BIG = true: the *nbig centres in big, spread over the
-// workgroups, each workgroup's histogram of 2^kHubBigBits slots at gh (3 words per slot; dynamic
-// LDS: T masks); a centre with more than `most` keys flags *overflow. Once any centre has flagged it,
-// the later ones stop early (the count is redone with fewer hubs).
-template <typename MT, int NW, bool BIG>
+// HB = kHubHashBits: centres a0 + blockIdx.x, histogram in LDS (dynamic LDS: T masks, then keys,
+// counts and counts above a, kHubHash each). HB > kHubHashBits: the *nin centres in in_ids, spread
+// over the workgroups, each workgroup's histogram of 2^HB slots at gh (3 words per slot; dynamic
+// LDS: T masks). A centre with more than `most` keys adds its light degree alone and is appended to
+// out_ids[*nout], or, without out_ids, flags *overflow; once that is flagged, the later centres
+// stop early (the count is redone with fewer hubs).
+template <typename MT, int NW, int HB>
 __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
     int32_t E, int32_t T, const uint32_t* __restrict__ eoff, const int32_t* __restrict__ enodes,
     const uint32_t* __restrict__ voff, const int32_t* __restrict__ vedges, const int8_t* __restrict__ hub,
     const uint32_t* __restrict__ hmask, const uint8_t* __restrict__ lsz, uint32_t* __restrict__ dL,
     unsigned long long* __restrict__ tkeys, unsigned long long* __restrict__ tcnts,
     unsigned long long* __restrict__ ntent, unsigned long long cap, uint32_t* __restrict__ overflow,
-    unsigned long long* __restrict__ accC, int32_t a0, int keys, int32_t* __restrict__ big,
-    uint32_t* __restrict__ nbig, uint32_t* __restrict__ gh, uint32_t most) {
-  constexpr int HB = BIG ? kHubBigBits : kHubHashBits;
+    unsigned long long* __restrict__ accC, int32_t a0, int keys, const int32_t* __restrict__ in_ids,
+    const uint32_t* __restrict__ nin, int32_t* __restrict__ out_ids, uint32_t* __restrict__ nout,
+    uint32_t* __restrict__ gh, uint32_t most) {
+  constexpr bool BIG = HB > kHubHashBits;
   constexpr int HS = 1 << HB;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ unsigned long long hs[kHubIds * kHubCopies];
@@ -252,10 +257,10 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
   uint32_t* hk = BIG ? gh + (size_t)blockIdx.x * 3 * HS : reinterpret_cast<uint32_t*>(smem + region);
   uint32_t* hn = hk + HS;
   uint32_t* hu = hn + HS;
-  const int32_t nwork = BIG ? (int32_t)*nbig : (int32_t)blockIdx.x + 1;
+  const int32_t nwork = BIG ? (int32_t)*nin : (int32_t)blockIdx.x + 1;
   for (int32_t it = (int32_t)blockIdx.x; it < nwork; it += BIG ? (int32_t)gridDim.x : 1) {
     __syncthreads();  // the previous centre's reads of the shared state are done
-    const int32_t a = BIG ? big[it] : a0 + it;
+    const int32_t a = BIG ? in_ids[it] : a0 + it;
     const int la = lsz[a];
     if (la == 0) {  // no light node: no light neighbour
       if (threadIdx.x == 0) dL[a] = 0;
@@ -322,8 +327,8 @@ __global__ __launch_bounds__(NW * kWave) void k_hub_centre(
       for (int w = 0; w < NW; ++w) dd += s_d[w];
       dL[a] = dd;
       if (s_full) {
-        if (BIG) atomicOr(overflow, 1u);
-        else big[atomicAdd(nbig, 1u)] = a;
+        if (out_ids) out_ids[atomicAdd(nout, 1u)] = a;
+        else atomicOr(overflow, 1u);
       }
     }
     __syncthreads();                         // s_d is reused by the compaction below
@@ -973,6 +978,7 @@ hm_status hub_choose(hm_graph* g, int32_t* h_out, double* cost_out) {
 // rows beyond which the tiled path sizes them by the upper light degrees (one more fill pass)
 // instead of the light degrees.
 constexpr unsigned long long kHubTypeCap = 1ull << 28;
+constexpr int32_t kHubCentreSpan = 1 << 19;  // centres of one range at most (each range reports its end)
 constexpr unsigned long long kHubRowsByDegree = 1ull << 32;
 // Per-launch (a, b) state of the tiled light-triangle kernel: centres are launched in ranges whose
 // upper rows hold at most this many entries in all (one centre's row may exceed it alone).
@@ -1061,8 +1067,10 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
                                          : (int32_t)(((size_t)lim - kHubTriStatic - 1024) / kTriKey) & ~255;
   if (!dense && (size_t)Tt * kTriKey + kHubTriStatic > (size_t)lim)
     return fail(HM_ERR_BAD_INPUT, "hub-aware count: the forced tile exceeds the LDS limit");
-  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves, false>, lds_centre));
-  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves, true>, (size_t)Tc * sizeof(MT)));
+  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves, kHubHashBits>, lds_centre));
+  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves, kHubBigBits[0]>, (size_t)Tc * sizeof(MT)));
+  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves, kHubBigBits[1]>, (size_t)Tc * sizeof(MT)));
+  HM_TRY(allow_lds((const void*)k_hub_centre<MT, kHubWaves, kHubBigBits[2]>, (size_t)Tc * sizeof(MT)));
   if (dense) {
     HM_TRY(allow_lds((const void*)k_hub_fill<MT>, (size_t)E * sizeof(MT)));
     HM_TRY(allow_lds((const void*)k_hub_tri<MT, kHubWaves>, (size_t)E * kTriKey));
@@ -1191,21 +1199,25 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
   unsigned long long nt_all = 0, nT = 0;
   if (!run.abc) {
     k_hub_zero<<<1, 64, 0, st>>>(runs + 1, 3);
-    k_hub_centre<MT, kHubWaves, false><<<E, kHubWaves * kWave, lds_centre, st>>>(
+    k_hub_centre<MT, kHubWaves, kHubHashBits><<<E, kHubWaves * kWave, lds_centre, st>>>(
         E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
         hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(), nullptr,
-        nullptr, runs + 1, 0ull, reinterpret_cast<uint32_t*>(runs + 3), accC, 0, 0, nullptr, nullptr, nullptr, 0u);
+        nullptr, runs + 1, 0ull, reinterpret_cast<uint32_t*>(runs + 3), accC, 0, 0, nullptr, nullptr, nullptr,
+        nullptr, nullptr, 0u);
     HM_HIP(hipGetLastError());
   } else {
     HM_TRY(hb.accr.ensure(sizeof(unsigned long long) * kAccCount));
     HM_TRY(hb.big.ensure(sizeof(int32_t) * (size_t)E));
+    HM_TRY(hb.big2.ensure(sizeof(int32_t) * (size_t)E));
     const uint32_t lds_keys = (uint32_t)std::max<unsigned long long>(1, hub_limit("HM_HUB_LDS_KEYS", kHubKeysMax));
+    // keys of the global histograms before the last level (lowered by HM_HUB_BIG_KEYS in tests)
+    const uint32_t big_keys = (uint32_t)std::max<unsigned long long>(1, hub_limit("HM_HUB_BIG_KEYS", kHubBigKeysMax));
     unsigned long long* accR = hb.accr.as<unsigned long long>();
     const unsigned long long type_cap = std::max<unsigned long long>(1, hub_limit("HM_HUB_TYPE_CAP", kHubTypeCap));
     unsigned long long cap = std::max<unsigned long long>(
         1, std::min<unsigned long long>({lp_bound, std::max<unsigned long long>((unsigned long long)E * 1024, 1u << 20),
                                          type_cap}));
-    int32_t a0 = 0, span = E;
+    int32_t a0 = 0, span = std::min<int32_t>(E, kHubCentreSpan);
     while (a0 < E) {
       const int32_t n = std::min(span, E - a0);
       HM_TRY(hb.types.ensure(sizeof(unsigned long long) * cap));
@@ -1214,33 +1226,60 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
                                          type_cap);
       k_hub_zero<<<1, 64, 0, st>>>(accR, kAccCount);
       k_hub_zero<<<1, 64, 0, st>>>(runs + 1, 3);
-      k_hub_zero<<<1, 64, 0, st>>>(runs + 5, 1);
-      k_hub_centre<MT, kHubWaves, false><<<(unsigned)n, kHubWaves * kWave, lds_centre, st>>>(
+      k_hub_zero<<<1, 64, 0, st>>>(runs + 5, 2);
+      uint32_t* const cnt_a = reinterpret_cast<uint32_t*>(runs + 5);
+      uint32_t* const cnt_b = reinterpret_cast<uint32_t*>(runs + 6);
+      k_hub_centre<MT, kHubWaves, kHubHashBits><<<(unsigned)n, kHubWaves * kWave, lds_centre, st>>>(
           E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
           hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
           hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), runs + 1, cap,
-          reinterpret_cast<uint32_t*>(runs + 3), accR, a0, 1, hb.big.as<int32_t>(),
-          reinterpret_cast<uint32_t*>(runs + 5), nullptr, lds_keys);
+          reinterpret_cast<uint32_t*>(runs + 3), accR, a0, 1, nullptr, nullptr, hb.big.as<int32_t>(), cnt_a,
+          nullptr, lds_keys);
       HM_HIP(hipGetLastError());
-      unsigned long long r2[5] = {0, 0, 0, 0, 0};
+      unsigned long long r2[6] = {0, 0, 0, 0, 0, 0};
       HM_HIP(hipMemcpyAsync(r2, runs + 1, sizeof(r2), hipMemcpyDeviceToHost, st));
       HM_HIP(hipStreamSynchronize(st));
       const uint32_t nbig = (uint32_t)r2[4];
-      if (nbig > 0) {  // the centres the LDS histogram could not hold, with histograms in global memory
-        const int32_t slices = std::min<int32_t>((int32_t)nbig, kHubBigSlices);
-        HM_TRY(hb.bighash.ensure(sizeof(uint32_t) * 3 * ((size_t)1 << kHubBigBits) * (size_t)slices));
-        k_hub_centre<MT, kHubWaves, true><<<(unsigned)slices, kHubWaves * kWave, (size_t)Tc * sizeof(MT), st>>>(
-            E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
-            hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
-            hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), runs + 1, cap,
-            reinterpret_cast<uint32_t*>(runs + 3), accR, a0, 1, hb.big.as<int32_t>(),
-            reinterpret_cast<uint32_t*>(runs + 5), hb.bighash.as<uint32_t>(), kHubBigKeysMax);
+      // the centres a histogram could not hold, again with the next larger one in global memory
+      uint32_t nlev = nbig;
+      int32_t* ids_in = hb.big.as<int32_t>();
+      int32_t* ids_out = hb.big2.as<int32_t>();
+      uint32_t* n_in = cnt_a;
+      uint32_t* n_out = cnt_b;
+      for (int lv = 0; lv < kHubBigLevels && nlev > 0; ++lv) {
+        const int32_t slices = std::min<int32_t>((int32_t)nlev, kHubBigSlices[lv]);
+        HM_TRY(hb.bighash.ensure(sizeof(uint32_t) * 3 * ((size_t)1 << kHubBigBits[lv]) * (size_t)slices));
+        const bool last = lv + 1 == kHubBigLevels;
+        k_hub_zero<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long*>(n_out), 1);
+        auto launch = [&](auto lvl) {
+          constexpr int HBv = decltype(lvl)::value;
+          k_hub_centre<MT, kHubWaves, HBv><<<(unsigned)slices, kHubWaves * kWave, (size_t)Tc * sizeof(MT), st>>>(
+              E, Tc, c.eoff.as<uint32_t>(), c.enodes.as<int32_t>(), c.voff.as<uint32_t>(), c.vedges.as<int32_t>(),
+              hb.hubidx.as<int8_t>(), hb.hmask.as<uint32_t>(), hb.lsz.as<uint8_t>(), s.deg.as<uint32_t>(),
+              hb.types.as<unsigned long long>(), hb.tcraw.as<unsigned long long>(), runs + 1, cap,
+              reinterpret_cast<uint32_t*>(runs + 3), accR, a0, 1, ids_in, n_in, last ? nullptr : ids_out,
+              last ? nullptr : n_out, hb.bighash.as<uint32_t>(),
+              last ? (1u << HBv) - (1u << HBv) / 8 : std::min<uint32_t>((1u << HBv) - (1u << HBv) / 8, big_keys));
+        };
+        if (lv == 0) launch(std::integral_constant<int, kHubBigBits[0]>{});
+        else if (lv == 1) launch(std::integral_constant<int, kHubBigBits[1]>{});
+        else launch(std::integral_constant<int, kHubBigBits[2]>{});
         HM_HIP(hipGetLastError());
+        uint32_t nnext = 0;
+        if (!last) {
+          HM_HIP(hipMemcpyAsync(&nnext, n_out, sizeof(nnext), hipMemcpyDeviceToHost, st));
+          HM_HIP(hipStreamSynchronize(st));
+        }
+        if (verbose)
+          std::fprintf(stderr, "[hm hub] centres %d..%d: %u with more keys than %s holds, in %d-slot histograms\n",
+                       a0, a0 + n, nlev, lv == 0 ? "LDS" : "the last level", 1 << kHubBigBits[lv]);
+        nlev = nnext;
+        std::swap(ids_in, ids_out);
+        std::swap(n_in, n_out);
+      }
+      if (nbig > 0) {
         HM_HIP(hipMemcpyAsync(r2, runs + 1, sizeof(r2), hipMemcpyDeviceToHost, st));
         HM_HIP(hipStreamSynchronize(st));
-        if (verbose)
-          std::fprintf(stderr, "[hm hub] centres %d..%d: %u with more keys than LDS holds, in global memory\n",
-                       a0, a0 + n, nbig);
       }
       if (r2[2]) {
         *key_overflow = true;
@@ -1250,6 +1289,9 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
       }
       const unsigned long long nt = r2[0];
       if (nt > cap) {  // the range again: a larger buffer, or fewer centres
+        if (verbose)
+          std::fprintf(stderr, "[hm hub] centres %d..%d: %llu pair-type entries outgrew %llu, again\n", a0, a0 + n,
+                       nt, cap);
         if (nt <= type_cap) {
           cap = nt;
         } else {
@@ -1324,6 +1366,10 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
                                                   acc, t0);
       HM_HIP(hipGetLastError());
     }
+  }
+  if (verbose) {
+    HM_HIP(hipStreamSynchronize(st));
+    std::fprintf(stderr, "[hm hub] class triples and %llu pair types x %d classes done\n", nT, K);
   }
   HM_HIP(hipEventRecord(g->ev[3], st));
 
@@ -1486,7 +1532,10 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
   for (int k = 0; k < kStaticBins; ++k) h_acc[kAccHist + k] += hC[kAccHist + k];
   // projection statistics from the projected degrees alone (the wedge identity
   // sum(open) + 3 sum(closed) = sum_a C(d_a, 2) checks the bins against them)
-  if (run.stats) HM_TRY(projection_stats(g, h_acc));
+  if (run.stats) {
+    HM_TRY(projection_stats(g, h_acc));
+    if (verbose) std::fprintf(stderr, "[hm hub] projection statistics done\n");
+  }
   (void)dmax;
   g->hub_stats.light_triangles = (int64_t)h_d[kAccLightTri];
   float m01 = 0, m12 = 0, m23 = 0, m34 = 0, m54 = 0;

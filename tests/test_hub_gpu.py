@@ -205,15 +205,19 @@ def test_split_paths_match_oracle(gpu_ready, monkeypatch, seed):
 
 @pytest.mark.parametrize("seed", range(4))
 def test_global_memory_histograms_match_oracle(gpu_ready, monkeypatch, seed):
-    """Centres whose neighbour keys outgrow the LDS histogram are counted again with a histogram in
-    global memory: with the LDS table cut to 3 keys (HM_HUB_LDS_KEYS) nearly every centre takes that
-    path, in one table and in tiles, and the bins still equal the oracle's."""
+    """Centres whose neighbour keys outgrow the LDS histogram are counted again with histograms in
+    global memory, level by level: with the LDS table cut to 3 keys (HM_HUB_LDS_KEYS) nearly every
+    centre takes that path, and with the first two global levels cut to 3 keys as well
+    (HM_HUB_BIG_KEYS, odd seeds) nearly every centre reaches the last one; in one table and in
+    tiles, the bins still equal the oracle's."""
     rng = np.random.default_rng(9500 + seed)
     kmax = [6, 40][seed % 2]
     off, nodes = random_hypergraph(rng, num_edges=int(rng.integers(150, 500)),
                                    num_nodes=int(rng.integers(kmax + 10, 120)), kmax=kmax, dup_frac=0.2)
     want = O.static_mochy(off, nodes)
     monkeypatch.setenv("HM_HUB_LDS_KEYS", "3")
+    if seed % 2:
+        monkeypatch.setenv("HM_HUB_BIG_KEYS", "3")
     for tile in (0, 64):
         for h in (1, 4, 12):
             print(f"seed={seed} E={off.size - 1} tile={tile} hubs={h}", flush=True)
