@@ -1,0 +1,27 @@
+"""The committed parts of the 2M-hyperedge static count (tests/golden/static_c2_parts, made on an
+MI355X by tests/golden/make_static_c2.py) combine to tests/golden/static_c2_hub.json: one part holds
+the class, pair-type and light-wedge parts, the light-rank ranges tile [0, E), and the bins satisfy
+sum(open) + 3 sum(closed) = sum_a C(d_a, 2) against the projected degrees."""
+import glob
+import json
+import os
+
+from conftest import GOLDEN
+
+OPEN = range(20, 26)
+
+
+def test_static_c2_parts_combine_to_the_fixture():
+    parts = sorted((json.load(open(p)) for p in glob.glob(os.path.join(GOLDEN, "static_c2_parts", "*.json"))),
+                   key=lambda r: r["begin"])
+    fx = json.load(open(os.path.join(GOLDEN, "static_c2_hub.json")))
+    E = fx["params"]["num_edges"]
+    assert sum(r["other_parts"] for r in parts) == 1
+    assert parts[0]["begin"] == 0 and parts[-1]["end"] == E
+    assert all(a["end"] == b["begin"] for a, b in zip(parts, parts[1:]))
+    assert all(r["hubs"] == fx["hubs"] and r["hyperedges"] == E for r in parts)
+    bins = [sum(r["bins"][k] for r in parts) % (1 << 64) for k in range(30)]
+    assert bins == fx["counts30"] and sum(bins) == fx["triples"]
+    n_open = sum(bins[k] for k in OPEN)
+    assert n_open + 3 * (sum(bins) - n_open) == fx["wedges"]
+    assert sum(r["light_triangles"] for r in parts) == fx["light_triangles"]
