@@ -12,9 +12,10 @@ OPEN = range(20, 26)
 
 
 def test_static_c2_parts_combine_to_the_fixture():
-    parts = sorted((json.load(open(p)) for p in glob.glob(os.path.join(GOLDEN, "static_c2_parts", "*.json"))),
-                   key=lambda r: r["begin"])
     fx = json.load(open(os.path.join(GOLDEN, "static_c2_hub.json")))
+    parts = sorted((json.load(open(p)) for p in
+                    glob.glob(os.path.join(GOLDEN, "static_c2_parts", f"c2_h{fx['hubs']}_part_*.json"))),
+                   key=lambda r: r["begin"])
     E = fx["params"]["num_edges"]
     assert sum(r["other_parts"] for r in parts) == 1
     assert parts[0]["begin"] == 0 and parts[-1]["end"] == E
