@@ -17,12 +17,13 @@ Workloads (inputs are built and copied to HBM before the timed region):
   static_c3_base         MoCHy's 30 static bins of the dynamic workload's full 500,000-hyperedge
                          base graph (3.16e15 triples, its largest node in 198,573 hyperedges) by
                          the hub-aware count (hm_graph_set_hubs, hub count from the cost model); a
-                         step takes ~42 s, so this workload defaults to one step, no warmup.
-The full static 1M-node / 2M-hyperedge graph stays out of reach: even the hub-aware count leaves
-~4.3e14 light triangles with 12 hubs (~47 min at the measured rate) and more light-row state than
-HBM holds. The temporal workload is the largest full-size configuration whose step is short
-enough for repeated timing; the default line adds the static counts of c1 and of a 60,000-
-hyperedge graph of the 2M-hyperedge graph's family by the hub-aware count (config.static_hub).
+                         step takes ~24 s, so this workload defaults to one step, no warmup.
+The full static 1M-node / 2M-hyperedge graph counts exactly with the hub-aware count in ~21 min of
+GPU time, split into time-bounded parts (hm_graph_count_hub_part; tests/golden/make_static_c2.py,
+tests/golden/static_c2_hub.json): beyond a bench step. The temporal workload is the largest
+full-size configuration whose step is short enough for repeated timing; the default line adds the
+static counts of c1 and of a 60,000-hyperedge graph of the 2M-hyperedge graph's family by the
+hub-aware count (config.static_hub).
 
 value = h-motif instances classified per second = (sum of the bins) / step time.
 roofline = the dominant kernel (k_tcount, k_tri or k_delta), bound "hbm": achieved = the HBM bytes
@@ -78,11 +79,11 @@ DESC = {
                       "hub-aware count (hubs from the cost model)",
 }
 WHY_DEFAULT = ("the largest full-size configuration with a step short enough for repeated timing: "
-               "the 2M-hyperedge static graph (one node in 780,282 hyperedges, > 7.9e16 triples) is "
-               "beyond per-triple enumeration and, with the hub-aware count, still leaves ~4.3e14 "
-               "light triangles at 12 hubs (~47 min) and more light-row state than HBM; the "
-               "500k-hyperedge dynamic base counts exactly in ~42 s per step (--workload "
-               "static_c3_base)")
+               "the 2M-hyperedge static graph (one node in 780,282 hyperedges, 1.96e17 triples) "
+               "counts exactly with the hub-aware count in ~21 min of GPU time, split into two "
+               "time-bounded parts (tests/golden/static_c2_hub.json; 22 and 23 hubs agree bin for "
+               "bin), beyond a bench step; the 500k-hyperedge dynamic base counts exactly in ~24 s "
+               "per step (--workload static_c3_base)")
 # Bounded CPU samples: a prefix of the same input sized for ~10-30 s of reference-program time.
 CPU_SAMPLE_EDGES = {"temporal_c4": 700, "static_c1": 4000, "dynamic_c3": 4600}
 DEFAULT_STEPS = {"temporal_c4": 3, "static_c1": 20, "dynamic_c3": 10, "serving_c5": 1, "static_c3_base": 1}
