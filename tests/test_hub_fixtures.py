@@ -26,3 +26,19 @@ def test_static_c2_parts_combine_to_the_fixture():
     n_open = sum(bins[k] for k in OPEN)
     assert n_open + 3 * (sum(bins) - n_open) == fx["wedges"]
     assert sum(r["light_triangles"] for r in parts) == fx["light_triangles"]
+
+
+def test_serving_c5_large_counts_hold_the_wedge_identity():
+    """Every graph of the serving configuration above the served band was counted at its own size
+    (tests/golden/make_c5_large_counts.py), and its bins satisfy the wedge identity."""
+    import sys
+
+    sys.path.insert(0, GOLDEN)
+    import make_c5_large_counts as M
+
+    fx = json.load(open(os.path.join(GOLDEN, "serving_c5_large_counts.json")))
+    assert [(r["graph"], r["hyperedges"]) for r in fx["counts"]] == M.large_graphs()
+    for r in fx["counts"]:
+        c = r["counts30"]
+        n_open = sum(c[k] for k in OPEN)
+        assert sum(c) == r["triples"] and n_open + 3 * (sum(c) - n_open) == r["wedges"], r["graph"]
