@@ -178,7 +178,8 @@ hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out);
  * With hubs != 0, hm_graph_update's delta is hub-aware too: the class, pair-type and light-wedge
  * parts of the graph before and after the batch, and the light triangles that hold a changed
  * hyperedge (deleted ones before the batch, inserted ones after it), with one hub set for both
- * graphs (chosen on the graph before; the automatic hub count is kept across updates). If the
+ * graphs (chosen on the graph before; the automatic hub count, at most 12 for a delta, is kept
+ * across updates). If the
  * graph after the batch overflows a neighbour-key histogram with that hub set, the batch stays
  * applied and the update fails (recount the graph). */
 hm_status hm_graph_set_hubs(hm_graph* g, int32_t hubs);

@@ -279,6 +279,7 @@ namespace hm {
 // Most hubs the hub-aware count takes (a neighbour key holds the hub pattern in 23 bits beside the
 // overlap and a flag).
 constexpr int32_t kHubMax = 23;
+constexpr int32_t kHubDeltaHubs = 12;  // most hubs of an automatic hub-aware delta
 // One run of the hub-aware pipeline: an optional pinned hub set, ids ranked first in the light
 // order (with only_first, (D) runs over those centres alone: the light triangles holding one of
 // them), and whether the projection statistics are computed.
