@@ -164,11 +164,20 @@ struct HubScratch {
   DevBuf big;      // int32[E]    centres whose keys outgrew the LDS histogram (and the next levels')
   DevBuf big2;     // int32[E]    the other list of the level-by-level passes
   DevBuf bighash;  // uint32[]    their histograms in global memory (3 * 2^HB words per workgroup)
+  // every buffer above, for release() and the device-memory accounting alike
+  template <typename Self, typename F>
+  static void each_of(Self& s, F f) {
+    for (auto* b : {&s.hubidx, &s.hmask, &s.lsz, &s.ckey, &s.ckey2, &s.ukey, &s.ucnt, &s.nruns, &s.hmr, &s.lszr,
+                    &s.types, &s.tcraw, &s.types2, &s.tcsrt, &s.tkey, &s.tcnt, &s.rkey, &s.first, &s.firstids,
+                    &s.accd, &s.accr, &s.big, &s.big2, &s.bighash})
+      f(*b);
+  }
+  template <typename F>
+  void each(F f) { each_of(*this, f); }
+  template <typename F>
+  void each(F f) const { each_of(*this, f); }
   void release() {
-    for (DevBuf* b : {&hubidx, &hmask, &lsz, &ckey, &ckey2, &ukey, &ucnt, &nruns, &hmr, &lszr, &types,
-                      &tcraw, &types2, &tcsrt, &tkey, &tcnt, &rkey, &first, &firstids, &accd, &accr, &big,
-                      &big2, &bighash})
-      b->release();
+    each([](DevBuf& b) { b.release(); });
   }
   // the pair-type buffers, once (B) has run (they are the largest of a big graph's count)
   void release_types() {

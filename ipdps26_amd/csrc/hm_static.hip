@@ -870,10 +870,7 @@ int64_t graph_device_bytes(const hm_graph* g) {
                           &d.srt_node, &d.srt_edge, &d.nvoff, &d.slen, &d.soff, &d.scur, &d.spm, &d.ord})
     add(*b);
   add(g->uplan.in);
-  const HubScratch& h = g->hub;
-  for (const DevBuf* b : {&h.hubidx, &h.hmask, &h.lsz, &h.ckey, &h.ckey2, &h.ukey, &h.ucnt, &h.nruns, &h.hmr,
-                          &h.lszr, &h.types, &h.types2, &h.tkey, &h.tcnt})
-    add(*b);
+  g->hub.each([&](const DevBuf& b) { add(b); });
   return n;
 }
 
