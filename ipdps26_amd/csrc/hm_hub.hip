@@ -1217,7 +1217,9 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
     unsigned long long cap = std::max<unsigned long long>(
         1, std::min<unsigned long long>({lp_bound, std::max<unsigned long long>((unsigned long long)E * 1024, 1u << 20),
                                          type_cap}));
-    int32_t a0 = 0, span = std::min<int32_t>(E, kHubCentreSpan);
+    // one range up to 2^16 centres; beyond, the first range is an eighth of the longest and each
+    // later one is sized from the entries per centre of the one before, to fill ~80% of the buffer
+    int32_t a0 = 0, span = E <= (1 << 16) ? E : std::max<int32_t>(1, std::min<int32_t>(E, kHubCentreSpan) / 8);
     while (a0 < E) {
       const int32_t n = std::min(span, E - a0);
       HM_TRY(hb.types.ensure(sizeof(unsigned long long) * cap));
@@ -1341,6 +1343,8 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
         std::fprintf(stderr, "[hm hub] light neighbours: centres %d..%d of %d done, %llu pair types\n", a0, a0 + n,
                      E, nT);
       a0 += n;
+      span = (int32_t)std::min<double>(kHubCentreSpan,
+                                       std::max(1.0, (double)n * 0.8 * (double)cap / (double)std::max(nt, 1ull)));
     }
   }
   g->hub_stats.light_pairs = -1;
