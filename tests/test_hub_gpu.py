@@ -370,3 +370,23 @@ def test_dynamic_workload_full_size_first_batches(gpu_ready):
             print(f"batch {b}: sum|delta| {int(np.abs(delta).sum())}, {gr.update_stats()}", flush=True)
             assert delta.tolist() == g["deltas30"][b - 1]
             live = np.concatenate([np.setdiff1d(live, dels), ids])
+
+
+def test_census_graph_counts_match_the_fixture(gpu_ready):
+    """The smallest of the serving configuration's graphs above the served band (191,373
+    hyperedges): the automatic hub-aware count and a two-part count (hm_graph_count_hub_part)
+    equal the census vector (tests/golden/serving_c5_large_counts.json)."""
+    fx = json.load(open(os.path.join(GOLDEN, "serving_c5_large_counts.json")))["counts"][0]
+    off, nodes = W._serving_graph(fx["hyperedges"], fx["graph"], 5)
+    assert off.size - 1 == fx["hyperedges"]
+    with hm.Graph(off, nodes) as g:
+        g.set_hubs(-1)
+        c = g.count_static().astype(np.int64)
+        hs = g.hub_stats()
+        print(f"graph {fx['graph']}: {int(c.sum())} triples, {hs}", flush=True)
+        assert c.tolist() == fx["counts30"]
+        h = hs["hubs"]
+        E = off.size - 1
+        a, _ = g.count_hub_part(h, 0, E // 3, other_parts=True)
+        b, _ = g.count_hub_part(h, E // 3, -1, other_parts=False)
+        assert (a + b).astype(np.int64).tolist() == fx["counts30"]
