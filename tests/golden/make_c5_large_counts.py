@@ -86,9 +86,11 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--first", type=int, default=0, help="first graph (in ascending size) to count")
     ap.add_argument("--seconds", type=float, default=900.0, help="no new graph starts after this long")
-    ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out", "c5large"))
+    ap.add_argument("--out", help="directory for the counts' JSON files")
     ap.add_argument("--combine", metavar="DIR")
     args = ap.parse_args()
+    if not args.combine and not args.out:
+        ap.error("--out is required for counting")
     return combine(args) if args.combine else run(args)
 
 

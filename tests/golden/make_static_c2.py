@@ -114,9 +114,11 @@ def main() -> int:
     ap.add_argument("--other", action="store_true", help="also count the class, pair-type and light-wedge parts")
     ap.add_argument("--begin", type=int, default=0, help="first light rank of the light-triangle part")
     ap.add_argument("--seconds", type=float, default=0.0, help="time limit of the light-triangle part")
-    ap.add_argument("--out", default=os.path.join(ROOT, "gpurun_out", "c2"))
+    ap.add_argument("--out", help="directory for the part's JSON file")
     ap.add_argument("--combine", metavar="DIR", help="combine the parts in DIR")
     args = ap.parse_args()
+    if not args.combine and not args.out:
+        ap.error("--out is required for a part")
     return combine(args) if args.combine else run_part(args)
 
 
