@@ -164,12 +164,14 @@ struct HubScratch {
   DevBuf big;      // int32[E]    centres whose keys outgrew the LDS histogram (and the next levels')
   DevBuf big2;     // int32[E]    the other list of the level-by-level passes
   DevBuf bighash;  // uint32[]    their histograms in global memory (3 * 2^HB words per workgroup)
+  DevBuf gtab;     // uint32[2^(h+1)] class hyperedges by hub-mask subset sums, per light flag
+  DevBuf slow;     // uint32[]    pair types with too many hubs for the subset sums
   // every buffer above, for release() and the device-memory accounting alike
   template <typename Self, typename F>
   static void each_of(Self& s, F f) {
     for (auto* b : {&s.hubidx, &s.hmask, &s.lsz, &s.ckey, &s.ckey2, &s.ukey, &s.ucnt, &s.nruns, &s.hmr, &s.lszr,
                     &s.types, &s.tcraw, &s.types2, &s.tcsrt, &s.tkey, &s.tcnt, &s.rkey, &s.first, &s.firstids,
-                    &s.accd, &s.accr, &s.big, &s.big2, &s.bighash})
+                    &s.accd, &s.accr, &s.big, &s.big2, &s.bighash, &s.gtab, &s.slow})
       f(*b);
   }
   template <typename F>

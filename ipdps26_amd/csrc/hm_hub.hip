@@ -429,11 +429,13 @@ __global__ __launch_bounds__(kBlock) void k_hub_types(int32_t K, int32_t k0,
                                                       const uint32_t* __restrict__ ucnt,
                                                       const unsigned long long* __restrict__ tkey,
                                                       const unsigned long long* __restrict__ tcnt,
-                                                      unsigned long long* __restrict__ acc, unsigned long long t0) {
+                                                      unsigned long long* __restrict__ acc, unsigned long long t0,
+                                                      const uint32_t* __restrict__ list) {
   __shared__ long long hs[kHubIds * kHubCopies];
   for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0;
   __syncthreads();
-  const unsigned long long ti = t0 + blockIdx.x;  // type t0 + blockIdx.x (launches of kHubLaunch types)
+  // type t0 + blockIdx.x (launches of kHubLaunch types), or the list's entry there
+  const unsigned long long ti = list ? (unsigned long long)list[t0 + blockIdx.x] : t0 + blockIdx.x;
   const unsigned long long t = tkey[ti];
   const uint32_t s1 = (uint32_t)(t >> 32), s2 = (uint32_t)t;
   const uint32_t Aa = s1 >> 1, Ab = s2 >> 1;
@@ -460,6 +462,95 @@ __global__ __launch_bounds__(kBlock) void k_hub_types(int32_t K, int32_t k0,
   }
   __syncthreads();
   fold_ids(hs, (unsigned long long)tcnt[ti], acc);
+}
+
+// Subset sums of the classes over hub masks, for (B) without a pass over the classes per type:
+// gtab[l << h | X] = hyperedges of the classes (A, l) with A a subset of X (X < 2^h).
+__global__ void k_hub_gtab_scatter(int32_t K, int hh, const unsigned long long* __restrict__ ukey,
+                                   const uint32_t* __restrict__ ucnt, uint32_t* __restrict__ gtab) {
+  const int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= K) return;
+  const unsigned long long key = ukey[c];  // hub mask << 1 | [light part non-empty]; keys distinct
+  gtab[((size_t)(key & 1ull) << hh) | (size_t)(key >> 1)] = ucnt[c];
+}
+// One subset-sum step over bit b: every mask with bit b adds the mask without it (which this step
+// does not change).
+__global__ void k_hub_gtab_sos(int hh, int b, uint32_t* __restrict__ gtab) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ((size_t)2 << hh)) return;
+  if ((i >> b) & 1u) gtab[i] += gtab[i ^ ((size_t)1 << b)];
+}
+
+constexpr int kHubSosMaxBits = 7;  // types whose two hub masks hold more hubs go through k_hub_types
+
+// (B) by subset sums: thread per pair type t = (Aa, fa, Ab, fb) with nt light pairs. The classes'
+// contribution depends on A(c) only through S = A(c) ∩ (Aa ∪ Ab) and [A(c) has a hub outside], so
+// the hyperedges per (S, outside, l) come from gtab by inclusion-exclusion over the subsets of S
+// (3^|Aa ∪ Ab| lookups); each adds nt * n (F(P1ab) - F(P0)) as k_hub_types does class by class.
+// Types with more than kHubSosMaxBits hubs in Aa ∪ Ab are listed for k_hub_types instead.
+__global__ __launch_bounds__(kBlock) void k_hub_types_sos(unsigned long long t0, unsigned long long nT, int hh,
+                                                          const unsigned long long* __restrict__ tkey,
+                                                          const unsigned long long* __restrict__ tcnt,
+                                                          const uint32_t* __restrict__ gtab,
+                                                          uint32_t* __restrict__ slow, uint32_t* __restrict__ nslow,
+                                                          unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long hs[kHubIds * kHubCopies];
+  for (int k = threadIdx.x; k < kHubIds * kHubCopies; k += blockDim.x) hs[k] = 0ull;
+  __syncthreads();
+  unsigned long long* hc = hs + (lane_id() & (kHubCopies - 1));
+  const uint32_t full = (hh >= 32) ? 0xFFFFFFFFu : ((1u << hh) - 1u);
+  const uint32_t* g0 = gtab;
+  const uint32_t* g1 = gtab + ((size_t)1 << hh);
+  for (unsigned long long t = t0 + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < t0 + nT;
+       t += (unsigned long long)gridDim.x * blockDim.x) {
+    const unsigned long long key = tkey[t];
+    const uint32_t s1 = (uint32_t)(key >> 32), s2 = (uint32_t)key;
+    const uint32_t Aa = s1 >> 1, Ab = s2 >> 1;
+    const unsigned fa = s1 & 1u, fb = s2 & 1u;
+    const uint32_t U = Aa | Ab;
+    if (__popc(U) > kHubSosMaxBits) {
+      slow[atomicAdd(nslow, 1u)] = (uint32_t)t;
+      continue;
+    }
+    const unsigned long long w = tcnt[t];
+    const uint32_t Ro = Aa & ~Ab, Rb = Ab & ~Aa, Rab = Aa & Ab, NU = full & ~U;
+    auto add = [&](unsigned id, unsigned long long v) { atomicAdd(&hc[id * kHubCopies], v); };
+    for (uint32_t S = U;; S = (S - 1u) & U) {
+      long long in0 = 0, in1 = 0, all0 = 0, all1 = 0;  // classes with A ∩ U = S: inside U / any
+      const int ps = __popc(S);
+      for (uint32_t T = S;; T = (T - 1u) & S) {
+        const long long sg = ((ps - __popc(T)) & 1) ? -1 : 1;
+        in0 += sg * (long long)g0[T];
+        in1 += sg * (long long)g1[T];
+        all0 += sg * (long long)g0[T | NU];
+        all1 += sg * (long long)g1[T | NU];
+        if (T == 0u) break;
+      }
+      const unsigned hin = nzm(Ro & ~S) | nzm(Rb & ~S) << 1 | nzm(Rab & ~S) << 3 | nzm(Rb & S) << 4 |
+                           nzm(Ro & S) << 5 | nzm(Rab & S) << 6;
+      const long long n[2][2] = {{in0, all0 - in0}, {in1, all1 - in1}};  // [l][outside]
+      for (unsigned l = 0; l < 2; ++l)
+        for (unsigned out = 0; out < 2; ++out) {
+          const long long c = n[l][out];
+          if (c == 0) continue;
+          const unsigned h = hin | out << 2;
+          const unsigned long long v = w * (unsigned long long)c;
+          add(h | 8u | fa | fb << 1 | l << 2, v);
+          add(h | 3u | l << 2, 0ull - v);
+        }
+      if (S == 0u) break;
+    }
+    // the pair's own two hyperedges (classes (Aa, 1) and (Ab, 1)) are not third hyperedges
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t Ac = side ? Ab : Aa;
+      const unsigned h = nzm(Ro & ~Ac) | nzm(Rb & ~Ac) << 1 | nzm(Ac & ~U) << 2 | nzm(Rab & ~Ac) << 3 |
+                         nzm(Rb & Ac) << 4 | nzm(Ro & Ac) << 5 | nzm(Rab & Ac) << 6;
+      add(h | 8u | fa | fb << 1 | 4u, 0ull - w);
+      add(h | 3u | 4u, w);
+    }
+  }
+  __syncthreads();
+  fold_ids(hs, 1ull, acc);
 }
 
 // Sort keys of the light-projection order: ascending light degree, with the hyperedges flagged in
@@ -1362,13 +1453,39 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
 
   // (B) every distinct pair type against every class
   g->hub_stats.pair_types = (int64_t)nT;
-  if (run.abc && K > k0) {
-    for (unsigned long long t0 = 0; t0 < nT; t0 += kHubLaunch) {
-      const unsigned long long n = std::min(kHubLaunch, nT - t0);
-      k_hub_types<<<(unsigned)n, kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(), hb.ucnt.as<uint32_t>(),
-                                                  hb.tkey.as<unsigned long long>(), hb.tcnt.as<unsigned long long>(),
-                                                  acc, t0);
+  if (run.abc && K > k0 && nT > 0) {
+    // subset sums of the classes over hub masks, then the types in chunks: those with few hubs by
+    // the sums, the others listed and run against every class
+    HM_TRY(hb.gtab.ensure(sizeof(uint32_t) * ((size_t)2 << hh)));
+    HM_HIP(hipMemsetAsync(hb.gtab.p, 0, sizeof(uint32_t) * ((size_t)2 << hh), st));
+    k_hub_gtab_scatter<<<(unsigned)((K + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+        K, hh, hb.ukey.as<unsigned long long>(), hb.ucnt.as<uint32_t>(), hb.gtab.as<uint32_t>());
+    HM_HIP(hipGetLastError());
+    for (int b = 0; b < hh; ++b) {
+      k_hub_gtab_sos<<<(unsigned)((((size_t)2 << hh) + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+          hh, b, hb.gtab.as<uint32_t>());
       HM_HIP(hipGetLastError());
+    }
+    const unsigned long long chunk = kHubLaunch * 4;
+    HM_TRY(hb.slow.ensure(sizeof(uint32_t) * (size_t)std::min(nT, chunk)));
+    uint32_t* nslow = reinterpret_cast<uint32_t*>(runs + 7);
+    for (unsigned long long t0 = 0; t0 < nT; t0 += chunk) {
+      const unsigned long long n = std::min(chunk, nT - t0);
+      k_hub_zero<<<1, 64, 0, st>>>(runs + 7, 1);
+      k_hub_types_sos<<<(unsigned)std::min<unsigned long long>((n + kBlock - 1) / kBlock, 4096), kBlock, 0, st>>>(
+          t0, n, hh, hb.tkey.as<unsigned long long>(), hb.tcnt.as<unsigned long long>(), hb.gtab.as<uint32_t>(),
+          hb.slow.as<uint32_t>(), nslow, acc);
+      HM_HIP(hipGetLastError());
+      uint32_t ns = 0;
+      HM_HIP(hipMemcpyAsync(&ns, nslow, sizeof(ns), hipMemcpyDeviceToHost, st));
+      HM_HIP(hipStreamSynchronize(st));
+      for (unsigned long long s0 = 0; s0 < ns; s0 += kHubLaunch) {
+        const unsigned long long m = std::min<unsigned long long>(kHubLaunch, ns - s0);
+        k_hub_types<<<(unsigned)m, kBlock, 0, st>>>(K, k0, hb.ukey.as<unsigned long long>(), hb.ucnt.as<uint32_t>(),
+                                                    hb.tkey.as<unsigned long long>(), hb.tcnt.as<unsigned long long>(),
+                                                    acc, s0, hb.slow.as<uint32_t>());
+        HM_HIP(hipGetLastError());
+      }
     }
   }
   if (verbose) {
