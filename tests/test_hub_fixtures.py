@@ -42,3 +42,38 @@ def test_serving_c5_large_counts_hold_the_wedge_identity():
         c = r["counts30"]
         n_open = sum(c[k] for k in OPEN)
         assert sum(c) == r["triples"] and n_open + 3 * (sum(c) - n_open) == r["wedges"], r["graph"]
+
+
+def _parts_module():
+    import importlib.util
+    import sys
+
+    sys.path.insert(0, GOLDEN)
+    spec = importlib.util.spec_from_file_location("make_static_c2", os.path.join(GOLDEN, "make_static_c2.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_static_c2_combine_refuses_bad_part_sets():
+    """The combine step of tests/golden/make_static_c2.py refuses parts that leave a gap between
+    ranges, count the other parts twice, or do not end at E, and accepts the committed set."""
+    import copy
+
+    import pytest
+
+    M = _parts_module()
+    parts = [json.load(open(p)) for p in glob.glob(os.path.join(GOLDEN, "static_c2_parts", "c2_h23_part_*.json"))]
+    M.combine_hubs(copy.deepcopy(parts))  # the committed set holds
+    gap = copy.deepcopy(parts)
+    min(gap, key=lambda r: r["begin"])["end"] -= 1
+    twice = copy.deepcopy(parts)
+    for r in twice:
+        r["other_parts"] = True
+    short = copy.deepcopy(parts)
+    max(short, key=lambda r: r["begin"])["end"] -= 1
+    wrong = copy.deepcopy(parts)
+    wrong[0]["bins"][3] += 1
+    for bad in (gap, twice, short, wrong):
+        with pytest.raises(AssertionError):
+            M.combine_hubs(bad)
