@@ -171,7 +171,8 @@ hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out);
  * pairs of light neighbours of every hyperedge grouped by key, and the light triangles, without
  * enumerating the cliques that hubs put into the projection; hubs = -1: the hub count is chosen by
  * a cost model (and lowered by a quarter while some hyperedge has more distinct neighbour keys
- * than its LDS histogram holds; if even one hub overflows, the triangle path counts). The 30 bins are identical
+ * than the largest histogram holds — 7,168 in LDS, then up to 917,504 in global memory; if even
+ * one hub overflows, the triangle path counts). The 30 bins are identical
  * either way. A forced hub count whose histograms overflow fails with HM_ERR_UNSUPPORTED. After a
  * hub-aware count hm_graph_stats holds the projection statistics computed from the projected
  * degrees alone (no rows: upper-row totals are not kept).

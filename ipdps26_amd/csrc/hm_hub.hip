@@ -1236,7 +1236,9 @@ static hm_status count_static_hub_impl(hm_graph* g, int32_t h, const HubRun& run
   unsigned long long* accC = s.acc2.as<unsigned long long>();
   HM_TRY(hb.accd.ensure(sizeof(unsigned long long) * kAccCount));
   unsigned long long* accD = hb.accd.as<unsigned long long>();
-  // runs: [0] classes, [1] pair-type entries, [2] distinct pair types, [3] key-overflow flag
+  // runs: [0] classes, [1] pair-type entries, [2] distinct pair types, [3] key-overflow flag,
+  // [4] fill capacity flag, [5] / [6] centres listed for the next key-histogram level, [7] pair
+  // types listed for the class-by-class kernel
   unsigned long long* runs = hb.nruns.as<unsigned long long>();
   const unsigned eb = (unsigned)((E + kBlock - 1) / kBlock);
   const int tbits = 32 + hh + 1;

@@ -1077,8 +1077,9 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
     HM_HIP(hipSetDevice(g->device));
     g->hub_stats.hubs = 0;
     if (g->hubs != 0) {
-      // forced hub count: its result or its error; automatic: the cost model's choice, halved
-      // while some hyperedge's neighbour keys overflow their histogram, else the triangle path
+      // forced hub count: its result or its error; automatic: the cost model's choice, lowered by
+      // a quarter while some hyperedge's neighbour keys overflow the largest global histogram
+      // (917,504 keys), else the triangle path
       int32_t h = g->hubs;
       if (h < 0) {  // the cost model's choice is kept while the store is unchanged
         if (g->hub_auto_version != g->version || g->hub_auto_h <= 0) {
