@@ -17,7 +17,7 @@ Workloads (inputs are built and copied to HBM before the timed region):
   static_c3_base         MoCHy's 30 static bins of the dynamic workload's full 500,000-hyperedge
                          base graph (3.16e15 triples, its largest node in 198,573 hyperedges) by
                          the hub-aware count (hm_graph_set_hubs, hub count from the cost model); a
-                         step takes ~20 s, so this workload defaults to one step, no warmup.
+                         step takes ~18 s, so this workload defaults to one step, no warmup.
 The full static 1M-node / 2M-hyperedge graph counts exactly with the hub-aware count in ~21 min of
 GPU time, split into time-bounded parts (hm_graph_count_hub_part; tests/golden/make_static_c2.py,
 tests/golden/static_c2_hub.json): beyond a bench step. The temporal workload is the largest
@@ -82,7 +82,7 @@ WHY_DEFAULT = ("the largest full-size configuration with a step short enough for
                "the 2M-hyperedge static graph (one node in 780,282 hyperedges, 1.96e17 triples) "
                "counts exactly with the hub-aware count in ~21 min of GPU time, split into two "
                "time-bounded parts (tests/golden/static_c2_hub.json; 22 and 23 hubs agree bin for "
-               "bin), beyond a bench step; the 500k-hyperedge dynamic base counts exactly in ~20 s "
+               "bin), beyond a bench step; the 500k-hyperedge dynamic base counts exactly in ~18 s "
                "per step (--workload static_c3_base)")
 # Bounded CPU samples: a prefix of the same input sized for ~10-30 s of reference-program time.
 CPU_SAMPLE_EDGES = {"temporal_c4": 700, "static_c1": 4000, "dynamic_c3": 4600}

@@ -175,7 +175,8 @@ hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out);
  * one hub overflows, the triangle path counts). The 30 bins are identical
  * either way. A forced hub count whose histograms overflow fails with HM_ERR_UNSUPPORTED. After a
  * hub-aware count hm_graph_stats holds the projection statistics computed from the projected
- * degrees alone (no rows: upper-row totals are not kept).
+ * degrees alone (no rows: upper-row totals are not kept), by the first hm_graph_get_stats after
+ * the count (they are not part of the bins, so the count itself does not compute them).
  * With hubs != 0, hm_graph_update's delta is hub-aware too: the class, pair-type and light-wedge
  * parts of the graph before and after the batch, and the light triangles that hold a changed
  * hyperedge (deleted ones before the batch, inserted ones after it), with one hub set for both

@@ -1530,7 +1530,7 @@ static hm_status update_hub(hm_graph* g, DynScratch& d, const int32_t* del_ids, 
   g->ms_upd_delta = (float)(ms_before + std::chrono::duration<double, std::milli>(clk::now() - t1).count());
   HM_HIP(hipEventSynchronize(g->ev[7]));
   HM_HIP(hipEventElapsedTime(&g->ms_upd_store, g->ev[6], g->ev[7]));
-  g->have_stats = false;
+  g->have_stats = g->stats_lazy = false;
   return HM_OK;
 }
 
@@ -1599,7 +1599,7 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
       g->upd_removed = (int64_t)h[kDynAccRemoved];
       g->upd_added = (int64_t)h[kDynAccAdded];
       g->upd_entries = (int64_t)h[kDynAccEntries];
-      g->have_stats = false;
+      g->have_stats = g->stats_lazy = false;
       return HM_OK;
     }
   }
@@ -1640,7 +1640,7 @@ hm_status hm_graph_update(hm_graph* g, const int32_t* del_ids, int32_t n_del,
   }
   HM_HIP(hipEventSynchronize(g->ev[7]));
   HM_HIP(hipEventElapsedTime(&g->ms_upd_store, g->ev[3], g->ev[7]));
-  g->have_stats = false;
+  g->have_stats = g->stats_lazy = false;
   return HM_OK;
 }
 

@@ -257,6 +257,10 @@ struct hm_graph {
   std::vector<uint8_t> live;
   hm_graph_stats stats{};
   bool have_stats = false;
+  // a hub-aware count leaves its projection statistics to the first hm_graph_get_stats (they are
+  // not part of the bins): stats_lazy with the count's bins, for that call to complete `stats`
+  bool stats_lazy = false;
+  uint64_t lazy_bins[HM_STATIC_BINS] = {};
   float ms_project = 0.f, ms_triangles = 0.f;
   // id tile of the static count: 0 = automatic (one LDS table when E fits), > 0 = forced tiles
   int32_t tile = 0;
@@ -315,7 +319,8 @@ hm_status hub_parts(hm_graph* g, int32_t h, const HubRun& run, unsigned long lon
 // triangle path (30 bins; the projection statistics are not computed). *key_overflow is set when
 // some hyperedge's light neighbours hold more distinct keys than its LDS histogram (fewer hubs
 // then fit).
-hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow);
+hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow,
+                           bool stats = true);
 // Hub count minimising the hub-aware cost model.
 hm_status hub_choose(hm_graph* g, int32_t* h_out, double* cost_out);
 // Projection statistics (sum and max of the projected degrees, wedges, oriented row sum, largest

@@ -1684,9 +1684,10 @@ hm_status hub_parts(hm_graph* g, int32_t h, const HubRun& run, unsigned long lon
                               : count_static_hub_impl<uint32_t>(g, h, run, h_abc, h_d, key_overflow);
 }
 
-hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow) {
+hm_status count_static_hub(hm_graph* g, int32_t h, unsigned long long* h_acc, bool* key_overflow, bool stats) {
   HubRun run;
   run.hubs_out = &g->hub_nodes;
+  run.stats = stats;
   unsigned long long d[kAccCount];
   HM_TRY(hub_parts(g, h, run, h_acc, d, key_overflow));
   for (int k = 0; k < kStaticBins; ++k) h_acc[kAccHist + k] += d[kAccHist + k];

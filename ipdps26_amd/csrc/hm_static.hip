@@ -940,6 +940,7 @@ static void fill_count_stats(hm_graph* g, const uint64_t out[HM_STATIC_BINS], co
   st.triangles = closed;
   st.max_node_degree = (int64_t)acc[kAccMaxNodeDeg];
   g->have_stats = true;
+  g->stats_lazy = false;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1091,10 +1092,11 @@ hm_status count_static(hm_graph* g, uint64_t out[HM_STATIC_BINS]) {
       }
       for (;;) {
         bool overflow = false;
-        const hm_status st = count_static_hub(g, h, acc, &overflow);
+        const hm_status st = count_static_hub(g, h, acc, &overflow, /*stats*/ false);
         if (st == HM_OK) {
-          for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = acc[kAccHist + k];
-          fill_count_stats(g, out, acc);
+          for (int k = 0; k < HM_STATIC_BINS; ++k) out[k] = g->lazy_bins[k] = acc[kAccHist + k];
+          g->have_stats = true;  // completed by the first hm_graph_get_stats
+          g->stats_lazy = true;
           return HM_OK;
         }
         if (g->hubs > 0 || !overflow) return st;
@@ -1319,6 +1321,12 @@ hm_status hm_graph_get_stats(hm_graph* g, hm_graph_stats* out) {
   hm::clear_error();
   if (!g || !out) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_get_stats: null argument");
   if (!g->have_stats) return hm::fail(HM_ERR_BAD_INPUT, "hm_graph_get_stats: no count has run");
+  if (g->stats_lazy) {  // the projection statistics of the last (hub-aware) count, from the degrees
+    unsigned long long acc[hm::kAccCount] = {0};
+    HM_HIP(hipSetDevice(g->device));
+    HM_TRY(hm::projection_stats(g, acc));
+    hm::fill_count_stats(g, g->lazy_bins, acc);
+  }
   *out = g->stats;
   return HM_OK;
 }
